@@ -1,0 +1,269 @@
+/*
+ * pga.h - C ABI of the MI355X-native shape-grammar derivation engine.
+ *
+ * The ABI replaces the derivation path of the reference's `pga` C++ API:
+ *   - PGA::SinglePhaseEvaluator<ProcList, ...>::initialize/execute/release
+ *     (include/pga/core/GPUEvaluator.cuh:475-536, CPUEvaluator.h:393-459)
+ *   - the interpreter facade initializePGA/executePGA/releasePGA/destroyPGA
+ *     (src/interpreter/PGAFacade.h:17-21, src/interpreter/PGAFacade.cu:897-923)
+ *   - the dispatch table (include/pga/core/DispatchTable.h:16-104,
+ *     DispatchTableEntry.h:8-36) and procedure list
+ *     (include/pga/compiler/ProcedureList.h:18-69).
+ *
+ * Plain pointers and sizes only: no HIP or torch types cross this boundary.
+ * Device pointers are passed as `void*`/typed pointers that the caller
+ * obtained from hipMalloc (or torch); a stream is passed as `void*`
+ * holding a hipStream_t.
+ */
+#ifndef PGA_H
+#define PGA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define PGA_API __attribute__((visibility("default")))
+#else
+#define PGA_API
+#endif
+
+/* ---- limits: PGA::Constants (include/pga/core/GlobalConstants.h:9-14) ---- */
+#define PGA_MAX_NUM_PARAMETERS 17
+#define PGA_MAX_NUM_PARAMETER_VALUES 16
+#define PGA_MAX_NUM_SUCCESSORS 20
+#define PGA_MAX_NUM_AXIOM_SEEDS 32
+#define PGA_MAX_EXP_STACK 4
+/* instanced meshes per scene (include/pga/rendering/RenderingConstants.h) */
+#define PGA_MAX_TERMINALS 64
+
+/* ---- status codes (the reference traps or throws; see each code) ---- */
+typedef enum pga_status {
+    PGA_OK = 0,
+    PGA_ERR_INVALID_ARG = 1,
+    PGA_ERR_OVERFLOW_QUEUE = 2,      /* queueDistLocks.cuh:65-70 trap */
+    PGA_ERR_OVERFLOW_INSTANCES = 3,  /* GenerationFunctions.cuh:175 wrap */
+    PGA_ERR_BAD_PROGRAM = 4,         /* invariant checks, Proc.cuh:40-49 */
+    PGA_ERR_SUBDIV_OVERCONSTRAINED = 5, /* Subdivide.cuh:208-216 trap */
+    PGA_ERR_REPEAT_BAD_EXTENT = 6,   /* Repeat.cuh:73-96 trap */
+    PGA_ERR_HIP = 7,                 /* CUDAException.h:8-56 */
+    PGA_ERR_MAX_DEPTH = 8,           /* CPUEvaluator MaxDerivationSteps */
+    PGA_ERR_UNSUPPORTED = 9,
+    PGA_ERR_NOT_INITIALIZED = 10
+} pga_status;
+
+/* ---- PGA::ParameterType (include/pga/core/ParameterType.h) ---- */
+typedef enum pga_parameter_type {
+    PGA_PT_SCALAR = 0,
+    PGA_PT_RAND = 1,
+    PGA_PT_SHAPE_ATTR = 2,
+    PGA_PT_EXP = 3,
+    PGA_PT_VEC2 = 4,
+    PGA_PT_VEC4 = 5,
+    /* project extensions (no reference counterpart):
+     *  PHILOX      value[0]=min, value[1]=max, value[2]=stream id;
+     *              min + (max-min)*u, u from Philox4x32-10 keyed by the
+     *              context key, counter (axiom, path, depth, stream).
+     *  AXIOM_PARAM value[0]=slot; the per-axiom parameter
+     *              params[axiom * n_params + slot] (differentiable). */
+    PGA_PT_PHILOX = 64,
+    PGA_PT_AXIOM_PARAM = 65
+} pga_parameter_type;
+
+/* ---- expression operand/operation codes (OperandType.h, OperationType.h) */
+typedef enum pga_operand_type {
+    PGA_ORT_SCALAR = 0,
+    PGA_ORT_RAND = 1,
+    PGA_ORT_SHAPE_ATTR = 2,
+    PGA_ORT_OP = 3
+} pga_operand_type;
+
+typedef enum pga_operation_type {
+    PGA_OPT_ADD = 0, PGA_OPT_SUB, PGA_OPT_DIV, PGA_OPT_MULTI,
+    PGA_OPT_EQ, PGA_OPT_NEQ, PGA_OPT_LT, PGA_OPT_GT,
+    PGA_OPT_LEQ, PGA_OPT_GEQ, PGA_OPT_AND, PGA_OPT_OR
+} pga_operation_type;
+
+/* ---- PGA::Compiler::OperatorType (include/pga/compiler/OperatorType.h) */
+typedef enum pga_operator_type {
+    PGA_OP_TRANSLATE = 1,
+    PGA_OP_ROTATE = 2,
+    PGA_OP_SCALE = 3,
+    PGA_OP_EXTRUDE = 4,
+    PGA_OP_COMPSPLIT = 5,
+    PGA_OP_SUBDIV = 6,
+    PGA_OP_REPEAT = 7,
+    PGA_OP_DISCARD = 8,
+    PGA_OP_IF = 9,
+    PGA_OP_IFSIZELESS = 10,
+    PGA_OP_IFCOLLIDES = 11,
+    PGA_OP_GENERATE = 12,
+    PGA_OP_STOCHASTIC = 13,
+    PGA_OP_SET_AS_DYNAMIC_CONVEX_POLYGON = 14,
+    PGA_OP_SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM = 15,
+    PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON = 16,
+    PGA_OP_SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM = 17,
+    PGA_OP_COLLIDER = 18,
+    PGA_OP_SWAPSIZE = 19,
+    PGA_OP_REPLICATE = 20
+} pga_operator_type;
+
+/* ---- PGA::Compiler::ShapeType (include/pga/compiler/ShapeType.h) ---- */
+typedef enum pga_shape_type {
+    PGA_SHAPE_TRIANGLE = 0,
+    PGA_SHAPE_QUAD = 1,
+    PGA_SHAPE_BOX = 9
+} pga_shape_type;
+
+/* ---- PGA::RepeatMode (include/pga/core/RepeatMode.h) ---- */
+typedef enum pga_repeat_mode {
+    PGA_ANCHOR_TO_START = 0,
+    PGA_ANCHOR_TO_END = 1,
+    PGA_ADJUST_TO_FILL = 2
+} pga_repeat_mode;
+
+/* ---- PGA::GPU::Technique (include/pga/core/GPUTechnique.h) ---- */
+typedef enum pga_technique {
+    PGA_TECHNIQUE_KERNELS = 0,    /* one launch per derivation level, host loop */
+    PGA_TECHNIQUE_DYN_PAR = 1,    /* not available on HIP: PGA_ERR_UNSUPPORTED */
+    PGA_TECHNIQUE_MEGAKERNEL = 2, /* one persistent launch for the whole derivation */
+    PGA_TECHNIQUE_MEGAKERNEL_LOCAL_QUEUES = 3 /* alias of MEGAKERNEL */
+} pga_technique;
+
+/* Byte-identical to PGA::DispatchTableEntry (DispatchTableEntry.h:8-36):
+ * 1420 bytes, so a reference caller can pass
+ * DispatchTable::toDispatchTableEntriesPtr().get() unchanged. */
+typedef struct pga_dispatch_parameter {
+    float value[PGA_MAX_NUM_PARAMETER_VALUES];
+    int32_t type; /* pga_parameter_type */
+} pga_dispatch_parameter;
+
+typedef struct pga_dispatch_successor {
+    int32_t entry_index;
+    int32_t phase_index;
+} pga_dispatch_successor;
+
+typedef struct pga_dispatch_entry {
+    int32_t operator_code; /* index into the procedure list */
+    uint32_t num_parameters;
+    pga_dispatch_parameter parameters[PGA_MAX_NUM_PARAMETERS];
+    uint32_t num_successors;
+    uint8_t rule_tag_id;
+    pga_dispatch_successor successors[PGA_MAX_NUM_SUCCESSORS];
+    uint32_t num_edge_indices;
+    int32_t edge_indices[PGA_MAX_NUM_SUCCESSORS];
+    int32_t sub_graph_index;
+} pga_dispatch_entry;
+
+/* One procedure of the procedure list: PGA::Compiler::SingleOperatorProcedure
+ * (ProcedureList.h:18-40) plus what the Proc<> template carries.
+ * flags: PGA_PROC_STATIC marks the static-parameter form of an operator
+ * (Subdivide<Axis, Pair<F,Next>...>, RandomRule<Chance, Pair<P,Next>...>)
+ * whose semantics differ from the <DynParams> form (Subdivide.cuh:22-168 vs
+ * 170-248; RandomRule.cuh:20-111 vs 121-147). */
+#define PGA_PROC_STATIC 1u
+typedef struct pga_procedure {
+    int32_t op_type;      /* pga_operator_type */
+    int32_t shape_type;   /* pga_shape_type of the shapes the queue holds */
+    int32_t gen_func_idx; /* Generate only: 1 = instanced triangle mesh */
+    uint32_t flags;
+} pga_procedure;
+
+/* An axiom: the shape record an AxiomGenerator would dispatch
+ * (Grid.cuh:188-206, src/interpreter/PGAFacade.cu:697-729).
+ * model is the row-major 3x4 model matrix (translation in column 3). */
+typedef struct pga_axiom {
+    float model[12];
+    float size[3];
+    float seed;
+    float custom;
+    int32_t entry_index; /* -1: skipped (PGAFacade.cu:708) */
+    int32_t shape_type;  /* pga_shape_type */
+    uint32_t reserved;
+} pga_axiom; /* 80 bytes */
+
+/* A terminal instance (64 bytes). model holds rows 0..2 of
+ * model4 * scale(size) (GenerationFunctions.cuh:180-184, row 3 is 0 0 0 1);
+ * the reference stores its transpose as a float4x4 plus `custom` (80 B,
+ * InstancedTriangleMeshData.h:9-26). axiom/path/meta identify the derivation
+ * path that produced the instance: meta = (depth << 16) | generate entry. */
+typedef struct pga_instance {
+    float model[12];
+    float custom;
+    uint32_t axiom;
+    uint32_t path;
+    uint32_t meta;
+} pga_instance;
+
+typedef struct pga_config {
+    int32_t technique;       /* pga_technique */
+    uint32_t queue_capacity; /* shape records per frontier buffer (0: default) */
+    uint32_t block_size;     /* threads per workgroup, multiple of 64 (0: 256) */
+    uint32_t max_levels;     /* derivation-level guard (0: 4096) */
+    uint32_t grid_blocks;    /* persistent grid size (0: occupancy-derived) */
+    int32_t device;          /* HIP device ordinal (-1: current) */
+    void* stream;            /* hipStream_t, NULL: the context creates one */
+} pga_config;
+
+typedef struct pga_stats {
+    uint64_t levels;          /* derivation levels executed */
+    uint64_t records;         /* shape records written to frontiers */
+    uint64_t terminals;       /* instances written, all terminal types */
+    double seconds;           /* device time of the last execute */
+} pga_stats;
+
+typedef struct pga_context pga_context;
+
+/* -- context (replaces SinglePhaseEvaluator construction/release) -- */
+PGA_API pga_status pga_create(const pga_config* config, pga_context** out);
+PGA_API void pga_destroy(pga_context* ctx);
+PGA_API const char* pga_last_error(const pga_context* ctx);
+PGA_API const char* pga_status_string(pga_status status);
+PGA_API uint32_t pga_abi_version(void);
+
+/* -- program: procedure list + dispatch table
+ *    (SinglePhaseEvaluator::initialize, GPUEvaluator.cuh:523-528;
+ *     initializePGA, src/interpreter/PGAFacade.cu:904-911) -- */
+PGA_API pga_status pga_initialize(pga_context* ctx,
+                                  const pga_procedure* procedures, uint32_t num_procedures,
+                                  const pga_dispatch_entry* entries, uint32_t num_entries);
+
+/* -- axioms (AxiomGenerator::generateAxiom, Grid.cuh:236-284;
+ *    Device::setAxioms, src/interpreter/GlobalVariables.cuh:47-59) -- */
+PGA_API pga_status pga_set_axioms(pga_context* ctx, const pga_axiom* axioms, uint32_t num_axioms);
+/* the caller keeps `device_axioms` alive until the next execute returns */
+PGA_API pga_status pga_set_axioms_device(pga_context* ctx, const pga_axiom* device_axioms,
+                                         uint32_t num_axioms);
+/* PGA::Device::setSeeds (GlobalVariables.cuh:28-32): copies
+ * (n % 33) seeds into a 32-slot table, as the reference does. */
+PGA_API pga_status pga_set_seeds(pga_context* ctx, const float* seeds, uint32_t num_seeds);
+PGA_API pga_status pga_get_seeds(const pga_context* ctx, float* seeds32);
+/* per-axiom continuous parameters for PGA_PT_AXIOM_PARAM (host memory) */
+PGA_API pga_status pga_set_axiom_params(pga_context* ctx, const float* params,
+                                        uint32_t num_axioms, uint32_t params_per_axiom);
+PGA_API pga_status pga_set_philox_key(pga_context* ctx, uint32_t key0, uint32_t key1);
+
+/* -- terminal output (Rendering::Device::setInstancedTriangleMeshes,
+ *    RenderingGlobalVariables.cuh:26-59): caller-owned device buffers -- */
+PGA_API pga_status pga_bind_instances(pga_context* ctx, uint32_t terminal,
+                                      pga_instance* device_buffer, uint64_t capacity);
+PGA_API pga_status pga_get_counts(pga_context* ctx, uint64_t* counts, uint32_t num_terminals);
+
+/* -- derivation (SinglePhaseEvaluator::execute, GPUEvaluator.cuh:508-521;
+ *    executePGA, src/interpreter/PGAFacade.cu:921-923).
+ *    Derives axioms [first, first + count) of the bound axiom array and
+ *    appends instances after the counts left by earlier calls when
+ *    `append` is nonzero (counts start at zero otherwise). -- */
+PGA_API pga_status pga_execute(pga_context* ctx, uint32_t num_axioms, double* seconds);
+PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first_axiom, uint32_t num_axioms,
+                                     int32_t append, double* seconds);
+PGA_API pga_status pga_get_stats(const pga_context* ctx, pga_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PGA_H */

@@ -1,0 +1,64 @@
+"""Builds libpga_amd.so (the C ABI over the gfx950 kernels) in-tree with hipcc.
+
+Flags that matter for parity and for the hardware:
+  -ffp-contract=off        no FMA contraction: FP32 results follow the
+                           reference's separately rounded products and sums;
+  -fhip-fp32-correctly-rounded-divide-sqrt
+                           IEEE division (Repeat/Subdivide divide sizes);
+  -target-feature -packed-fp32-ops
+                           no packed FP32 instructions.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libpga_amd.so")
+SOURCES = ["pga_kernels.hip", "pga_api.cpp"]
+HEADERS = ["pga_device.h", "pga_ops.h", "pga_kernels.h"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("PGA_OFFLOAD_ARCH", "gfx950")
+
+COMMON_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fvisibility=hidden", "-Wall", "-Wno-unused-function", "-Wno-unused-result",
+]
+
+
+def _run(cmd):
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed: " + " ".join(cmd))
+    return res
+
+
+def build(verbose=False):
+    """Compiles every source for gfx950 into operatorgraph_amd/libpga_amd.so."""
+    objs = []
+    build_dir = os.path.join(HERE, "_build")
+    os.makedirs(build_dir, exist_ok=True)
+    for src in SOURCES:
+        obj = os.path.join(build_dir, src + ".o")
+        cmd = [HIPCC, f"--offload-arch={ARCH}", *COMMON_FLAGS, "-I", os.path.join(ROOT, "include"),
+               "-c", os.path.join(CSRC, src), "-o", obj]
+        if src.endswith(".cpp"):
+            cmd.insert(1, "-x")
+            cmd.insert(2, "hip")
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,609 @@
+// pga_api.cpp - the C ABI (include/pga.h) over the gfx950 derivation kernels.
+//
+// Owns the device copy of the program, the frontier buffers, the counters and
+// the per-context stream. Validation that the reference performs with device
+// traps or host exceptions happens here on load, or returns a status after
+// execution from the device error word.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "pga_kernels.h"
+
+using pga::DevEntry;
+using pga::DevParam;
+
+struct pga_context {
+    pga_config cfg{};
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string error;
+
+    // program
+    bool initialized = false;
+    std::vector<DevEntry> h_entries;
+    std::vector<DevParam> h_params;
+    std::vector<int32_t> h_succ;
+    std::vector<uint32_t> h_exp;
+    DevEntry* d_entries = nullptr;
+    DevParam* d_params = nullptr;
+    int32_t* d_succ = nullptr;
+    uint32_t* d_exp = nullptr;
+    int32_t max_param_slot = -1;
+
+    // axioms and parameters
+    const pga_axiom* axioms = nullptr;
+    pga_axiom* d_axioms_owned = nullptr;
+    size_t axioms_owned_cap = 0;
+    uint32_t n_axioms = 0;
+    float* d_aparams = nullptr;
+    size_t aparams_cap = 0;
+    uint32_t aparams_per = 0, aparams_axioms = 0;
+    uint32_t key[2] = {0, 0};
+    float seeds[PGA_MAX_NUM_AXIOM_SEEDS] = {0};
+
+    // terminal buffers
+    pga_instance* inst_ptr[PGA_MAX_TERMINALS] = {nullptr};
+    unsigned long long inst_cap[PGA_MAX_TERMINALS] = {0};
+    bool bind_dirty = true;
+    pga_instance** d_inst_ptr = nullptr;
+    unsigned long long* d_inst_cap = nullptr;
+
+    // work buffers
+    pga::Rec* d_front[2] = {nullptr, nullptr};
+    uint32_t cap = 0;
+    unsigned long long* d_term_counts = nullptr;
+    uint32_t* d_level_counts = nullptr;
+    uint32_t* d_err = nullptr;
+    uint32_t* d_bar = nullptr;
+    uint32_t* h_pinned = nullptr;   // count readback
+    uint32_t mega_blocks = 0;
+
+    pga_stats stats{};
+};
+
+namespace {
+
+const uint32_t kDefaultCapacity = 1u << 22;
+const uint32_t kDefaultMaxLevels = 4096;
+const uint32_t kMaxChain = 256;
+
+pga_status fail(pga_context* ctx, pga_status st, const std::string& msg) {
+    if (ctx) ctx->error = msg;
+    return st;
+}
+
+pga_status hip_fail(pga_context* ctx, hipError_t e, const char* what) {
+    return fail(ctx, PGA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                   \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+    } while (0)
+
+template <class T> void hfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+bool is_single(uint8_t k) {
+    return k == pga::K_IFSIZELESS || k == pga::K_IF || k == pga::K_RANDOM || k == pga::K_TRANSLATE ||
+           k == pga::K_ROTATE || k == pga::K_SCALE || k == pga::K_SWAPSIZE || k == pga::K_EXTRUDE;
+}
+bool is_fan(uint8_t k) {
+    return k == pga::K_SUBDIV || k == pga::K_REPEAT || k == pga::K_COMPSPLIT || k == pga::K_REPLICATE;
+}
+
+// Checks an expression parameter (Parameters.cuh:563-664): length, operand
+// tags, integral operation codes and attribute indices, a stack that never
+// exceeds four entries nor underflows.
+bool check_exp(const pga_dispatch_parameter& p, std::string& why) {
+    float lenf = p.value[0];
+    if (!(lenf >= 1.0f) || lenf != std::floor(lenf) || lenf > (float)(PGA_MAX_NUM_PARAMETER_VALUES - 1)) {
+        why = "bad expression length";
+        return false;
+    }
+    uint32_t len = (uint32_t)lenf;
+    int depth = 0;
+    for (uint32_t i = 1; i <= len; ++i) {
+        uint32_t raw;
+        std::memcpy(&raw, &p.value[i], 4);
+        uint32_t tag = raw >> 30;
+        uint32_t xb = raw << 2;
+        float x;
+        std::memcpy(&x, &xb, 4);
+        if (tag == PGA_ORT_OP) {
+            if (x != std::floor(x) || x < 0.0f || x > 11.0f) { why = "bad expression operation"; return false; }
+            if (depth < 2) { why = "expression stack underflow"; return false; }
+            depth -= 1;
+        } else {
+            if (tag == PGA_ORT_SHAPE_ATTR && (x != std::floor(x) || x < 0.0f || x > 16.0f)) {
+                why = "bad expression shape attribute";
+                return false;
+            }
+            if (++depth > PGA_MAX_EXP_STACK) { why = "expression stack overflow"; return false; }
+        }
+    }
+    if (depth < 1) { why = "empty expression"; return false; }
+    return true;
+}
+
+pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t nprocs,
+                         const pga_dispatch_entry* entries, uint32_t n) {
+    if (!procs || !entries || nprocs == 0 || n == 0) return fail(ctx, PGA_ERR_INVALID_ARG, "empty program");
+    if (n > 0xFFFFu) return fail(ctx, PGA_ERR_BAD_PROGRAM, "more than 65535 dispatch entries");
+    std::vector<DevEntry> E(n);
+    std::vector<DevParam> Pm;
+    std::vector<int32_t> Sc;
+    std::vector<uint32_t> Ex;
+    int32_t max_slot = -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        const pga_dispatch_entry& e = entries[i];
+        std::string at = "entry " + std::to_string(i) + ": ";
+        if (e.operator_code < 0 || (uint32_t)e.operator_code >= nprocs)
+            return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "operator code out of range");
+        if (e.num_parameters > PGA_MAX_NUM_PARAMETERS || e.num_successors > PGA_MAX_NUM_SUCCESSORS)
+            return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "too many parameters or successors");
+        const pga_procedure& p = procs[e.operator_code];
+        DevEntry d{};
+        uint32_t need_p = 0, need_s = 0;
+        switch (p.op_type) {
+        case PGA_OP_TRANSLATE: d.kind = pga::K_TRANSLATE; need_p = 3; need_s = 1; break;
+        case PGA_OP_ROTATE: d.kind = pga::K_ROTATE; need_p = 3; need_s = 1; break;
+        case PGA_OP_SCALE: d.kind = pga::K_SCALE; need_p = 3; need_s = 1; break;
+        case PGA_OP_SWAPSIZE: d.kind = pga::K_SWAPSIZE; need_p = 3; need_s = 1; break;
+        case PGA_OP_EXTRUDE:
+            if (p.shape_type != PGA_SHAPE_QUAD) return fail(ctx, PGA_ERR_UNSUPPORTED, at + "Extrude needs a Quad procedure");
+            d.kind = pga::K_EXTRUDE; need_p = 2; need_s = 1; break;
+        case PGA_OP_COMPSPLIT:
+            if (p.shape_type != PGA_SHAPE_BOX) return fail(ctx, PGA_ERR_UNSUPPORTED, at + "ComponentSplit needs a Box procedure");
+            d.kind = pga::K_COMPSPLIT; need_p = 0; need_s = 3; break;
+        case PGA_OP_SUBDIV:
+            d.kind = pga::K_SUBDIV; need_p = 1;
+            need_s = e.num_parameters > 0 ? e.num_parameters - 1 : 0;
+            break;
+        case PGA_OP_REPEAT: d.kind = pga::K_REPEAT; need_p = 3; need_s = 1; break;
+        case PGA_OP_DISCARD: d.kind = pga::K_DISCARD; break;
+        case PGA_OP_IF: d.kind = pga::K_IF; need_p = 1; need_s = 2; break;
+        case PGA_OP_IFSIZELESS: d.kind = pga::K_IFSIZELESS; need_p = 2; need_s = 2; break;
+        case PGA_OP_STOCHASTIC:
+            d.kind = pga::K_RANDOM; need_p = 2;
+            need_s = e.num_parameters > 0 ? e.num_parameters - 1 : 0;
+            break;
+        case PGA_OP_REPLICATE: d.kind = pga::K_REPLICATE; break;
+        case PGA_OP_GENERATE:
+            if (p.gen_func_idx != 1)
+                return fail(ctx, PGA_ERR_UNSUPPORTED, at + "only instanced generation (GenFunc 1) is supported");
+            d.kind = pga::K_GENERATE; need_p = 1; break;
+        default:
+            return fail(ctx, PGA_ERR_UNSUPPORTED,
+                        at + "unsupported operator type " + std::to_string(p.op_type));
+        }
+        if (p.shape_type != PGA_SHAPE_BOX && p.shape_type != PGA_SHAPE_QUAD)
+            return fail(ctx, PGA_ERR_UNSUPPORTED, at + "only Box and Quad shapes are supported");
+        if (e.num_parameters < need_p) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "too few parameters");
+        if (e.num_successors < need_s) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "too few successors");
+        d.shape = (uint8_t)p.shape_type;
+        d.flags = (p.flags & PGA_PROC_STATIC) ? pga::EF_STATIC : 0;
+        d.nparams = (uint8_t)e.num_parameters;
+        d.nsucc = (uint8_t)e.num_successors;
+        d.pbase = (uint32_t)Pm.size();
+        d.sbase = (uint32_t)Sc.size();
+        for (uint32_t k = 0; k < e.num_parameters; ++k) {
+            const pga_dispatch_parameter& q = e.parameters[k];
+            DevParam dp{};
+            dp.type = (uint32_t)q.type;
+            switch (q.type) {
+            case PGA_PT_SCALAR: dp.a = q.value[0]; break;
+            case PGA_PT_RAND: dp.a = q.value[0]; dp.b = q.value[1]; break;
+            case PGA_PT_SHAPE_ATTR:
+                if (!(q.value[0] >= 0.0f) || (uint32_t)q.value[0] > 16u)
+                    return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "shape attribute index out of range");
+                dp.a = q.value[0];
+                break;
+            case PGA_PT_EXP: {
+                std::string why;
+                if (!check_exp(q, why)) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + why);
+                uint32_t off = (uint32_t)Ex.size();
+                uint32_t len = (uint32_t)q.value[0];
+                for (uint32_t v = 0; v <= len; ++v) {
+                    uint32_t raw;
+                    std::memcpy(&raw, &q.value[v], 4);
+                    Ex.push_back(raw);
+                }
+                std::memcpy(&dp.a, &off, 4);
+                break;
+            }
+            case PGA_PT_PHILOX:
+                if (!(q.value[2] >= 0.0f)) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "bad Philox stream");
+                dp.a = q.value[0]; dp.b = q.value[1]; dp.c = q.value[2];
+                break;
+            case PGA_PT_AXIOM_PARAM:
+                if (!(q.value[0] >= 0.0f) || q.value[0] > 65535.0f)
+                    return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "bad axiom parameter slot");
+                dp.a = q.value[0];
+                max_slot = std::max(max_slot, (int32_t)(uint32_t)q.value[0]);
+                break;
+            default:
+                return fail(ctx, PGA_ERR_UNSUPPORTED, at + "unsupported parameter type " + std::to_string(q.type));
+            }
+            Pm.push_back(dp);
+        }
+        for (uint32_t k = 0; k < e.num_successors; ++k) {
+            int32_t s = e.successors[k].entry_index;
+            if (s < -1 || s >= (int32_t)n) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "successor out of range");
+            Sc.push_back(s);
+        }
+        E[i] = d;
+    }
+    // successor shape consistency: the queue a child lands in holds its shape
+    for (uint32_t i = 0; i < n; ++i) {
+        const DevEntry& d = E[i];
+        uint8_t child_shape = d.kind == pga::K_COMPSPLIT ? (uint8_t)PGA_SHAPE_QUAD
+                            : d.kind == pga::K_EXTRUDE ? (uint8_t)PGA_SHAPE_BOX : d.shape;
+        for (uint32_t k = 0; k < d.nsucc; ++k) {
+            int32_t s = Sc[d.sbase + k];
+            if (s >= 0 && E[s].shape != child_shape)
+                return fail(ctx, PGA_ERR_BAD_PROGRAM,
+                            "entry " + std::to_string(i) + ": successor " + std::to_string(s) + " expects another shape type");
+        }
+    }
+    // leaf fan-outs: every successor reaches Generate/Discard through
+    // single-child operators only (no cycle, bounded chain)
+    std::vector<int8_t> memo(n, -1);   // -1 unknown, 0 no, 1 yes, 2 visiting
+    std::function<bool(int32_t, uint32_t)> ends = [&](int32_t s, uint32_t len) -> bool {
+        if (s < 0) return true;
+        if (len > kMaxChain) return false;
+        if (memo[s] == 0 || memo[s] == 1) return memo[s] == 1;
+        if (memo[s] == 2) return false;
+        const DevEntry& d = E[s];
+        if (d.kind == pga::K_GENERATE || d.kind == pga::K_DISCARD) { memo[s] = 1; return true; }
+        if (!is_single(d.kind)) { memo[s] = 0; return false; }
+        memo[s] = 2;
+        bool ok = true;
+        uint32_t ns = (d.kind == pga::K_IFSIZELESS || d.kind == pga::K_IF || d.kind == pga::K_RANDOM) ? d.nsucc : 1u;
+        for (uint32_t k = 0; k < ns && ok; ++k) ok = ends(Sc[d.sbase + k], len + 1);
+        memo[s] = ok ? 1 : 0;
+        return ok;
+    };
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!is_fan(E[i].kind)) continue;
+        bool leaf = true;
+        for (uint32_t k = 0; k < E[i].nsucc && leaf; ++k) leaf = ends(Sc[E[i].sbase + k], 1);
+        if (leaf) E[i].flags |= pga::EF_LEAF;
+    }
+    if (Sc.empty()) Sc.push_back(-1);
+    if (Pm.empty()) Pm.push_back(DevParam{});
+    if (Ex.empty()) Ex.push_back(0u);
+    hfree(ctx->d_entries); hfree(ctx->d_params); hfree(ctx->d_succ); hfree(ctx->d_exp);
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_entries, E.size() * sizeof(DevEntry)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_params, Pm.size() * sizeof(DevParam)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_succ, Sc.size() * sizeof(int32_t)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_exp, Ex.size() * sizeof(uint32_t)));
+    HIP_TRY(ctx, hipMemcpy(ctx->d_entries, E.data(), E.size() * sizeof(DevEntry), hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(ctx->d_params, Pm.data(), Pm.size() * sizeof(DevParam), hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(ctx->d_succ, Sc.data(), Sc.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(ctx->d_exp, Ex.data(), Ex.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    ctx->h_entries = E;
+    ctx->h_params = Pm;
+    ctx->h_succ = Sc;
+    ctx->h_exp = Ex;
+    ctx->max_param_slot = max_slot;
+    ctx->initialized = true;
+    return PGA_OK;
+}
+
+pga_status ensure_buffers(pga_context* ctx) {
+    uint32_t cap = ctx->cfg.queue_capacity ? ctx->cfg.queue_capacity : kDefaultCapacity;
+    if (ctx->cap != cap) {
+        hfree(ctx->d_front[0]);
+        hfree(ctx->d_front[1]);
+        HIP_TRY(ctx, hipMalloc((void**)&ctx->d_front[0], (size_t)cap * sizeof(pga::Rec)));
+        HIP_TRY(ctx, hipMalloc((void**)&ctx->d_front[1], (size_t)cap * sizeof(pga::Rec)));
+        ctx->cap = cap;
+    }
+    if (ctx->bind_dirty) {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_inst_ptr, ctx->inst_ptr, sizeof(ctx->inst_ptr), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_inst_cap, ctx->inst_cap, sizeof(ctx->inst_cap), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->bind_dirty = false;
+    }
+    return PGA_OK;
+}
+
+pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
+    pga::KArgs a{};
+    a.prog.entries = ctx->d_entries;
+    a.prog.params = ctx->d_params;
+    a.prog.succ = ctx->d_succ;
+    a.prog.exp = ctx->d_exp;
+    a.prog.nentries = (uint32_t)ctx->h_entries.size();
+    a.axioms = ctx->axioms;
+    a.first_axiom = first;
+    a.n_axioms = count;
+    a.aparams = ctx->d_aparams;
+    a.aparams_per = ctx->aparams_per;
+    a.naxioms_total = ctx->n_axioms;
+    a.key0 = ctx->key[0];
+    a.key1 = ctx->key[1];
+    a.inst = ctx->d_inst_ptr;
+    a.inst_cap = ctx->d_inst_cap;
+    a.term_counts = ctx->d_term_counts;
+    a.front0 = ctx->d_front[0];
+    a.front1 = ctx->d_front[1];
+    a.cap = ctx->cap;
+    a.level_counts = ctx->d_level_counts;
+    a.err = ctx->d_err;
+    a.bar = ctx->d_bar;
+    a.max_levels = ctx->cfg.max_levels;
+    a.max_chain = kMaxChain;
+    return a;
+}
+
+pga_status status_from_err(pga_context* ctx, uint32_t err) {
+    if (!err) return PGA_OK;
+    static const pga_status order[] = {PGA_ERR_BAD_PROGRAM, PGA_ERR_SUBDIV_OVERCONSTRAINED,
+                                       PGA_ERR_REPEAT_BAD_EXTENT, PGA_ERR_MAX_DEPTH,
+                                       PGA_ERR_OVERFLOW_QUEUE, PGA_ERR_OVERFLOW_INSTANCES};
+    std::string msg = "device reported:";
+    pga_status first = PGA_OK;
+    for (pga_status s : order)
+        if (err & (1u << s)) {
+            msg += std::string(" ") + pga_status_string(s);
+            if (first == PGA_OK) first = s;
+        }
+    if (first == PGA_OK) first = PGA_ERR_BAD_PROGRAM;
+    ctx->error = msg;
+    return first;
+}
+
+} // namespace
+
+extern "C" {
+
+PGA_API uint32_t pga_abi_version(void) { return 1u; }
+
+PGA_API const char* pga_status_string(pga_status s) {
+    switch (s) {
+    case PGA_OK: return "OK";
+    case PGA_ERR_INVALID_ARG: return "INVALID_ARG";
+    case PGA_ERR_OVERFLOW_QUEUE: return "OVERFLOW_QUEUE";
+    case PGA_ERR_OVERFLOW_INSTANCES: return "OVERFLOW_INSTANCES";
+    case PGA_ERR_BAD_PROGRAM: return "BAD_PROGRAM";
+    case PGA_ERR_SUBDIV_OVERCONSTRAINED: return "SUBDIV_OVERCONSTRAINED";
+    case PGA_ERR_REPEAT_BAD_EXTENT: return "REPEAT_BAD_EXTENT";
+    case PGA_ERR_HIP: return "HIP_ERROR";
+    case PGA_ERR_MAX_DEPTH: return "MAX_DEPTH";
+    case PGA_ERR_UNSUPPORTED: return "UNSUPPORTED";
+    case PGA_ERR_NOT_INITIALIZED: return "NOT_INITIALIZED";
+    }
+    return "UNKNOWN";
+}
+
+PGA_API pga_status pga_create(const pga_config* config, pga_context** out) {
+    if (!out) return PGA_ERR_INVALID_ARG;
+    *out = nullptr;
+    pga_context* ctx = new pga_context();
+    if (config) ctx->cfg = *config;
+    else ctx->cfg.technique = PGA_TECHNIQUE_MEGAKERNEL;
+    if (ctx->cfg.technique == PGA_TECHNIQUE_DYN_PAR) {
+        delete ctx;
+        return PGA_ERR_UNSUPPORTED;
+    }
+    if (ctx->cfg.technique == PGA_TECHNIQUE_MEGAKERNEL_LOCAL_QUEUES) ctx->cfg.technique = PGA_TECHNIQUE_MEGAKERNEL;
+    if (ctx->cfg.technique != PGA_TECHNIQUE_KERNELS && ctx->cfg.technique != PGA_TECHNIQUE_MEGAKERNEL) {
+        delete ctx;
+        return PGA_ERR_INVALID_ARG;
+    }
+    if (ctx->cfg.block_size == 0) ctx->cfg.block_size = 256;
+    if (ctx->cfg.block_size % 64 != 0 || ctx->cfg.block_size > 256) {
+        delete ctx;
+        return PGA_ERR_INVALID_ARG;
+    }
+    if (ctx->cfg.max_levels == 0) ctx->cfg.max_levels = kDefaultMaxLevels;
+    *out = ctx;
+    if (ctx->cfg.device >= 0) HIP_TRY(ctx, hipSetDevice(ctx->cfg.device));
+    HIP_TRY(ctx, hipGetDevice(&ctx->device));
+    HIP_TRY(ctx, hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    if (ctx->cfg.stream) {
+        ctx->stream = (hipStream_t)ctx->cfg.stream;
+    } else {
+        HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ctx->own_stream = true;
+    }
+    HIP_TRY(ctx, hipEventCreate(&ctx->ev0));
+    HIP_TRY(ctx, hipEventCreate(&ctx->ev1));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_term_counts, PGA_MAX_TERMINALS * sizeof(unsigned long long)));
+    HIP_TRY(ctx, hipMemset(ctx->d_term_counts, 0, PGA_MAX_TERMINALS * sizeof(unsigned long long)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_level_counts, (ctx->cfg.max_levels + 2) * sizeof(uint32_t)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_err, sizeof(uint32_t)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_bar, 2 * sizeof(uint32_t)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_inst_ptr, PGA_MAX_TERMINALS * sizeof(pga_instance*)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_inst_cap, PGA_MAX_TERMINALS * sizeof(unsigned long long)));
+    HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_pinned, 64 * sizeof(uint32_t), hipHostMallocDefault));
+    int per_cu = 0;
+    HIP_TRY(ctx, pga_mega_occupancy(ctx->cfg.block_size, &per_cu));
+    if (per_cu < 1) per_cu = 1;
+    uint32_t max_blocks = (uint32_t)per_cu * (uint32_t)ctx->num_cus;
+    ctx->mega_blocks = ctx->cfg.grid_blocks ? std::min(ctx->cfg.grid_blocks, max_blocks) : max_blocks;
+    return PGA_OK;
+}
+
+PGA_API void pga_destroy(pga_context* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    hfree(ctx->d_entries); hfree(ctx->d_params); hfree(ctx->d_succ); hfree(ctx->d_exp);
+    hfree(ctx->d_axioms_owned); hfree(ctx->d_aparams);
+    hfree(ctx->d_inst_ptr); hfree(ctx->d_inst_cap);
+    hfree(ctx->d_front[0]); hfree(ctx->d_front[1]);
+    hfree(ctx->d_term_counts); hfree(ctx->d_level_counts); hfree(ctx->d_err); hfree(ctx->d_bar);
+    if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+PGA_API const char* pga_last_error(const pga_context* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+PGA_API pga_status pga_initialize(pga_context* ctx, const pga_procedure* procedures, uint32_t num_procedures,
+                                  const pga_dispatch_entry* entries, uint32_t num_entries) {
+    if (!ctx) return PGA_ERR_INVALID_ARG;
+    ctx->initialized = false;
+    return build_program(ctx, procedures, num_procedures, entries, num_entries);
+}
+
+PGA_API pga_status pga_set_axioms(pga_context* ctx, const pga_axiom* axioms, uint32_t n) {
+    if (!ctx || (!axioms && n)) return fail(ctx, PGA_ERR_INVALID_ARG, "null axioms");
+    size_t bytes = (size_t)std::max<uint32_t>(n, 1) * sizeof(pga_axiom);
+    if (bytes > ctx->axioms_owned_cap) {
+        hfree(ctx->d_axioms_owned);
+        HIP_TRY(ctx, hipMalloc((void**)&ctx->d_axioms_owned, bytes));
+        ctx->axioms_owned_cap = bytes;
+    }
+    if (n) HIP_TRY(ctx, hipMemcpy(ctx->d_axioms_owned, axioms, (size_t)n * sizeof(pga_axiom), hipMemcpyHostToDevice));
+    ctx->axioms = ctx->d_axioms_owned;
+    ctx->n_axioms = n;
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_set_axioms_device(pga_context* ctx, const pga_axiom* device_axioms, uint32_t n) {
+    if (!ctx || (!device_axioms && n)) return fail(ctx, PGA_ERR_INVALID_ARG, "null axioms");
+    ctx->axioms = device_axioms;
+    ctx->n_axioms = n;
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_set_seeds(pga_context* ctx, const float* seeds, uint32_t n) {
+    if (!ctx || (!seeds && n)) return fail(ctx, PGA_ERR_INVALID_ARG, "null seeds");
+    uint32_t k = n % (PGA_MAX_NUM_AXIOM_SEEDS + 1);      // GlobalVariables.cuh:28-32
+    if (k) std::memcpy(ctx->seeds, seeds, k * sizeof(float));
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_get_seeds(const pga_context* ctx, float* seeds32) {
+    if (!ctx || !seeds32) return PGA_ERR_INVALID_ARG;
+    std::memcpy(seeds32, ctx->seeds, sizeof(ctx->seeds));
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_set_axiom_params(pga_context* ctx, const float* params, uint32_t num_axioms,
+                                        uint32_t per_axiom) {
+    if (!ctx || (!params && num_axioms && per_axiom)) return fail(ctx, PGA_ERR_INVALID_ARG, "null parameters");
+    size_t bytes = (size_t)num_axioms * per_axiom * sizeof(float);
+    if (bytes > ctx->aparams_cap) {
+        hfree(ctx->d_aparams);
+        HIP_TRY(ctx, hipMalloc((void**)&ctx->d_aparams, bytes));
+        ctx->aparams_cap = bytes;
+    }
+    if (bytes) HIP_TRY(ctx, hipMemcpy(ctx->d_aparams, params, bytes, hipMemcpyHostToDevice));
+    ctx->aparams_per = per_axiom;
+    ctx->aparams_axioms = num_axioms;
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_set_philox_key(pga_context* ctx, uint32_t k0, uint32_t k1) {
+    if (!ctx) return PGA_ERR_INVALID_ARG;
+    ctx->key[0] = k0;
+    ctx->key[1] = k1;
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_bind_instances(pga_context* ctx, uint32_t terminal, pga_instance* buf, uint64_t capacity) {
+    if (!ctx || terminal >= PGA_MAX_TERMINALS) return fail(ctx, PGA_ERR_INVALID_ARG, "terminal index out of range");
+    if (!buf && capacity) return fail(ctx, PGA_ERR_INVALID_ARG, "null buffer with nonzero capacity");
+    ctx->inst_ptr[terminal] = buf;
+    ctx->inst_cap[terminal] = capacity;
+    ctx->bind_dirty = true;
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_get_counts(pga_context* ctx, uint64_t* counts, uint32_t n) {
+    if (!ctx || !counts || n > PGA_MAX_TERMINALS) return fail(ctx, PGA_ERR_INVALID_ARG, "bad counts request");
+    unsigned long long tmp[PGA_MAX_TERMINALS];
+    HIP_TRY(ctx, hipMemcpyAsync(tmp, ctx->d_term_counts, sizeof(tmp), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (uint32_t i = 0; i < n; ++i) counts[i] = tmp[i];
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first, uint32_t count, int32_t append,
+                                     double* seconds) {
+    if (!ctx) return PGA_ERR_INVALID_ARG;
+    if (!ctx->initialized) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no program loaded");
+    if ((uint64_t)first + count > ctx->n_axioms || (count && !ctx->axioms))
+        return fail(ctx, PGA_ERR_INVALID_ARG, "axiom range exceeds the bound axioms");
+    if (ctx->max_param_slot >= 0 &&
+        ((uint32_t)ctx->max_param_slot >= ctx->aparams_per || ctx->aparams_axioms < ctx->n_axioms))
+        return fail(ctx, PGA_ERR_INVALID_ARG, "program reads axiom parameters that are not set");
+    pga_status st = ensure_buffers(ctx);
+    if (st) return st;
+    pga::KArgs a = make_args(ctx, first, count);
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, pga_launch_reset(a, append ? 0 : 1, ctx->stream));
+    uint32_t levels = 0;
+    const uint32_t bs = ctx->cfg.block_size;
+    if (ctx->cfg.technique == PGA_TECHNIQUE_KERNELS) {
+        uint32_t n = count;
+        while (n > 0) {
+            if (levels >= ctx->cfg.max_levels) {
+                HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                return fail(ctx, PGA_ERR_MAX_DEPTH, "maximum number of derivation levels reached");
+            }
+            uint64_t blocks = ((uint64_t)n + bs - 1) / bs;
+            blocks = std::min<uint64_t>(blocks, 1u << 16);
+            HIP_TRY(ctx, pga_launch_level(a, levels, (uint32_t)blocks, bs, ctx->stream));
+            HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->d_level_counts + levels + 1, sizeof(uint32_t),
+                                        hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            n = std::min(ctx->h_pinned[0], ctx->cap);
+            ++levels;
+        }
+    } else if (count > 0) {
+        HIP_TRY(ctx, pga_launch_mega(a, ctx->mega_blocks, bs, ctx->stream));
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    uint32_t err = 0;
+    HIP_TRY(ctx, hipMemcpy(&err, ctx->d_err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> lc(ctx->cfg.max_levels + 2);
+    HIP_TRY(ctx, hipMemcpy(lc.data(), ctx->d_level_counts, lc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    unsigned long long tc[PGA_MAX_TERMINALS];
+    HIP_TRY(ctx, hipMemcpy(tc, ctx->d_term_counts, sizeof(tc), hipMemcpyDeviceToHost));
+    if (ctx->cfg.technique == PGA_TECHNIQUE_MEGAKERNEL && count > 0) levels = lc[ctx->cfg.max_levels + 1];
+    ctx->stats.levels = levels;
+    ctx->stats.records = 0;
+    for (uint32_t l = 1; l <= levels && l <= ctx->cfg.max_levels; ++l) ctx->stats.records += lc[l];
+    ctx->stats.terminals = 0;
+    for (int t = 0; t < PGA_MAX_TERMINALS; ++t) ctx->stats.terminals += tc[t];
+    ctx->stats.seconds = ms * 1e-3;
+    if (seconds) *seconds = ms * 1e-3;
+    for (uint32_t l = 1; l <= levels && l <= ctx->cfg.max_levels; ++l)
+        if (lc[l] > ctx->cap) err |= 1u << PGA_ERR_OVERFLOW_QUEUE;
+    return status_from_err(ctx, err);
+}
+
+PGA_API pga_status pga_execute(pga_context* ctx, uint32_t num_axioms, double* seconds) {
+    return pga_execute_range(ctx, 0, num_axioms, 0, seconds);
+}
+
+PGA_API pga_status pga_get_stats(const pga_context* ctx, pga_stats* stats) {
+    if (!ctx || !stats) return PGA_ERR_INVALID_ARG;
+    *stats = ctx->stats;
+    return PGA_OK;
+}
+
+} // extern "C"

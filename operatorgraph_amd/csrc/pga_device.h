@@ -1,0 +1,318 @@
+// pga_device.h - device-side data layout and shape-grammar operators for
+// gfx950 (wave64). Included by the kernel translation unit only.
+//
+// Operator arithmetic follows the reference's FP32 operation order exactly
+// (the engine is compiled with -ffp-contract=off), so instance records are
+// bit-identical to a host derivation of the same grammar.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pga.h"
+
+namespace pga {
+
+// ---- compact program (built from procedures + dispatch entries) ----
+enum Kind : uint8_t {
+    K_DISCARD = 0,
+    K_GENERATE,
+    K_TRANSLATE,
+    K_ROTATE,
+    K_SCALE,
+    K_SWAPSIZE,
+    K_EXTRUDE,
+    K_IFSIZELESS,
+    K_IF,
+    K_RANDOM,
+    K_SUBDIV,
+    K_REPEAT,
+    K_COMPSPLIT,
+    K_REPLICATE,
+};
+enum EntryFlags : uint8_t {
+    EF_STATIC = 1,    // static-parameter operator form
+    EF_LEAF = 2,      // fan-out whose children never reach another fan-out
+    EF_SHAPE_DEP = 4, // some parameter reads the shape (dyn Subdivide re-evaluation)
+};
+
+struct DevEntry {       // 16 bytes
+    uint8_t kind;
+    uint8_t shape;      // pga_shape_type of the procedure (BOX/QUAD)
+    uint8_t flags;
+    uint8_t nparams;
+    uint8_t nsucc;
+    uint8_t pad0, pad1, pad2;
+    uint32_t pbase;     // first DevParam
+    uint32_t sbase;     // first successor in the successor pool
+};
+
+struct DevParam {       // 16 bytes
+    uint32_t type;      // pga_parameter_type
+    float a, b, c;      // SCALAR a; RAND a,b; SHAPE_ATTR a; PHILOX a,b,c; AXIOM_PARAM a;
+                        // EXP: bits of a = offset of the expression in the pool
+};
+
+struct Prog {
+    const DevEntry* __restrict__ entries;
+    const DevParam* __restrict__ params;
+    const int32_t* __restrict__ succ;
+    const uint32_t* __restrict__ exp;   // raw bits of packed expressions
+    uint32_t nentries;
+};
+
+// ---- shape record (80 bytes, 5 x 16 B) ----
+struct __align__(16) Rec {
+    float m[12];        // row-major 3x4 model, row 3 implicit (0 0 0 1)
+    float s[3];         // size
+    float seed;
+    float custom;
+    uint32_t ed;        // entry | depth << 16
+    uint32_t axiom;
+    uint32_t path;
+};
+static_assert(sizeof(Rec) == 80, "Rec must be 80 bytes");
+
+__device__ __forceinline__ uint32_t rec_entry(const Rec& r) { return r.ed & 0xFFFFu; }
+__device__ __forceinline__ uint32_t rec_depth(const Rec& r) { return r.ed >> 16; }
+
+// Per-launch context shared by every lane.
+struct Ctx {
+    Prog prog;
+    const float* __restrict__ aparams;  // per-axiom parameters
+    uint32_t aparams_per;
+    uint32_t naxioms_total;
+    uint32_t key0, key1;
+    pga_instance* const* inst;          // [PGA_MAX_TERMINALS]
+    const unsigned long long* inst_cap; // [PGA_MAX_TERMINALS]
+    unsigned long long* term_counts;    // [PGA_MAX_TERMINALS]
+    Rec* out;                           // next-level frontier
+    uint32_t* out_count;
+    uint32_t out_cap;
+    uint32_t* err;                      // bit (1 << pga_status) per error raised
+    uint32_t max_chain;
+};
+
+__device__ __forceinline__ void raise(const Ctx& c, int status) {
+    atomicOr(c.err, 1u << status);
+}
+
+// ---- float <-> bits, conversions ----
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }
+
+// round-down float->uint with saturation (the reference's __float2uint_rd)
+__device__ __forceinline__ uint32_t f2u_rd(float x) {
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+// ---- hybrid Tausworthe seed chain (Random.cuh:15-57) ----
+__device__ __forceinline__ uint32_t taus_step(uint32_t z, int s1, int s2, int s3, uint32_t m) {
+    uint32_t b = (((z << s1) ^ z) >> s2);
+    return (((z & m) << s3) ^ b);
+}
+__device__ __forceinline__ float next_seed(float base, uint32_t modifier) {
+    const uint32_t um = 0xFFFFFFFFu;
+    uint32_t z = (f2u_rd(base) * um) + (modifier * 41231278u) % um;
+    uint32_t z1 = (z + 7873234u) % um;
+    uint32_t z2 = (z1 + 1237217827u) % um;
+    uint32_t z3 = (z2 + 87382u) % um;
+    uint32_t z4 = (z3 + 1410022u) % um;
+    uint32_t t = taus_step(z1, 13, 19, 12, 4294967294u) ^ taus_step(z2, 2, 25, 4, 4294967288u) ^
+                 taus_step(z3, 3, 11, 17, 4294967280u) ^ (1664525u * z4 + 1013904223u);
+    return (float)(2.3283064365387e-10 * (double)t);
+}
+__device__ __forceinline__ float range(float seed, float mn, float mx) {
+    if (seed < 1.0f) return seed * (mx - mn) + mn;
+    return mx;
+}
+
+// ---- Philox4x32-10 (Salmon et al., SC'11) ----
+__device__ __forceinline__ uint32_t philox_x0(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    return c0;
+}
+
+// Order-independent identifier of child `ordinal` of a shape (murmur3 fmix32).
+__device__ __forceinline__ uint32_t path_mix(uint32_t path, uint32_t ordinal) {
+    uint32_t h = path * 0x01000193u ^ (ordinal + 1u) * 0x9E3779B9u;
+    h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16;
+    return h;
+}
+
+// ---- sin/cos: Cody-Waite reduction by pi/2 (three-float split) and Taylor
+// polynomials of sin (to r^9) and cos (to r^10) on |r| <= pi/4 ----
+__device__ __forceinline__ void sincos_f32(float x, float* s_out, float* c_out) {
+    const float two_over_pi = 0.636619772367581343f;
+    const float p1 = 1.5703125f;
+    const float p2 = 4.83751296997070312e-4f;
+    const float p3 = 7.54978995489188216e-8f;
+    float k = __builtin_rintf(x * two_over_pi);
+    float r = ((x - k * p1) - k * p2) - k * p3;
+    float r2 = r * r;
+    float sp = r + r * r2 * (-1.66666666666666667e-1f + r2 * (8.33333333333333333e-3f +
+                r2 * (-1.98412698412698413e-4f + r2 * 2.75573192239858907e-6f)));
+    float cp = 1.0f + r2 * (-0.5f + r2 * (4.16666666666666667e-2f + r2 * (-1.38888888888888889e-3f +
+                r2 * (2.48015873015873016e-5f + r2 * -2.75573192239858907e-7f))));
+    int q = (int)((long long)k & 3);
+    float s, c;
+    if (q == 0) { s = sp; c = cp; }
+    else if (q == 1) { s = cp; c = -sp; }
+    else if (q == 2) { s = -sp; c = -cp; }
+    else { s = -cp; c = sp; }
+    *s_out = s; *c_out = c;
+}
+
+// ---- matrices: c = a(3x4, implicit row 0001) * b(4x4), sums left to right
+// as PGA's math::float4x4 operator* (matrix.h:776-781) ----
+__device__ __forceinline__ void mul34x44(const float a[12], const float b[16], float c[12]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            c[i * 4 + j] = ((a[i * 4 + 0] * b[j] + a[i * 4 + 1] * b[4 + j]) + a[i * 4 + 2] * b[8 + j]) +
+                           a[i * 4 + 3] * b[12 + j];
+}
+// model4 * translate(t) (matrix.h:904-910): the general product with the
+// translate matrix's 0/1 entries, kept term by term for exact signed zeros.
+__device__ __forceinline__ void mul_translate(const float a[12], float tx, float ty, float tz, float c[12]) {
+    const float b[16] = {1.0f, 0.0f, 0.0f, tx, 0.0f, 1.0f, 0.0f, ty, 0.0f, 0.0f, 1.0f, tz,
+                         0.0f, 0.0f, 0.0f, 1.0f};
+    mul34x44(a, b, c);
+}
+
+__device__ __forceinline__ float shape_at(const Rec& r, uint32_t i) {
+    // Shape::at (Shape.cuh:120-171); indices are validated on load
+    float v = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) if (i == (uint32_t)k) v = r.m[k];
+    if (i == 12) v = r.s[0];
+    if (i == 13) v = r.s[1];
+    if (i == 14) v = r.s[2];
+    if (i == 15) v = r.seed;
+    if (i == 16) v = r.custom;
+    return v;
+}
+
+// unpackFloat2 (PackUtils.h:39-43)
+__device__ __forceinline__ void unpack_float2(float a, float* x, float* y) {
+    *y = fmodf(a, 4096.0f) / (4096.0f - 1.0f);
+    *x = floorf(a / 4096.0f) / (4096.0f - 1.0f);
+}
+
+__device__ __forceinline__ float exp_operate(uint32_t op, float p1, float p2) {
+    switch (op) {
+    case PGA_OPT_ADD: return p1 + p2;
+    case PGA_OPT_SUB: return p1 - p2;
+    case PGA_OPT_DIV: return p1 / p2;
+    case PGA_OPT_MULTI: return p1 * p2;
+    case PGA_OPT_EQ: return (float)(p1 == p2);
+    case PGA_OPT_NEQ: return (float)(p1 != p2);
+    case PGA_OPT_LT: return (float)(p1 < p2);
+    case PGA_OPT_GT: return (float)(p1 > p2);
+    case PGA_OPT_LEQ: return (float)(p1 <= p2);
+    case PGA_OPT_GEQ: return (float)(p1 >= p2);
+    case PGA_OPT_AND: return (float)(p1 != 0.0f && p2 != 0.0f);
+    default: return (float)(p1 != 0.0f || p2 != 0.0f);
+    }
+}
+
+// EncExp::eval (Parameters.cuh:563-664). Expressions are validated on load
+// (length <= 15, stack depth <= 4, indices in range), so the four stack
+// slots live in registers. Out of line: the shape attributes come in as
+// scalars so the record never has to be addressable.
+__device__ __noinline__ float exp_eval(const uint32_t* __restrict__ b, float m0, float m1, float m2, float m3,
+                                       float m4, float m5, float m6, float m7, float m8, float m9, float m10,
+                                       float m11, float s0x, float s1x, float s2x, float seed, float custom) {
+    float st0 = 0.0f, st1 = 0.0f, st2 = 0.0f, st3 = 0.0f;
+    int top = 0;
+    uint32_t size = (uint32_t)bitsf(b[0]);
+    for (uint32_t i = 1; i <= size; ++i) {
+        uint32_t raw = b[i];
+        uint32_t type = raw >> 30;
+        float x = bitsf(raw << 2);
+        float v;
+        if (type == PGA_ORT_OP) {
+            uint32_t op = (uint32_t)__builtin_rintf(x);
+            float p2 = top == 1 ? st0 : top == 2 ? st1 : top == 3 ? st2 : st3;
+            float p1 = top == 2 ? st0 : top == 3 ? st1 : st2;
+            top -= 2;
+            v = exp_operate(op, p1, p2);
+        } else if (type == PGA_ORT_SCALAR) {
+            v = x;
+        } else if (type == PGA_ORT_RAND) {
+            float mn, mx;
+            unpack_float2(x, &mn, &mx);
+            v = range(seed, mn, mx);
+        } else {
+            uint32_t idx = (uint32_t)__builtin_rintf(x);
+            const float a[17] = {m0, m1, m2, m3, m4, m5, m6, m7, m8, m9, m10, m11, s0x, s1x, s2x, seed, custom};
+            v = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 17; ++k) v = idx == (uint32_t)k ? a[k] : v;
+        }
+        if (top == 0) st0 = v; else if (top == 1) st1 = v; else if (top == 2) st2 = v; else st3 = v;
+        ++top;
+    }
+    return top == 1 ? st0 : top == 2 ? st1 : top == 3 ? st2 : st3;
+}
+
+__device__ __noinline__ float philox_uniform(uint32_t axiom, uint32_t path, uint32_t depth, uint32_t stream,
+                                             uint32_t k0, uint32_t k1) {
+    uint32_t u = philox_x0(axiom, path, depth, stream, k0, k1);
+    return (float)(u >> 8) * 5.9604644775390625e-8f;   // top 24 bits, exact
+}
+
+// dynEval (Parameters.cuh:669-711) and the project parameter types.
+// (axiom, path, depth) identify the shape whose operator runs.
+__device__ __forceinline__ float eval_param(const Ctx& c, const DevParam& p, const Rec& r,
+                                            uint32_t axiom, uint32_t path, uint32_t depth) {
+    const uint32_t t = p.type;
+    if (t == PGA_PT_SCALAR) return p.a;
+    if (t == PGA_PT_AXIOM_PARAM) return c.aparams[(size_t)axiom * c.aparams_per + (uint32_t)p.a];
+    if (t == PGA_PT_RAND) return range(r.seed, p.a, p.b);
+    if (t == PGA_PT_SHAPE_ATTR) return shape_at(r, (uint32_t)p.a);
+    if (t == PGA_PT_PHILOX) {
+        float uf = philox_uniform(axiom, path, depth, (uint32_t)p.c, c.key0, c.key1);
+        return (p.b - p.a) * uf + p.a;
+    }
+    return exp_eval(c.prog.exp + fbits(p.a), r.m[0], r.m[1], r.m[2], r.m[3], r.m[4], r.m[5], r.m[6], r.m[7],
+                    r.m[8], r.m[9], r.m[10], r.m[11], r.s[0], r.s[1], r.s[2], r.seed, r.custom);
+}
+
+__device__ __forceinline__ bool shape_dependent(uint32_t type) {
+    return type == PGA_PT_RAND || type == PGA_PT_SHAPE_ATTR || type == PGA_PT_EXP;
+}
+
+__device__ __forceinline__ float P(const Ctx& c, const DevEntry& e, uint32_t k, const Rec& r,
+                                   uint32_t axiom, uint32_t path, uint32_t depth) {
+    return eval_param(c, c.prog.params[e.pbase + k], r, axiom, path, depth);
+}
+
+// Evaluates the first min(nparams, 3) parameters through one call site.
+__device__ __forceinline__ void eval3(const Ctx& c, const DevEntry& e, const Rec& r, float& v0, float& v1,
+                                      float& v2) {
+    const uint32_t np = e.nparams < 3 ? e.nparams : 3u;
+    const uint32_t ax = r.axiom, pa = r.path, de = r.ed >> 16;
+    v0 = v1 = v2 = 0.0f;
+#pragma unroll 1
+    for (uint32_t k = 0; k < np; ++k) {
+        float x = P(c, e, k, r, ax, pa, de);
+        v0 = k == 0 ? x : v0;
+        v1 = k == 1 ? x : v1;
+        v2 = k == 2 ? x : v2;
+    }
+}
+
+} // namespace pga

@@ -1,0 +1,39 @@
+// pga_kernels.h - kernel arguments and launchers shared by the kernel and
+// host translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pga_ops.h"
+
+namespace pga {
+
+struct KArgs {
+    Prog prog;
+    const pga_axiom* axioms;
+    uint32_t first_axiom;
+    uint32_t n_axioms;
+    const float* aparams;
+    uint32_t aparams_per;
+    uint32_t naxioms_total;
+    uint32_t key0, key1;
+    pga_instance* const* inst;          // device array [PGA_MAX_TERMINALS]
+    const unsigned long long* inst_cap; // device array [PGA_MAX_TERMINALS]
+    unsigned long long* term_counts;    // device array [PGA_MAX_TERMINALS]
+    Rec* front0;
+    Rec* front1;
+    uint32_t cap;                       // records per frontier buffer
+    uint32_t* level_counts;             // [max_levels + 2]; [max_levels + 1] = levels run
+    uint32_t* err;
+    uint32_t* bar;                      // grid barrier words
+    uint32_t max_levels;
+    uint32_t max_chain;
+};
+
+} // namespace pga
+
+hipError_t pga_launch_reset(const pga::KArgs& a, int reset_terms, hipStream_t s);
+hipError_t pga_launch_level(const pga::KArgs& a, uint32_t level, uint32_t blocks, uint32_t block_size,
+                            hipStream_t s);
+hipError_t pga_mega_occupancy(uint32_t block_size, int* blocks_per_cu);
+hipError_t pga_launch_mega(const pga::KArgs& a, uint32_t blocks, uint32_t block_size, hipStream_t s);

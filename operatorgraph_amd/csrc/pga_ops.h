@@ -1,0 +1,352 @@
+// pga_ops.h - operator execution on one shape record per lane.
+//
+// Single-child operators (IfSizeLess, If, RandomRule, Translate, Rotate,
+// Scale, SwapSize, Extrude) transform the record in registers and continue
+// with the successor entry (a fused chain). Fan-out operators (Subdivide,
+// Repeat, ComponentSplit, Replicate) are split into `fan_init` (per-parent
+// prelude: axis, relative-to-absolute scale, repetition counts) and
+// `fan_child` (child j, called for j = 0, 1, ... in order, carrying the
+// running offset exactly as the reference's sequential loops do).
+//
+// Parameters are evaluated through few call sites (eval3 and short loops)
+// so the fused chain stays small enough to inline.
+#pragma once
+
+#include "pga_device.h"
+
+namespace pga {
+
+enum Settle : int { ST_DROP = 0, ST_TERMINAL = 1, ST_FANOUT = 2, ST_OVERLONG = 3 };
+
+__device__ __forceinline__ bool advance(const Ctx& c, Rec& r, int32_t next, uint32_t ordinal) {
+    if (next < 0) return false;   // unused dynamic edge: nothing dispatched
+    uint32_t depth = rec_depth(r) + 1u;
+    if (depth > 0xFFFFu) { raise(c, PGA_ERR_MAX_DEPTH); return false; }
+    r.ed = (uint32_t)next | (depth << 16);
+    r.path = path_mix(r.path, ordinal);
+    return true;
+}
+
+__device__ __forceinline__ uint32_t axis_of(const Ctx& c, float v) {
+    uint32_t a = f2u_rd(v);
+    if (a > 2u) { raise(c, PGA_ERR_BAD_PROGRAM); a = 0u; }
+    return a;
+}
+
+__device__ __forceinline__ float sel3(const float s[3], uint32_t a) {
+    return a == 0 ? s[0] : (a == 1 ? s[1] : s[2]);
+}
+
+__device__ __forceinline__ void set_model(Rec& r, const float m[12]) {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) r.m[i] = m[i];
+}
+
+// RandomRule<DynParams> (RandomRule.cuh:121-147) and the static form
+// (RandomRule.cuh:20-111: right-fold sum, zero-probability branches skipped,
+// last branch unconditional). Returns the chosen branch or -1.
+__device__ __forceinline__ int32_t random_choice(const Ctx& c, const DevEntry& e, float chance, uint32_t axiom,
+                                                 uint32_t path, uint32_t depth, const Rec& rr) {
+    const uint32_t n = e.nparams;
+    const bool is_static = (e.flags & EF_STATIC) != 0;
+    float sum = 0.0f;
+    if (!is_static) {
+        for (uint32_t i = 1; i < n; ++i) sum += P(c, e, i, rr, axiom, path, depth);
+    } else {
+        for (uint32_t i = n - 1; i >= 1; --i) {
+            float p = P(c, e, i, rr, axiom, path, depth);
+            sum = (i == n - 1) ? p : p + sum;
+        }
+    }
+    float acc = 0.0f;
+    for (uint32_t i = 1; i < n; ++i) {
+        if (is_static && i == n - 1) return (int32_t)i - 1;
+        float np = P(c, e, i, rr, axiom, path, depth) / sum;
+        acc += np;
+        if ((!is_static || np != 0.0f) && chance <= acc) return (int32_t)i - 1;
+    }
+    return -1;
+}
+
+// Applies one single-child operator in place. Returns the successor entry
+// (or -1 when nothing is dispatched) and the child ordinal.
+__device__ __forceinline__ int32_t apply_single(const Ctx& c, const DevEntry& e, Rec& r, uint32_t* ordinal) {
+    const int32_t* succ = c.prog.succ + e.sbase;
+    float v0, v1, v2;
+    eval3(c, e, r, v0, v1, v2);
+    *ordinal = 0;
+    const uint8_t k = e.kind;
+    if (k == K_IFSIZELESS) {                   // If.cuh:61-98
+        uint32_t axis = axis_of(c, v0);
+        uint32_t t = (sel3(r.s, axis) < v1) ? 0u : 1u;
+        *ordinal = t;
+        return succ[t];
+    }
+    if (k == K_IF) {                           // If.cuh:23-58 (NaN counts as true)
+        uint32_t t = (v0 != 0.0f) ? 0u : 1u;
+        *ordinal = t;
+        return succ[t];
+    }
+    if (k == K_RANDOM) {
+        float chance = v0 - floorf(v0);        // math::frac (math.h:243-246)
+        int32_t ch = random_choice(c, e, chance, r.axiom, r.path, rec_depth(r), r);
+        if (ch < 0) return -1;                 // dyn form: round-off can choose nothing
+        *ordinal = (uint32_t)ch;
+        return succ[ch];
+    }
+    if (k == K_TRANSLATE) {                    // Translate.cuh:24-50
+        float m[12];
+        mul_translate(r.m, v0, v1, v2, m);
+        set_model(r, m);
+        return succ[0];
+    }
+    if (k == K_ROTATE) {                       // Rotate.cuh:24-51, matrix.h:876-888
+        const float kd = 3.1415926535897932384626434f / 180.0f;
+        float sa, sb, sg, ca, cb, cg;
+        sincos_f32(v0 * kd, &sa, &ca);
+        sincos_f32(v1 * kd, &sb, &cb);
+        sincos_f32(v2 * kd, &sg, &cg);
+        const float b[16] = {cb * cg, cg * sa * sb - ca * sg, ca * cg * sb + sa * sg, 0.0f,
+                             cb * sg, ca * cg + sa * sb * sg, -cg * sa + ca * sb * sg, 0.0f,
+                             -sb, cb * sa, ca * cb, 0.0f,
+                             0.0f, 0.0f, 0.0f, 1.0f};
+        float m[12];
+        mul34x44(r.m, b, m);
+        set_model(r, m);
+        return succ[0];
+    }
+    if (k == K_SCALE) {                        // Scale.cuh:23-50
+        const float v[3] = {v0, v1, v2};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            r.s[i] = (((float)(v[i] < 0.0f) * r.s[i]) * v[i]) * -1.0f + ((float)(v[i] > 0.0f) * v[i]);
+        return succ[0];
+    }
+    if (k == K_SWAPSIZE) {                     // SwapSize.cuh:25-58
+        uint32_t a0 = axis_of(c, v0), a1 = axis_of(c, v1), a2 = axis_of(c, v2);
+        float s0 = sel3(r.s, a0), s1 = sel3(r.s, a1), s2 = sel3(r.s, a2);
+        r.s[0] = s0; r.s[1] = s1; r.s[2] = s2;
+        return succ[0];
+    }
+    // K_EXTRUDE, Extrude.cuh:25-71 (Quad -> Box); parameter 0 (axis) unused
+    const float b[16] = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1.0f, 0.0f,
+                         0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+    float m[12];
+    mul34x44(r.m, b, m);
+    set_model(r, m);
+    float sx = r.s[0], sy = r.s[1];
+    r.s[0] = sx; r.s[1] = v1; r.s[2] = sy;
+    r.seed = next_seed(r.seed, 0u);
+    return succ[0];
+}
+
+__device__ __forceinline__ bool is_fanout(uint8_t k) {
+    return k == K_SUBDIV || k == K_REPEAT || k == K_COMPSPLIT || k == K_REPLICATE;
+}
+
+// Runs single-child operators until the record reaches a terminal, a
+// discard, a fan-out operator, or `max_chain` steps (then it is queued to
+// continue on the next level).
+__device__ __forceinline__ int settle(const Ctx& c, Rec& r) {
+    for (uint32_t step = 0; step < c.max_chain; ++step) {
+        const DevEntry e = c.prog.entries[rec_entry(r)];
+        if (e.kind == K_DISCARD) return ST_DROP;
+        if (e.kind == K_GENERATE) return ST_TERMINAL;
+        if (is_fanout(e.kind)) return ST_FANOUT;
+        uint32_t ord;
+        int32_t next = apply_single(c, e, r, &ord);
+        if (!advance(c, r, next, ord)) return ST_DROP;
+    }
+    return ST_OVERLONG;
+}
+
+// ---- fan-out operators ----
+struct Fan {
+    uint32_t kind_flags; // kind | flags << 8
+    uint32_t n;          // children to iterate (j = 0 .. n-1)
+    uint32_t axis;
+    float total;
+    float r2a;           // Subdivide: relative-to-absolute scale
+    float off;           // running offset on the axis
+    float e1, e2, e3;    // Repeat extents (start/mid/end); ComponentSplit half extents
+    float prev_ns;       // Subdivide: size and offset of the previous child
+    float prev_o;
+    uint32_t reps;       // Repeat repetitions
+    uint32_t sbase, pbase;
+};
+
+__device__ __forceinline__ void fan_init(const Ctx& c, const Rec& r, Fan& f) {
+    const DevEntry e = c.prog.entries[rec_entry(r)];
+    f.kind_flags = (uint32_t)e.kind | ((uint32_t)e.flags << 8);
+    f.sbase = e.sbase; f.pbase = e.pbase;
+    f.n = 0;
+    f.reps = 0;
+    f.prev_ns = 0.0f; f.prev_o = 0.0f;
+    if (e.kind == K_SUBDIV) {                  // Subdivide.cuh:142-153 / 181-216
+        const uint32_t ax = r.axiom, pa = r.path, de = rec_depth(r);
+        float abs_sum = 0.0f, rel_sum = 0.0f;
+        uint32_t axis = 0;
+#pragma unroll 1
+        for (uint32_t i = 0; i < e.nparams; ++i) {
+            float fac = P(c, e, i, r, ax, pa, de);
+            if (i == 0) {
+                axis = f2u_rd(fac);
+            } else {
+                abs_sum += (float)(fac > 0.0f) * fac;
+                rel_sum += (float)(fac < 0.0f) * (fac * -1.0f);
+            }
+        }
+        if (axis > 2u) { raise(c, PGA_ERR_BAD_PROGRAM); return; }
+        f.axis = axis;
+        f.total = sel3(r.s, axis);
+        f.off = f.total * -0.5f;
+        f.r2a = (rel_sum == 0.0f) ? 0.0f : (f.total - abs_sum) / rel_sum;
+        if (!(e.flags & EF_STATIC) && f.r2a < 0.0f) {
+            raise(c, PGA_ERR_SUBDIV_OVERCONSTRAINED);
+            return;
+        }
+        f.n = e.nparams - 1;
+    } else if (e.kind == K_REPEAT) {           // Repeat.cuh:70-149
+        float v0, ex, v2;
+        eval3(c, e, r, v0, ex, v2);
+        uint32_t axis = axis_of(c, v0);
+        uint32_t mode = f2u_rd(v2);
+        float sz = sel3(r.s, axis);
+        float extent = (((float)(ex < 0.0f) * sz) * fabsf(ex)) + ((float)(ex > 0.0f) * ex);
+        if (extent <= 0.0f || sz < 0.0f) { raise(c, PGA_ERR_REPEAT_BAD_EXTENT); return; }
+        f.axis = axis;
+        f.total = sz;
+        float division = sz / extent;
+        float e1, e2, e3;
+        uint32_t reps;
+        if (mode == PGA_ANCHOR_TO_START || mode == PGA_ANCHOR_TO_END) {
+            float rest = fmodf(sz, extent);
+            float exact = (float)(rest == 0.0f);
+            float cd = ceilf(division);
+            reps = cd >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)cd;
+            e1 = exact * extent + (1.0f - exact) * rest;
+            e2 = e3 = extent;
+        } else if (mode == PGA_ADJUST_TO_FILL) {
+            float fd = floorf(division);
+            reps = fd >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)fd;
+            if (reps > 0) e1 = e2 = e3 = sz / (float)reps;
+            else e1 = e2 = e3 = 0.0f;
+        } else {
+            raise(c, PGA_ERR_BAD_PROGRAM);
+            return;
+        }
+        if (reps > (1u << 24)) { raise(c, PGA_ERR_OVERFLOW_QUEUE); return; }
+        bool start = mode == PGA_ANCHOR_TO_START;
+        f.e1 = (float)start * e1 + (float)(!start) * e3;   // start piece
+        f.e2 = e2;                                          // middle pieces
+        f.e3 = (float)start * e3 + (float)(!start) * e1;   // end piece
+        f.reps = reps;
+        f.n = reps > 1 ? reps : 1u;
+        f.off = 0.0f;
+    } else if (e.kind == K_COMPSPLIT) {        // ComponentSplit.cuh:234-267
+        f.e1 = r.s[0] * 0.5f; f.e2 = r.s[1] * 0.5f; f.e3 = r.s[2] * 0.5f;
+        f.n = 6;
+    } else {                                   // K_REPLICATE, Replicate.cuh:102-146
+        f.n = e.nsucc;
+    }
+}
+
+__device__ __forceinline__ void axis_vec(uint32_t axis, float o, float& tx, float& ty, float& tz) {
+    tx = axis == 0 ? o : 0.0f;
+    ty = axis == 1 ? o : 0.0f;
+    tz = axis == 2 ? o : 0.0f;
+}
+
+// Child j of the fan-out parent `r` (j = 0, 1, ... in order). Returns false
+// when the child is not dispatched.
+__device__ __forceinline__ bool fan_child(const Ctx& c, const Rec& r, Fan& f, uint32_t j, Rec& ch) {
+    const int32_t* succ = c.prog.succ + f.sbase;
+    const uint32_t kind = f.kind_flags & 0xFFu;
+    ch = r;
+    if (kind == K_SUBDIV) {                    // Subdivide.cuh:49-69, 218-236
+        const DevParam p = c.prog.params[f.pbase + 1 + j];
+        const bool is_static = ((f.kind_flags >> 8) & EF_STATIC) != 0;
+        float fac;
+        if (!is_static && j > 0 && shape_dependent(p.type)) {
+            // the DynParams form evaluates factor j on the symbol that child
+            // j-1 left behind (size, model and seed overwritten in place)
+            Rec sym = r;
+            float tx, ty, tz;
+            axis_vec(f.axis, f.prev_o, tx, ty, tz);
+            mul_translate(r.m, tx, ty, tz, sym.m);
+            if (f.axis == 0) sym.s[0] = f.prev_ns; else if (f.axis == 1) sym.s[1] = f.prev_ns; else sym.s[2] = f.prev_ns;
+            sym.seed = next_seed(r.seed, j - 1u);
+            fac = eval_param(c, p, sym, r.axiom, r.path, rec_depth(r));
+        } else {
+            fac = eval_param(c, p, r, r.axiom, r.path, rec_depth(r));
+        }
+        if (is_static && fac == 0.0f) return false;     // static form skips zero factors
+        float ns = (((float)(fac < 0.0f) * f.r2a) * fac) * -1.0f + ((float)(fac > 0.0f) * fac);
+        float o = f.off + (ns * 0.5f);
+        float tx, ty, tz;
+        axis_vec(f.axis, o, tx, ty, tz);
+        mul_translate(r.m, tx, ty, tz, ch.m);
+        if (f.axis == 0) ch.s[0] = ns; else if (f.axis == 1) ch.s[1] = ns; else ch.s[2] = ns;
+        ch.seed = next_seed(r.seed, j);
+        f.prev_ns = ns;
+        f.prev_o = o;
+        f.off = f.off + ns;
+        return advance(c, ch, succ[j], j);
+    }
+    if (kind == K_REPEAT) {                    // Repeat.cuh:146-197
+        float ext;
+        uint32_t mod, ord;
+        if (j == 0) {
+            ext = f.e1;
+            f.off = (f.total * -0.5f) + (ext * 0.5f);
+            mod = 1u; ord = 0u;
+        } else {
+            if (j == 1) f.off = f.off + f.e1 * 0.5f;
+            const bool middle = j + 1 < f.reps;   // pieces i = j + 1 in 2 .. reps-1
+            ext = middle ? f.e2 : f.e3;
+            f.off = f.off + ext * 0.5f;
+            mod = middle ? j + 1 : f.reps;
+            ord = middle ? j : f.reps - 1;
+        }
+        float tx, ty, tz;
+        axis_vec(f.axis, f.off, tx, ty, tz);
+        mul_translate(r.m, tx, ty, tz, ch.m);
+        if (f.axis == 0) ch.s[0] = ext; else if (f.axis == 1) ch.s[1] = ext; else ch.s[2] = ext;
+        ch.seed = next_seed(r.seed, mod);
+        if (j > 0 && j + 1 < f.reps) f.off = f.off + ext * 0.5f;
+        return advance(c, ch, succ[0], ord);
+    }
+    if (kind == K_COMPSPLIT) {                 // FaceSplitter<Box> (ComponentSplit.cuh:77-174)
+        float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+        // face frame b (4x4), identity unless set below
+        float b00 = 1.0f, b02 = 0.0f, b06 = 0.0f, b09 = 0.0f, b08 = 0.0f, b10 = 1.0f, b05 = 1.0f;
+        bool rotate = true;
+        if (j == 0) { tz = -f.e3; b00 = -1.0f; b10 = -1.0f; }                        // back
+        else if (j == 1) { tz = f.e3; rotate = false; }                              // front
+        else if (j == 2) { tx = -f.e1; b00 = 0.0f; b02 = -1.0f; b08 = 1.0f; b10 = 0.0f; }  // left
+        else if (j == 3) { tx = f.e1; b00 = 0.0f; b02 = 1.0f; b08 = -1.0f; b10 = 0.0f; }   // right
+        else if (j == 4) { ty = -f.e2; b05 = 0.0f; b06 = -1.0f; b09 = 1.0f; b10 = 0.0f; }  // bottom
+        else { ty = f.e2; b05 = 0.0f; b06 = 1.0f; b09 = -1.0f; b10 = 0.0f; }               // top
+        float tmp[12];
+        mul_translate(r.m, tx, ty, tz, tmp);
+        if (rotate) {
+            const float b[16] = {b00, 0.0f, b02, 0.0f, 0.0f, b05, b06, 0.0f,
+                                 b08, b09, b10, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+            mul34x44(tmp, b, ch.m);
+        } else {
+            set_model(ch, tmp);
+        }
+        const float sx = r.s[0], sy = r.s[1], sz = r.s[2];
+        ch.s[0] = (j == 2 || j == 3) ? sz : sx;
+        ch.s[1] = j < 4 ? sy : sz;
+        ch.s[2] = 1.0f;
+        const uint32_t mod = j < 4 ? j + 2u : (j == 4 ? 1u : 0u);
+        ch.seed = next_seed(r.seed, mod);
+        const int32_t target = j < 4 ? succ[2] : (j == 4 ? succ[1] : succ[0]);
+        return advance(c, ch, target, j);
+    }
+    // K_REPLICATE
+    return advance(c, ch, succ[j], j);
+}
+
+} // namespace pga

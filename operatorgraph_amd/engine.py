@@ -1,0 +1,213 @@
+"""ctypes binding of libpga_amd.so (include/pga.h).
+
+`Engine` mirrors the reference's evaluator life cycle
+(SinglePhaseEvaluator::initialize / execute / release,
+include/pga/core/GPUEvaluator.cuh:475-536): load a program, bind axioms and
+terminal buffers, execute. Device memory comes from torch (plumbing only);
+every derivation runs in the HIP kernels of libpga_amd.so. There is no CPU
+fallback: without the built library, or without a GPU, construction raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpga_amd.so")
+
+_lib = None
+
+
+class PGAError(RuntimeError):
+    def __init__(self, status, message=""):
+        name = abi.STATUS_NAMES.get(status, str(status))
+        super().__init__(f"{name}: {message}" if message else name)
+        self.status = status
+
+
+def load_library():
+    """Loads the in-tree libpga_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PGAError(abi.ERR_NOT_INITIALIZED,
+                       f"{LIB_PATH} is missing: build it with `python -m operatorgraph_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    sigs = {
+        "pga_create": (C.c_int, [C.POINTER(abi.Config), C.POINTER(P)]),
+        "pga_destroy": (None, [P]),
+        "pga_last_error": (C.c_char_p, [P]),
+        "pga_status_string": (C.c_char_p, [C.c_int]),
+        "pga_abi_version": (C.c_uint32, []),
+        "pga_initialize": (C.c_int, [P, P, C.c_uint32, P, C.c_uint32]),
+        "pga_set_axioms": (C.c_int, [P, P, C.c_uint32]),
+        "pga_set_axioms_device": (C.c_int, [P, P, C.c_uint32]),
+        "pga_set_seeds": (C.c_int, [P, P, C.c_uint32]),
+        "pga_get_seeds": (C.c_int, [P, P]),
+        "pga_set_axiom_params": (C.c_int, [P, P, C.c_uint32, C.c_uint32]),
+        "pga_set_philox_key": (C.c_int, [P, C.c_uint32, C.c_uint32]),
+        "pga_bind_instances": (C.c_int, [P, C.c_uint32, P, C.c_uint64]),
+        "pga_get_counts": (C.c_int, [P, P, C.c_uint32]),
+        "pga_execute": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_double)]),
+        "pga_execute_range": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_double)]),
+        "pga_get_stats": (C.c_int, [P, C.POINTER(abi.Stats)]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class Engine:
+    """One derivation context (pga_context)."""
+
+    def __init__(self, technique=abi.MEGAKERNEL, queue_capacity=0, block_size=0, max_levels=0,
+                 grid_blocks=0, device=-1, stream=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise PGAError(abi.ERR_HIP, "no GPU visible: the engine has no CPU path")
+        self.torch = torch
+        self.L = load_library()
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        cfg = abi.Config(technique, queue_capacity, block_size, max_levels, grid_blocks, device,
+                         C.c_void_p(stream))
+        h = C.c_void_p()
+        st = self.L.pga_create(C.byref(cfg), C.byref(h))
+        self._h = h.value
+        self._check(st)
+        self._keep = {}
+        self.n_axioms = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.pga_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != abi.OK:
+            msg = self.L.pga_last_error(self._h).decode() if getattr(self, "_h", None) else ""
+            raise PGAError(st, msg)
+
+    # -- program --
+    def load(self, program):
+        procs, entries = program.arrays()
+        self._keep["program"] = (procs, entries)
+        self._check(self.L.pga_initialize(self._h, C.addressof(procs), len(program.procedures),
+                                          C.addressof(entries), len(program.table.entries)))
+        self.program = program
+        return self
+
+    # -- inputs --
+    def set_axioms(self, axioms):
+        """axioms: numpy AXIOM_DTYPE array (copied) or a torch CUDA tensor of
+        80-byte records (uint8 [n, 80] or float32 [n, 20], not copied)."""
+        torch = self.torch
+        if isinstance(axioms, torch.Tensor):
+            assert axioms.is_cuda and axioms.is_contiguous()
+            n = axioms.numel() * axioms.element_size() // 80
+            self._keep["axioms"] = axioms
+            self._check(self.L.pga_set_axioms_device(self._h, C.c_void_p(axioms.data_ptr()), n))
+        else:
+            a = np.ascontiguousarray(axioms, dtype=abi.AXIOM_DTYPE)
+            n = len(a)
+            self._check(self.L.pga_set_axioms(self._h, a.ctypes.data, n))
+        self.n_axioms = n
+        return self
+
+    def set_axiom_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self.L.pga_set_axiom_params(self._h, p.ctypes.data, p.shape[0], p.shape[1]))
+        return self
+
+    def set_philox_key(self, key):
+        self._check(self.L.pga_set_philox_key(self._h, int(key[0]), int(key[1])))
+        return self
+
+    def set_seeds(self, seeds):
+        s = np.ascontiguousarray(seeds, dtype=np.float32)
+        self._check(self.L.pga_set_seeds(self._h, s.ctypes.data, len(s)))
+
+    def seeds(self):
+        out = np.zeros(abi.MAX_NUM_AXIOM_SEEDS, dtype=np.float32)
+        self._check(self.L.pga_get_seeds(self._h, out.ctypes.data))
+        return out
+
+    # -- outputs --
+    def bind(self, terminal, capacity, buffer=None):
+        """Binds a [capacity, 16] float32 CUDA tensor as terminal buffer
+        (allocated when `buffer` is None). capacity 0 counts only."""
+        torch = self.torch
+        if capacity == 0:
+            self._keep[("inst", terminal)] = None
+            self._check(self.L.pga_bind_instances(self._h, terminal, None, 0))
+            return None
+        if buffer is None:
+            buffer = torch.empty((capacity, 16), dtype=torch.float32, device="cuda")
+        assert buffer.is_cuda and buffer.is_contiguous() and buffer.numel() * 4 >= capacity * 64
+        self._keep[("inst", terminal)] = buffer
+        self._check(self.L.pga_bind_instances(self._h, terminal, C.c_void_p(buffer.data_ptr()), capacity))
+        return buffer
+
+    def buffer(self, terminal):
+        return self._keep.get(("inst", terminal))
+
+    def counts(self, n=abi.MAX_TERMINALS):
+        out = np.zeros(n, dtype=np.uint64)
+        self._check(self.L.pga_get_counts(self._h, out.ctypes.data, n))
+        return out
+
+    # -- execution --
+    def execute(self, n=None, first=0, append=False):
+        """Derives axioms [first, first + n); returns device seconds."""
+        self.torch.cuda.current_stream().synchronize()
+        if n is None:
+            n = self.n_axioms - first
+        sec = C.c_double(0.0)
+        self._check(self.L.pga_execute_range(self._h, first, n, 1 if append else 0, C.byref(sec)))
+        return sec.value
+
+    def stats(self):
+        s = abi.Stats()
+        self._check(self.L.pga_get_stats(self._h, C.byref(s)))
+        return {"levels": s.levels, "records": s.records, "terminals": s.terminals, "seconds": s.seconds}
+
+    def instances(self, terminal, count=None):
+        """Copies the instances of `terminal` to host as an INSTANCE_DTYPE array."""
+        buf = self.buffer(terminal)
+        if count is None:
+            count = int(self.counts(terminal + 1)[terminal])
+        if buf is None or count == 0:
+            return np.zeros(0, dtype=abi.INSTANCE_DTYPE)
+        count = min(count, buf.shape[0])
+        host = buf[:count].cpu().numpy()
+        return host.view(abi.INSTANCE_DTYPE).reshape(count)
+
+
+def derive(program, axioms, capacities, technique=abi.MEGAKERNEL, params=None, key=None, **cfg):
+    """One-shot derivation: returns (list of instance arrays, stats)."""
+    eng = Engine(technique=technique, **cfg)
+    eng.load(program).set_axioms(axioms)
+    if params is not None:
+        eng.set_axiom_params(params)
+    if key is not None:
+        eng.set_philox_key(key)
+    for t, cap in enumerate(capacities):
+        eng.bind(t, cap)
+    eng.execute()
+    out = [eng.instances(t) for t in range(len(capacities))]
+    st = eng.stats()
+    eng.close()
+    return out, st

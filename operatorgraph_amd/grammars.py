@@ -1,0 +1,203 @@
+"""Grammars and procedure lists used as workloads and fixtures.
+
+* `INTERPRETER_PROCEDURES`: the interpreter's fixed Box/Quad procedure list
+  (src/interpreter/PGAFacade.cu:24-62); operator codes of interpreter
+  dispatch tables index this list.
+* `menger_interpreter(threshold)`: src/interpreter/grammars/menger_sponge_3/
+  grammar.pga compiled by hand into an interpreter dispatch table.
+* `menger_test_runner()`: the procedure list and dispatch table of
+  src/test_runner/MengerSponge.cuh:28-45 (the "Menger-5" scene when the
+  axiom has size 81).
+* `facade()`: the batched-facade grammar (two terminal types).
+* `stochastic()`: per-shape Philox rule choice and split ratios.
+* `facade_params()`: the facade with 8 per-lot differentiable parameters.
+"""
+from . import abi
+from .program import (AxiomParam, DispatchTable, Entry, Parameter, Philox, Procedure,
+                      ProcedureList, Program, Scalar)
+
+B, Q = abi.BOX, abi.QUAD
+
+# src/interpreter/PGAFacade.cu:24-62 (Box/Quad only)
+INTERPRETER_PROCEDURES = ProcedureList([
+    # QUAD
+    Procedure(abi.DISCARD, Q), Procedure(abi.EXTRUDE, Q), Procedure(abi.GENERATE, Q, 0),
+    Procedure(abi.GENERATE, Q, 1), Procedure(abi.IF, Q), Procedure(abi.IFSIZELESS, Q),
+    Procedure(abi.STOCHASTIC, Q), Procedure(abi.REPEAT, Q), Procedure(abi.REPLICATE, Q),
+    Procedure(abi.ROTATE, Q), Procedure(abi.SCALE, Q),
+    Procedure(abi.SET_AS_DYNAMIC_CONVEX_POLYGON, Q),
+    Procedure(abi.SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM, Q),
+    Procedure(abi.SET_AS_DYNAMIC_CONCAVE_POLYGON, Q),
+    Procedure(abi.SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM, Q),
+    Procedure(abi.SUBDIV, Q), Procedure(abi.SWAPSIZE, Q), Procedure(abi.TRANSLATE, Q),
+    # BOX
+    Procedure(abi.COMPSPLIT, B), Procedure(abi.DISCARD, B), Procedure(abi.GENERATE, B, 0),
+    Procedure(abi.GENERATE, B, 1), Procedure(abi.IF, B), Procedure(abi.IFSIZELESS, B),
+    Procedure(abi.STOCHASTIC, B), Procedure(abi.REPEAT, B), Procedure(abi.REPLICATE, B),
+    Procedure(abi.ROTATE, B), Procedure(abi.SCALE, B),
+    Procedure(abi.SET_AS_DYNAMIC_CONVEX_POLYGON, B),
+    Procedure(abi.SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM, B),
+    Procedure(abi.SET_AS_DYNAMIC_CONCAVE_POLYGON, B),
+    Procedure(abi.SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM, B),
+    Procedure(abi.SUBDIV, B), Procedure(abi.SWAPSIZE, B), Procedure(abi.TRANSLATE, B),
+])
+
+
+def op(op_type, shape, gen_func=0):
+    """Operator code of an interpreter procedure (ProcedureList::indexOf)."""
+    idx = INTERPRETER_PROCEDURES.index_of(Procedure(op_type, shape, gen_func))
+    if idx < 0:
+        raise KeyError((op_type, shape, gen_func))
+    return idx
+
+
+def subdiv_entry(shape, axis, factors, successors):
+    """DispatchTableGenerator::buildSubdivideEntry
+    (src/compiler/DispatchTableGenerator.cpp:355-383): parameter 0 holds the
+    axis and the absolute/relative sums, parameter j the factor and the
+    running sums before it."""
+    params = []
+    abs_sum = rel_sum = 0.0
+    for f in factors:
+        params.append(Parameter(abi.PT_SCALAR, [f, abs_sum, rel_sum]))
+        if f > 0:
+            abs_sum += f
+        else:
+            rel_sum -= f
+    p0 = Parameter(abi.PT_SCALAR, [axis, abs_sum, rel_sum])
+    return Entry(op(abi.SUBDIV, shape), [p0] + params, [(s, 0) for s in successors])
+
+
+def menger_interpreter(threshold=0.111):
+    """menger_sponge_3/grammar.pga:
+        A = IfSizeLess(X, 0.111) { B | SubX };
+        ZDiscard = SubDiv(Z) { -1: A | -1: Discard() | -1: A };
+        YDiscard = SubDiv(Y) { -1: ZDiscard | -1: Discard() | -1: ZDiscard };
+        SubZ = SubDiv(Z) { -1: A | -1: A | -1: A };
+        SubY = SubDiv(Y) { -1: SubZ | -1: ZDiscard | -1: SubZ };
+        SubX = SubDiv(X) { -1: SubY | -1: YDiscard | -1: SubY };
+    terminal B (1,0). Axiom entry 0."""
+    A, Bt, SUBX, SUBY, YDISC, SUBZ, ZDISC, DISC = range(8)
+    r = [-1.0, -1.0, -1.0]
+    table = DispatchTable([
+        Entry(op(abi.IFSIZELESS, B), [Scalar(abi.X), Scalar(threshold)], [(Bt, 0), (SUBX, 0)]),
+        Entry(op(abi.GENERATE, B, 1), [Scalar(0)], []),
+        subdiv_entry(B, abi.X, r, [SUBY, YDISC, SUBY]),
+        subdiv_entry(B, abi.Y, r, [SUBZ, ZDISC, SUBZ]),
+        subdiv_entry(B, abi.Y, r, [ZDISC, DISC, ZDISC]),
+        subdiv_entry(B, abi.Z, r, [A, A, A]),
+        subdiv_entry(B, abi.Z, r, [A, DISC, A]),
+        Entry(op(abi.DISCARD, B), [], []),
+    ])
+    return Program(INTERPRETER_PROCEDURES, table, "menger_interpreter")
+
+
+def menger_test_runner():
+    """src/test_runner/MengerSponge.cuh:28-45, verbatim: a four-procedure
+    list (static-parameter Subdivide, Discard, IfSizeLess, instanced Generate)
+    and a nine-entry table; axioms start at entry 2."""
+    procs = ProcedureList([
+        Procedure(abi.SUBDIV, B, 0, abi.PROC_STATIC),
+        Procedure(abi.DISCARD, B),
+        Procedure(abi.IFSIZELESS, B),
+        Procedure(abi.GENERATE, B, 1),
+    ])
+    S = abi.PT_SCALAR
+    table = DispatchTable([
+        Entry(2, [(S, [0]), (S, [1])], [(1, 0), (2, 0)]),
+        Entry(3, [(S, [0])], []),
+        Entry(0, [(S, [0]), (S, [-1]), (S, [-1]), (S, [-1])], [(3, 0), (7, 0), (3, 0)]),
+        Entry(0, [(S, [1]), (S, [-1]), (S, [-1]), (S, [-1])], [(4, 0), (5, 0), (4, 0)]),
+        Entry(0, [(S, [2]), (S, [-1]), (S, [-1]), (S, [-1])], [(0, 0), (0, 0), (0, 0)]),
+        Entry(0, [(S, [2]), (S, [-1]), (S, [-1]), (S, [-1])], [(0, 0), (6, 0), (0, 0)]),
+        Entry(1, [], []),
+        Entry(0, [(S, [1]), (S, [-1]), (S, [-1]), (S, [-1])], [(5, 0), (8, 0), (5, 0)]),
+        Entry(1, [], []),
+    ])
+    return Program(procs, table, "menger_test_runner")
+
+
+MENGER_TEST_RUNNER_AXIOM_ENTRY = 2
+
+
+def facade():
+    """Batched-facade grammar (two terminal types, Wall=0 and Window=1):
+        Lot    = CompSplit() { Discard() | Discard() | Facade };
+        Facade = Repeat(Y, 3, ADJUST_TO_FILL) Floor;
+        Floor  = Repeat(X, 2.5, ADJUST_TO_FILL) Tile;
+        Tile   = IfSizeLess(X, 1.6) { Wall | SubDiv(X) { -1: Wall | 1.2: Window | -1: Wall } };
+    Axiom entry 0 (a Box lot)."""
+    LOT, QDISC, FACADE, FLOOR, TILE, WALL, SUBWIN, WINDOW = range(8)
+    table = DispatchTable([
+        Entry(op(abi.COMPSPLIT, B), [], [(QDISC, 0), (QDISC, 0), (FACADE, 0)]),
+        Entry(op(abi.DISCARD, Q), [], []),
+        Entry(op(abi.REPEAT, Q), [Scalar(abi.Y), Scalar(3.0), Scalar(abi.ADJUST_TO_FILL)], [(FLOOR, 0)]),
+        Entry(op(abi.REPEAT, Q), [Scalar(abi.X), Scalar(2.5), Scalar(abi.ADJUST_TO_FILL)], [(TILE, 0)]),
+        Entry(op(abi.IFSIZELESS, Q), [Scalar(abi.X), Scalar(1.6)], [(WALL, 0), (SUBWIN, 0)]),
+        Entry(op(abi.GENERATE, Q, 1), [Scalar(0)], []),
+        subdiv_entry(Q, abi.X, [-1.0, 1.2, -1.0], [WALL, WINDOW, WALL]),
+        Entry(op(abi.GENERATE, Q, 1), [Scalar(1)], []),
+    ])
+    return Program(INTERPRETER_PROCEDURES, table, "facade")
+
+
+def stochastic():
+    """Stochastic grammar (one terminal type). The axiom's Y size is its
+    depth budget: every split halves Y, derivation stops once Y < 1.5.
+        S      = IfSizeLess(Y, 1.5) { Leaf | Choose };
+        Choose = RandomRule(Philox(0,1,#0)) { 0.2: Leaf | 0.4: SplitX | 0.4: SplitZ };
+        SplitX = SubDiv(X) { Philox(-0.8,-0.2,#1): Half | -0.5: Half };
+        SplitZ = SubDiv(Z) { Philox(-0.8,-0.2,#1): Half | -0.5: Half };
+        Half   = Scale(-1, -0.5, -1) S;
+        Leaf   = Generate(0);
+    Axiom entry 0."""
+    S_, CHOOSE, SPLITX, SPLITZ, HALF, LEAF = range(6)
+    split = lambda axis: Entry(op(abi.SUBDIV, B), [Scalar(axis), Philox(-0.8, -0.2, 1), Scalar(-0.5)],
+                               [(HALF, 0), (HALF, 0)])
+    table = DispatchTable([
+        Entry(op(abi.IFSIZELESS, B), [Scalar(abi.Y), Scalar(1.5)], [(LEAF, 0), (CHOOSE, 0)]),
+        Entry(op(abi.STOCHASTIC, B), [Philox(0.0, 1.0, 0), Scalar(0.2), Scalar(0.4), Scalar(0.4)],
+              [(LEAF, 0), (SPLITX, 0), (SPLITZ, 0)]),
+        split(abi.X),
+        split(abi.Z),
+        Entry(op(abi.SCALE, B), [Scalar(-1.0), Scalar(-0.5), Scalar(-1.0)], [(S_, 0)]),
+        Entry(op(abi.GENERATE, B, 1), [Scalar(0)], []),
+    ])
+    return Program(INTERPRETER_PROCEDURES, table, "stochastic")
+
+
+# per-lot parameter slots of facade_params()
+FACADE_PARAM_NAMES = ("left_weight", "right_weight", "floor_height", "tile_width",
+                      "window_width", "window_recess", "sill_height", "lintel_height")
+FACADE_PARAM_DEFAULTS = (-1.0, -1.0, 2.9, 2.45, 1.2, 0.15, 0.9, 0.4)
+
+
+def facade_params():
+    """Facade with 8 per-lot parameters (PT_AXIOM_PARAM slots):
+        Lot    = CompSplit() { Discard() | Discard() | Facade };
+        Facade = Repeat(Y, p2, ADJUST_TO_FILL) Floor;
+        Floor  = Repeat(X, p3, ADJUST_TO_FILL) Tile;
+        Tile   = SubDiv(X) { p0: Wall | p4: Column | p1: Wall };
+        Column = SubDiv(Y) { p6: Wall | -1: Recess | p7: Wall };
+        Recess = Translate(0, 0, p5) Window;
+    Slots (FACADE_PARAM_NAMES): p0/p1 relative wall weights (negative
+    values), p2 floor height, p3 tile width, p4 window width, p5 window
+    recess offset, p6 sill height, p7 lintel height. ADJUST_TO_FILL keeps
+    every floor >= p2 and every tile >= p3, so the absolute factors always
+    fit; the repetition counts are discrete, so dL/dp2 and dL/dp3 vanish
+    almost everywhere. Terminals: Wall=0, Window=1."""
+    LOT, QDISC, FACADE, FLOOR, TILE, WALL, COLUMN, RECESS, WINDOW = range(9)
+    table = DispatchTable([
+        Entry(op(abi.COMPSPLIT, B), [], [(QDISC, 0), (QDISC, 0), (FACADE, 0)]),
+        Entry(op(abi.DISCARD, Q), [], []),
+        Entry(op(abi.REPEAT, Q), [Scalar(abi.Y), AxiomParam(2), Scalar(abi.ADJUST_TO_FILL)], [(FLOOR, 0)]),
+        Entry(op(abi.REPEAT, Q), [Scalar(abi.X), AxiomParam(3), Scalar(abi.ADJUST_TO_FILL)], [(TILE, 0)]),
+        Entry(op(abi.SUBDIV, Q), [Scalar(abi.X), AxiomParam(0), AxiomParam(4), AxiomParam(1)],
+              [(WALL, 0), (COLUMN, 0), (WALL, 0)]),
+        Entry(op(abi.GENERATE, Q, 1), [Scalar(0)], []),
+        Entry(op(abi.SUBDIV, Q), [Scalar(abi.Y), AxiomParam(6), Scalar(-1.0), AxiomParam(7)],
+              [(WALL, 0), (RECESS, 0), (WALL, 0)]),
+        Entry(op(abi.TRANSLATE, Q), [Scalar(0.0), Scalar(0.0), AxiomParam(5)], [(WINDOW, 0)]),
+        Entry(op(abi.GENERATE, Q, 1), [Scalar(1)], []),
+    ])
+    return Program(INTERPRETER_PROCEDURES, table, "facade_params")
