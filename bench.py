@@ -1,0 +1,207 @@
+#!/usr/bin/env python
+"""bench.py - shape-grammar derivation throughput on one MI355X.
+
+Workload (one step): batched derivation of the facade grammar over 1,024
+seeded building lots (W in [10,40] m, D in [10,30] m, 3..20 floors; Philox
+key 0x5EED0002) in one persistent launch, inputs resident in HBM, about
+1.2M instanced terminals of 64 bytes per step.
+
+Prints ONE JSON line. Beside the headline (batched terminals/s) it reports:
+  roofline      - HBM roofline of the derivation kernel: algorithmic bytes
+                  (64 B per instance written + 80 B per axiom read) over the
+                  kernel's average duration (HIP events on its stream);
+  plain         - the one-axiom-per-call path (pga_execute per lot), the
+                  reference's executePGA usage, on a sample of lots;
+  batched_16384 - the same grammar over 16,384 lots in one launch;
+  cpu_baseline  - the CPU restatement (oracle) on the host, one thread.
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "terminals/sec batched vs plain at 1 GPU; fwd+bwd grads/sec; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+AXIOM_BYTES = 80
+INSTANCE_BYTES = 64
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--lots", type=int, default=1024)
+    p.add_argument("--big-lots", type=int, default=16384)
+    p.add_argument("--plain-sample", type=int, default=256)
+    p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--only-kernel", action="store_true",
+                   help="run the timed batched steps only (for profilers)")
+    return p.parse_args()
+
+
+def make_engine(torch, engine_mod, grammars, axioms, n_lots, key=None):
+    from operatorgraph_amd import abi
+    lots = axioms.facade_lots(n_lots)
+    walls, windows = axioms.facade_lot_terminals(lots)
+    T = int(walls.sum() + windows.sum())
+    # frontier: floors of all lots (tiles are expanded in place)
+    queue = max(1 << 16, int(n_lots * 4 * 24 * 1.25))
+    eng = engine_mod.Engine(technique=abi.MEGAKERNEL, queue_capacity=queue)
+    eng.load(grammars.facade())
+    dev_axioms = torch.from_numpy(lots.view(np.uint8).copy()).cuda()
+    eng.set_axioms(dev_axioms)
+    eng.bind(0, int(walls.sum()))
+    eng.bind(1, int(windows.sum()))
+    return eng, lots, T
+
+
+def timed_steps(torch, eng, steps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.execute_async()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    eng.synchronize()     # raises on any device error of the last step
+    return (t1 - t0) / steps
+
+
+def kernel_seconds(torch, eng, steps):
+    times = []
+    for _ in range(steps):
+        eng.execute_async()
+        eng.synchronize()
+        times.append(eng.kernel_seconds())
+    return float(np.mean(times)), times
+
+
+def cpu_baseline(lots_fn, seconds, threads=1):
+    from operatorgraph_amd import axioms, grammars
+    from oracle import oracle as O
+    prog = grammars.facade()
+    lots = lots_fn()
+    parts = np.array_split(np.arange(len(lots)), threads)
+    done = []
+    t0 = time.perf_counter()
+    while True:
+        counts = [0] * threads
+
+        def work(i):
+            sub = lots[parts[i]]
+            r = O.derive(prog, sub)
+            counts[i] = r.count(0) + r.count(1)
+
+        ts = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+        s = time.perf_counter()
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        done.append((sum(counts), time.perf_counter() - s))
+        if time.perf_counter() - t0 >= seconds:
+            break
+    rates = [c / dt for c, dt in done]
+    return float(np.median(rates)), len(done), sum(c for c, _ in done)
+
+
+def load_traffic(workload):
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("traffic_bytes_per_launch")
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the derivation shards by axiom with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from operatorgraph_amd import axioms, engine, grammars
+
+    eng, lots, T = make_engine(torch, engine, grammars, axioms, args.lots)
+    A = len(lots)
+    for _ in range(args.warmup):
+        eng.execute_async()
+    eng.synchronize()
+    assert eng.stats()["terminals"] == T, (eng.stats(), T)
+    sec = timed_steps(torch, eng, args.steps)
+    value = T / sec
+    if args.only_kernel:
+        print(json.dumps({"terminals_per_step": T, "ms_per_step": sec * 1e3}), flush=True)
+        return
+    ksec, _ = kernel_seconds(torch, eng, max(5, args.steps // 2))
+    alg_bytes = INSTANCE_BYTES * T + AXIOM_BYTES * A
+    achieved = alg_bytes / ksec / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(f"facade_lots_{args.lots}"),
+                "kernel": "pga::k_mega", "kernel_ms": round(ksec * 1e3, 4),
+                "algorithmic_bytes_per_launch": alg_bytes}
+
+    # plain: one axiom per call (synchronous pga_execute_range per lot)
+    n_plain = min(args.plain_sample, A)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_T = 0
+    for i in range(n_plain):
+        eng.execute(1, first=i)
+        plain_T += int(eng.stats()["terminals"])
+    plain_sec = time.perf_counter() - t0
+    plain_rate = plain_T / plain_sec
+    eng.close()
+
+    big = None
+    if args.big_lots:
+        beng, blots, BT = make_engine(torch, engine, grammars, axioms, args.big_lots)
+        for _ in range(2):
+            beng.execute_async()
+        beng.synchronize()
+        bsec = timed_steps(torch, beng, max(3, args.steps // 4))
+        bk, _ = kernel_seconds(torch, beng, 3)
+        bbytes = INSTANCE_BYTES * BT + AXIOM_BYTES * len(blots)
+        big = {"lots": len(blots), "terminals_per_step": BT, "ms_per_step": round(bsec * 1e3, 4),
+               "terminals_per_s": round(BT / bsec), "kernel_ms": round(bk * 1e3, 4),
+               "hbm_frac": round(bbytes / bk / 1e9 / HBM_PEAK_GBS, 4),
+               "vs_plain": round((BT / bsec) / plain_rate, 1)}
+        beng.close()
+
+    cpu = None
+    if not args.no_cpu:
+        rate, runs, total = cpu_baseline(lambda: axioms.facade_lots(args.lots), args.cpu_seconds, 1)
+        cpu = {"value": round(rate), "unit": "terminals/s", "cores": 1, "kind": "port",
+               "sample": f"facade grammar over the same {args.lots} lots, {runs} full derivations "
+                         f"({total} terminals) by the oracle on one host thread, median rate"}
+
+    out = {
+        "metric": METRIC, "value": round(value), "unit": "terminals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"facade_lots_{args.lots}", "lots": A, "terminals_per_step": T,
+                   "grammar": "Lot=CompSplit{Discard|Discard|Facade}; Facade=Repeat(Y,3); "
+                              "Floor=Repeat(X,2.5); Tile=IfSizeLess(X,1.6){Wall|SubDiv(X){-1|1.2|-1}}",
+                   "technique": "MEGAKERNEL (cooperative persistent launch)"},
+        "roofline": roofline,
+        "plain": {"lots": n_plain, "terminals": plain_T, "terminals_per_s": round(plain_rate),
+                  "ms_per_lot": round(plain_sec / n_plain * 1e3, 4)},
+        "batched_vs_plain": round(value / plain_rate, 1),
+        "batched_16384": big,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

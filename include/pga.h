@@ -262,6 +262,16 @@ PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first_axiom, uin
                                      int32_t append, double* seconds);
 PGA_API pga_status pga_get_stats(const pga_context* ctx, pga_stats* stats);
 
+/* -- asynchronous form: enqueues the derivation on the context stream and
+ *    returns without waiting (MEGAKERNEL technique only). pga_synchronize
+ *    waits, then reports the device error word and refreshes the stats;
+ *    pga_kernel_seconds gives the device time of the last derivation
+ *    kernel alone (events around the persistent launch). -- */
+PGA_API pga_status pga_execute_async(pga_context* ctx, uint32_t first_axiom, uint32_t num_axioms,
+                                     int32_t append);
+PGA_API pga_status pga_synchronize(pga_context* ctx);
+PGA_API pga_status pga_kernel_seconds(pga_context* ctx, double* seconds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,9 @@ struct pga_context {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evk0 = nullptr, evk1 = nullptr;   // around the derivation kernel only
     std::string error;
+    uint32_t pending_count = 0;   // axioms of the last asynchronous launch
 
     // program
     bool initialized = false;
@@ -424,6 +426,8 @@ PGA_API pga_status pga_create(const pga_config* config, pga_context** out) {
     }
     HIP_TRY(ctx, hipEventCreate(&ctx->ev0));
     HIP_TRY(ctx, hipEventCreate(&ctx->ev1));
+    HIP_TRY(ctx, hipEventCreate(&ctx->evk0));
+    HIP_TRY(ctx, hipEventCreate(&ctx->evk1));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_term_counts, PGA_MAX_TERMINALS * sizeof(unsigned long long)));
     HIP_TRY(ctx, hipMemset(ctx->d_term_counts, 0, PGA_MAX_TERMINALS * sizeof(unsigned long long)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_level_counts, (ctx->cfg.max_levels + 2) * sizeof(uint32_t)));
@@ -451,6 +455,8 @@ PGA_API void pga_destroy(pga_context* ctx) {
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->evk0) (void)hipEventDestroy(ctx->evk0);
+    if (ctx->evk1) (void)hipEventDestroy(ctx->evk1);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -538,8 +544,7 @@ PGA_API pga_status pga_get_counts(pga_context* ctx, uint64_t* counts, uint32_t n
     return PGA_OK;
 }
 
-PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first, uint32_t count, int32_t append,
-                                     double* seconds) {
+static pga_status check_launch(pga_context* ctx, uint32_t first, uint32_t count) {
     if (!ctx) return PGA_ERR_INVALID_ARG;
     if (!ctx->initialized) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no program loaded");
     if ((uint64_t)first + count > ctx->n_axioms || (count && !ctx->axioms))
@@ -547,13 +552,70 @@ PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first, uint32_t 
     if (ctx->max_param_slot >= 0 &&
         ((uint32_t)ctx->max_param_slot >= ctx->aparams_per || ctx->aparams_axioms < ctx->n_axioms))
         return fail(ctx, PGA_ERR_INVALID_ARG, "program reads axiom parameters that are not set");
-    pga_status st = ensure_buffers(ctx);
+    return ensure_buffers(ctx);
+}
+
+// Reads the error word, level and terminal counters of the finished run.
+static pga_status finish(pga_context* ctx, uint32_t count, uint32_t host_levels) {
+    uint32_t err = 0;
+    HIP_TRY(ctx, hipMemcpy(&err, ctx->d_err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> lc(ctx->cfg.max_levels + 2);
+    HIP_TRY(ctx, hipMemcpy(lc.data(), ctx->d_level_counts, lc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    unsigned long long tc[PGA_MAX_TERMINALS];
+    HIP_TRY(ctx, hipMemcpy(tc, ctx->d_term_counts, sizeof(tc), hipMemcpyDeviceToHost));
+    uint32_t levels = host_levels;
+    if (ctx->cfg.technique == PGA_TECHNIQUE_MEGAKERNEL && count > 0) levels = lc[ctx->cfg.max_levels + 1];
+    ctx->stats.levels = levels;
+    ctx->stats.records = 0;
+    for (uint32_t l = 1; l <= levels && l <= ctx->cfg.max_levels; ++l) ctx->stats.records += lc[l];
+    ctx->stats.terminals = 0;
+    for (int t = 0; t < PGA_MAX_TERMINALS; ++t) ctx->stats.terminals += tc[t];
+    for (uint32_t l = 1; l <= levels && l <= ctx->cfg.max_levels; ++l)
+        if (lc[l] > ctx->cap) err |= 1u << PGA_ERR_OVERFLOW_QUEUE;
+    return status_from_err(ctx, err);
+}
+
+PGA_API pga_status pga_execute_async(pga_context* ctx, uint32_t first, uint32_t count, int32_t append) {
+    pga_status st = check_launch(ctx, first, count);
+    if (st) return st;
+    if (ctx->cfg.technique != PGA_TECHNIQUE_MEGAKERNEL)
+        return fail(ctx, PGA_ERR_UNSUPPORTED, "asynchronous execution needs the MEGAKERNEL technique");
+    pga::KArgs a = make_args(ctx, first, count);
+    HIP_TRY(ctx, pga_launch_reset(a, append ? 0 : 1, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->evk0, ctx->stream));
+    if (count > 0) HIP_TRY(ctx, pga_launch_mega(a, ctx->mega_blocks, ctx->cfg.block_size, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->evk1, ctx->stream));
+    ctx->pending_count = count;
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_synchronize(pga_context* ctx) {
+    if (!ctx) return PGA_ERR_INVALID_ARG;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+    ctx->stats.seconds = ms * 1e-3;
+    return finish(ctx, ctx->pending_count, 0);
+}
+
+PGA_API pga_status pga_kernel_seconds(pga_context* ctx, double* seconds) {
+    if (!ctx || !seconds) return PGA_ERR_INVALID_ARG;
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+    *seconds = ms * 1e-3;
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first, uint32_t count, int32_t append,
+                                     double* seconds) {
+    pga_status st = check_launch(ctx, first, count);
     if (st) return st;
     pga::KArgs a = make_args(ctx, first, count);
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     HIP_TRY(ctx, pga_launch_reset(a, append ? 0 : 1, ctx->stream));
     uint32_t levels = 0;
     const uint32_t bs = ctx->cfg.block_size;
+    HIP_TRY(ctx, hipEventRecord(ctx->evk0, ctx->stream));
     if (ctx->cfg.technique == PGA_TECHNIQUE_KERNELS) {
         uint32_t n = count;
         while (n > 0) {
@@ -573,27 +635,14 @@ PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first, uint32_t 
     } else if (count > 0) {
         HIP_TRY(ctx, pga_launch_mega(a, ctx->mega_blocks, bs, ctx->stream));
     }
+    HIP_TRY(ctx, hipEventRecord(ctx->evk1, ctx->stream));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.0f;
     HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    uint32_t err = 0;
-    HIP_TRY(ctx, hipMemcpy(&err, ctx->d_err, sizeof(uint32_t), hipMemcpyDeviceToHost));
-    std::vector<uint32_t> lc(ctx->cfg.max_levels + 2);
-    HIP_TRY(ctx, hipMemcpy(lc.data(), ctx->d_level_counts, lc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    unsigned long long tc[PGA_MAX_TERMINALS];
-    HIP_TRY(ctx, hipMemcpy(tc, ctx->d_term_counts, sizeof(tc), hipMemcpyDeviceToHost));
-    if (ctx->cfg.technique == PGA_TECHNIQUE_MEGAKERNEL && count > 0) levels = lc[ctx->cfg.max_levels + 1];
-    ctx->stats.levels = levels;
-    ctx->stats.records = 0;
-    for (uint32_t l = 1; l <= levels && l <= ctx->cfg.max_levels; ++l) ctx->stats.records += lc[l];
-    ctx->stats.terminals = 0;
-    for (int t = 0; t < PGA_MAX_TERMINALS; ++t) ctx->stats.terminals += tc[t];
     ctx->stats.seconds = ms * 1e-3;
     if (seconds) *seconds = ms * 1e-3;
-    for (uint32_t l = 1; l <= levels && l <= ctx->cfg.max_levels; ++l)
-        if (lc[l] > ctx->cap) err |= 1u << PGA_ERR_OVERFLOW_QUEUE;
-    return status_from_err(ctx, err);
+    return finish(ctx, count, levels);
 }
 
 PGA_API pga_status pga_execute(pga_context* ctx, uint32_t num_axioms, double* seconds) {

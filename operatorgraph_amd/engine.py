@@ -55,6 +55,9 @@ def load_library():
         "pga_execute": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_double)]),
         "pga_execute_range": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_double)]),
         "pga_get_stats": (C.c_int, [P, C.POINTER(abi.Stats)]),
+        "pga_execute_async": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int32]),
+        "pga_synchronize": (C.c_int, [P]),
+        "pga_kernel_seconds": (C.c_int, [P, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -177,6 +180,20 @@ class Engine:
             n = self.n_axioms - first
         sec = C.c_double(0.0)
         self._check(self.L.pga_execute_range(self._h, first, n, 1 if append else 0, C.byref(sec)))
+        return sec.value
+
+    def execute_async(self, n=None, first=0, append=False):
+        """Enqueues the derivation on the engine's stream without waiting."""
+        if n is None:
+            n = self.n_axioms - first
+        self._check(self.L.pga_execute_async(self._h, first, n, 1 if append else 0))
+
+    def synchronize(self):
+        self._check(self.L.pga_synchronize(self._h))
+
+    def kernel_seconds(self):
+        sec = C.c_double(0.0)
+        self._check(self.L.pga_kernel_seconds(self._h, C.byref(sec)))
         return sec.value
 
     def stats(self):
