@@ -53,8 +53,8 @@ def make_engine(torch, engine_mod, grammars, axioms, n_lots, key=None):
     lots = axioms.facade_lots(n_lots)
     walls, windows = axioms.facade_lot_terminals(lots)
     T = int(walls.sum() + windows.sum())
-    # frontier: floors of all lots (tiles are expanded in place)
-    queue = max(1 << 16, int(n_lots * 4 * 24 * 1.25))
+    # largest level: one child descriptor per tile (= one per window)
+    queue = max(1 << 16, int(windows.sum() * 1.05))
     eng = engine_mod.Engine(technique=abi.MEGAKERNEL, queue_capacity=queue)
     eng.load(grammars.facade())
     dev_axioms = torch.from_numpy(lots.view(np.uint8).copy()).cuda()

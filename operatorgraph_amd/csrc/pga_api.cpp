@@ -40,6 +40,8 @@ struct pga_context {
     DevParam* d_params = nullptr;
     int32_t* d_succ = nullptr;
     uint32_t* d_exp = nullptr;
+    uint8_t* d_slots = nullptr;       // [0,64): terminal -> slot, [64,128): slot -> terminal
+    uint32_t nslots = 0;
     int32_t max_param_slot = -1;
 
     // axioms and parameters
@@ -61,12 +63,15 @@ struct pga_context {
     unsigned long long* d_inst_cap = nullptr;
 
     // work buffers
-    pga::Rec* d_front[2] = {nullptr, nullptr};
+    pga::Rec* d_par[2] = {nullptr, nullptr};
+    pga::Desc* d_desc[2] = {nullptr, nullptr};
     uint32_t cap = 0;
     unsigned long long* d_term_counts = nullptr;
-    uint32_t* d_level_counts = nullptr;
+    uint32_t* d_par_counts = nullptr;
+    uint32_t* d_desc_counts = nullptr;
     uint32_t* d_err = nullptr;
     uint32_t* d_bar = nullptr;
+    uint32_t* d_block_totals = nullptr;
     uint32_t* h_pinned = nullptr;   // count readback
     uint32_t mega_blocks = 0;
 
@@ -285,10 +290,38 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         for (uint32_t k = 0; k < E[i].nsucc && leaf; ++k) leaf = ends(Sc[E[i].sbase + k], 1);
         if (leaf) E[i].flags |= pga::EF_LEAF;
     }
+    // dense terminal slots: the distinct constant terminal indices of the
+    // Generate entries; a computed terminal index needs all 64 slots
+    uint8_t slots[2 * PGA_MAX_TERMINALS];
+    std::memset(slots, 0xFF, sizeof(slots));
+    bool dynamic_terminal = false;
+    std::vector<uint32_t> used;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (E[i].kind != pga::K_GENERATE) continue;
+        const pga_dispatch_parameter& q = entries[i].parameters[0];
+        if (q.type != PGA_PT_SCALAR) { dynamic_terminal = true; continue; }
+        float v = q.value[0];
+        uint32_t t = v > 0.0f ? (uint32_t)v : 0u;
+        if (!(v < (float)PGA_MAX_TERMINALS))
+            return fail(ctx, PGA_ERR_BAD_PROGRAM, "entry " + std::to_string(i) + ": terminal index out of range");
+        if (std::find(used.begin(), used.end(), t) == used.end()) used.push_back(t);
+    }
+    if (dynamic_terminal) {
+        used.clear();
+        for (uint32_t t = 0; t < PGA_MAX_TERMINALS; ++t) used.push_back(t);
+    }
+    std::sort(used.begin(), used.end());
+    for (uint32_t s = 0; s < used.size(); ++s) {
+        slots[used[s]] = (uint8_t)s;
+        slots[PGA_MAX_TERMINALS + s] = (uint8_t)used[s];
+    }
     if (Sc.empty()) Sc.push_back(-1);
     if (Pm.empty()) Pm.push_back(DevParam{});
     if (Ex.empty()) Ex.push_back(0u);
-    hfree(ctx->d_entries); hfree(ctx->d_params); hfree(ctx->d_succ); hfree(ctx->d_exp);
+    hfree(ctx->d_entries); hfree(ctx->d_params); hfree(ctx->d_succ); hfree(ctx->d_exp); hfree(ctx->d_slots);
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_slots, sizeof(slots)));
+    HIP_TRY(ctx, hipMemcpy(ctx->d_slots, slots, sizeof(slots), hipMemcpyHostToDevice));
+    ctx->nslots = (uint32_t)used.size();
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_entries, E.size() * sizeof(DevEntry)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_params, Pm.size() * sizeof(DevParam)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_succ, Sc.size() * sizeof(int32_t)));
@@ -309,10 +342,12 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
 pga_status ensure_buffers(pga_context* ctx) {
     uint32_t cap = ctx->cfg.queue_capacity ? ctx->cfg.queue_capacity : kDefaultCapacity;
     if (ctx->cap != cap) {
-        hfree(ctx->d_front[0]);
-        hfree(ctx->d_front[1]);
-        HIP_TRY(ctx, hipMalloc((void**)&ctx->d_front[0], (size_t)cap * sizeof(pga::Rec)));
-        HIP_TRY(ctx, hipMalloc((void**)&ctx->d_front[1], (size_t)cap * sizeof(pga::Rec)));
+        for (int b = 0; b < 2; ++b) {
+            hfree(ctx->d_par[b]);
+            hfree(ctx->d_desc[b]);
+            HIP_TRY(ctx, hipMalloc((void**)&ctx->d_par[b], (size_t)cap * sizeof(pga::Rec)));
+            HIP_TRY(ctx, hipMalloc((void**)&ctx->d_desc[b], (size_t)cap * sizeof(pga::Desc)));
+        }
         ctx->cap = cap;
     }
     if (ctx->bind_dirty) {
@@ -331,6 +366,10 @@ pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
     a.prog.succ = ctx->d_succ;
     a.prog.exp = ctx->d_exp;
     a.prog.nentries = (uint32_t)ctx->h_entries.size();
+    a.prog.term_slot = ctx->d_slots;
+    a.prog.slot_term = ctx->d_slots + PGA_MAX_TERMINALS;
+    a.prog.nslots = ctx->nslots;
+    a.block_totals = ctx->d_block_totals;
     a.axioms = ctx->axioms;
     a.first_axiom = first;
     a.n_axioms = count;
@@ -342,10 +381,14 @@ pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
     a.inst = ctx->d_inst_ptr;
     a.inst_cap = ctx->d_inst_cap;
     a.term_counts = ctx->d_term_counts;
-    a.front0 = ctx->d_front[0];
-    a.front1 = ctx->d_front[1];
-    a.cap = ctx->cap;
-    a.level_counts = ctx->d_level_counts;
+    a.par0 = ctx->d_par[0];
+    a.par1 = ctx->d_par[1];
+    a.desc0 = ctx->d_desc[0];
+    a.desc1 = ctx->d_desc[1];
+    a.cap_par = ctx->cap;
+    a.cap_desc = ctx->cap;
+    a.par_counts = ctx->d_par_counts;
+    a.desc_counts = ctx->d_desc_counts;
     a.err = ctx->d_err;
     a.bar = ctx->d_bar;
     a.max_levels = ctx->cfg.max_levels;
@@ -430,9 +473,11 @@ PGA_API pga_status pga_create(const pga_config* config, pga_context** out) {
     HIP_TRY(ctx, hipEventCreate(&ctx->evk1));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_term_counts, PGA_MAX_TERMINALS * sizeof(unsigned long long)));
     HIP_TRY(ctx, hipMemset(ctx->d_term_counts, 0, PGA_MAX_TERMINALS * sizeof(unsigned long long)));
-    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_level_counts, (ctx->cfg.max_levels + 2) * sizeof(uint32_t)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_par_counts, (ctx->cfg.max_levels + 2) * sizeof(uint32_t)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_desc_counts, (ctx->cfg.max_levels + 2) * sizeof(uint32_t)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_err, sizeof(uint32_t)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_bar, 2 * sizeof(uint32_t)));
+    HIP_TRY(ctx, hipMemset(ctx->d_bar, 0, 2 * sizeof(uint32_t)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_inst_ptr, PGA_MAX_TERMINALS * sizeof(pga_instance*)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_inst_cap, PGA_MAX_TERMINALS * sizeof(unsigned long long)));
     HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_pinned, 64 * sizeof(uint32_t), hipHostMallocDefault));
@@ -441,6 +486,8 @@ PGA_API pga_status pga_create(const pga_config* config, pga_context** out) {
     if (per_cu < 1) per_cu = 1;
     uint32_t max_blocks = (uint32_t)per_cu * (uint32_t)ctx->num_cus;
     ctx->mega_blocks = ctx->cfg.grid_blocks ? std::min(ctx->cfg.grid_blocks, max_blocks) : max_blocks;
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_block_totals,
+                           (size_t)ctx->mega_blocks * (PGA_MAX_TERMINALS + 2) * sizeof(uint32_t)));
     return PGA_OK;
 }
 
@@ -450,8 +497,9 @@ PGA_API void pga_destroy(pga_context* ctx) {
     hfree(ctx->d_entries); hfree(ctx->d_params); hfree(ctx->d_succ); hfree(ctx->d_exp);
     hfree(ctx->d_axioms_owned); hfree(ctx->d_aparams);
     hfree(ctx->d_inst_ptr); hfree(ctx->d_inst_cap);
-    hfree(ctx->d_front[0]); hfree(ctx->d_front[1]);
-    hfree(ctx->d_term_counts); hfree(ctx->d_level_counts); hfree(ctx->d_err); hfree(ctx->d_bar);
+    for (int b = 0; b < 2; ++b) { hfree(ctx->d_par[b]); hfree(ctx->d_desc[b]); }
+    hfree(ctx->d_term_counts); hfree(ctx->d_par_counts); hfree(ctx->d_desc_counts);
+    hfree(ctx->d_err); hfree(ctx->d_bar); hfree(ctx->d_block_totals); hfree(ctx->d_slots);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -559,19 +607,21 @@ static pga_status check_launch(pga_context* ctx, uint32_t first, uint32_t count)
 static pga_status finish(pga_context* ctx, uint32_t count, uint32_t host_levels) {
     uint32_t err = 0;
     HIP_TRY(ctx, hipMemcpy(&err, ctx->d_err, sizeof(uint32_t), hipMemcpyDeviceToHost));
-    std::vector<uint32_t> lc(ctx->cfg.max_levels + 2);
-    HIP_TRY(ctx, hipMemcpy(lc.data(), ctx->d_level_counts, lc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> lc(ctx->cfg.max_levels + 2), pc(ctx->cfg.max_levels + 2);
+    HIP_TRY(ctx, hipMemcpy(lc.data(), ctx->d_desc_counts, lc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(pc.data(), ctx->d_par_counts, pc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     unsigned long long tc[PGA_MAX_TERMINALS];
     HIP_TRY(ctx, hipMemcpy(tc, ctx->d_term_counts, sizeof(tc), hipMemcpyDeviceToHost));
     uint32_t levels = host_levels;
     if (ctx->cfg.technique == PGA_TECHNIQUE_MEGAKERNEL && count > 0) levels = lc[ctx->cfg.max_levels + 1];
     ctx->stats.levels = levels;
     ctx->stats.records = 0;
-    for (uint32_t l = 1; l <= levels && l <= ctx->cfg.max_levels; ++l) ctx->stats.records += lc[l];
+    for (uint32_t l = 0; l < levels && l < ctx->cfg.max_levels; ++l) {
+        ctx->stats.records += pc[l];
+        if (pc[l] > ctx->cap || lc[l] > ctx->cap) err |= 1u << PGA_ERR_OVERFLOW_QUEUE;
+    }
     ctx->stats.terminals = 0;
     for (int t = 0; t < PGA_MAX_TERMINALS; ++t) ctx->stats.terminals += tc[t];
-    for (uint32_t l = 1; l <= levels && l <= ctx->cfg.max_levels; ++l)
-        if (lc[l] > ctx->cap) err |= 1u << PGA_ERR_OVERFLOW_QUEUE;
     return status_from_err(ctx, err);
 }
 
@@ -623,10 +673,8 @@ PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first, uint32_t 
                 HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
                 return fail(ctx, PGA_ERR_MAX_DEPTH, "maximum number of derivation levels reached");
             }
-            uint64_t blocks = ((uint64_t)n + bs - 1) / bs;
-            blocks = std::min<uint64_t>(blocks, 1u << 16);
-            HIP_TRY(ctx, pga_launch_level(a, levels, (uint32_t)blocks, bs, ctx->stream));
-            HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->d_level_counts + levels + 1, sizeof(uint32_t),
+            HIP_TRY(ctx, pga_launch_level(a, levels, ctx->mega_blocks, bs, ctx->stream));
+            HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->d_desc_counts + levels, sizeof(uint32_t),
                                         hipMemcpyDeviceToHost, ctx->stream));
             HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
             n = std::min(ctx->h_pinned[0], ctx->cap);

@@ -58,7 +58,10 @@ struct Prog {
     const DevParam* __restrict__ params;
     const int32_t* __restrict__ succ;
     const uint32_t* __restrict__ exp;   // raw bits of packed expressions
+    const uint8_t* __restrict__ term_slot;  // terminal index -> dense slot (0xFF: none)
+    const uint8_t* __restrict__ slot_term;  // dense slot -> terminal index
     uint32_t nentries;
+    uint32_t nslots;                     // terminal slots used by the program
 };
 
 // ---- shape record (80 bytes, 5 x 16 B) ----
@@ -76,6 +79,8 @@ static_assert(sizeof(Rec) == 80, "Rec must be 80 bytes");
 __device__ __forceinline__ uint32_t rec_entry(const Rec& r) { return r.ed & 0xFFFFu; }
 __device__ __forceinline__ uint32_t rec_depth(const Rec& r) { return r.ed >> 16; }
 
+struct Desc;
+
 // Per-launch context shared by every lane.
 struct Ctx {
     Prog prog;
@@ -86,9 +91,11 @@ struct Ctx {
     pga_instance* const* inst;          // [PGA_MAX_TERMINALS]
     const unsigned long long* inst_cap; // [PGA_MAX_TERMINALS]
     unsigned long long* term_counts;    // [PGA_MAX_TERMINALS]
-    Rec* out;                           // next-level frontier
-    uint32_t* out_count;
-    uint32_t out_cap;
+    Rec* par_out;                       // parent records of this level
+    struct Desc* desc_out;              // child descriptors for the next level
+    uint32_t* par_count;
+    uint32_t* desc_count;
+    uint32_t par_cap, desc_cap;
     uint32_t* err;                      // bit (1 << pga_status) per error raised
     uint32_t max_chain;
 };

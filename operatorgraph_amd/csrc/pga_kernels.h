@@ -20,10 +20,15 @@ struct KArgs {
     pga_instance* const* inst;          // device array [PGA_MAX_TERMINALS]
     const unsigned long long* inst_cap; // device array [PGA_MAX_TERMINALS]
     unsigned long long* term_counts;    // device array [PGA_MAX_TERMINALS]
-    Rec* front0;
-    Rec* front1;
-    uint32_t cap;                       // records per frontier buffer
-    uint32_t* level_counts;             // [max_levels + 2]; [max_levels + 1] = levels run
+    Rec* par0;                          // parent records, written by even / odd levels
+    Rec* par1;
+    Desc* desc0;                        // child descriptors, written by even / odd levels
+    Desc* desc1;
+    uint32_t cap_par, cap_desc;
+    uint32_t* par_counts;               // [max_levels + 2]: parents written by level L
+    uint32_t* desc_counts;              // [max_levels + 2]: children of level L + 1;
+                                        // [max_levels + 1] = levels run by k_mega
+    uint32_t* block_totals;             // [grid blocks][nslots + 2] counts of one pass
     uint32_t* err;
     uint32_t* bar;                      // grid barrier words
     uint32_t max_levels;

@@ -181,6 +181,7 @@ __device__ __forceinline__ void fan_init(const Ctx& c, const Rec& r, Fan& f) {
     f.sbase = e.sbase; f.pbase = e.pbase;
     f.n = 0;
     f.reps = 0;
+    f.axis = 0;
     f.prev_ns = 0.0f; f.prev_o = 0.0f;
     if (e.kind == K_SUBDIV) {                  // Subdivide.cuh:142-153 / 181-216
         const uint32_t ax = r.axiom, pa = r.path, de = rec_depth(r);
@@ -251,19 +252,37 @@ __device__ __forceinline__ void fan_init(const Ctx& c, const Rec& r, Fan& f) {
     }
 }
 
+// ---- child descriptors (one per fan-out child, 16 bytes) ----
+// jx = child index | axis << 30; o = translation of the child along the axis
+// (Subdivide, Repeat), ns = the child's size along the axis. The fan-out lane
+// computes o and ns with the reference's sequential running-offset sums; the
+// child lane rebuilds the child from the parent record and the descriptor.
+struct __align__(16) Desc {
+    uint32_t parent;
+    uint32_t jx;
+    float o;
+    float ns;
+};
+static_assert(sizeof(Desc) == 16, "Desc must be 16 bytes");
+const uint32_t kDescSelf = 0x3FFFFFFFu;   // child = the parent record itself (chain continues)
+const uint32_t kDescNone = 0x3FFFFFFEu;   // no child (static Subdivide zero factor)
+
 __device__ __forceinline__ void axis_vec(uint32_t axis, float o, float& tx, float& ty, float& tz) {
     tx = axis == 0 ? o : 0.0f;
     ty = axis == 1 ? o : 0.0f;
     tz = axis == 2 ? o : 0.0f;
 }
 
-// Child j of the fan-out parent `r` (j = 0, 1, ... in order). Returns false
-// when the child is not dispatched.
-__device__ __forceinline__ bool fan_child(const Ctx& c, const Rec& r, Fan& f, uint32_t j, Rec& ch) {
-    const int32_t* succ = c.prog.succ + f.sbase;
+// Geometry of child j of the fan-out parent `r`, j = 0, 1, ... in order:
+// the translation o and size ns along the axis, from the reference's
+// sequential running offsets (Subdivide.cuh:49-69, 218-236; Repeat.cuh:
+// 146-197). Returns false when the child is not dispatched (a zero factor of
+// the static Subdivide form).
+__device__ __forceinline__ bool fan_geom(const Ctx& c, const Rec& r, Fan& f, uint32_t j, float& o, float& ns) {
     const uint32_t kind = f.kind_flags & 0xFFu;
-    ch = r;
-    if (kind == K_SUBDIV) {                    // Subdivide.cuh:49-69, 218-236
+    o = 0.0f;
+    ns = 0.0f;
+    if (kind == K_SUBDIV) {
         const DevParam p = c.prog.params[f.pbase + 1 + j];
         const bool is_static = ((f.kind_flags >> 8) & EF_STATIC) != 0;
         float fac;
@@ -280,53 +299,59 @@ __device__ __forceinline__ bool fan_child(const Ctx& c, const Rec& r, Fan& f, ui
         } else {
             fac = eval_param(c, p, r, r.axiom, r.path, rec_depth(r));
         }
-        if (is_static && fac == 0.0f) return false;     // static form skips zero factors
-        float ns = (((float)(fac < 0.0f) * f.r2a) * fac) * -1.0f + ((float)(fac > 0.0f) * fac);
-        float o = f.off + (ns * 0.5f);
-        float tx, ty, tz;
-        axis_vec(f.axis, o, tx, ty, tz);
-        mul_translate(r.m, tx, ty, tz, ch.m);
-        if (f.axis == 0) ch.s[0] = ns; else if (f.axis == 1) ch.s[1] = ns; else ch.s[2] = ns;
-        ch.seed = next_seed(r.seed, j);
+        if (is_static && fac == 0.0f) return false;
+        ns = (((float)(fac < 0.0f) * f.r2a) * fac) * -1.0f + ((float)(fac > 0.0f) * fac);
+        o = f.off + (ns * 0.5f);
         f.prev_ns = ns;
         f.prev_o = o;
         f.off = f.off + ns;
-        return advance(c, ch, succ[j], j);
+        return true;
     }
-    if (kind == K_REPEAT) {                    // Repeat.cuh:146-197
-        float ext;
-        uint32_t mod, ord;
+    if (kind == K_REPEAT) {
         if (j == 0) {
-            ext = f.e1;
-            f.off = (f.total * -0.5f) + (ext * 0.5f);
-            mod = 1u; ord = 0u;
+            ns = f.e1;
+            f.off = (f.total * -0.5f) + (ns * 0.5f);
         } else {
             if (j == 1) f.off = f.off + f.e1 * 0.5f;
             const bool middle = j + 1 < f.reps;   // pieces i = j + 1 in 2 .. reps-1
-            ext = middle ? f.e2 : f.e3;
-            f.off = f.off + ext * 0.5f;
-            mod = middle ? j + 1 : f.reps;
-            ord = middle ? j : f.reps - 1;
+            ns = middle ? f.e2 : f.e3;
+            f.off = f.off + ns * 0.5f;
         }
+        o = f.off;
+        if (j > 0 && j + 1 < f.reps) f.off = f.off + ns * 0.5f;
+        return true;
+    }
+    return true;
+}
+
+// Child j of a fan-out parent from its geometry (o, ns along `axis`).
+// Repeat pieces use seed modifier j + 1 (1 for the first piece, i for the
+// middle pieces i = 2 .. reps-1, reps for the last, Repeat.cuh:156-184).
+__device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, uint32_t kind, uint32_t sbase, uint32_t j,
+                                            uint32_t axis, float o, float ns, Rec& ch) {
+    const int32_t* succ = c.prog.succ + sbase;
+    ch = r;
+    if (kind == K_SUBDIV || kind == K_REPEAT) {
         float tx, ty, tz;
-        axis_vec(f.axis, f.off, tx, ty, tz);
+        axis_vec(axis, o, tx, ty, tz);
         mul_translate(r.m, tx, ty, tz, ch.m);
-        if (f.axis == 0) ch.s[0] = ext; else if (f.axis == 1) ch.s[1] = ext; else ch.s[2] = ext;
-        ch.seed = next_seed(r.seed, mod);
-        if (j > 0 && j + 1 < f.reps) f.off = f.off + ext * 0.5f;
-        return advance(c, ch, succ[0], ord);
+        if (axis == 0) ch.s[0] = ns; else if (axis == 1) ch.s[1] = ns; else ch.s[2] = ns;
+        const bool sub = kind == K_SUBDIV;
+        ch.seed = next_seed(r.seed, sub ? j : j + 1u);
+        return advance(c, ch, succ[sub ? j : 0u], j);
     }
     if (kind == K_COMPSPLIT) {                 // FaceSplitter<Box> (ComponentSplit.cuh:77-174)
+        const float hx = r.s[0] * 0.5f, hy = r.s[1] * 0.5f, hz = r.s[2] * 0.5f;
         float tx = 0.0f, ty = 0.0f, tz = 0.0f;
         // face frame b (4x4), identity unless set below
         float b00 = 1.0f, b02 = 0.0f, b06 = 0.0f, b09 = 0.0f, b08 = 0.0f, b10 = 1.0f, b05 = 1.0f;
         bool rotate = true;
-        if (j == 0) { tz = -f.e3; b00 = -1.0f; b10 = -1.0f; }                        // back
-        else if (j == 1) { tz = f.e3; rotate = false; }                              // front
-        else if (j == 2) { tx = -f.e1; b00 = 0.0f; b02 = -1.0f; b08 = 1.0f; b10 = 0.0f; }  // left
-        else if (j == 3) { tx = f.e1; b00 = 0.0f; b02 = 1.0f; b08 = -1.0f; b10 = 0.0f; }   // right
-        else if (j == 4) { ty = -f.e2; b05 = 0.0f; b06 = -1.0f; b09 = 1.0f; b10 = 0.0f; }  // bottom
-        else { ty = f.e2; b05 = 0.0f; b06 = 1.0f; b09 = -1.0f; b10 = 0.0f; }               // top
+        if (j == 0) { tz = -hz; b00 = -1.0f; b10 = -1.0f; }                          // back
+        else if (j == 1) { tz = hz; rotate = false; }                                // front
+        else if (j == 2) { tx = -hx; b00 = 0.0f; b02 = -1.0f; b08 = 1.0f; b10 = 0.0f; }    // left
+        else if (j == 3) { tx = hx; b00 = 0.0f; b02 = 1.0f; b08 = -1.0f; b10 = 0.0f; }     // right
+        else if (j == 4) { ty = -hy; b05 = 0.0f; b06 = -1.0f; b09 = 1.0f; b10 = 0.0f; }    // bottom
+        else { ty = hy; b05 = 0.0f; b06 = 1.0f; b09 = -1.0f; b10 = 0.0f; }                 // top
         float tmp[12];
         mul_translate(r.m, tx, ty, tz, tmp);
         if (rotate) {
@@ -347,6 +372,23 @@ __device__ __forceinline__ bool fan_child(const Ctx& c, const Rec& r, Fan& f, ui
     }
     // K_REPLICATE
     return advance(c, ch, succ[j], j);
+}
+
+// Child j of the fan-out parent `r` (j = 0, 1, ... in order). Returns false
+// when the child is not dispatched.
+__device__ __forceinline__ bool fan_child(const Ctx& c, const Rec& r, Fan& f, uint32_t j, Rec& ch) {
+    float o, ns;
+    if (!fan_geom(c, r, f, j, o, ns)) return false;
+    return build_child(c, r, f.kind_flags & 0xFFu, f.sbase, j, f.axis, o, ns, ch);
+}
+
+// Child described by `d` of the parent record `r`.
+__device__ __forceinline__ bool desc_child(const Ctx& c, const Rec& r, const Desc& d, Rec& ch) {
+    const uint32_t j = d.jx & 0x3FFFFFFFu;
+    if (j == kDescSelf) { ch = r; return true; }
+    if (j == kDescNone) return false;
+    const DevEntry e = c.prog.entries[rec_entry(r)];
+    return build_child(c, r, e.kind, e.sbase, j, d.jx >> 30, d.o, d.ns, ch);
 }
 
 } // namespace pga
