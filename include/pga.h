@@ -272,6 +272,16 @@ PGA_API pga_status pga_execute_async(pga_context* ctx, uint32_t first_axiom, uin
 PGA_API pga_status pga_synchronize(pga_context* ctx);
 PGA_API pga_status pga_kernel_seconds(pga_context* ctx, double* seconds);
 
+/* -- instrumentation (the reference's clock64 edge/procedure timers,
+ *    Instrumentation.cuh:47-76, become phase timestamps): when a device
+ *    buffer is set, workgroup 0 of the derivation kernel writes the device
+ *    wall clock (ticks of pga_timestamp_hz) at every phase boundary of every
+ *    level: [level * 4 + 0] level start, + 1 count pass done, + 2 prefix
+ *    sums done, + 3 write pass done; up to capacity / 4 levels. NULL turns
+ *    it off. -- */
+PGA_API pga_status pga_set_timestamps(pga_context* ctx, uint64_t* device_buffer, uint32_t capacity);
+PGA_API double pga_timestamp_hz(pga_context* ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,7 +36,10 @@ struct pga_context {
     std::vector<DevParam> h_params;
     std::vector<int32_t> h_succ;
     std::vector<uint32_t> h_exp;
-    DevEntry* d_entries = nullptr;
+    uint8_t* d_image = nullptr;       // entries | params | succ | exp | slots (16-byte sections)
+    uint32_t image_bytes = 0, off_params = 0, off_succ = 0, off_exp = 0, off_slots = 0;
+    uint32_t lds_bytes = 0;           // dynamic LDS of the derivation kernels, 0 = program read from HBM
+    DevEntry* d_entries = nullptr;    // views into d_image
     DevParam* d_params = nullptr;
     int32_t* d_succ = nullptr;
     uint32_t* d_exp = nullptr;
@@ -72,6 +75,8 @@ struct pga_context {
     uint32_t* d_err = nullptr;
     uint32_t* d_bar = nullptr;
     uint32_t* d_block_totals = nullptr;
+    uint64_t* ticks = nullptr;      // caller-owned timestamp buffer
+    uint32_t ticks_cap = 0;
     uint32_t* h_pinned = nullptr;   // count readback
     uint32_t mega_blocks = 0;
 
@@ -83,6 +88,8 @@ namespace {
 const uint32_t kDefaultCapacity = 1u << 22;
 const uint32_t kDefaultMaxLevels = 4096;
 const uint32_t kMaxChain = 256;
+const uint32_t kMaxProgramLds = 48u * 1024u;   // larger images stay in HBM
+const uint32_t kMaxBlocksPerCu = 8;
 
 pga_status fail(pga_context* ctx, pga_status st, const std::string& msg) {
     if (ctx) ctx->error = msg;
@@ -315,21 +322,63 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         slots[used[s]] = (uint8_t)s;
         slots[PGA_MAX_TERMINALS + s] = (uint8_t)used[s];
     }
+    // inline constants, successors and terminal slots of each entry
+    for (uint32_t i = 0; i < n; ++i) {
+        DevEntry& d = E[i];
+        d.inl = 0;
+        d.tslot = 0xFF;
+        for (uint32_t k = 0; k < 3; ++k) {
+            d.pv[k] = 0.0f;
+            d.sv[k] = k < d.nsucc ? Sc[d.sbase + k] : -1;
+            if (k < d.nparams && Pm[d.pbase + k].type == PGA_PT_SCALAR) {
+                d.inl |= (uint8_t)(1u << k);
+                d.pv[k] = Pm[d.pbase + k].a;
+            }
+        }
+        if (d.kind == pga::K_GENERATE && (d.inl & 1u)) {
+            float v = d.pv[0];
+            uint32_t t = v > 0.0f ? (uint32_t)v : 0u;
+            d.tslot = slots[t];
+        }
+    }
     if (Sc.empty()) Sc.push_back(-1);
     if (Pm.empty()) Pm.push_back(DevParam{});
     if (Ex.empty()) Ex.push_back(0u);
-    hfree(ctx->d_entries); hfree(ctx->d_params); hfree(ctx->d_succ); hfree(ctx->d_exp); hfree(ctx->d_slots);
-    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_slots, sizeof(slots)));
-    HIP_TRY(ctx, hipMemcpy(ctx->d_slots, slots, sizeof(slots), hipMemcpyHostToDevice));
+    // one program image: entries | params | succ | exp | slots, 16-byte sections
+    auto pad16 = [](size_t b) { return (b + 15u) & ~(size_t)15u; };
+    const size_t off_params = pad16(E.size() * sizeof(DevEntry));
+    const size_t off_succ = off_params + pad16(Pm.size() * sizeof(DevParam));
+    const size_t off_exp = off_succ + pad16(Sc.size() * sizeof(int32_t));
+    const size_t off_slots = off_exp + pad16(Ex.size() * sizeof(uint32_t));
+    const size_t image_bytes = off_slots + sizeof(slots);
+    std::vector<uint8_t> image(image_bytes, 0);
+    std::memcpy(image.data(), E.data(), E.size() * sizeof(DevEntry));
+    std::memcpy(image.data() + off_params, Pm.data(), Pm.size() * sizeof(DevParam));
+    std::memcpy(image.data() + off_succ, Sc.data(), Sc.size() * sizeof(int32_t));
+    std::memcpy(image.data() + off_exp, Ex.data(), Ex.size() * sizeof(uint32_t));
+    std::memcpy(image.data() + off_slots, slots, sizeof(slots));
+    hfree(ctx->d_image);
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_image, image_bytes));
+    HIP_TRY(ctx, hipMemcpy(ctx->d_image, image.data(), image_bytes, hipMemcpyHostToDevice));
+    ctx->image_bytes = (uint32_t)image_bytes;
+    ctx->off_params = (uint32_t)off_params;
+    ctx->off_succ = (uint32_t)off_succ;
+    ctx->off_exp = (uint32_t)off_exp;
+    ctx->off_slots = (uint32_t)off_slots;
+    ctx->d_entries = reinterpret_cast<DevEntry*>(ctx->d_image);
+    ctx->d_params = reinterpret_cast<DevParam*>(ctx->d_image + off_params);
+    ctx->d_succ = reinterpret_cast<int32_t*>(ctx->d_image + off_succ);
+    ctx->d_exp = reinterpret_cast<uint32_t*>(ctx->d_image + off_exp);
+    ctx->d_slots = ctx->d_image + off_slots;
     ctx->nslots = (uint32_t)used.size();
-    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_entries, E.size() * sizeof(DevEntry)));
-    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_params, Pm.size() * sizeof(DevParam)));
-    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_succ, Sc.size() * sizeof(int32_t)));
-    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_exp, Ex.size() * sizeof(uint32_t)));
-    HIP_TRY(ctx, hipMemcpy(ctx->d_entries, E.data(), E.size() * sizeof(DevEntry), hipMemcpyHostToDevice));
-    HIP_TRY(ctx, hipMemcpy(ctx->d_params, Pm.data(), Pm.size() * sizeof(DevParam), hipMemcpyHostToDevice));
-    HIP_TRY(ctx, hipMemcpy(ctx->d_succ, Sc.data(), Sc.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIP_TRY(ctx, hipMemcpy(ctx->d_exp, Ex.data(), Ex.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    // the image and the terminal tables go to LDS when they fit
+    const uint32_t lds = (uint32_t)image_bytes + 16u * PGA_MAX_TERMINALS;
+    ctx->lds_bytes = lds <= kMaxProgramLds ? lds : 0u;
+    int per_cu = 0;
+    HIP_TRY(ctx, pga_mega_occupancy(ctx->cfg.block_size, ctx->lds_bytes, &per_cu));
+    if (per_cu < 1) return fail(ctx, PGA_ERR_UNSUPPORTED, "derivation kernel does not fit on a compute unit");
+    const uint32_t max_blocks = std::min((uint32_t)per_cu, kMaxBlocksPerCu) * (uint32_t)ctx->num_cus;
+    ctx->mega_blocks = ctx->cfg.grid_blocks ? std::min(ctx->cfg.grid_blocks, max_blocks) : max_blocks;
     ctx->h_entries = E;
     ctx->h_params = Pm;
     ctx->h_succ = Sc;
@@ -369,7 +418,16 @@ pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
     a.prog.term_slot = ctx->d_slots;
     a.prog.slot_term = ctx->d_slots + PGA_MAX_TERMINALS;
     a.prog.nslots = ctx->nslots;
+    a.image = reinterpret_cast<const uint4*>(ctx->d_image);
+    a.image_bytes = ctx->image_bytes;
+    a.lds_bytes = ctx->lds_bytes;
+    a.off_params = ctx->off_params;
+    a.off_succ = ctx->off_succ;
+    a.off_exp = ctx->off_exp;
+    a.off_slots = ctx->off_slots;
     a.block_totals = ctx->d_block_totals;
+    a.ticks = reinterpret_cast<unsigned long long*>(ctx->ticks);
+    a.ticks_cap = ctx->ticks_cap;
     a.axioms = ctx->axioms;
     a.first_axiom = first;
     a.n_axioms = count;
@@ -481,25 +539,20 @@ PGA_API pga_status pga_create(const pga_config* config, pga_context** out) {
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_inst_ptr, PGA_MAX_TERMINALS * sizeof(pga_instance*)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_inst_cap, PGA_MAX_TERMINALS * sizeof(unsigned long long)));
     HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_pinned, 64 * sizeof(uint32_t), hipHostMallocDefault));
-    int per_cu = 0;
-    HIP_TRY(ctx, pga_mega_occupancy(ctx->cfg.block_size, &per_cu));
-    if (per_cu < 1) per_cu = 1;
-    uint32_t max_blocks = (uint32_t)per_cu * (uint32_t)ctx->num_cus;
-    ctx->mega_blocks = ctx->cfg.grid_blocks ? std::min(ctx->cfg.grid_blocks, max_blocks) : max_blocks;
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_block_totals,
-                           (size_t)ctx->mega_blocks * (PGA_MAX_TERMINALS + 2) * sizeof(uint32_t)));
+                           (size_t)ctx->num_cus * kMaxBlocksPerCu * (PGA_MAX_TERMINALS + 2) * sizeof(uint32_t)));
     return PGA_OK;
 }
 
 PGA_API void pga_destroy(pga_context* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    hfree(ctx->d_entries); hfree(ctx->d_params); hfree(ctx->d_succ); hfree(ctx->d_exp);
+    hfree(ctx->d_image);
     hfree(ctx->d_axioms_owned); hfree(ctx->d_aparams);
     hfree(ctx->d_inst_ptr); hfree(ctx->d_inst_cap);
     for (int b = 0; b < 2; ++b) { hfree(ctx->d_par[b]); hfree(ctx->d_desc[b]); }
     hfree(ctx->d_term_counts); hfree(ctx->d_par_counts); hfree(ctx->d_desc_counts);
-    hfree(ctx->d_err); hfree(ctx->d_bar); hfree(ctx->d_block_totals); hfree(ctx->d_slots);
+    hfree(ctx->d_err); hfree(ctx->d_bar); hfree(ctx->d_block_totals);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -695,6 +748,20 @@ PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first, uint32_t 
 
 PGA_API pga_status pga_execute(pga_context* ctx, uint32_t num_axioms, double* seconds) {
     return pga_execute_range(ctx, 0, num_axioms, 0, seconds);
+}
+
+PGA_API pga_status pga_set_timestamps(pga_context* ctx, uint64_t* device_buffer, uint32_t capacity) {
+    if (!ctx || (!device_buffer && capacity)) return fail(ctx, PGA_ERR_INVALID_ARG, "null timestamp buffer");
+    ctx->ticks = device_buffer;
+    ctx->ticks_cap = device_buffer ? capacity : 0u;
+    return PGA_OK;
+}
+
+PGA_API double pga_timestamp_hz(pga_context* ctx) {
+    if (!ctx) return 0.0;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess) return 0.0;
+    return (double)khz * 1e3;
 }
 
 PGA_API pga_status pga_get_stats(const pga_context* ctx, pga_stats* stats) {

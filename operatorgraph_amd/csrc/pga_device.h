@@ -36,15 +36,23 @@ enum EntryFlags : uint8_t {
     EF_SHAPE_DEP = 4, // some parameter reads the shape (dyn Subdivide re-evaluation)
 };
 
-struct DevEntry {       // 16 bytes
+// One dispatch entry, 48 bytes. The first three parameters are held inline
+// when they are constants (bit k of `inl`), and the first three successors
+// always, so most operators cost a single table read.
+struct __align__(16) DevEntry {
     uint8_t kind;
     uint8_t shape;      // pga_shape_type of the procedure (BOX/QUAD)
     uint8_t flags;
     uint8_t nparams;
     uint8_t nsucc;
-    uint8_t pad0, pad1, pad2;
+    uint8_t inl;        // bit k: parameter k (< 3) is the constant pv[k]
+    uint8_t tslot;      // Generate with a constant terminal: its dense slot (else 0xFF)
+    uint8_t pad0;
     uint32_t pbase;     // first DevParam
     uint32_t sbase;     // first successor in the successor pool
+    float pv[3];        // inline constant parameters
+    int32_t sv[3];      // successors 0..2
+    uint32_t pad1;
 };
 
 struct DevParam {       // 16 bytes
@@ -304,18 +312,29 @@ __device__ __forceinline__ bool shape_dependent(uint32_t type) {
 
 __device__ __forceinline__ float P(const Ctx& c, const DevEntry& e, uint32_t k, const Rec& r,
                                    uint32_t axiom, uint32_t path, uint32_t depth) {
+    if (k < 3u && ((e.inl >> k) & 1u)) return k == 0 ? e.pv[0] : (k == 1 ? e.pv[1] : e.pv[2]);
     return eval_param(c, c.prog.params[e.pbase + k], r, axiom, path, depth);
 }
 
-// Evaluates the first min(nparams, 3) parameters through one call site.
+// Successor k of an entry (the first three are held in the entry).
+__device__ __forceinline__ int32_t succ_of(const Ctx& c, const DevEntry& e, uint32_t k) {
+    if (k < 3u) return k == 0 ? e.sv[0] : (k == 1 ? e.sv[1] : e.sv[2]);
+    return c.prog.succ[e.sbase + k];
+}
+
+// Evaluates the first min(nparams, 3) parameters; constants come from the
+// entry, the others through one evaluation site.
 __device__ __forceinline__ void eval3(const Ctx& c, const DevEntry& e, const Rec& r, float& v0, float& v1,
                                       float& v2) {
     const uint32_t np = e.nparams < 3 ? e.nparams : 3u;
+    v0 = e.pv[0]; v1 = e.pv[1]; v2 = e.pv[2];
+    const uint32_t todo = ((1u << np) - 1u) & ~(uint32_t)e.inl;
+    if (todo == 0u) return;
     const uint32_t ax = r.axiom, pa = r.path, de = r.ed >> 16;
-    v0 = v1 = v2 = 0.0f;
 #pragma unroll 1
     for (uint32_t k = 0; k < np; ++k) {
-        float x = P(c, e, k, r, ax, pa, de);
+        if (!((todo >> k) & 1u)) continue;
+        float x = eval_param(c, c.prog.params[e.pbase + k], r, ax, pa, de);
         v0 = k == 0 ? x : v0;
         v1 = k == 1 ? x : v1;
         v2 = k == 2 ? x : v2;

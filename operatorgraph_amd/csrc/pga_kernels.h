@@ -9,7 +9,11 @@
 namespace pga {
 
 struct KArgs {
-    Prog prog;
+    Prog prog;                          // tables in the global program image
+    const uint4* image;                 // the program image (entries|params|succ|exp|slots)
+    uint32_t image_bytes;               // multiple of 16
+    uint32_t lds_bytes;                 // dynamic LDS per workgroup: image + terminal tables, 0 = none
+    uint32_t off_params, off_succ, off_exp, off_slots;
     const pga_axiom* axioms;
     uint32_t first_axiom;
     uint32_t n_axioms;
@@ -29,6 +33,8 @@ struct KArgs {
     uint32_t* desc_counts;              // [max_levels + 2]: children of level L + 1;
                                         // [max_levels + 1] = levels run by k_mega
     uint32_t* block_totals;             // [grid blocks][nslots + 2] counts of one pass
+    unsigned long long* ticks;          // phase timestamps (NULL: off)
+    uint32_t ticks_cap;
     uint32_t* err;
     uint32_t* bar;                      // grid barrier words
     uint32_t max_levels;
@@ -40,5 +46,5 @@ struct KArgs {
 hipError_t pga_launch_reset(const pga::KArgs& a, int reset_terms, hipStream_t s);
 hipError_t pga_launch_level(const pga::KArgs& a, uint32_t level, uint32_t blocks, uint32_t block_size,
                             hipStream_t s);
-hipError_t pga_mega_occupancy(uint32_t block_size, int* blocks_per_cu);
+hipError_t pga_mega_occupancy(uint32_t block_size, uint32_t lds_bytes, int* blocks_per_cu);
 hipError_t pga_launch_mega(const pga::KArgs& a, uint32_t blocks, uint32_t block_size, hipStream_t s);

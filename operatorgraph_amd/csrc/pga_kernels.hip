@@ -90,8 +90,11 @@ __device__ __forceinline__ void emit_terminal(const Ctx& c, LevelLDS& L, bool ha
     uint32_t slot = 0;
     if (has) {
         const DevEntry e = c.prog.entries[rec_entry(r)];
-        const uint32_t t = f2u_rd(P(c, e, 0, r, r.axiom, r.path, rec_depth(r)));
-        slot = t < PGA_MAX_TERMINALS ? c.prog.term_slot[t] : 0xFFu;
+        slot = e.tslot;
+        if (slot == 0xFFu) {                   // terminal index computed per shape
+            const uint32_t t = f2u_rd(P(c, e, 0, r, r.axiom, r.path, rec_depth(r)));
+            slot = t < PGA_MAX_TERMINALS ? c.prog.term_slot[t] : 0xFFu;
+        }
         if (slot >= c.prog.nslots) { raise(c, PGA_ERR_BAD_PROGRAM); slot = 0; }
     }
     while (pending) {
@@ -222,16 +225,45 @@ __device__ void process_child(const Ctx& c, LevelLDS& L, bool valid, Rec r) {
     if (__ballot(queue) != 0ull) emit_fanout<WRITE>(c, L, queue, st == ST_OVERLONG, r);
 }
 
+// Program image and terminal tables staged in LDS (dynamic shared memory).
+extern __shared__ __align__(16) unsigned char pga_smem[];
+
+// Copies the program image and the terminal buffer tables into LDS when
+// they fit (lds_bytes != 0); every operator then reads its entry,
+// parameters and successors from LDS instead of the cache hierarchy.
+__device__ __forceinline__ void stage_program(const KArgs& a) {
+    if (a.lds_bytes == 0u) return;
+    uint4* dst = reinterpret_cast<uint4*>(pga_smem);
+    const uint32_t n16 = a.image_bytes / 16u;
+    for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = a.image[i];
+    unsigned long long* tab = reinterpret_cast<unsigned long long*>(pga_smem + a.image_bytes);
+    for (uint32_t i = threadIdx.x; i < PGA_MAX_TERMINALS; i += blockDim.x) {
+        tab[i] = reinterpret_cast<unsigned long long>(a.inst[i]);
+        tab[PGA_MAX_TERMINALS + i] = a.inst_cap[i];
+    }
+    __syncthreads();
+}
+
 __device__ __forceinline__ Ctx make_ctx(const KArgs& a, uint32_t level) {
     Ctx c;
     c.prog = a.prog;
+    c.inst = a.inst;
+    c.inst_cap = a.inst_cap;
+    if (a.lds_bytes != 0u) {
+        c.prog.entries = reinterpret_cast<const DevEntry*>(pga_smem);
+        c.prog.params = reinterpret_cast<const DevParam*>(pga_smem + a.off_params);
+        c.prog.succ = reinterpret_cast<const int32_t*>(pga_smem + a.off_succ);
+        c.prog.exp = reinterpret_cast<const uint32_t*>(pga_smem + a.off_exp);
+        c.prog.term_slot = pga_smem + a.off_slots;
+        c.prog.slot_term = pga_smem + a.off_slots + PGA_MAX_TERMINALS;
+        c.inst = reinterpret_cast<pga_instance* const*>(pga_smem + a.image_bytes);
+        c.inst_cap = reinterpret_cast<const unsigned long long*>(pga_smem + a.image_bytes + 8u * PGA_MAX_TERMINALS);
+    }
     c.aparams = a.aparams;
     c.aparams_per = a.aparams_per;
     c.naxioms_total = a.naxioms_total;
     c.key0 = a.key0;
     c.key1 = a.key1;
-    c.inst = a.inst;
-    c.inst_cap = a.inst_cap;
     c.term_counts = a.term_counts;
     c.par_out = (level & 1u) ? a.par1 : a.par0;
     c.desc_out = (level & 1u) ? a.desc1 : a.desc0;
@@ -327,6 +359,14 @@ __device__ __forceinline__ uint32_t block_sum(LevelLDS& L, uint32_t v, int which
     return s;
 }
 
+// Phase timestamp of workgroup 0 (instrumentation; off when ticks is NULL).
+__device__ __forceinline__ void stamp(const KArgs& a, uint32_t level, uint32_t phase) {
+    if (a.ticks != nullptr && blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t i = level * 4u + phase;
+        if (i < a.ticks_cap) a.ticks[i] = wall_clock64();
+    }
+}
+
 // One derivation level: count pass, publish, barrier, scan, write pass.
 // Leaves L.next_n = children of the next level; block 0 records the level's
 // parent and descriptor counts.
@@ -335,9 +375,11 @@ __device__ void run_level(const KArgs& a, LevelLDS& L, uint32_t level, uint32_t 
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t w = wave_in_block();
+    stamp(a, level, 0);
     for (uint32_t k = lane_id(); k < ncnt; k += 64u) L.cnt[w][k] = 0u;
     run_pass<false>(a, L, level, n, wave, nwaves);
     __syncthreads();
+    stamp(a, level, 1);
     // block totals; in-block wave offsets go to the cursors
     uint32_t* totals = a.block_totals + (size_t)blockIdx.x * ncnt;
     for (uint32_t k = threadIdx.x; k < ncnt; k += blockDim.x) {
@@ -375,8 +417,10 @@ __device__ void run_level(const KArgs& a, LevelLDS& L, uint32_t level, uint32_t 
         }
     }
     __syncthreads();
+    stamp(a, level, 2);
     run_pass<true>(a, L, level, n, wave, nwaves);
     __syncthreads();
+    stamp(a, level, 3);
     if (threadIdx.x == 0)
         for (uint32_t k = 0; k < nslots; ++k) L.term_base[k] += L.cnt[0][k];
     __syncthreads();
@@ -401,6 +445,7 @@ __device__ __forceinline__ uint32_t first_level_size(const KArgs& a, uint32_t le
 // KERNELS technique: one level per (cooperative) launch.
 __global__ void __launch_bounds__(256) k_level(KArgs a, uint32_t level) {
     __shared__ LevelLDS L;
+    stage_program(a);
     init_bases(a, L);
     const uint32_t n = first_level_size(a, level);
     if (n != 0u) run_level(a, L, level, n);
@@ -411,6 +456,7 @@ __global__ void __launch_bounds__(256) k_level(KArgs a, uint32_t level) {
 // MEGAKERNEL technique: all levels in one persistent launch.
 __global__ void __launch_bounds__(256) k_mega(KArgs a) {
     __shared__ LevelLDS L;
+    stage_program(a);
     init_bases(a, L);
     uint32_t level = 0;
     uint32_t n = a.n_axioms;
@@ -451,14 +497,15 @@ hipError_t pga_launch_level(const pga::KArgs& a, uint32_t level, uint32_t blocks
                             hipStream_t s) {
     pga::KArgs copy = a;
     void* args[] = {&copy, &level};
-    return hipLaunchCooperativeKernel((const void*)pga::k_level, dim3(blocks), dim3(block_size), args, 0, s);
+    return hipLaunchCooperativeKernel((const void*)pga::k_level, dim3(blocks), dim3(block_size), args,
+                                      a.lds_bytes, s);
 }
 
-hipError_t pga_mega_occupancy(uint32_t block_size, int* blocks_per_cu) {
+hipError_t pga_mega_occupancy(uint32_t block_size, uint32_t lds_bytes, int* blocks_per_cu) {
     int a = 0, b = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, pga::k_mega, (int)block_size, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, pga::k_mega, (int)block_size, lds_bytes);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pga::k_level, (int)block_size, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pga::k_level, (int)block_size, lds_bytes);
     *blocks_per_cu = a < b ? a : b;
     return e;
 }
@@ -466,5 +513,6 @@ hipError_t pga_mega_occupancy(uint32_t block_size, int* blocks_per_cu) {
 hipError_t pga_launch_mega(const pga::KArgs& a, uint32_t blocks, uint32_t block_size, hipStream_t s) {
     pga::KArgs copy = a;
     void* args[] = {&copy};
-    return hipLaunchCooperativeKernel((const void*)pga::k_mega, dim3(blocks), dim3(block_size), args, 0, s);
+    return hipLaunchCooperativeKernel((const void*)pga::k_mega, dim3(blocks), dim3(block_size), args,
+                                      a.lds_bytes, s);
 }

@@ -71,7 +71,6 @@ __device__ __forceinline__ int32_t random_choice(const Ctx& c, const DevEntry& e
 // Applies one single-child operator in place. Returns the successor entry
 // (or -1 when nothing is dispatched) and the child ordinal.
 __device__ __forceinline__ int32_t apply_single(const Ctx& c, const DevEntry& e, Rec& r, uint32_t* ordinal) {
-    const int32_t* succ = c.prog.succ + e.sbase;
     float v0, v1, v2;
     eval3(c, e, r, v0, v1, v2);
     *ordinal = 0;
@@ -80,25 +79,25 @@ __device__ __forceinline__ int32_t apply_single(const Ctx& c, const DevEntry& e,
         uint32_t axis = axis_of(c, v0);
         uint32_t t = (sel3(r.s, axis) < v1) ? 0u : 1u;
         *ordinal = t;
-        return succ[t];
+        return succ_of(c, e, t);
     }
     if (k == K_IF) {                           // If.cuh:23-58 (NaN counts as true)
         uint32_t t = (v0 != 0.0f) ? 0u : 1u;
         *ordinal = t;
-        return succ[t];
+        return succ_of(c, e, t);
     }
     if (k == K_RANDOM) {
         float chance = v0 - floorf(v0);        // math::frac (math.h:243-246)
         int32_t ch = random_choice(c, e, chance, r.axiom, r.path, rec_depth(r), r);
         if (ch < 0) return -1;                 // dyn form: round-off can choose nothing
         *ordinal = (uint32_t)ch;
-        return succ[ch];
+        return succ_of(c, e, (uint32_t)ch);
     }
     if (k == K_TRANSLATE) {                    // Translate.cuh:24-50
         float m[12];
         mul_translate(r.m, v0, v1, v2, m);
         set_model(r, m);
-        return succ[0];
+        return succ_of(c, e, 0);
     }
     if (k == K_ROTATE) {                       // Rotate.cuh:24-51, matrix.h:876-888
         const float kd = 3.1415926535897932384626434f / 180.0f;
@@ -113,20 +112,20 @@ __device__ __forceinline__ int32_t apply_single(const Ctx& c, const DevEntry& e,
         float m[12];
         mul34x44(r.m, b, m);
         set_model(r, m);
-        return succ[0];
+        return succ_of(c, e, 0);
     }
     if (k == K_SCALE) {                        // Scale.cuh:23-50
         const float v[3] = {v0, v1, v2};
 #pragma unroll
         for (int i = 0; i < 3; ++i)
             r.s[i] = (((float)(v[i] < 0.0f) * r.s[i]) * v[i]) * -1.0f + ((float)(v[i] > 0.0f) * v[i]);
-        return succ[0];
+        return succ_of(c, e, 0);
     }
     if (k == K_SWAPSIZE) {                     // SwapSize.cuh:25-58
         uint32_t a0 = axis_of(c, v0), a1 = axis_of(c, v1), a2 = axis_of(c, v2);
         float s0 = sel3(r.s, a0), s1 = sel3(r.s, a1), s2 = sel3(r.s, a2);
         r.s[0] = s0; r.s[1] = s1; r.s[2] = s2;
-        return succ[0];
+        return succ_of(c, e, 0);
     }
     // K_EXTRUDE, Extrude.cuh:25-71 (Quad -> Box); parameter 0 (axis) unused
     const float b[16] = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1.0f, 0.0f,
@@ -137,7 +136,7 @@ __device__ __forceinline__ int32_t apply_single(const Ctx& c, const DevEntry& e,
     float sx = r.s[0], sy = r.s[1];
     r.s[0] = sx; r.s[1] = v1; r.s[2] = sy;
     r.seed = next_seed(r.seed, 0u);
-    return succ[0];
+    return succ_of(c, e, 0);
 }
 
 __device__ __forceinline__ bool is_fanout(uint8_t k) {
@@ -327,9 +326,9 @@ __device__ __forceinline__ bool fan_geom(const Ctx& c, const Rec& r, Fan& f, uin
 // Child j of a fan-out parent from its geometry (o, ns along `axis`).
 // Repeat pieces use seed modifier j + 1 (1 for the first piece, i for the
 // middle pieces i = 2 .. reps-1, reps for the last, Repeat.cuh:156-184).
-__device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, uint32_t kind, uint32_t sbase, uint32_t j,
+__device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, const DevEntry& e, uint32_t j,
                                             uint32_t axis, float o, float ns, Rec& ch) {
-    const int32_t* succ = c.prog.succ + sbase;
+    const uint32_t kind = e.kind;
     ch = r;
     if (kind == K_SUBDIV || kind == K_REPEAT) {
         float tx, ty, tz;
@@ -338,7 +337,7 @@ __device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, uint32_t
         if (axis == 0) ch.s[0] = ns; else if (axis == 1) ch.s[1] = ns; else ch.s[2] = ns;
         const bool sub = kind == K_SUBDIV;
         ch.seed = next_seed(r.seed, sub ? j : j + 1u);
-        return advance(c, ch, succ[sub ? j : 0u], j);
+        return advance(c, ch, succ_of(c, e, sub ? j : 0u), j);
     }
     if (kind == K_COMPSPLIT) {                 // FaceSplitter<Box> (ComponentSplit.cuh:77-174)
         const float hx = r.s[0] * 0.5f, hy = r.s[1] * 0.5f, hz = r.s[2] * 0.5f;
@@ -367,11 +366,11 @@ __device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, uint32_t
         ch.s[2] = 1.0f;
         const uint32_t mod = j < 4 ? j + 2u : (j == 4 ? 1u : 0u);
         ch.seed = next_seed(r.seed, mod);
-        const int32_t target = j < 4 ? succ[2] : (j == 4 ? succ[1] : succ[0]);
+        const int32_t target = j < 4 ? e.sv[2] : (j == 4 ? e.sv[1] : e.sv[0]);
         return advance(c, ch, target, j);
     }
     // K_REPLICATE
-    return advance(c, ch, succ[j], j);
+    return advance(c, ch, succ_of(c, e, j), j);
 }
 
 // Child j of the fan-out parent `r` (j = 0, 1, ... in order). Returns false
@@ -379,7 +378,7 @@ __device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, uint32_t
 __device__ __forceinline__ bool fan_child(const Ctx& c, const Rec& r, Fan& f, uint32_t j, Rec& ch) {
     float o, ns;
     if (!fan_geom(c, r, f, j, o, ns)) return false;
-    return build_child(c, r, f.kind_flags & 0xFFu, f.sbase, j, f.axis, o, ns, ch);
+    return build_child(c, r, c.prog.entries[rec_entry(r)], j, f.axis, o, ns, ch);
 }
 
 // Child described by `d` of the parent record `r`.
@@ -388,7 +387,7 @@ __device__ __forceinline__ bool desc_child(const Ctx& c, const Rec& r, const Des
     if (j == kDescSelf) { ch = r; return true; }
     if (j == kDescNone) return false;
     const DevEntry e = c.prog.entries[rec_entry(r)];
-    return build_child(c, r, e.kind, e.sbase, j, d.jx >> 30, d.o, d.ns, ch);
+    return build_child(c, r, e, j, d.jx >> 30, d.o, d.ns, ch);
 }
 
 } // namespace pga

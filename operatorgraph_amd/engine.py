@@ -58,6 +58,8 @@ def load_library():
         "pga_execute_async": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int32]),
         "pga_synchronize": (C.c_int, [P]),
         "pga_kernel_seconds": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "pga_set_timestamps": (C.c_int, [P, P, C.c_uint32]),
+        "pga_timestamp_hz": (C.c_double, [P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -195,6 +197,31 @@ class Engine:
         sec = C.c_double(0.0)
         self._check(self.L.pga_kernel_seconds(self._h, C.byref(sec)))
         return sec.value
+
+    def enable_timestamps(self, capacity=4 * 64):
+        """Phase timestamps of the derivation kernel (see pga_set_timestamps)."""
+        torch = self.torch
+        buf = torch.zeros(capacity, dtype=torch.int64, device="cuda")
+        self._keep["ticks"] = buf
+        self._check(self.L.pga_set_timestamps(self._h, C.c_void_p(buf.data_ptr()), capacity))
+        return buf
+
+    def phase_times(self):
+        """Per-level (count, scan, write) durations in microseconds of the
+        last run with timestamps enabled."""
+        buf = self._keep.get("ticks")
+        if buf is None:
+            return []
+        hz = self.L.pga_timestamp_hz(self._h)
+        t = buf.cpu().numpy().reshape(-1, 4).astype(np.float64)
+        out = []
+        for lv in range(len(t)):
+            if t[lv, 0] == 0:
+                break
+            d = np.diff(t[lv]) / hz * 1e6
+            gap = (t[lv + 1, 0] - t[lv, 3]) / hz * 1e6 if lv + 1 < len(t) and t[lv + 1, 0] else 0.0
+            out.append({"count_us": d[0], "scan_us": d[1], "write_us": d[2], "barrier_us": gap})
+        return out
 
     def stats(self):
         s = abi.Stats()
