@@ -45,6 +45,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--only-kernel", action="store_true",
                    help="run the timed batched steps only (for profilers)")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="run only device-to-device copies of MIB MiB (scales HBM byte counters)")
     return p.parse_args()
 
 
@@ -128,6 +130,16 @@ def main():
               "(the derivation shards by axiom with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     import torch
+
+    if args.copy_calibration:
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": args.steps}), flush=True)
+        return
 
     from operatorgraph_amd import axioms, engine, grammars
 

@@ -201,7 +201,7 @@ typedef struct pga_instance {
 typedef struct pga_config {
     int32_t technique;       /* pga_technique */
     uint32_t queue_capacity; /* shape records per frontier buffer (0: default) */
-    uint32_t block_size;     /* threads per workgroup, multiple of 64 (0: 256) */
+    uint32_t block_size;     /* threads per workgroup: 256, 512, 768 or 1024 (0: 512) */
     uint32_t max_levels;     /* derivation-level guard (0: 4096) */
     uint32_t grid_blocks;    /* persistent grid size (0: occupancy-derived) */
     int32_t device;          /* HIP device ordinal (-1: current) */

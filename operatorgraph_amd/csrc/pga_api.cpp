@@ -38,7 +38,8 @@ struct pga_context {
     std::vector<uint32_t> h_exp;
     uint8_t* d_image = nullptr;       // entries | params | succ | exp | slots (16-byte sections)
     uint32_t image_bytes = 0, off_params = 0, off_succ = 0, off_exp = 0, off_slots = 0;
-    uint32_t lds_bytes = 0;           // dynamic LDS of the derivation kernels, 0 = program read from HBM
+    uint32_t lds_bytes = 0;           // dynamic LDS of the derivation kernels
+    uint32_t count_geom = 0;          // 1: some decision of the program reads the model matrix
     DevEntry* d_entries = nullptr;    // views into d_image
     DevParam* d_params = nullptr;
     int32_t* d_succ = nullptr;
@@ -88,8 +89,9 @@ namespace {
 const uint32_t kDefaultCapacity = 1u << 22;
 const uint32_t kDefaultMaxLevels = 4096;
 const uint32_t kMaxChain = 256;
-const uint32_t kMaxProgramLds = 48u * 1024u;   // larger images stay in HBM
+const uint32_t kMaxProgramLds = 60u * 1024u;   // program image + terminal tables
 const uint32_t kMaxBlocksPerCu = 8;
+const uint32_t kDefaultBlockSize = 512;        // one workgroup of 8 waves per compute unit
 
 pga_status fail(pga_context* ctx, pga_status st, const std::string& msg) {
     if (ctx) ctx->error = msg;
@@ -162,6 +164,7 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
     std::vector<int32_t> Sc;
     std::vector<uint32_t> Ex;
     int32_t max_slot = -1;
+    bool count_geom = false;
     for (uint32_t i = 0; i < n; ++i) {
         const pga_dispatch_entry& e = entries[i];
         std::string at = "entry " + std::to_string(i) + ": ";
@@ -225,10 +228,12 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
                 if (!(q.value[0] >= 0.0f) || (uint32_t)q.value[0] > 16u)
                     return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "shape attribute index out of range");
                 dp.a = q.value[0];
+                if ((uint32_t)q.value[0] < 12u) count_geom = true;   // a model-matrix element
                 break;
             case PGA_PT_EXP: {
                 std::string why;
                 if (!check_exp(q, why)) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + why);
+                count_geom = true;   // conservatively: an expression may read the model matrix
                 uint32_t off = (uint32_t)Ex.size();
                 uint32_t len = (uint32_t)q.value[0];
                 for (uint32_t v = 0; v <= len; ++v) {
@@ -341,6 +346,50 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
             d.tslot = slots[t];
         }
     }
+    // direct leaf fan-outs: every child is a constant-terminal Generate or a
+    // Discard, so a counting pass takes its terminal counts from a table
+    for (uint32_t i = 0; i < n; ++i) {
+        DevEntry& d = E[i];
+        d.ndirect = 0;
+        std::memset(d.dslot, 0, sizeof(d.dslot));
+        std::memset(d.dcnt, 0, sizeof(d.dcnt));
+        if (!(d.flags & pga::EF_LEAF)) continue;
+        std::vector<std::pair<int32_t, uint32_t>> kids;   // (successor, children that take it)
+        bool ok = true;
+        if (d.kind == pga::K_SUBDIV) {
+            for (uint32_t j = 0; j + 1 < d.nparams; ++j) {
+                const DevParam& q = Pm[d.pbase + 1 + j];
+                // the static form skips zero factors: only constant non-zero ones have fixed counts
+                if ((d.flags & pga::EF_STATIC) && !(q.type == PGA_PT_SCALAR && q.a != 0.0f)) ok = false;
+                kids.push_back({Sc[d.sbase + j], 1u});
+            }
+        } else if (d.kind == pga::K_REPEAT) {
+            kids.push_back({Sc[d.sbase], 1u});
+        } else if (d.kind == pga::K_COMPSPLIT) {
+            kids.push_back({Sc[d.sbase + 2], 4u});
+            kids.push_back({Sc[d.sbase + 1], 1u});
+            kids.push_back({Sc[d.sbase + 0], 1u});
+        } else {
+            for (uint32_t k = 0; k < d.nsucc; ++k) kids.push_back({Sc[d.sbase + k], 1u});
+        }
+        uint32_t cnt[PGA_MAX_TERMINALS] = {0};
+        for (auto& kc : kids) {
+            if (kc.first < 0 || E[kc.first].kind == pga::K_DISCARD) continue;
+            if (E[kc.first].kind != pga::K_GENERATE || E[kc.first].tslot == 0xFF) { ok = false; break; }
+            cnt[E[kc.first].tslot] += kc.second;
+        }
+        uint32_t nd = 0;
+        for (uint32_t t = 0; t < PGA_MAX_TERMINALS && ok; ++t) {
+            if (cnt[t] == 0) continue;
+            if (nd == 4 || cnt[t] > 255) { ok = false; break; }
+            d.dslot[nd] = (uint8_t)t;
+            d.dcnt[nd] = (uint8_t)cnt[t];
+            ++nd;
+        }
+        if (!ok) continue;
+        d.ndirect = (uint8_t)nd;
+        d.flags |= pga::EF_DIRECT;
+    }
     if (Sc.empty()) Sc.push_back(-1);
     if (Pm.empty()) Pm.push_back(DevParam{});
     if (Ex.empty()) Ex.push_back(0u);
@@ -371,9 +420,14 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
     ctx->d_exp = reinterpret_cast<uint32_t*>(ctx->d_image + off_exp);
     ctx->d_slots = ctx->d_image + off_slots;
     ctx->nslots = (uint32_t)used.size();
-    // the image and the terminal tables go to LDS when they fit
+    // the image and the terminal tables live in LDS during a derivation
     const uint32_t lds = (uint32_t)image_bytes + 16u * PGA_MAX_TERMINALS;
-    ctx->lds_bytes = lds <= kMaxProgramLds ? lds : 0u;
+    if (lds > kMaxProgramLds)
+        return fail(ctx, PGA_ERR_UNSUPPORTED, "program image of " + std::to_string(image_bytes) +
+                    " bytes exceeds the " + std::to_string(kMaxProgramLds - 16u * PGA_MAX_TERMINALS) +
+                    " bytes the derivation kernels stage in LDS");
+    ctx->lds_bytes = lds;
+    ctx->count_geom = count_geom ? 1u : 0u;
     int per_cu = 0;
     HIP_TRY(ctx, pga_mega_occupancy(ctx->cfg.block_size, ctx->lds_bytes, &per_cu));
     if (per_cu < 1) return fail(ctx, PGA_ERR_UNSUPPORTED, "derivation kernel does not fit on a compute unit");
@@ -451,6 +505,7 @@ pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
     a.bar = ctx->d_bar;
     a.max_levels = ctx->cfg.max_levels;
     a.max_chain = kMaxChain;
+    a.count_geom = ctx->count_geom;
     return a;
 }
 
@@ -509,8 +564,8 @@ PGA_API pga_status pga_create(const pga_config* config, pga_context** out) {
         delete ctx;
         return PGA_ERR_INVALID_ARG;
     }
-    if (ctx->cfg.block_size == 0) ctx->cfg.block_size = 256;
-    if (ctx->cfg.block_size % 64 != 0 || ctx->cfg.block_size > 256) {
+    if (ctx->cfg.block_size == 0) ctx->cfg.block_size = kDefaultBlockSize;
+    if (!pga_block_size_supported(ctx->cfg.block_size)) {
         delete ctx;
         return PGA_ERR_INVALID_ARG;
     }
@@ -534,8 +589,8 @@ PGA_API pga_status pga_create(const pga_config* config, pga_context** out) {
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_par_counts, (ctx->cfg.max_levels + 2) * sizeof(uint32_t)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_desc_counts, (ctx->cfg.max_levels + 2) * sizeof(uint32_t)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_err, sizeof(uint32_t)));
-    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_bar, 2 * sizeof(uint32_t)));
-    HIP_TRY(ctx, hipMemset(ctx->d_bar, 0, 2 * sizeof(uint32_t)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_bar, pga::kBarWords * sizeof(uint32_t)));
+    HIP_TRY(ctx, hipMemset(ctx->d_bar, 0, pga::kBarWords * sizeof(uint32_t)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_inst_ptr, PGA_MAX_TERMINALS * sizeof(pga_instance*)));
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_inst_cap, PGA_MAX_TERMINALS * sizeof(unsigned long long)));
     HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_pinned, 64 * sizeof(uint32_t), hipHostMallocDefault));

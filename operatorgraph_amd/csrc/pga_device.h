@@ -34,6 +34,8 @@ enum EntryFlags : uint8_t {
     EF_STATIC = 1,    // static-parameter operator form
     EF_LEAF = 2,      // fan-out whose children never reach another fan-out
     EF_SHAPE_DEP = 4, // some parameter reads the shape (dyn Subdivide re-evaluation)
+    EF_DIRECT = 8,    // leaf fan-out whose children are all Generate (constant terminal) or
+                      // Discard: its terminal counts are a table (dslot, dcnt)
 };
 
 // One dispatch entry, 48 bytes. The first three parameters are held inline
@@ -47,12 +49,13 @@ struct __align__(16) DevEntry {
     uint8_t nsucc;
     uint8_t inl;        // bit k: parameter k (< 3) is the constant pv[k]
     uint8_t tslot;      // Generate with a constant terminal: its dense slot (else 0xFF)
-    uint8_t pad0;
+    uint8_t ndirect;    // direct fan-out: pairs in dslot/dcnt
     uint32_t pbase;     // first DevParam
     uint32_t sbase;     // first successor in the successor pool
     float pv[3];        // inline constant parameters
     int32_t sv[3];      // successors 0..2
-    uint32_t pad1;
+    uint8_t dslot[4];   // direct fan-out: terminal slot k receives dcnt[k] instances
+    uint8_t dcnt[4];    //   per child set (times the repetitions of a Repeat)
 };
 
 struct DevParam {       // 16 bytes
@@ -106,6 +109,7 @@ struct Ctx {
     uint32_t par_cap, desc_cap;
     uint32_t* err;                      // bit (1 << pga_status) per error raised
     uint32_t max_chain;
+    uint32_t geom;                      // 0: skip model-matrix arithmetic (counting only)
 };
 
 __device__ __forceinline__ void raise(const Ctx& c, int status) {
@@ -128,13 +132,15 @@ __device__ __forceinline__ uint32_t taus_step(uint32_t z, int s1, int s2, int s3
     uint32_t b = (((z << s1) ^ z) >> s2);
     return (((z & m) << s3) ^ b);
 }
+// x % 0xFFFFFFFF for a 32-bit x: only x = 0xFFFFFFFF wraps
+__device__ __forceinline__ uint32_t mod_um(uint32_t x) { return x == 0xFFFFFFFFu ? 0u : x; }
 __device__ __forceinline__ float next_seed(float base, uint32_t modifier) {
     const uint32_t um = 0xFFFFFFFFu;
-    uint32_t z = (f2u_rd(base) * um) + (modifier * 41231278u) % um;
-    uint32_t z1 = (z + 7873234u) % um;
-    uint32_t z2 = (z1 + 1237217827u) % um;
-    uint32_t z3 = (z2 + 87382u) % um;
-    uint32_t z4 = (z3 + 1410022u) % um;
+    uint32_t z = (f2u_rd(base) * um) + mod_um(modifier * 41231278u);
+    uint32_t z1 = mod_um(z + 7873234u);
+    uint32_t z2 = mod_um(z1 + 1237217827u);
+    uint32_t z3 = mod_um(z2 + 87382u);
+    uint32_t z4 = mod_um(z3 + 1410022u);
     uint32_t t = taus_step(z1, 13, 19, 12, 4294967294u) ^ taus_step(z2, 2, 25, 4, 4294967288u) ^
                  taus_step(z3, 3, 11, 17, 4294967280u) ^ (1664525u * z4 + 1013904223u);
     return (float)(2.3283064365387e-10 * (double)t);
@@ -308,37 +314,6 @@ __device__ __forceinline__ float eval_param(const Ctx& c, const DevParam& p, con
 
 __device__ __forceinline__ bool shape_dependent(uint32_t type) {
     return type == PGA_PT_RAND || type == PGA_PT_SHAPE_ATTR || type == PGA_PT_EXP;
-}
-
-__device__ __forceinline__ float P(const Ctx& c, const DevEntry& e, uint32_t k, const Rec& r,
-                                   uint32_t axiom, uint32_t path, uint32_t depth) {
-    if (k < 3u && ((e.inl >> k) & 1u)) return k == 0 ? e.pv[0] : (k == 1 ? e.pv[1] : e.pv[2]);
-    return eval_param(c, c.prog.params[e.pbase + k], r, axiom, path, depth);
-}
-
-// Successor k of an entry (the first three are held in the entry).
-__device__ __forceinline__ int32_t succ_of(const Ctx& c, const DevEntry& e, uint32_t k) {
-    if (k < 3u) return k == 0 ? e.sv[0] : (k == 1 ? e.sv[1] : e.sv[2]);
-    return c.prog.succ[e.sbase + k];
-}
-
-// Evaluates the first min(nparams, 3) parameters; constants come from the
-// entry, the others through one evaluation site.
-__device__ __forceinline__ void eval3(const Ctx& c, const DevEntry& e, const Rec& r, float& v0, float& v1,
-                                      float& v2) {
-    const uint32_t np = e.nparams < 3 ? e.nparams : 3u;
-    v0 = e.pv[0]; v1 = e.pv[1]; v2 = e.pv[2];
-    const uint32_t todo = ((1u << np) - 1u) & ~(uint32_t)e.inl;
-    if (todo == 0u) return;
-    const uint32_t ax = r.axiom, pa = r.path, de = r.ed >> 16;
-#pragma unroll 1
-    for (uint32_t k = 0; k < np; ++k) {
-        if (!((todo >> k) & 1u)) continue;
-        float x = eval_param(c, c.prog.params[e.pbase + k], r, ax, pa, de);
-        v0 = k == 0 ? x : v0;
-        v1 = k == 1 ? x : v1;
-        v2 = k == 2 ? x : v2;
-    }
 }
 
 } // namespace pga

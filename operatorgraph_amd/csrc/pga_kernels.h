@@ -8,6 +8,8 @@
 
 namespace pga {
 
+constexpr uint32_t kBarWords = 160;     // grid barrier: generation + 9 counters, 64 bytes apart
+
 struct KArgs {
     Prog prog;                          // tables in the global program image
     const uint4* image;                 // the program image (entries|params|succ|exp|slots)
@@ -39,10 +41,12 @@ struct KArgs {
     uint32_t* bar;                      // grid barrier words
     uint32_t max_levels;
     uint32_t max_chain;
+    uint32_t count_geom;                // 1: some decision reads the model matrix
 };
 
 } // namespace pga
 
+bool pga_block_size_supported(uint32_t block_size);   // 256, 512, 768 or 1024
 hipError_t pga_launch_reset(const pga::KArgs& a, int reset_terms, hipStream_t s);
 hipError_t pga_launch_level(const pga::KArgs& a, uint32_t level, uint32_t blocks, uint32_t block_size,
                             hipStream_t s);

@@ -1,13 +1,18 @@
 // pga_kernels.hip - derivation kernels for gfx950 (wave64).
 //
-// A derivation level processes the level's children, one per lane: the
-// child is rebuilt from its parent record and 16-byte descriptor, runs its
-// fused single-child chain, and then either
-//   - writes a terminal instance,
-//   - expands a leaf fan-out in place (all its children are terminals), or
-//   - becomes a parent of the next level: its record is written once and one
-//     descriptor per child (with the sequential running offsets computed
-//     here) is appended.
+// A derivation level processes the level's children, one per lane. Each
+// lane runs a small state machine over its child, one operator per step:
+//   SETTLE - apply the single-child operator of the current record, or
+//            finish it: a Generate emits a terminal instance, a Discard
+//            drops it, a fan-out starts FAN;
+//   FAN    - child j of the fan-out parent: a leaf fan-out (all of whose
+//            children end in terminals) builds the child and settles it in
+//            place; any other fan-out writes its parent record once and one
+//            16-byte descriptor per child (with the sequential running
+//            offsets computed here) for the next level.
+// Every operator body exists once in the code, and the two passes of a
+// level (below) share it, so the kernel stays small enough for the
+// instruction cache and the register file (four waves per SIMD).
 // Level 0's children are the axioms.
 //
 // Output slots come from exact prefix sums, not atomics: a count pass
@@ -16,7 +21,8 @@
 // scans the totals of the blocks before it, and a write pass recomputes the
 // children and writes at exact offsets. Same-address global atomics sustain
 // only ~1e8/s on this device, so per-wave atomics would bound the kernel;
-// the scan also makes the output order deterministic.
+// the scan also makes the output order deterministic. The count pass skips
+// model-matrix arithmetic when no decision of the program reads the matrix.
 //
 // Two techniques share that level body:
 //   k_level  - one cooperative launch per level, the host reads the next
@@ -27,16 +33,21 @@
 
 namespace pga {
 
-constexpr int kWaves = 4;                        // waves per workgroup (256 threads)
+constexpr int kWaves = 16;                       // most waves per workgroup (1024 threads)
 constexpr int kCounters = PGA_MAX_TERMINALS + 2; // terminal slots, parents, descriptors
 
 struct LevelLDS {
     uint32_t cnt[kWaves][kCounters];   // count pass: per-wave totals
     uint32_t cur[kWaves][kCounters];   // write pass: per-wave cursors
+    uint32_t before[kCounters];        // totals of the preceding blocks
+    uint32_t level_total[kCounters];   // totals of the level over the grid
     unsigned long long term_base[PGA_MAX_TERMINALS];  // running instance count per slot
-    uint32_t red[kWaves][2];
     uint32_t next_n;                   // children of the next level (all blocks agree)
 };
+
+enum Mode : uint32_t { M_DONE = 0, M_SETTLE = 1, M_FAN = 2 };
+
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint32_t wave_in_block() { return threadIdx.x >> 6; }
@@ -64,14 +75,13 @@ __device__ __forceinline__ void load_rec(const Rec* src, Rec& r) {
 }
 
 // Claims popc(mask) consecutive slots of counter k for the lanes in `mask`:
-// count pass adds to the wave's total, write pass returns the wave cursor
-// (the lane's slot is cursor + mbcnt(mask)) and advances it.
-template <bool WRITE>
-__device__ __forceinline__ uint32_t claim(LevelLDS& L, uint32_t k, unsigned long long mask, int leader) {
+// the count pass adds to the wave's total, the write pass returns the wave
+// cursor (the lane's slot is cursor + mbcnt(mask)) and advances it.
+__device__ __forceinline__ uint32_t claim(LevelLDS& L, bool wr, uint32_t k, unsigned long long mask, int leader) {
     const uint32_t w = wave_in_block();
     const uint32_t cnt = (uint32_t)__popcll(mask);
     uint32_t base = 0;
-    if (WRITE) {
+    if (wr) {
         base = L.cur[w][k];
         if ((int)lane_id() == leader) L.cur[w][k] = base + cnt;
     } else {
@@ -83,156 +93,90 @@ __device__ __forceinline__ uint32_t claim(LevelLDS& L, uint32_t k, unsigned long
 // GenerationFunction<GenFuncFilter, 1>::perShape (GenerationFunctions.cuh:
 // 155-189): instance = model4 * scale(size), custom attribute. A full buffer
 // is an error (the reference wraps modulo capacity); capacity 0 counts only.
-template <bool WRITE>
-__device__ __forceinline__ void emit_terminal(const Ctx& c, LevelLDS& L, bool has, const Rec& r) {
+__device__ __forceinline__ void emit_terminal(const Ctx& c, LevelLDS& L, bool wr, bool has, uint32_t slot,
+                                              const Rec& r) {
     unsigned long long pending = __ballot(has);
-    if (pending == 0ull) return;
-    uint32_t slot = 0;
-    if (has) {
-        const DevEntry e = c.prog.entries[rec_entry(r)];
-        slot = e.tslot;
-        if (slot == 0xFFu) {                   // terminal index computed per shape
-            const uint32_t t = f2u_rd(P(c, e, 0, r, r.axiom, r.path, rec_depth(r)));
-            slot = t < PGA_MAX_TERMINALS ? c.prog.term_slot[t] : 0xFFu;
-        }
-        if (slot >= c.prog.nslots) { raise(c, PGA_ERR_BAD_PROGRAM); slot = 0; }
-    }
-    while (pending) {
+    unsigned long long idx = 0;
+    while (pending) {                          // one claim per distinct slot in the wave
         const int leader = __ffsll((long long)pending) - 1;
         const uint32_t sl = __shfl(slot, leader);
         const bool mine = has && slot == sl;
         const unsigned long long m = __ballot(mine);
-        const uint32_t base = claim<WRITE>(L, sl, m, leader);
-        if (WRITE && mine) {
-            const unsigned long long idx = L.term_base[sl] + base + mbcnt(m);
-            const uint32_t t = c.prog.slot_term[sl];
-            const unsigned long long cap = c.inst_cap[t];
-            if (idx < cap) {
-                const float b[16] = {r.s[0], 0.0f, 0.0f, 0.0f, 0.0f, r.s[1], 0.0f, 0.0f,
-                                     0.0f, 0.0f, r.s[2], 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
-                float mm[12];
-                mul34x44(r.m, b, mm);
-                float4* d = reinterpret_cast<float4*>(c.inst[t] + idx);
-                d[0] = make_float4(mm[0], mm[1], mm[2], mm[3]);
-                d[1] = make_float4(mm[4], mm[5], mm[6], mm[7]);
-                d[2] = make_float4(mm[8], mm[9], mm[10], mm[11]);
-                d[3] = make_float4(r.custom, __uint_as_float(r.axiom), __uint_as_float(r.path),
-                                   __uint_as_float(r.ed));
-            } else if (cap != 0ull) {
-                raise(c, PGA_ERR_OVERFLOW_INSTANCES);
-            }
-        }
+        const uint32_t base = claim(L, wr, sl, m, leader);
+        if (mine) idx = L.term_base[sl] + base + mbcnt(m);
         pending &= ~m;
     }
-}
-
-__device__ __forceinline__ bool entry_is_leaf(const Ctx& c, const Rec& r) {
-    return (c.prog.entries[rec_entry(r)].flags & EF_LEAF) != 0;
-}
-
-// Writes the parent (fan-out) records of the active lanes and one
-// descriptor per child; `self` continues an over-long chain on the next
-// level (one descriptor that stands for the record itself).
-template <bool WRITE>
-__device__ __forceinline__ void emit_fanout(const Ctx& c, LevelLDS& L, bool active, bool self, const Rec& r) {
-    const uint32_t kP = c.prog.nslots, kD = c.prog.nslots + 1u;
-    const unsigned long long pm = __ballot(active);
-    if (pm == 0ull) return;
-    const int pl = __ffsll((long long)pm) - 1;
-    const uint32_t pbase = claim<WRITE>(L, kP, pm, pl);
-    uint32_t pidx = ~0u;
-    if (WRITE && active) {
-        pidx = pbase + mbcnt(pm);
-        if (pidx < c.par_cap) store_rec(c.par_out + pidx, r);
-        else { raise(c, PGA_ERR_OVERFLOW_QUEUE); pidx = ~0u; }
-    }
-    Fan f;
-    uint32_t n = 0;
-    if (active) {
-        if (self) {
-            n = 1;
-        } else {
-            fan_init(c, r, f);
-            n = f.n;
+    if (wr && has) {
+        const uint32_t t = c.prog.slot_term[slot];
+        const unsigned long long cap = c.inst_cap[t];
+        if (idx < cap) {
+            const float b[16] = {r.s[0], 0.0f, 0.0f, 0.0f, 0.0f, r.s[1], 0.0f, 0.0f,
+                                 0.0f, 0.0f, r.s[2], 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+            float mm[12];
+            mul34x44(r.m, b, mm);
+            f4* d = reinterpret_cast<f4*>(c.inst[t] + idx);
+            d[0] = f4{mm[0], mm[1], mm[2], mm[3]};
+            d[1] = f4{mm[4], mm[5], mm[6], mm[7]};
+            d[2] = f4{mm[8], mm[9], mm[10], mm[11]};
+            d[3] = f4{r.custom, __uint_as_float(r.axiom), __uint_as_float(r.path), __uint_as_float(r.ed)};
+        } else if (cap != 0ull) {
+            raise(c, PGA_ERR_OVERFLOW_INSTANCES);
         }
     }
-    // exclusive prefix of n over the wave
-    uint32_t incl = n;
+}
+
+// Counting pass over a direct leaf fan-out: its children's terminals per
+// slot come from the entry's table (times the repetitions of a Repeat).
+__device__ __forceinline__ void count_direct(const Ctx& c, LevelLDS& L, const EntryV& e, const Fan& f) {
+    const uint32_t mult = e.kind() == K_REPEAT ? f.n : (f.n != 0u ? 1u : 0u);
+    const uint2 tab = *reinterpret_cast<const uint2*>(&c.prog.entries[e.idx].dslot[0]);
+    const uint32_t w = wave_in_block();
+    for (uint32_t k = 0; k < e.ndirect(); ++k) {
+        const uint32_t slot = (tab.x >> (8u * k)) & 0xFFu;
+        const unsigned long long add = (unsigned long long)((tab.y >> (8u * k)) & 0xFFu) * mult;
+        if (add >= 0x80000000ull) { raise(c, PGA_ERR_OVERFLOW_INSTANCES); continue; }
+        atomicAdd(&L.cnt[w][slot], (uint32_t)add);
+    }
+}
+
+// Starts the non-leaf fan-outs of the active lanes: claims and writes each
+// parent record (pidx = its index, ~0u when the queue is full) and claims
+// `n` consecutive descriptor slots from `dslot` on.
+__device__ __forceinline__ void start_fanout(const Ctx& c, LevelLDS& L, bool wr, bool active, const Rec& par,
+                                             uint32_t n, uint32_t& pidx, uint32_t& dslot) {
+    const uint32_t kP = c.prog.nslots, kD = c.prog.nslots + 1u;
+    const unsigned long long pm = __ballot(active);
+    const int pl = __ffsll((long long)pm) - 1;
+    const uint32_t pbase = claim(L, wr, kP, pm, pl);
+    if (wr && active) {
+        pidx = pbase + mbcnt(pm);
+        if (pidx < c.par_cap) store_rec(c.par_out + pidx, par);
+        else { raise(c, PGA_ERR_OVERFLOW_QUEUE); pidx = ~0u; }
+    }
+    const uint32_t nn = active ? n : 0u;
+    uint32_t incl = nn;       // inclusive prefix of n over the wave
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        uint32_t v = __shfl_up(incl, d);
+        const uint32_t v = __shfl_up(incl, d);
         if ((int)lane_id() >= d) incl += v;
     }
     const uint32_t total = __shfl(incl, 63);
     const uint32_t w = wave_in_block();
-    if (!WRITE) {
+    if (!wr) {
         if (lane_id() == 0) L.cnt[w][kD] += total;
         return;
     }
     const uint32_t dbase = L.cur[w][kD];
     if (lane_id() == 0) L.cur[w][kD] = dbase + total;
-    uint32_t slot = dbase + incl - n;
-    for (uint32_t j = 0; j < n; ++j, ++slot) {
-        uint32_t jx;
-        float o = 0.0f, ns = 0.0f;
-        if (pidx == ~0u) {
-            jx = kDescNone;                      // parent dropped on overflow: no gap
-        } else if (self) {
-            jx = kDescSelf;
-        } else {
-            const bool keep = fan_geom(c, r, f, j, o, ns);
-            jx = keep ? (j | (f.axis << 30)) : kDescNone;
-        }
-        if (slot < c.desc_cap)
-            *reinterpret_cast<float4*>(c.desc_out + slot) =
-                make_float4(__uint_as_float(pidx), __uint_as_float(jx), o, ns);
-        else
-            raise(c, PGA_ERR_OVERFLOW_QUEUE);
-    }
-}
-
-// Expands the leaf fan-outs of the active lanes in place: every child is a
-// terminal or a discard after its fused chain.
-template <bool WRITE>
-__device__ void expand_leaf(const Ctx& c, LevelLDS& L, bool active, const Rec& r) {
-    Fan f;
-    uint32_t n = 0;
-    if (active) {
-        fan_init(c, r, f);
-        n = f.n;
-    }
-    for (uint32_t j = 0; __ballot(j < n) != 0ull; ++j) {
-        bool has = j < n;
-        Rec ch;
-        if (has) has = fan_child(c, r, f, j, ch);
-        int cs = has ? settle(c, ch) : ST_DROP;
-        emit_terminal<WRITE>(c, L, cs == ST_TERMINAL, ch);
-        // the load-time leaf analysis bounds every chain below a leaf by the
-        // chain limit and excludes fan-outs there
-        if (WRITE && (cs == ST_FANOUT || cs == ST_OVERLONG)) raise(c, PGA_ERR_BAD_PROGRAM);
-    }
-}
-
-// One child per lane: settle, then route.
-template <bool WRITE>
-__device__ void process_child(const Ctx& c, LevelLDS& L, bool valid, Rec r) {
-    int st = valid ? settle(c, r) : ST_DROP;
-    emit_terminal<WRITE>(c, L, st == ST_TERMINAL, r);
-    const bool fan = st == ST_FANOUT;
-    const bool leaf = fan && entry_is_leaf(c, r);
-    if (__ballot(leaf) != 0ull) expand_leaf<WRITE>(c, L, leaf, r);
-    const bool queue = (fan && !leaf) || st == ST_OVERLONG;
-    if (__ballot(queue) != 0ull) emit_fanout<WRITE>(c, L, queue, st == ST_OVERLONG, r);
+    if (active) dslot = dbase + incl - nn;
 }
 
 // Program image and terminal tables staged in LDS (dynamic shared memory).
 extern __shared__ __align__(16) unsigned char pga_smem[];
 
-// Copies the program image and the terminal buffer tables into LDS when
-// they fit (lds_bytes != 0); every operator then reads its entry,
-// parameters and successors from LDS instead of the cache hierarchy.
+// Copies the program image and the terminal buffer tables into LDS; every
+// operator then reads its entry, parameters and successors from LDS.
 __device__ __forceinline__ void stage_program(const KArgs& a) {
-    if (a.lds_bytes == 0u) return;
     uint4* dst = reinterpret_cast<uint4*>(pga_smem);
     const uint32_t n16 = a.image_bytes / 16u;
     for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = a.image[i];
@@ -244,21 +188,18 @@ __device__ __forceinline__ void stage_program(const KArgs& a) {
     __syncthreads();
 }
 
-__device__ __forceinline__ Ctx make_ctx(const KArgs& a, uint32_t level) {
+__device__ __forceinline__ Ctx make_ctx(const KArgs& a, uint32_t level, bool wr) {
     Ctx c;
-    c.prog = a.prog;
-    c.inst = a.inst;
-    c.inst_cap = a.inst_cap;
-    if (a.lds_bytes != 0u) {
-        c.prog.entries = reinterpret_cast<const DevEntry*>(pga_smem);
-        c.prog.params = reinterpret_cast<const DevParam*>(pga_smem + a.off_params);
-        c.prog.succ = reinterpret_cast<const int32_t*>(pga_smem + a.off_succ);
-        c.prog.exp = reinterpret_cast<const uint32_t*>(pga_smem + a.off_exp);
-        c.prog.term_slot = pga_smem + a.off_slots;
-        c.prog.slot_term = pga_smem + a.off_slots + PGA_MAX_TERMINALS;
-        c.inst = reinterpret_cast<pga_instance* const*>(pga_smem + a.image_bytes);
-        c.inst_cap = reinterpret_cast<const unsigned long long*>(pga_smem + a.image_bytes + 8u * PGA_MAX_TERMINALS);
-    }
+    c.prog.entries = reinterpret_cast<const DevEntry*>(pga_smem);
+    c.prog.params = reinterpret_cast<const DevParam*>(pga_smem + a.off_params);
+    c.prog.succ = reinterpret_cast<const int32_t*>(pga_smem + a.off_succ);
+    c.prog.exp = reinterpret_cast<const uint32_t*>(pga_smem + a.off_exp);
+    c.prog.term_slot = pga_smem + a.off_slots;
+    c.prog.slot_term = pga_smem + a.off_slots + PGA_MAX_TERMINALS;
+    c.prog.nentries = a.prog.nentries;
+    c.prog.nslots = a.prog.nslots;
+    c.inst = reinterpret_cast<pga_instance* const*>(pga_smem + a.image_bytes);
+    c.inst_cap = reinterpret_cast<const unsigned long long*>(pga_smem + a.image_bytes + 8u * PGA_MAX_TERMINALS);
     c.aparams = a.aparams;
     c.aparams_per = a.aparams_per;
     c.naxioms_total = a.naxioms_total;
@@ -273,90 +214,184 @@ __device__ __forceinline__ Ctx make_ctx(const KArgs& a, uint32_t level) {
     c.desc_cap = a.cap_desc;
     c.err = a.err;
     c.max_chain = a.max_chain;
+    c.geom = wr ? 1u : a.count_geom;
     return c;
+}
+
+// Child i of the level: an axiom (level 0) or the child that a descriptor
+// of the previous level names. Returns false when there is none.
+__device__ __forceinline__ bool load_child(const KArgs& a, const Ctx& c, bool wr, uint32_t level, uint32_t i,
+                                           Rec& r) {
+    if (level == 0u) {
+        const pga_axiom& ax = a.axioms[a.first_axiom + i];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) r.m[k] = ax.model[k];
+        r.s[0] = ax.size[0]; r.s[1] = ax.size[1]; r.s[2] = ax.size[2];
+        r.seed = ax.seed;
+        r.custom = ax.custom;
+        r.axiom = a.first_axiom + i;
+        r.path = 0u;
+        r.ed = 0u;
+        const int32_t ent = ax.entry_index;
+        if (ent < 0) return false;              // PGAFacade.cu:708
+        if ((uint32_t)ent >= c.prog.nentries || (int32_t)load_entry(c, (uint32_t)ent).shape() != ax.shape_type) {
+            if (wr) raise(c, PGA_ERR_BAD_PROGRAM);
+            return false;
+        }
+        r.ed = (uint32_t)ent;
+        return true;
+    }
+    const Rec* par_in = (level & 1u) ? a.par0 : a.par1;      // written by level - 1
+    const Desc* desc_in = (level & 1u) ? a.desc0 : a.desc1;
+    const uint4 dv = *reinterpret_cast<const uint4*>(desc_in + i);
+    const uint32_t parent = dv.x, jx = dv.y;
+    if (parent >= a.cap_par) return false;       // parent was dropped on overflow
+    const uint32_t j = jx & 0x3FFFFFFFu;
+    if (j == kDescNone) return false;
+    Rec p;
+    load_rec(par_in + parent, p);
+    if (j == kDescSelf) { r = p; return true; }
+    return build_child(c, p, load_entry(c, rec_entry(p)), j, jx >> 30, __uint_as_float(dv.z),
+                       __uint_as_float(dv.w), r);
 }
 
 // One pass over the children of a level; waves take 64 consecutive
 // children at a time, in a fixed order, so both passes see the same lanes.
-template <bool WRITE>
-__device__ void run_pass(const KArgs& a, LevelLDS& L, uint32_t level, uint32_t n, uint32_t wave, uint32_t nwaves) {
-    const Ctx c = make_ctx(a, level);
-    const Rec* par_in = (level & 1u) ? a.par0 : a.par1;      // written by level - 1
-    const Desc* desc_in = (level & 1u) ? a.desc0 : a.desc1;
+__device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS& L, bool wr, uint32_t level,
+                                         uint32_t n, uint32_t wave, uint32_t nwaves) {
+#pragma unroll 1
     for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u) {
         const uint32_t i = base + lane_id();
-        bool valid = i < n;
-        Rec r;
-        if (level == 0u) {
-            if (valid) {
-                const pga_axiom& ax = a.axioms[a.first_axiom + i];
-                for (int k = 0; k < 12; ++k) r.m[k] = ax.model[k];
-                r.s[0] = ax.size[0]; r.s[1] = ax.size[1]; r.s[2] = ax.size[2];
-                r.seed = ax.seed;
-                r.custom = ax.custom;
-                r.axiom = a.first_axiom + i;
-                r.path = 0u;
-                r.ed = 0u;
-                const int32_t ent = ax.entry_index;
-                if (ent < 0) {
-                    valid = false;                  // PGAFacade.cu:708
-                } else if ((uint32_t)ent >= a.prog.nentries ||
-                           (int32_t)a.prog.entries[ent].shape != ax.shape_type) {
-                    if (WRITE) raise(c, PGA_ERR_BAD_PROGRAM);
-                    valid = false;
+        Rec cur;
+        const bool valid = i < n && load_child(a, c, wr, level, i, cur);
+        uint32_t mode = valid ? M_SETTLE : M_DONE;
+        bool leaf = false, self = false;
+        uint32_t steps = 0, j = 0, dslot = 0, pidx = ~0u;
+        Rec par = cur;
+        Fan f;
+        f.kind_flags = 0; f.n = 0; f.axis = 0; f.total = 0.0f; f.r2a = 0.0f; f.off = 0.0f;
+        f.e1 = f.e2 = f.e3 = 0.0f; f.prev_ns = 0.0f; f.prev_o = 0.0f; f.reps = 0; f.pbase = 0;
+#pragma unroll 1
+        while (__ballot(mode != M_DONE) != 0ull) {
+            bool emit = false, start = false;
+            uint32_t slot = 0;
+            if (mode == M_FAN) {
+                if (j < f.n) {
+                    float o = 0.0f, ns = 0.0f;
+                    const bool keep = self ? true : fan_geom(c, par, f, j, o, ns);
+                    if (leaf) {
+                        if (keep && build_child(c, par, load_entry(c, rec_entry(par)), j, f.axis, o, ns, cur)) {
+                            mode = M_SETTLE;
+                            steps = 0;
+                        }
+                    } else {   // write pass only: the count pass claimed f.n slots at the start
+                        const uint32_t jx = pidx == ~0u ? kDescNone
+                                          : self        ? kDescSelf
+                                          : keep        ? (j | (f.axis << 30))
+                                                        : kDescNone;
+                        if (dslot < c.desc_cap)
+                            *reinterpret_cast<uint4*>(c.desc_out + dslot) =
+                                make_uint4(pidx, jx, __float_as_uint(o), __float_as_uint(ns));
+                        else
+                            raise(c, PGA_ERR_OVERFLOW_QUEUE);
+                        ++dslot;
+                    }
+                    ++j;
                 } else {
-                    r.ed = (uint32_t)ent;
+                    mode = M_DONE;
                 }
             }
-        } else if (valid) {
-            const float4 dv = *reinterpret_cast<const float4*>(desc_in + i);
-            Desc d;
-            d.parent = __float_as_uint(dv.x); d.jx = __float_as_uint(dv.y); d.o = dv.z; d.ns = dv.w;
-            if (d.parent >= a.cap_par) {
-                valid = false;                      // parent was dropped on overflow
-            } else {
-                Rec p;
-                load_rec(par_in + d.parent, p);
-                valid = desc_child(c, p, d, r);
+            if (mode == M_SETTLE) {
+                const EntryV e = load_entry(c, rec_entry(cur));
+                const uint32_t k = e.kind();
+                const uint32_t back = leaf ? M_FAN : M_DONE;
+                if (k == K_GENERATE) {
+                    slot = e.tslot();
+                    if (slot == 0xFFu) {            // terminal index computed per shape
+                        const uint32_t t = f2u_rd(P(c, e, 0, cur, cur.axiom, cur.path, rec_depth(cur)));
+                        slot = t < PGA_MAX_TERMINALS ? c.prog.term_slot[t] : 0xFFu;
+                    }
+                    if (slot >= c.prog.nslots) { raise(c, PGA_ERR_BAD_PROGRAM); slot = 0; }
+                    emit = true;
+                    mode = back;
+                } else if (k == K_DISCARD) {
+                    mode = back;
+                } else if (is_fanout(k)) {
+                    if (leaf) {
+                        // the load-time leaf analysis excludes fan-outs below a leaf
+                        raise(c, PGA_ERR_BAD_PROGRAM);
+                        mode = M_FAN;
+                    } else {
+                        par = cur;
+                        fan_init(c, par, e, f);
+                        j = 0;
+                        leaf = (e.flags() & EF_LEAF) != 0;
+                        start = !leaf;
+                        if (!wr && (e.flags() & EF_DIRECT)) {
+                            count_direct(c, L, e, f);
+                            mode = M_DONE;
+                        } else {
+                            mode = (leaf || wr) ? M_FAN : M_DONE;
+                        }
+                    }
+                } else if (steps >= c.max_chain) {
+                    if (leaf) {
+                        // the leaf analysis bounds every chain below a leaf by the limit
+                        raise(c, PGA_ERR_BAD_PROGRAM);
+                        mode = M_FAN;
+                    } else {
+                        // over-long chain: continues on the next level
+                        par = cur;
+                        self = true;
+                        f.n = 1;
+                        j = 0;
+                        start = true;
+                        mode = wr ? M_FAN : M_DONE;
+                    }
+                } else {
+                    uint32_t ord;
+                    const int32_t next = apply_single(c, e, cur, &ord);
+                    if (!advance(c, cur, next, ord)) mode = back;
+                    ++steps;
+                }
             }
+            if (__ballot(emit) != 0ull) emit_terminal(c, L, wr, emit, slot, cur);
+            if (__ballot(start) != 0ull) start_fanout(c, L, wr, start, par, f.n, pidx, dslot);
         }
-        process_child<WRITE>(c, L, valid, r);
     }
 }
 
-// Grid-wide barrier of a cooperative launch: arrive counter bar[0] and
-// generation bar[1]. The words return to (0, g + 1) after every barrier.
-// One agent-scope release fence per block before arriving, relaxed
-// polling, one acquire fence after.
+// Grid-wide barrier of a cooperative launch. Words (64 bytes apart):
+// bar[0] generation, bar[16 (g + 1)] arrivals of block group g = block % 8,
+// bar[16 * 9] arrivals of the groups. Spreading the arrivals over nine
+// counters keeps same-address atomics short; every counter is back at 0 and
+// the generation one higher after each barrier. One agent-scope release
+// fence per block before arriving, relaxed polling, one acquire fence after.
 __device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t nblocks) {
     __syncthreads();
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const uint32_t gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblocks - 1u) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t gen = __hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t g = blockIdx.x & 7u;
+        const uint32_t in_group = (nblocks - g + 7u) >> 3;          // blocks b < nblocks with b % 8 == g
+        const uint32_t groups = nblocks < 8u ? nblocks : 8u;
+        bool last = false;
+        uint32_t* gc = &bar[16u * (g + 1u)];
+        if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == in_group - 1u) {
+            __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t* top = &bar[16u * 9u];
+            last = __hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == groups - 1u;
+            if (last) __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (last) {
+            __hip_atomic_store(&bar[0], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
+            while (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
                 __builtin_amdgcn_s_sleep(1);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-}
-
-// Sum over the workgroup of one value per thread (all threads call it).
-__device__ __forceinline__ uint32_t block_sum(LevelLDS& L, uint32_t v, int which) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
-    if (lane_id() == 0) L.red[wave_in_block()][which] = v;
-    __syncthreads();
-    uint32_t s = 0;
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) s += L.red[w][which];
-    __syncthreads();
-    return s;
 }
 
 // Phase timestamp of workgroup 0 (instrumentation; off when ticks is NULL).
@@ -367,74 +402,89 @@ __device__ __forceinline__ void stamp(const KArgs& a, uint32_t level, uint32_t p
     }
 }
 
-// One derivation level: count pass, publish, barrier, scan, write pass.
-// Leaves L.next_n = children of the next level; block 0 records the level's
-// parent and descriptor counts.
-__device__ void run_level(const KArgs& a, LevelLDS& L, uint32_t level, uint32_t n) {
+// Between the passes: publish this block's totals, grid barrier, exclusive
+// prefix over the preceding blocks into the write cursors, level totals.
+__device__ __forceinline__ void scan_counts(const KArgs& a, LevelLDS& L, uint32_t level) {
     const uint32_t nslots = a.prog.nslots, ncnt = nslots + 2u;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t w = wave_in_block();
-    stamp(a, level, 0);
-    for (uint32_t k = lane_id(); k < ncnt; k += 64u) L.cnt[w][k] = 0u;
-    run_pass<false>(a, L, level, n, wave, nwaves);
-    __syncthreads();
     stamp(a, level, 1);
-    // block totals; in-block wave offsets go to the cursors
+    const uint32_t nw = blockDim.x >> 6;
     uint32_t* totals = a.block_totals + (size_t)blockIdx.x * ncnt;
     for (uint32_t k = threadIdx.x; k < ncnt; k += blockDim.x) {
         uint32_t s = 0;
-        for (int ww = 0; ww < kWaves; ++ww) {
+        for (uint32_t ww = 0; ww < nw; ++ww) {
             L.cur[ww][k] = s;
             s += L.cnt[ww][k];
         }
         totals[k] = s;
     }
     grid_barrier(a.bar, gridDim.x);
-    // exclusive prefix over the blocks before this one, and the grand total
-    for (uint32_t k = 0; k < ncnt; ++k) {
+    // wave w sums counters k = w, w + nw, ... over the blocks, 64 blocks a step
+    for (uint32_t k = wave_in_block(); k < ncnt; k += nw) {
         uint32_t before = 0, all = 0;
-        for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+        for (uint32_t b = lane_id(); b < gridDim.x; b += 64u) {
             const uint32_t v = a.block_totals[(size_t)b * ncnt + k];
             all += v;
             before += b < blockIdx.x ? v : 0u;
         }
-        before = block_sum(L, before, 0);
-        all = block_sum(L, all, 1);
-        if (threadIdx.x < kWaves) L.cur[threadIdx.x][k] += before;
-        if (threadIdx.x == 0) {
-            if (k == nslots + 1u) L.next_n = all;
-            if (blockIdx.x == 0 && k >= nslots) {
-                if (k == nslots) a.par_counts[level] = all;
-                else a.desc_counts[level] = all;
-            }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            before += __shfl_xor(before, d);
+            all += __shfl_xor(all, d);
         }
-        __syncthreads();
-        if (k < nslots && threadIdx.x == 0) {
-            // terminals accumulate across levels: this level writes after
-            // the running base, which then grows by the level total
-            L.cnt[0][k] = all;   // stash for the base update after the write pass
+        if (lane_id() == 0) {
+            L.before[k] = before;
+            L.level_total[k] = all;
         }
     }
     __syncthreads();
-    stamp(a, level, 2);
-    run_pass<true>(a, L, level, n, wave, nwaves);
+    for (uint32_t k = threadIdx.x; k < ncnt; k += blockDim.x) {
+        const uint32_t before = L.before[k], all = L.level_total[k];
+        for (uint32_t ww = 0; ww < nw; ++ww) L.cur[ww][k] += before;
+        if (k == nslots + 1u) L.next_n = all;
+        if (blockIdx.x == 0 && k == nslots) a.par_counts[level] = all;
+        if (blockIdx.x == 0 && k == nslots + 1u) a.desc_counts[level] = all;
+    }
     __syncthreads();
+    stamp(a, level, 2);
+}
+
+// One derivation level: count pass, scan, write pass. Leaves L.next_n =
+// children of the next level; block 0 records the level's parent and
+// descriptor counts.
+__device__ __forceinline__ void run_level(const KArgs& a, LevelLDS& L, uint32_t level, uint32_t n) {
+    const uint32_t nslots = a.prog.nslots, ncnt = nslots + 2u;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t w = wave_in_block();
+    stamp(a, level, 0);
+    for (uint32_t k = lane_id(); k < ncnt; k += 64u) L.cnt[w][k] = 0u;
+#pragma unroll 1
+    for (uint32_t pass = 0; pass < 2u; ++pass) {
+        const bool wr = pass != 0u;
+        const Ctx c = make_ctx(a, level, wr);
+        run_pass(a, c, L, wr, level, n, wave, nwaves);
+        __syncthreads();
+        if (!wr) scan_counts(a, L, level);
+    }
     stamp(a, level, 3);
+    // terminals accumulate across levels: this level wrote after the running
+    // base, which now grows by the level total
     if (threadIdx.x == 0)
-        for (uint32_t k = 0; k < nslots; ++k) L.term_base[k] += L.cnt[0][k];
+        for (uint32_t k = 0; k < nslots; ++k) L.term_base[k] += L.level_total[k];
     __syncthreads();
 }
 
 __device__ __forceinline__ void init_bases(const KArgs& a, LevelLDS& L) {
     if (threadIdx.x == 0)
-        for (uint32_t k = 0; k < a.prog.nslots; ++k) L.term_base[k] = a.term_counts[a.prog.slot_term[k]];
+        for (uint32_t k = 0; k < a.prog.nslots; ++k)
+            L.term_base[k] = a.term_counts[pga_smem[a.off_slots + PGA_MAX_TERMINALS + k]];
     __syncthreads();
 }
 
 __device__ __forceinline__ void store_bases(const KArgs& a, LevelLDS& L) {
     if (blockIdx.x == 0 && threadIdx.x == 0)
-        for (uint32_t k = 0; k < a.prog.nslots; ++k) a.term_counts[a.prog.slot_term[k]] = L.term_base[k];
+        for (uint32_t k = 0; k < a.prog.nslots; ++k)
+            a.term_counts[pga_smem[a.off_slots + PGA_MAX_TERMINALS + k]] = L.term_base[k];
 }
 
 __device__ __forceinline__ uint32_t first_level_size(const KArgs& a, uint32_t level) {
@@ -442,8 +492,11 @@ __device__ __forceinline__ uint32_t first_level_size(const KArgs& a, uint32_t le
     return min(a.desc_counts[level - 1u], a.cap_desc);
 }
 
-// KERNELS technique: one level per (cooperative) launch.
-__global__ void __launch_bounds__(256) k_level(KArgs a, uint32_t level) {
+// KERNELS technique: one level per (cooperative) launch. BS = threads per
+// workgroup; the variants differ in the registers the compiler may use
+// (BS / 256 waves per SIMD at one workgroup per compute unit).
+template <int BS>
+__global__ void __launch_bounds__(BS) k_level(KArgs a, uint32_t level) {
     __shared__ LevelLDS L;
     stage_program(a);
     init_bases(a, L);
@@ -454,7 +507,8 @@ __global__ void __launch_bounds__(256) k_level(KArgs a, uint32_t level) {
 }
 
 // MEGAKERNEL technique: all levels in one persistent launch.
-__global__ void __launch_bounds__(256) k_mega(KArgs a) {
+template <int BS>
+__global__ void __launch_bounds__(BS) k_mega(KArgs a) {
     __shared__ LevelLDS L;
     stage_program(a);
     init_bases(a, L);
@@ -478,7 +532,8 @@ __global__ void k_reset(uint32_t* par_counts, uint32_t* desc_counts, uint32_t n_
                         unsigned long long* term_counts, uint32_t reset_terms) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_levels) { par_counts[i] = 0u; desc_counts[i] = 0u; }
-    if (i == 0) { *err = 0u; bar[0] = 0u; bar[1] = 0u; }
+    if (i == 0) *err = 0u;
+    if (i < kBarWords) bar[i] = 0u;
     if (reset_terms && i < PGA_MAX_TERMINALS) term_counts[i] = 0ull;
 }
 
@@ -493,26 +548,51 @@ hipError_t pga_launch_reset(const pga::KArgs& a, int reset_terms, hipStream_t s)
     return hipGetLastError();
 }
 
+namespace {
+
+template <class F>
+bool pick_variant(uint32_t block_size, F&& f) {
+    switch (block_size) {
+    case 256: f(pga::k_mega<256>, pga::k_level<256>); return true;
+    case 512: f(pga::k_mega<512>, pga::k_level<512>); return true;
+    case 768: f(pga::k_mega<768>, pga::k_level<768>); return true;
+    case 1024: f(pga::k_mega<1024>, pga::k_level<1024>); return true;
+    default: return false;
+    }
+}
+
+} // namespace
+
+bool pga_block_size_supported(uint32_t block_size) {
+    return pick_variant(block_size, [](auto, auto) {});
+}
+
 hipError_t pga_launch_level(const pga::KArgs& a, uint32_t level, uint32_t blocks, uint32_t block_size,
                             hipStream_t s) {
     pga::KArgs copy = a;
     void* args[] = {&copy, &level};
-    return hipLaunchCooperativeKernel((const void*)pga::k_level, dim3(blocks), dim3(block_size), args,
-                                      a.lds_bytes, s);
+    const void* fn = nullptr;
+    if (!pick_variant(block_size, [&](auto mega, auto lvl) { (void)mega; fn = (const void*)lvl; }))
+        return hipErrorInvalidValue;
+    return hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(block_size), args, a.lds_bytes, s);
 }
 
 hipError_t pga_mega_occupancy(uint32_t block_size, uint32_t lds_bytes, int* blocks_per_cu) {
-    int a = 0, b = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, pga::k_mega, (int)block_size, lds_bytes);
-    if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pga::k_level, (int)block_size, lds_bytes);
-    *blocks_per_cu = a < b ? a : b;
+    hipError_t e = hipErrorInvalidValue;
+    pick_variant(block_size, [&](auto mega, auto lvl) {
+        int a = 0, b = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, mega, (int)block_size, lds_bytes);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, lvl, (int)block_size, lds_bytes);
+        *blocks_per_cu = a < b ? a : b;
+    });
     return e;
 }
 
 hipError_t pga_launch_mega(const pga::KArgs& a, uint32_t blocks, uint32_t block_size, hipStream_t s) {
     pga::KArgs copy = a;
     void* args[] = {&copy};
-    return hipLaunchCooperativeKernel((const void*)pga::k_mega, dim3(blocks), dim3(block_size), args,
-                                      a.lds_bytes, s);
+    const void* fn = nullptr;
+    if (!pick_variant(block_size, [&](auto mega, auto lvl) { (void)lvl; fn = (const void*)mega; }))
+        return hipErrorInvalidValue;
+    return hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(block_size), args, a.lds_bytes, s);
 }

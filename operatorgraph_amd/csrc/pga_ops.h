@@ -2,21 +2,55 @@
 //
 // Single-child operators (IfSizeLess, If, RandomRule, Translate, Rotate,
 // Scale, SwapSize, Extrude) transform the record in registers and continue
-// with the successor entry (a fused chain). Fan-out operators (Subdivide,
-// Repeat, ComponentSplit, Replicate) are split into `fan_init` (per-parent
-// prelude: axis, relative-to-absolute scale, repetition counts) and
-// `fan_child` (child j, called for j = 0, 1, ... in order, carrying the
-// running offset exactly as the reference's sequential loops do).
+// with the successor entry. Fan-out operators (Subdivide, Repeat,
+// ComponentSplit, Replicate) are split into `fan_init` (per-parent prelude:
+// axis, relative-to-absolute scale, repetition counts), `fan_geom` (the
+// geometry of child j, called for j = 0, 1, ... in order, carrying the
+// running offset exactly as the reference's sequential loops do) and
+// `build_child` (child j's record from the parent and that geometry).
 //
-// Parameters are evaluated through few call sites (eval3 and short loops)
-// so the fused chain stays small enough to inline.
+// Entries are read as three 16-byte words and unpacked into scalars, so no
+// entry is ever an addressable local. `Ctx::geom` = 0 skips the model-matrix
+// arithmetic (the counting pass of a program whose decisions never read the
+// matrix).
 #pragma once
 
 #include "pga_device.h"
 
 namespace pga {
 
-enum Settle : int { ST_DROP = 0, ST_TERMINAL = 1, ST_FANOUT = 2, ST_OVERLONG = 3 };
+// The fixed fields of a dispatch entry, unpacked into registers. Fields
+// chosen by a run-time index (successor k, inline parameter k) are read from
+// the LDS entry by index: a select between register copies would be turned
+// into an indexed load from a stack copy of the entry.
+struct EntryV {
+    uint32_t idx;     // entry index
+    uint32_t w0;      // kind | shape << 8 | flags << 16 | nparams << 24
+    uint32_t w1;      // nsucc | inl << 8 | tslot << 16
+    uint32_t pbase, sbase;
+    float pv0, pv1, pv2;
+    int32_t sv0;
+    __device__ __forceinline__ uint32_t kind() const { return w0 & 0xFFu; }
+    __device__ __forceinline__ uint32_t shape() const { return (w0 >> 8) & 0xFFu; }
+    __device__ __forceinline__ uint32_t flags() const { return (w0 >> 16) & 0xFFu; }
+    __device__ __forceinline__ uint32_t nparams() const { return w0 >> 24; }
+    __device__ __forceinline__ uint32_t nsucc() const { return w1 & 0xFFu; }
+    __device__ __forceinline__ uint32_t inl() const { return (w1 >> 8) & 0xFFu; }
+    __device__ __forceinline__ uint32_t tslot() const { return (w1 >> 16) & 0xFFu; }
+    __device__ __forceinline__ uint32_t ndirect() const { return w1 >> 24; }
+};
+static_assert(sizeof(DevEntry) == 48, "DevEntry must be three 16-byte words");
+
+__device__ __forceinline__ EntryV load_entry(const Ctx& c, uint32_t i) {
+    const uint4* q = reinterpret_cast<const uint4*>(c.prog.entries) + i * 3u;
+    const uint4 a = q[0], b = q[1];
+    EntryV e;
+    e.idx = i;
+    e.w0 = a.x; e.w1 = a.y; e.pbase = a.z; e.sbase = a.w;
+    e.pv0 = __uint_as_float(b.x); e.pv1 = __uint_as_float(b.y); e.pv2 = __uint_as_float(b.z);
+    e.sv0 = (int32_t)b.w;
+    return e;
+}
 
 __device__ __forceinline__ bool advance(const Ctx& c, Rec& r, int32_t next, uint32_t ordinal) {
     if (next < 0) return false;   // unused dynamic edge: nothing dispatched
@@ -42,13 +76,45 @@ __device__ __forceinline__ void set_model(Rec& r, const float m[12]) {
     for (int i = 0; i < 12; ++i) r.m[i] = m[i];
 }
 
+// Parameter k of entry e for the shape r.
+__device__ __forceinline__ float P(const Ctx& c, const EntryV& e, uint32_t k, const Rec& r, uint32_t axiom,
+                                   uint32_t path, uint32_t depth) {
+    if (k < 3u && ((e.inl() >> k) & 1u)) return c.prog.entries[e.idx].pv[k];
+    return eval_param(c, c.prog.params[e.pbase + k], r, axiom, path, depth);
+}
+
+// Successor k of an entry (the first three are held in the entry).
+__device__ __forceinline__ int32_t succ_of(const Ctx& c, const EntryV& e, uint32_t k) {
+    if (k < 3u) return c.prog.entries[e.idx].sv[k];
+    return c.prog.succ[e.sbase + k];
+}
+
+// The first min(nparams, 3) parameters; constants come from the entry, the
+// others through one evaluation site.
+__device__ __forceinline__ void eval3(const Ctx& c, const EntryV& e, const Rec& r, float& v0, float& v1,
+                                      float& v2) {
+    const uint32_t np = e.nparams() < 3u ? e.nparams() : 3u;
+    v0 = e.pv0; v1 = e.pv1; v2 = e.pv2;
+    const uint32_t todo = ((1u << np) - 1u) & ~e.inl();
+    if (todo == 0u) return;
+    const uint32_t ax = r.axiom, pa = r.path, de = rec_depth(r);
+#pragma unroll 1
+    for (uint32_t k = 0; k < np; ++k) {
+        if (!((todo >> k) & 1u)) continue;
+        float x = eval_param(c, c.prog.params[e.pbase + k], r, ax, pa, de);
+        v0 = k == 0 ? x : v0;
+        v1 = k == 1 ? x : v1;
+        v2 = k == 2 ? x : v2;
+    }
+}
+
 // RandomRule<DynParams> (RandomRule.cuh:121-147) and the static form
 // (RandomRule.cuh:20-111: right-fold sum, zero-probability branches skipped,
 // last branch unconditional). Returns the chosen branch or -1.
-__device__ __forceinline__ int32_t random_choice(const Ctx& c, const DevEntry& e, float chance, uint32_t axiom,
-                                                 uint32_t path, uint32_t depth, const Rec& rr) {
-    const uint32_t n = e.nparams;
-    const bool is_static = (e.flags & EF_STATIC) != 0;
+__device__ __forceinline__ int32_t random_choice(const Ctx& c, const EntryV& e, float chance, const Rec& rr) {
+    const uint32_t n = e.nparams();
+    const bool is_static = (e.flags() & EF_STATIC) != 0;
+    const uint32_t axiom = rr.axiom, path = rr.path, depth = rec_depth(rr);
     float sum = 0.0f;
     if (!is_static) {
         for (uint32_t i = 1; i < n; ++i) sum += P(c, e, i, rr, axiom, path, depth);
@@ -70,11 +136,12 @@ __device__ __forceinline__ int32_t random_choice(const Ctx& c, const DevEntry& e
 
 // Applies one single-child operator in place. Returns the successor entry
 // (or -1 when nothing is dispatched) and the child ordinal.
-__device__ __forceinline__ int32_t apply_single(const Ctx& c, const DevEntry& e, Rec& r, uint32_t* ordinal) {
+__device__ __forceinline__ int32_t apply_single(const Ctx& c, const EntryV& e, Rec& r, uint32_t* ordinal) {
+    const uint32_t k = e.kind();
+    *ordinal = 0;
+    if (!c.geom && (k == K_TRANSLATE || k == K_ROTATE)) return e.sv0;   // model only
     float v0, v1, v2;
     eval3(c, e, r, v0, v1, v2);
-    *ordinal = 0;
-    const uint8_t k = e.kind;
     if (k == K_IFSIZELESS) {                   // If.cuh:61-98
         uint32_t axis = axis_of(c, v0);
         uint32_t t = (sel3(r.s, axis) < v1) ? 0u : 1u;
@@ -88,7 +155,7 @@ __device__ __forceinline__ int32_t apply_single(const Ctx& c, const DevEntry& e,
     }
     if (k == K_RANDOM) {
         float chance = v0 - floorf(v0);        // math::frac (math.h:243-246)
-        int32_t ch = random_choice(c, e, chance, r.axiom, r.path, rec_depth(r), r);
+        int32_t ch = random_choice(c, e, chance, r);
         if (ch < 0) return -1;                 // dyn form: round-off can choose nothing
         *ordinal = (uint32_t)ch;
         return succ_of(c, e, (uint32_t)ch);
@@ -97,7 +164,7 @@ __device__ __forceinline__ int32_t apply_single(const Ctx& c, const DevEntry& e,
         float m[12];
         mul_translate(r.m, v0, v1, v2, m);
         set_model(r, m);
-        return succ_of(c, e, 0);
+        return e.sv0;
     }
     if (k == K_ROTATE) {                       // Rotate.cuh:24-51, matrix.h:876-888
         const float kd = 3.1415926535897932384626434f / 180.0f;
@@ -112,51 +179,37 @@ __device__ __forceinline__ int32_t apply_single(const Ctx& c, const DevEntry& e,
         float m[12];
         mul34x44(r.m, b, m);
         set_model(r, m);
-        return succ_of(c, e, 0);
+        return e.sv0;
     }
     if (k == K_SCALE) {                        // Scale.cuh:23-50
         const float v[3] = {v0, v1, v2};
 #pragma unroll
         for (int i = 0; i < 3; ++i)
             r.s[i] = (((float)(v[i] < 0.0f) * r.s[i]) * v[i]) * -1.0f + ((float)(v[i] > 0.0f) * v[i]);
-        return succ_of(c, e, 0);
+        return e.sv0;
     }
     if (k == K_SWAPSIZE) {                     // SwapSize.cuh:25-58
         uint32_t a0 = axis_of(c, v0), a1 = axis_of(c, v1), a2 = axis_of(c, v2);
         float s0 = sel3(r.s, a0), s1 = sel3(r.s, a1), s2 = sel3(r.s, a2);
         r.s[0] = s0; r.s[1] = s1; r.s[2] = s2;
-        return succ_of(c, e, 0);
+        return e.sv0;
     }
     // K_EXTRUDE, Extrude.cuh:25-71 (Quad -> Box); parameter 0 (axis) unused
-    const float b[16] = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1.0f, 0.0f,
-                         0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
-    float m[12];
-    mul34x44(r.m, b, m);
-    set_model(r, m);
+    if (c.geom) {
+        const float b[16] = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1.0f, 0.0f,
+                             0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+        float m[12];
+        mul34x44(r.m, b, m);
+        set_model(r, m);
+    }
     float sx = r.s[0], sy = r.s[1];
     r.s[0] = sx; r.s[1] = v1; r.s[2] = sy;
     r.seed = next_seed(r.seed, 0u);
-    return succ_of(c, e, 0);
+    return e.sv0;
 }
 
-__device__ __forceinline__ bool is_fanout(uint8_t k) {
+__device__ __forceinline__ bool is_fanout(uint32_t k) {
     return k == K_SUBDIV || k == K_REPEAT || k == K_COMPSPLIT || k == K_REPLICATE;
-}
-
-// Runs single-child operators until the record reaches a terminal, a
-// discard, a fan-out operator, or `max_chain` steps (then it is queued to
-// continue on the next level).
-__device__ __forceinline__ int settle(const Ctx& c, Rec& r) {
-    for (uint32_t step = 0; step < c.max_chain; ++step) {
-        const DevEntry e = c.prog.entries[rec_entry(r)];
-        if (e.kind == K_DISCARD) return ST_DROP;
-        if (e.kind == K_GENERATE) return ST_TERMINAL;
-        if (is_fanout(e.kind)) return ST_FANOUT;
-        uint32_t ord;
-        int32_t next = apply_single(c, e, r, &ord);
-        if (!advance(c, r, next, ord)) return ST_DROP;
-    }
-    return ST_OVERLONG;
 }
 
 // ---- fan-out operators ----
@@ -167,27 +220,31 @@ struct Fan {
     float total;
     float r2a;           // Subdivide: relative-to-absolute scale
     float off;           // running offset on the axis
-    float e1, e2, e3;    // Repeat extents (start/mid/end); ComponentSplit half extents
+    float e1, e2, e3;    // Repeat extents (start/mid/end)
     float prev_ns;       // Subdivide: size and offset of the previous child
     float prev_o;
     uint32_t reps;       // Repeat repetitions
-    uint32_t sbase, pbase;
+    uint32_t pbase;
 };
 
-__device__ __forceinline__ void fan_init(const Ctx& c, const Rec& r, Fan& f) {
-    const DevEntry e = c.prog.entries[rec_entry(r)];
-    f.kind_flags = (uint32_t)e.kind | ((uint32_t)e.flags << 8);
-    f.sbase = e.sbase; f.pbase = e.pbase;
+__device__ __forceinline__ void fan_init(const Ctx& c, const Rec& r, const EntryV& e, Fan& f) {
+    const uint32_t kind = e.kind();
+    f.kind_flags = kind | (e.flags() << 8);
+    f.pbase = e.pbase;
     f.n = 0;
     f.reps = 0;
     f.axis = 0;
+    f.total = 0.0f;
+    f.r2a = 0.0f;
+    f.off = 0.0f;
+    f.e1 = f.e2 = f.e3 = 0.0f;
     f.prev_ns = 0.0f; f.prev_o = 0.0f;
-    if (e.kind == K_SUBDIV) {                  // Subdivide.cuh:142-153 / 181-216
+    if (kind == K_SUBDIV) {                    // Subdivide.cuh:142-153 / 181-216
         const uint32_t ax = r.axiom, pa = r.path, de = rec_depth(r);
         float abs_sum = 0.0f, rel_sum = 0.0f;
         uint32_t axis = 0;
 #pragma unroll 1
-        for (uint32_t i = 0; i < e.nparams; ++i) {
+        for (uint32_t i = 0; i < e.nparams(); ++i) {
             float fac = P(c, e, i, r, ax, pa, de);
             if (i == 0) {
                 axis = f2u_rd(fac);
@@ -201,12 +258,12 @@ __device__ __forceinline__ void fan_init(const Ctx& c, const Rec& r, Fan& f) {
         f.total = sel3(r.s, axis);
         f.off = f.total * -0.5f;
         f.r2a = (rel_sum == 0.0f) ? 0.0f : (f.total - abs_sum) / rel_sum;
-        if (!(e.flags & EF_STATIC) && f.r2a < 0.0f) {
+        if (!(e.flags() & EF_STATIC) && f.r2a < 0.0f) {
             raise(c, PGA_ERR_SUBDIV_OVERCONSTRAINED);
             return;
         }
-        f.n = e.nparams - 1;
-    } else if (e.kind == K_REPEAT) {           // Repeat.cuh:70-149
+        f.n = e.nparams() - 1;
+    } else if (kind == K_REPEAT) {             // Repeat.cuh:70-149
         float v0, ex, v2;
         eval3(c, e, r, v0, ex, v2);
         uint32_t axis = axis_of(c, v0);
@@ -243,11 +300,10 @@ __device__ __forceinline__ void fan_init(const Ctx& c, const Rec& r, Fan& f) {
         f.reps = reps;
         f.n = reps > 1 ? reps : 1u;
         f.off = 0.0f;
-    } else if (e.kind == K_COMPSPLIT) {        // ComponentSplit.cuh:234-267
-        f.e1 = r.s[0] * 0.5f; f.e2 = r.s[1] * 0.5f; f.e3 = r.s[2] * 0.5f;
+    } else if (kind == K_COMPSPLIT) {          // ComponentSplit.cuh:234-267
         f.n = 6;
     } else {                                   // K_REPLICATE, Replicate.cuh:102-146
-        f.n = e.nsucc;
+        f.n = e.nsucc();
     }
 }
 
@@ -285,13 +341,15 @@ __device__ __forceinline__ bool fan_geom(const Ctx& c, const Rec& r, Fan& f, uin
         const DevParam p = c.prog.params[f.pbase + 1 + j];
         const bool is_static = ((f.kind_flags >> 8) & EF_STATIC) != 0;
         float fac;
-        if (!is_static && j > 0 && shape_dependent(p.type)) {
+        if (p.type == PGA_PT_SCALAR) {
+            fac = p.a;
+        } else if (!is_static && j > 0 && shape_dependent(p.type)) {
             // the DynParams form evaluates factor j on the symbol that child
             // j-1 left behind (size, model and seed overwritten in place)
             Rec sym = r;
             float tx, ty, tz;
             axis_vec(f.axis, f.prev_o, tx, ty, tz);
-            mul_translate(r.m, tx, ty, tz, sym.m);
+            if (c.geom) mul_translate(r.m, tx, ty, tz, sym.m);
             if (f.axis == 0) sym.s[0] = f.prev_ns; else if (f.axis == 1) sym.s[1] = f.prev_ns; else sym.s[2] = f.prev_ns;
             sym.seed = next_seed(r.seed, j - 1u);
             fac = eval_param(c, p, sym, r.axiom, r.path, rec_depth(r));
@@ -326,39 +384,43 @@ __device__ __forceinline__ bool fan_geom(const Ctx& c, const Rec& r, Fan& f, uin
 // Child j of a fan-out parent from its geometry (o, ns along `axis`).
 // Repeat pieces use seed modifier j + 1 (1 for the first piece, i for the
 // middle pieces i = 2 .. reps-1, reps for the last, Repeat.cuh:156-184).
-__device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, const DevEntry& e, uint32_t j,
+__device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, const EntryV& e, uint32_t j,
                                             uint32_t axis, float o, float ns, Rec& ch) {
-    const uint32_t kind = e.kind;
+    const uint32_t kind = e.kind();
     ch = r;
     if (kind == K_SUBDIV || kind == K_REPEAT) {
-        float tx, ty, tz;
-        axis_vec(axis, o, tx, ty, tz);
-        mul_translate(r.m, tx, ty, tz, ch.m);
+        if (c.geom) {
+            float tx, ty, tz;
+            axis_vec(axis, o, tx, ty, tz);
+            mul_translate(r.m, tx, ty, tz, ch.m);
+        }
         if (axis == 0) ch.s[0] = ns; else if (axis == 1) ch.s[1] = ns; else ch.s[2] = ns;
         const bool sub = kind == K_SUBDIV;
         ch.seed = next_seed(r.seed, sub ? j : j + 1u);
         return advance(c, ch, succ_of(c, e, sub ? j : 0u), j);
     }
     if (kind == K_COMPSPLIT) {                 // FaceSplitter<Box> (ComponentSplit.cuh:77-174)
-        const float hx = r.s[0] * 0.5f, hy = r.s[1] * 0.5f, hz = r.s[2] * 0.5f;
-        float tx = 0.0f, ty = 0.0f, tz = 0.0f;
-        // face frame b (4x4), identity unless set below
-        float b00 = 1.0f, b02 = 0.0f, b06 = 0.0f, b09 = 0.0f, b08 = 0.0f, b10 = 1.0f, b05 = 1.0f;
-        bool rotate = true;
-        if (j == 0) { tz = -hz; b00 = -1.0f; b10 = -1.0f; }                          // back
-        else if (j == 1) { tz = hz; rotate = false; }                                // front
-        else if (j == 2) { tx = -hx; b00 = 0.0f; b02 = -1.0f; b08 = 1.0f; b10 = 0.0f; }    // left
-        else if (j == 3) { tx = hx; b00 = 0.0f; b02 = 1.0f; b08 = -1.0f; b10 = 0.0f; }     // right
-        else if (j == 4) { ty = -hy; b05 = 0.0f; b06 = -1.0f; b09 = 1.0f; b10 = 0.0f; }    // bottom
-        else { ty = hy; b05 = 0.0f; b06 = 1.0f; b09 = -1.0f; b10 = 0.0f; }                 // top
-        float tmp[12];
-        mul_translate(r.m, tx, ty, tz, tmp);
-        if (rotate) {
-            const float b[16] = {b00, 0.0f, b02, 0.0f, 0.0f, b05, b06, 0.0f,
-                                 b08, b09, b10, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
-            mul34x44(tmp, b, ch.m);
-        } else {
-            set_model(ch, tmp);
+        if (c.geom) {
+            const float hx = r.s[0] * 0.5f, hy = r.s[1] * 0.5f, hz = r.s[2] * 0.5f;
+            float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+            // face frame b (4x4), identity unless set below
+            float b00 = 1.0f, b02 = 0.0f, b06 = 0.0f, b09 = 0.0f, b08 = 0.0f, b10 = 1.0f, b05 = 1.0f;
+            bool rotate = true;
+            if (j == 0) { tz = -hz; b00 = -1.0f; b10 = -1.0f; }                          // back
+            else if (j == 1) { tz = hz; rotate = false; }                                // front
+            else if (j == 2) { tx = -hx; b00 = 0.0f; b02 = -1.0f; b08 = 1.0f; b10 = 0.0f; }    // left
+            else if (j == 3) { tx = hx; b00 = 0.0f; b02 = 1.0f; b08 = -1.0f; b10 = 0.0f; }     // right
+            else if (j == 4) { ty = -hy; b05 = 0.0f; b06 = -1.0f; b09 = 1.0f; b10 = 0.0f; }    // bottom
+            else { ty = hy; b05 = 0.0f; b06 = 1.0f; b09 = -1.0f; b10 = 0.0f; }                 // top
+            float tmp[12];
+            mul_translate(r.m, tx, ty, tz, tmp);
+            if (rotate) {
+                const float b[16] = {b00, 0.0f, b02, 0.0f, 0.0f, b05, b06, 0.0f,
+                                     b08, b09, b10, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+                mul34x44(tmp, b, ch.m);
+            } else {
+                set_model(ch, tmp);
+            }
         }
         const float sx = r.s[0], sy = r.s[1], sz = r.s[2];
         ch.s[0] = (j == 2 || j == 3) ? sz : sx;
@@ -366,28 +428,10 @@ __device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, const De
         ch.s[2] = 1.0f;
         const uint32_t mod = j < 4 ? j + 2u : (j == 4 ? 1u : 0u);
         ch.seed = next_seed(r.seed, mod);
-        const int32_t target = j < 4 ? e.sv[2] : (j == 4 ? e.sv[1] : e.sv[0]);
-        return advance(c, ch, target, j);
+        return advance(c, ch, succ_of(c, e, j < 4 ? 2u : (j == 4 ? 1u : 0u)), j);
     }
     // K_REPLICATE
     return advance(c, ch, succ_of(c, e, j), j);
-}
-
-// Child j of the fan-out parent `r` (j = 0, 1, ... in order). Returns false
-// when the child is not dispatched.
-__device__ __forceinline__ bool fan_child(const Ctx& c, const Rec& r, Fan& f, uint32_t j, Rec& ch) {
-    float o, ns;
-    if (!fan_geom(c, r, f, j, o, ns)) return false;
-    return build_child(c, r, c.prog.entries[rec_entry(r)], j, f.axis, o, ns, ch);
-}
-
-// Child described by `d` of the parent record `r`.
-__device__ __forceinline__ bool desc_child(const Ctx& c, const Rec& r, const Desc& d, Rec& ch) {
-    const uint32_t j = d.jx & 0x3FFFFFFFu;
-    if (j == kDescSelf) { ch = r; return true; }
-    if (j == kDescNone) return false;
-    const DevEntry e = c.prog.entries[rec_entry(r)];
-    return build_child(c, r, e, j, d.jx >> 30, d.o, d.ns, ch);
 }
 
 } // namespace pga
