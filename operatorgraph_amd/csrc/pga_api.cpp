@@ -40,6 +40,7 @@ struct pga_context {
     uint32_t image_bytes = 0, off_params = 0, off_succ = 0, off_exp = 0, off_slots = 0;
     uint32_t lds_bytes = 0;           // dynamic LDS of the derivation kernels
     uint32_t count_geom = 0;          // 1: some decision of the program reads the model matrix
+    uint32_t use_seed = 0;            // 1: some parameter reads a shape seed
     DevEntry* d_entries = nullptr;    // views into d_image
     DevParam* d_params = nullptr;
     int32_t* d_succ = nullptr;
@@ -165,6 +166,7 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
     std::vector<uint32_t> Ex;
     int32_t max_slot = -1;
     bool count_geom = false;
+    bool use_seed = false;
     for (uint32_t i = 0; i < n; ++i) {
         const pga_dispatch_entry& e = entries[i];
         std::string at = "entry " + std::to_string(i) + ": ";
@@ -223,17 +225,19 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
             dp.type = (uint32_t)q.type;
             switch (q.type) {
             case PGA_PT_SCALAR: dp.a = q.value[0]; break;
-            case PGA_PT_RAND: dp.a = q.value[0]; dp.b = q.value[1]; break;
+            case PGA_PT_RAND: dp.a = q.value[0]; dp.b = q.value[1]; use_seed = true; break;
             case PGA_PT_SHAPE_ATTR:
                 if (!(q.value[0] >= 0.0f) || (uint32_t)q.value[0] > 16u)
                     return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "shape attribute index out of range");
                 dp.a = q.value[0];
                 if ((uint32_t)q.value[0] < 12u) count_geom = true;   // a model-matrix element
+                if ((uint32_t)q.value[0] == 15u) use_seed = true;     // the seed
                 break;
             case PGA_PT_EXP: {
                 std::string why;
                 if (!check_exp(q, why)) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + why);
                 count_geom = true;   // conservatively: an expression may read the model matrix
+                use_seed = true;     // ... and the seed
                 uint32_t off = (uint32_t)Ex.size();
                 uint32_t len = (uint32_t)q.value[0];
                 for (uint32_t v = 0; v <= len; ++v) {
@@ -351,8 +355,21 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
     for (uint32_t i = 0; i < n; ++i) {
         DevEntry& d = E[i];
         d.ndirect = 0;
-        std::memset(d.dslot, 0, sizeof(d.dslot));
-        std::memset(d.dcnt, 0, sizeof(d.dcnt));
+        d.aux[0] = d.aux[1] = 0u;
+        if (!is_fan(d.kind)) continue;
+        // successor of each child, in child order
+        std::vector<int32_t> succ_j;
+        if (d.kind == pga::K_SUBDIV) {
+            for (uint32_t j = 0; j + 1 < d.nparams; ++j) succ_j.push_back(Sc[d.sbase + j]);
+        } else if (d.kind == pga::K_REPEAT) {
+            succ_j.push_back(Sc[d.sbase]);
+        } else if (d.kind == pga::K_COMPSPLIT) {
+            for (uint32_t j = 0; j < 6; ++j) succ_j.push_back(Sc[d.sbase + (j < 4 ? 2u : (j == 4 ? 1u : 0u))]);
+        } else {
+            for (uint32_t k = 0; k < d.nsucc; ++k) succ_j.push_back(Sc[d.sbase + k]);
+        }
+        for (uint32_t j = 0; j < succ_j.size() && j < 32; ++j)
+            if (succ_j[j] < 0 || E[succ_j[j]].kind == pga::K_DISCARD) d.aux[0] |= 1u << j;
         if (!(d.flags & pga::EF_LEAF)) continue;
         std::vector<std::pair<int32_t, uint32_t>> kids;   // (successor, children that take it)
         bool ok = true;
@@ -361,32 +378,24 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
                 const DevParam& q = Pm[d.pbase + 1 + j];
                 // the static form skips zero factors: only constant non-zero ones have fixed counts
                 if ((d.flags & pga::EF_STATIC) && !(q.type == PGA_PT_SCALAR && q.a != 0.0f)) ok = false;
-                kids.push_back({Sc[d.sbase + j], 1u});
             }
-        } else if (d.kind == pga::K_REPEAT) {
-            kids.push_back({Sc[d.sbase], 1u});
-        } else if (d.kind == pga::K_COMPSPLIT) {
-            kids.push_back({Sc[d.sbase + 2], 4u});
-            kids.push_back({Sc[d.sbase + 1], 1u});
-            kids.push_back({Sc[d.sbase + 0], 1u});
-        } else {
-            for (uint32_t k = 0; k < d.nsucc; ++k) kids.push_back({Sc[d.sbase + k], 1u});
         }
+        for (int32_t s : succ_j) kids.push_back({s, 1u});
         uint32_t cnt[PGA_MAX_TERMINALS] = {0};
         for (auto& kc : kids) {
             if (kc.first < 0 || E[kc.first].kind == pga::K_DISCARD) continue;
             if (E[kc.first].kind != pga::K_GENERATE || E[kc.first].tslot == 0xFF) { ok = false; break; }
             cnt[E[kc.first].tslot] += kc.second;
         }
-        uint32_t nd = 0;
+        uint32_t nd = 0, table = 0;
         for (uint32_t t = 0; t < PGA_MAX_TERMINALS && ok; ++t) {
             if (cnt[t] == 0) continue;
-            if (nd == 4 || cnt[t] > 255) { ok = false; break; }
-            d.dslot[nd] = (uint8_t)t;
-            d.dcnt[nd] = (uint8_t)cnt[t];
+            if (nd == 2 || cnt[t] > 255) { ok = false; break; }
+            table |= (t | (cnt[t] << 8)) << (16u * nd);
             ++nd;
         }
         if (!ok) continue;
+        d.aux[1] = table;
         d.ndirect = (uint8_t)nd;
         d.flags |= pga::EF_DIRECT;
     }
@@ -428,6 +437,7 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
                     " bytes the derivation kernels stage in LDS");
     ctx->lds_bytes = lds;
     ctx->count_geom = count_geom ? 1u : 0u;
+    ctx->use_seed = use_seed ? 1u : 0u;
     int per_cu = 0;
     HIP_TRY(ctx, pga_mega_occupancy(ctx->cfg.block_size, ctx->lds_bytes, &per_cu));
     if (per_cu < 1) return fail(ctx, PGA_ERR_UNSUPPORTED, "derivation kernel does not fit on a compute unit");
@@ -506,6 +516,7 @@ pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
     a.max_levels = ctx->cfg.max_levels;
     a.max_chain = kMaxChain;
     a.count_geom = ctx->count_geom;
+    a.use_seed = ctx->use_seed;
     return a;
 }
 

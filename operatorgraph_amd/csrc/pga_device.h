@@ -35,7 +35,7 @@ enum EntryFlags : uint8_t {
     EF_LEAF = 2,      // fan-out whose children never reach another fan-out
     EF_SHAPE_DEP = 4, // some parameter reads the shape (dyn Subdivide re-evaluation)
     EF_DIRECT = 8,    // leaf fan-out whose children are all Generate (constant terminal) or
-                      // Discard: its terminal counts are a table (dslot, dcnt)
+                      // Discard: its terminal counts are a table (aux[1])
 };
 
 // One dispatch entry, 48 bytes. The first three parameters are held inline
@@ -49,13 +49,16 @@ struct __align__(16) DevEntry {
     uint8_t nsucc;
     uint8_t inl;        // bit k: parameter k (< 3) is the constant pv[k]
     uint8_t tslot;      // Generate with a constant terminal: its dense slot (else 0xFF)
-    uint8_t ndirect;    // direct fan-out: pairs in dslot/dcnt
+    uint8_t ndirect;    // direct fan-out: (slot, count) pairs in aux
     uint32_t pbase;     // first DevParam
     uint32_t sbase;     // first successor in the successor pool
     float pv[3];        // inline constant parameters
     int32_t sv[3];      // successors 0..2
-    uint8_t dslot[4];   // direct fan-out: terminal slot k receives dcnt[k] instances
-    uint8_t dcnt[4];    //   per child set (times the repetitions of a Repeat)
+    // fan-outs: aux[0] = skip mask, bit j: child j goes to a Discard or
+    // nowhere (a Repeat's children all follow bit 0); direct fan-outs:
+    // aux[1] bytes k = (terminal slot, count) pair k, count per child set
+    // (times the repetitions of a Repeat)
+    uint32_t aux[2];
 };
 
 struct DevParam {       // 16 bytes
@@ -94,7 +97,7 @@ struct Desc;
 
 // Per-launch context shared by every lane.
 struct Ctx {
-    Prog prog;
+    Prog prog;                          // program tables (in LDS during a derivation)
     const float* __restrict__ aparams;  // per-axiom parameters
     uint32_t aparams_per;
     uint32_t naxioms_total;
@@ -110,6 +113,7 @@ struct Ctx {
     uint32_t* err;                      // bit (1 << pga_status) per error raised
     uint32_t max_chain;
     uint32_t geom;                      // 0: skip model-matrix arithmetic (counting only)
+    uint32_t seed;                      // 0: no parameter reads a seed, the seed chain is skipped
 };
 
 __device__ __forceinline__ void raise(const Ctx& c, int status) {

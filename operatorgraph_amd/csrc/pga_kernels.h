@@ -42,6 +42,7 @@ struct KArgs {
     uint32_t max_levels;
     uint32_t max_chain;
     uint32_t count_geom;                // 1: some decision reads the model matrix
+    uint32_t use_seed;                  // 1: some parameter reads a shape seed
 };
 
 } // namespace pga

@@ -129,11 +129,11 @@ __device__ __forceinline__ void emit_terminal(const Ctx& c, LevelLDS& L, bool wr
 // slot come from the entry's table (times the repetitions of a Repeat).
 __device__ __forceinline__ void count_direct(const Ctx& c, LevelLDS& L, const EntryV& e, const Fan& f) {
     const uint32_t mult = e.kind() == K_REPEAT ? f.n : (f.n != 0u ? 1u : 0u);
-    const uint2 tab = *reinterpret_cast<const uint2*>(&c.prog.entries[e.idx].dslot[0]);
+    const uint32_t tab = entry_aux(c, e, 1);
     const uint32_t w = wave_in_block();
     for (uint32_t k = 0; k < e.ndirect(); ++k) {
-        const uint32_t slot = (tab.x >> (8u * k)) & 0xFFu;
-        const unsigned long long add = (unsigned long long)((tab.y >> (8u * k)) & 0xFFu) * mult;
+        const uint32_t slot = (tab >> (16u * k)) & 0xFFu;
+        const unsigned long long add = (unsigned long long)((tab >> (16u * k + 8u)) & 0xFFu) * mult;
         if (add >= 0x80000000ull) { raise(c, PGA_ERR_OVERFLOW_INSTANCES); continue; }
         atomicAdd(&L.cnt[w][slot], (uint32_t)add);
     }
@@ -215,7 +215,33 @@ __device__ __forceinline__ Ctx make_ctx(const KArgs& a, uint32_t level, bool wr)
     c.err = a.err;
     c.max_chain = a.max_chain;
     c.geom = wr ? 1u : a.count_geom;
+    c.seed = a.use_seed;
     return c;
+}
+
+// The record of each lane's current fan-out parent lives in LDS (five
+// 16-byte words per lane, word k of all lanes contiguous), not in registers:
+// the state machine then fits more waves per SIMD.
+__device__ __forceinline__ void par_store(float4* PW, const Rec& r) {
+    float4* p = PW + threadIdx.x;
+    const uint32_t s = blockDim.x;
+    p[0] = make_float4(r.m[0], r.m[1], r.m[2], r.m[3]);
+    p[s] = make_float4(r.m[4], r.m[5], r.m[6], r.m[7]);
+    p[2u * s] = make_float4(r.m[8], r.m[9], r.m[10], r.m[11]);
+    p[3u * s] = make_float4(r.s[0], r.s[1], r.s[2], r.seed);
+    p[4u * s] = make_float4(r.custom, __uint_as_float(r.ed), __uint_as_float(r.axiom), __uint_as_float(r.path));
+}
+__device__ __forceinline__ Rec par_load(const float4* PW) {
+    const float4* p = PW + threadIdx.x;
+    const uint32_t s = blockDim.x;
+    const float4 a = p[0], b = p[s], c = p[2u * s], d = p[3u * s], e = p[4u * s];
+    Rec r;
+    r.m[0] = a.x; r.m[1] = a.y; r.m[2] = a.z; r.m[3] = a.w;
+    r.m[4] = b.x; r.m[5] = b.y; r.m[6] = b.z; r.m[7] = b.w;
+    r.m[8] = c.x; r.m[9] = c.y; r.m[10] = c.z; r.m[11] = c.w;
+    r.s[0] = d.x; r.s[1] = d.y; r.s[2] = d.z; r.seed = d.w;
+    r.custom = e.x; r.ed = __float_as_uint(e.y); r.axiom = __float_as_uint(e.z); r.path = __float_as_uint(e.w);
+    return r;
 }
 
 // Child i of the level: an axiom (level 0) or the child that a descriptor
@@ -234,7 +260,7 @@ __device__ __forceinline__ bool load_child(const KArgs& a, const Ctx& c, bool wr
         r.ed = 0u;
         const int32_t ent = ax.entry_index;
         if (ent < 0) return false;              // PGAFacade.cu:708
-        if ((uint32_t)ent >= c.prog.nentries || (int32_t)load_entry(c, (uint32_t)ent).shape() != ax.shape_type) {
+        if ((uint32_t)ent >= c.prog.nentries || (int32_t)entry_shape(c, (uint32_t)ent) != ax.shape_type) {
             if (wr) raise(c, PGA_ERR_BAD_PROGRAM);
             return false;
         }
@@ -257,8 +283,8 @@ __device__ __forceinline__ bool load_child(const KArgs& a, const Ctx& c, bool wr
 
 // One pass over the children of a level; waves take 64 consecutive
 // children at a time, in a fixed order, so both passes see the same lanes.
-__device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS& L, bool wr, uint32_t level,
-                                         uint32_t n, uint32_t wave, uint32_t nwaves) {
+__device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS& L, float4* PW, bool wr,
+                                         uint32_t level, uint32_t n, uint32_t wave, uint32_t nwaves) {
 #pragma unroll 1
     for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u) {
         const uint32_t i = base + lane_id();
@@ -267,10 +293,10 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
         uint32_t mode = valid ? M_SETTLE : M_DONE;
         bool leaf = false, self = false;
         uint32_t steps = 0, j = 0, dslot = 0, pidx = ~0u;
-        Rec par = cur;
+        uint32_t skipm = 0, nd = 0;   // children that go to a Discard; descriptors to write
         Fan f;
-        f.kind_flags = 0; f.n = 0; f.axis = 0; f.total = 0.0f; f.r2a = 0.0f; f.off = 0.0f;
-        f.e1 = f.e2 = f.e3 = 0.0f; f.prev_ns = 0.0f; f.prev_o = 0.0f; f.reps = 0; f.pbase = 0;
+        f.n = 0; f.axis = 0; f.total = 0.0f; f.r2a = 0.0f; f.off = 0.0f;
+        f.e1 = f.e2 = f.e3 = 0.0f; f.prev_ns = 0.0f; f.prev_o = 0.0f; f.reps = 0;
 #pragma unroll 1
         while (__ballot(mode != M_DONE) != 0ull) {
             bool emit = false, start = false;
@@ -278,13 +304,16 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
             if (mode == M_FAN) {
                 if (j < f.n) {
                     float o = 0.0f, ns = 0.0f;
-                    const bool keep = self ? true : fan_geom(c, par, f, j, o, ns);
+                    const Rec par = par_load(PW);
+                    const EntryV pe = load_entry(c, rec_entry(par));
+                    const bool keep = self ? true : fan_geom(c, par, pe, f, j, o, ns);
+                    const bool skip = j < 32u ? ((skipm >> j) & 1u) != 0u : skipm == 0xFFFFFFFFu;
                     if (leaf) {
-                        if (keep && build_child(c, par, load_entry(c, rec_entry(par)), j, f.axis, o, ns, cur)) {
+                        if (keep && !skip && build_child(c, par, pe, j, f.axis, o, ns, cur)) {
                             mode = M_SETTLE;
                             steps = 0;
                         }
-                    } else {   // write pass only: the count pass claimed f.n slots at the start
+                    } else if (!skip) {   // write pass only: the count pass claimed nd slots at the start
                         const uint32_t jx = pidx == ~0u ? kDescNone
                                           : self        ? kDescSelf
                                           : keep        ? (j | (f.axis << 30))
@@ -322,11 +351,19 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
                         raise(c, PGA_ERR_BAD_PROGRAM);
                         mode = M_FAN;
                     } else {
-                        par = cur;
-                        fan_init(c, par, e, f);
+                        par_store(PW, cur);
+                        fan_init(c, cur, e, f);
                         j = 0;
                         leaf = (e.flags() & EF_LEAF) != 0;
                         start = !leaf;
+                        const uint32_t sk = entry_aux(c, e, 0);
+                        if (k == K_REPEAT) {
+                            skipm = (sk & 1u) ? 0xFFFFFFFFu : 0u;
+                            nd = skipm ? 0u : f.n;
+                        } else {
+                            skipm = sk;
+                            nd = f.n - (uint32_t)__popc(skipm & (f.n < 32u ? (1u << f.n) - 1u : 0xFFFFFFFFu));
+                        }
                         if (!wr && (e.flags() & EF_DIRECT)) {
                             count_direct(c, L, e, f);
                             mode = M_DONE;
@@ -341,9 +378,11 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
                         mode = M_FAN;
                     } else {
                         // over-long chain: continues on the next level
-                        par = cur;
+                        par_store(PW, cur);
                         self = true;
                         f.n = 1;
+                        nd = 1;
+                        skipm = 0;
                         j = 0;
                         start = true;
                         mode = wr ? M_FAN : M_DONE;
@@ -356,7 +395,7 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
                 }
             }
             if (__ballot(emit) != 0ull) emit_terminal(c, L, wr, emit, slot, cur);
-            if (__ballot(start) != 0ull) start_fanout(c, L, wr, start, par, f.n, pidx, dslot);
+            if (__ballot(start) != 0ull) start_fanout(c, L, wr, start, par_load(PW), nd, pidx, dslot);
         }
     }
 }
@@ -451,7 +490,7 @@ __device__ __forceinline__ void scan_counts(const KArgs& a, LevelLDS& L, uint32_
 // One derivation level: count pass, scan, write pass. Leaves L.next_n =
 // children of the next level; block 0 records the level's parent and
 // descriptor counts.
-__device__ __forceinline__ void run_level(const KArgs& a, LevelLDS& L, uint32_t level, uint32_t n) {
+__device__ __forceinline__ void run_level(const KArgs& a, LevelLDS& L, float4* PW, uint32_t level, uint32_t n) {
     const uint32_t nslots = a.prog.nslots, ncnt = nslots + 2u;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -462,7 +501,7 @@ __device__ __forceinline__ void run_level(const KArgs& a, LevelLDS& L, uint32_t 
     for (uint32_t pass = 0; pass < 2u; ++pass) {
         const bool wr = pass != 0u;
         const Ctx c = make_ctx(a, level, wr);
-        run_pass(a, c, L, wr, level, n, wave, nwaves);
+        run_pass(a, c, L, PW, wr, level, n, wave, nwaves);
         __syncthreads();
         if (!wr) scan_counts(a, L, level);
     }
@@ -498,10 +537,11 @@ __device__ __forceinline__ uint32_t first_level_size(const KArgs& a, uint32_t le
 template <int BS>
 __global__ void __launch_bounds__(BS) k_level(KArgs a, uint32_t level) {
     __shared__ LevelLDS L;
+    __shared__ float4 PW[5 * BS];
     stage_program(a);
     init_bases(a, L);
     const uint32_t n = first_level_size(a, level);
-    if (n != 0u) run_level(a, L, level, n);
+    if (n != 0u) run_level(a, L, PW, level, n);
     // every block has read the old bases before the barrier inside run_level
     store_bases(a, L);
 }
@@ -510,12 +550,13 @@ __global__ void __launch_bounds__(BS) k_level(KArgs a, uint32_t level) {
 template <int BS>
 __global__ void __launch_bounds__(BS) k_mega(KArgs a) {
     __shared__ LevelLDS L;
+    __shared__ float4 PW[5 * BS];
     stage_program(a);
     init_bases(a, L);
     uint32_t level = 0;
     uint32_t n = a.n_axioms;
     for (; level < a.max_levels && n != 0u; ++level) {
-        run_level(a, L, level, n);
+        run_level(a, L, PW, level, n);
         n = min(L.next_n, a.cap_desc);
         grid_barrier(a.bar, gridDim.x);       // next level reads this level's records
     }

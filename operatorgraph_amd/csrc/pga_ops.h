@@ -20,13 +20,13 @@
 namespace pga {
 
 // The fixed fields of a dispatch entry, unpacked into registers. Fields
-// chosen by a run-time index (successor k, inline parameter k) are read from
-// the LDS entry by index: a select between register copies would be turned
-// into an indexed load from a stack copy of the entry.
+// chosen by a run-time index (successor k, inline parameter k, the aux words)
+// are read from the LDS entry by index: a select between register copies
+// would be turned into an indexed load from a stack copy of the entry.
 struct EntryV {
     uint32_t idx;     // entry index
     uint32_t w0;      // kind | shape << 8 | flags << 16 | nparams << 24
-    uint32_t w1;      // nsucc | inl << 8 | tslot << 16
+    uint32_t w1;      // nsucc | inl << 8 | tslot << 16 | ndirect << 24
     uint32_t pbase, sbase;
     float pv0, pv1, pv2;
     int32_t sv0;
@@ -41,6 +41,7 @@ struct EntryV {
 };
 static_assert(sizeof(DevEntry) == 48, "DevEntry must be three 16-byte words");
 
+// Entry i of this lane, from the LDS copy of the program.
 __device__ __forceinline__ EntryV load_entry(const Ctx& c, uint32_t i) {
     const uint4* q = reinterpret_cast<const uint4*>(c.prog.entries) + i * 3u;
     const uint4 a = q[0], b = q[1];
@@ -51,6 +52,12 @@ __device__ __forceinline__ EntryV load_entry(const Ctx& c, uint32_t i) {
     e.sv0 = (int32_t)b.w;
     return e;
 }
+__device__ __forceinline__ uint32_t entry_aux(const Ctx& c, const EntryV& e, uint32_t k) {
+    return c.prog.entries[e.idx].aux[k];
+}
+
+// Shape type of entry i.
+__device__ __forceinline__ uint32_t entry_shape(const Ctx& c, uint32_t i) { return c.prog.entries[i].shape; }
 
 __device__ __forceinline__ bool advance(const Ctx& c, Rec& r, int32_t next, uint32_t ordinal) {
     if (next < 0) return false;   // unused dynamic edge: nothing dispatched
@@ -204,7 +211,7 @@ __device__ __forceinline__ int32_t apply_single(const Ctx& c, const EntryV& e, R
     }
     float sx = r.s[0], sy = r.s[1];
     r.s[0] = sx; r.s[1] = v1; r.s[2] = sy;
-    r.seed = next_seed(r.seed, 0u);
+    if (c.seed) r.seed = next_seed(r.seed, 0u);
     return e.sv0;
 }
 
@@ -214,7 +221,6 @@ __device__ __forceinline__ bool is_fanout(uint32_t k) {
 
 // ---- fan-out operators ----
 struct Fan {
-    uint32_t kind_flags; // kind | flags << 8
     uint32_t n;          // children to iterate (j = 0 .. n-1)
     uint32_t axis;
     float total;
@@ -224,13 +230,10 @@ struct Fan {
     float prev_ns;       // Subdivide: size and offset of the previous child
     float prev_o;
     uint32_t reps;       // Repeat repetitions
-    uint32_t pbase;
 };
 
 __device__ __forceinline__ void fan_init(const Ctx& c, const Rec& r, const EntryV& e, Fan& f) {
     const uint32_t kind = e.kind();
-    f.kind_flags = kind | (e.flags() << 8);
-    f.pbase = e.pbase;
     f.n = 0;
     f.reps = 0;
     f.axis = 0;
@@ -333,13 +336,14 @@ __device__ __forceinline__ void axis_vec(uint32_t axis, float o, float& tx, floa
 // sequential running offsets (Subdivide.cuh:49-69, 218-236; Repeat.cuh:
 // 146-197). Returns false when the child is not dispatched (a zero factor of
 // the static Subdivide form).
-__device__ __forceinline__ bool fan_geom(const Ctx& c, const Rec& r, Fan& f, uint32_t j, float& o, float& ns) {
-    const uint32_t kind = f.kind_flags & 0xFFu;
+__device__ __forceinline__ bool fan_geom(const Ctx& c, const Rec& r, const EntryV& e, Fan& f, uint32_t j, float& o,
+                                         float& ns) {
+    const uint32_t kind = e.kind();
     o = 0.0f;
     ns = 0.0f;
     if (kind == K_SUBDIV) {
-        const DevParam p = c.prog.params[f.pbase + 1 + j];
-        const bool is_static = ((f.kind_flags >> 8) & EF_STATIC) != 0;
+        const DevParam p = c.prog.params[e.pbase + 1 + j];
+        const bool is_static = (e.flags() & EF_STATIC) != 0;
         float fac;
         if (p.type == PGA_PT_SCALAR) {
             fac = p.a;
@@ -351,7 +355,7 @@ __device__ __forceinline__ bool fan_geom(const Ctx& c, const Rec& r, Fan& f, uin
             axis_vec(f.axis, f.prev_o, tx, ty, tz);
             if (c.geom) mul_translate(r.m, tx, ty, tz, sym.m);
             if (f.axis == 0) sym.s[0] = f.prev_ns; else if (f.axis == 1) sym.s[1] = f.prev_ns; else sym.s[2] = f.prev_ns;
-            sym.seed = next_seed(r.seed, j - 1u);
+            if (c.seed) sym.seed = next_seed(r.seed, j - 1u);
             fac = eval_param(c, p, sym, r.axiom, r.path, rec_depth(r));
         } else {
             fac = eval_param(c, p, r, r.axiom, r.path, rec_depth(r));
@@ -396,7 +400,7 @@ __device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, const En
         }
         if (axis == 0) ch.s[0] = ns; else if (axis == 1) ch.s[1] = ns; else ch.s[2] = ns;
         const bool sub = kind == K_SUBDIV;
-        ch.seed = next_seed(r.seed, sub ? j : j + 1u);
+        if (c.seed) ch.seed = next_seed(r.seed, sub ? j : j + 1u);
         return advance(c, ch, succ_of(c, e, sub ? j : 0u), j);
     }
     if (kind == K_COMPSPLIT) {                 // FaceSplitter<Box> (ComponentSplit.cuh:77-174)
@@ -427,7 +431,7 @@ __device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, const En
         ch.s[1] = j < 4 ? sy : sz;
         ch.s[2] = 1.0f;
         const uint32_t mod = j < 4 ? j + 2u : (j == 4 ? 1u : 0u);
-        ch.seed = next_seed(r.seed, mod);
+        if (c.seed) ch.seed = next_seed(r.seed, mod);
         return advance(c, ch, succ_of(c, e, j < 4 ? 2u : (j == 4 ? 1u : 0u)), j);
     }
     // K_REPLICATE
