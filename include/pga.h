@@ -272,6 +272,26 @@ PGA_API pga_status pga_execute_async(pga_context* ctx, uint32_t first_axiom, uin
 PGA_API pga_status pga_synchronize(pga_context* ctx);
 PGA_API pga_status pga_kernel_seconds(pga_context* ctx, double* seconds);
 
+/* -- on-device autotuner (stands in for the reference's offline
+ *    auto-tuner, which times operator-graph partitions and writes their
+ *    configuration into generated code, auto_tuner/: here the candidates are
+ *    the launch configurations of this engine, timed on the device with the
+ *    caller's own axioms and bindings). Times every technique x workgroup
+ *    size x grid size, keeps the fastest for the context and reports the
+ *    largest frontiers seen, from which a caller sizes queue_capacity. The
+ *    timed derivations write the bound instance buffers. -- */
+typedef struct pga_tune_result {
+    uint32_t technique;            /* chosen technique */
+    uint32_t block_size;           /* chosen workgroup size */
+    uint32_t grid_blocks;          /* chosen workgroups per launch */
+    uint32_t candidates;           /* configurations timed */
+    double seconds;                /* best device time of the chosen configuration */
+    double initial_seconds;        /* best device time of the configuration before tuning */
+    uint64_t max_level_parents;    /* largest number of parent records written by one level */
+    uint64_t max_level_children;   /* largest number of children of one level */
+} pga_tune_result;
+PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t repeats, pga_tune_result* out);
+
 /* -- instrumentation (the reference's clock64 edge/procedure timers,
  *    Instrumentation.cuh:47-76, become phase timestamps): when a device
  *    buffer is set, workgroup 0 of the derivation kernel writes the device

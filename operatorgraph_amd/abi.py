@@ -114,6 +114,12 @@ class Stats(C.Structure):
                 ("terminals", C.c_uint64), ("seconds", C.c_double)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("technique", C.c_uint32), ("block_size", C.c_uint32), ("grid_blocks", C.c_uint32),
+                ("candidates", C.c_uint32), ("seconds", C.c_double), ("initial_seconds", C.c_double),
+                ("max_level_parents", C.c_uint64), ("max_level_children", C.c_uint64)]
+
+
 # numpy views of the 80-byte axiom and 64-byte instance records
 AXIOM_DTYPE = np.dtype([("model", "<f4", (12,)), ("size", "<f4", (3,)), ("seed", "<f4"),
                         ("custom", "<f4"), ("entry_index", "<i4"), ("shape_type", "<i4"),

@@ -81,6 +81,7 @@ struct pga_context {
     uint32_t ticks_cap = 0;
     uint32_t* h_pinned = nullptr;   // count readback
     uint32_t mega_blocks = 0;
+    uint64_t max_level_parents = 0, max_level_children = 0;   // of the last execute
 
     pga_stats stats{};
 };
@@ -735,8 +736,12 @@ static pga_status finish(pga_context* ctx, uint32_t count, uint32_t host_levels)
     if (ctx->cfg.technique == PGA_TECHNIQUE_MEGAKERNEL && count > 0) levels = lc[ctx->cfg.max_levels + 1];
     ctx->stats.levels = levels;
     ctx->stats.records = 0;
+    ctx->max_level_parents = 0;
+    ctx->max_level_children = 0;
     for (uint32_t l = 0; l < levels && l < ctx->cfg.max_levels; ++l) {
         ctx->stats.records += pc[l];
+        ctx->max_level_parents = std::max<uint64_t>(ctx->max_level_parents, pc[l]);
+        ctx->max_level_children = std::max<uint64_t>(ctx->max_level_children, lc[l]);
         if (pc[l] > ctx->cap || lc[l] > ctx->cap) err |= 1u << PGA_ERR_OVERFLOW_QUEUE;
     }
     ctx->stats.terminals = 0;
@@ -834,6 +839,82 @@ PGA_API pga_status pga_get_stats(const pga_context* ctx, pga_stats* stats) {
     if (!ctx || !stats) return PGA_ERR_INVALID_ARG;
     *stats = ctx->stats;
     return PGA_OK;
+}
+
+// Workgroups of one launch at workgroup size bs: all that fit on the device
+// at once (cooperative launch), at most kMaxBlocksPerCu per compute unit.
+static pga_status resident_blocks(pga_context* ctx, uint32_t bs, uint32_t* blocks) {
+    int per_cu = 0;
+    HIP_TRY(ctx, pga_mega_occupancy(bs, ctx->lds_bytes, &per_cu));
+    *blocks = per_cu < 1 ? 0u : std::min((uint32_t)per_cu, kMaxBlocksPerCu) * (uint32_t)ctx->num_cus;
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t repeats, pga_tune_result* out) {
+    if (!ctx || !out) return fail(ctx, PGA_ERR_INVALID_ARG, "null argument");
+    if (repeats == 0) repeats = 3;
+    const pga_config saved = ctx->cfg;
+    const uint32_t saved_blocks = ctx->mega_blocks;
+    auto time_config = [&](double* best) -> pga_status {
+        *best = 1e30;
+        for (uint32_t r = 0; r < repeats; ++r) {
+            double s = 0.0;
+            pga_status st = pga_execute_range(ctx, 0, num_axioms, 0, &s);
+            if (st) return st;
+            *best = std::min(*best, s);
+        }
+        return PGA_OK;
+    };
+    pga_tune_result res{};
+    pga_status st = time_config(&res.initial_seconds);
+    if (st) return st;
+    res.technique = saved.technique;
+    res.block_size = saved.block_size;
+    res.grid_blocks = saved_blocks;
+    res.seconds = res.initial_seconds;
+    res.max_level_parents = ctx->max_level_parents;
+    res.max_level_children = ctx->max_level_children;
+    res.candidates = 1;
+    const uint32_t techniques[] = {PGA_TECHNIQUE_MEGAKERNEL, PGA_TECHNIQUE_KERNELS};
+    const uint32_t sizes[] = {256u, 512u, 768u, 1024u};
+    for (uint32_t tech : techniques) {
+        for (uint32_t bs : sizes) {
+            uint32_t full = 0;
+            st = resident_blocks(ctx, bs, &full);
+            if (st) break;
+            if (full == 0) continue;
+            const uint32_t grids[] = {full, std::max(full / 2u, (uint32_t)ctx->num_cus)};
+            for (int g = 0; g < 2; ++g) {
+                if (g == 1 && grids[1] >= full) continue;
+                ctx->cfg.technique = tech;
+                ctx->cfg.block_size = bs;
+                ctx->mega_blocks = grids[g];
+                double s = 0.0;
+                st = time_config(&s);
+                if (st) break;
+                ++res.candidates;
+                if (s < res.seconds) {
+                    res.seconds = s;
+                    res.technique = tech;
+                    res.block_size = bs;
+                    res.grid_blocks = grids[g];
+                }
+            }
+            if (st) break;
+        }
+        if (st) break;
+    }
+    if (st) {   // leave the context as it was
+        ctx->cfg = saved;
+        ctx->mega_blocks = saved_blocks;
+        return st;
+    }
+    ctx->cfg.technique = res.technique;
+    ctx->cfg.block_size = res.block_size;
+    ctx->mega_blocks = res.grid_blocks;
+    st = pga_execute_range(ctx, 0, num_axioms, 0, nullptr);   // the buffers hold the chosen run
+    *out = res;
+    return st;
 }
 
 } // extern "C"

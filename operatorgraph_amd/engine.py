@@ -60,6 +60,7 @@ def load_library():
         "pga_kernel_seconds": (C.c_int, [P, C.POINTER(C.c_double)]),
         "pga_set_timestamps": (C.c_int, [P, P, C.c_uint32]),
         "pga_timestamp_hz": (C.c_double, [P]),
+        "pga_autotune": (C.c_int, [P, C.c_uint32, C.c_uint32, C.POINTER(abi.TuneResult)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -197,6 +198,13 @@ class Engine:
         sec = C.c_double(0.0)
         self._check(self.L.pga_kernel_seconds(self._h, C.byref(sec)))
         return sec.value
+
+    def autotune(self, n=None, repeats=3):
+        """Times every launch configuration on the bound axioms and keeps the
+        fastest (pga_autotune); returns the chosen configuration."""
+        r = abi.TuneResult()
+        self._check(self.L.pga_autotune(self._h, self.n_axioms if n is None else n, repeats, C.byref(r)))
+        return {name: getattr(r, name) for name, _ in abi.TuneResult._fields_}
 
     def enable_timestamps(self, capacity=4 * 64):
         """Phase timestamps of the derivation kernel (see pga_set_timestamps)."""

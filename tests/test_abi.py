@@ -21,14 +21,15 @@ PROBE = r"""
 #define O(t, f) printf(#t "." #f " %zu\n", offsetof(t, f));
 int main(void) {
   S(pga_dispatch_parameter) S(pga_dispatch_successor) S(pga_dispatch_entry) S(pga_procedure)
-  S(pga_axiom) S(pga_instance) S(pga_config) S(pga_stats)
+  S(pga_axiom) S(pga_instance) S(pga_config) S(pga_stats) S(pga_tune_result)
   O(pga_dispatch_entry, num_parameters) O(pga_dispatch_entry, parameters)
   O(pga_dispatch_entry, num_successors) O(pga_dispatch_entry, rule_tag_id)
   O(pga_dispatch_entry, successors) O(pga_dispatch_entry, num_edge_indices)
   O(pga_dispatch_entry, edge_indices) O(pga_dispatch_entry, sub_graph_index)
   O(pga_axiom, size) O(pga_axiom, seed) O(pga_axiom, custom) O(pga_axiom, entry_index)
   O(pga_axiom, shape_type) O(pga_instance, custom) O(pga_instance, axiom) O(pga_instance, path)
-  O(pga_instance, meta) O(pga_config, stream)
+  O(pga_instance, meta) O(pga_config, stream) O(pga_tune_result, seconds)
+  O(pga_tune_result, max_level_children)
   return 0;
 }
 """
@@ -37,6 +38,7 @@ MIRRORS = {
     "pga_dispatch_parameter": abi.DispatchParameter, "pga_dispatch_successor": abi.DispatchSuccessor,
     "pga_dispatch_entry": abi.DispatchEntry, "pga_procedure": abi.Procedure, "pga_axiom": abi.Axiom,
     "pga_instance": abi.Instance, "pga_config": abi.Config, "pga_stats": abi.Stats,
+    "pga_tune_result": abi.TuneResult,
 }
 
 

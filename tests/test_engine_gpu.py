@@ -45,10 +45,10 @@ def gpu_derive(E, program, ax, n_terms, technique, params=None, key=None, cap=No
     return out, st
 
 
-def check_against_oracle(E, program, ax, n_terms, technique, params=None, key=None, twins=True):
+def check_against_oracle(E, program, ax, n_terms, technique, params=None, key=None, twins=True, **cfg):
     from oracle import oracle as O
     r = O.derive(program, ax, params=params, key=key or (0, 0), twins=O.TWIN_F64 if twins else 0)
-    got, st = gpu_derive(E, program, ax, n_terms, technique, params=params, key=key)
+    got, st = gpu_derive(E, program, ax, n_terms, technique, params=params, key=key, **cfg)
     for t in range(n_terms):
         ref = r.instances(t)
         assert_same_multiset(got[t], ref, f"terminal {t}")
@@ -140,6 +140,26 @@ def test_kitchen_sink_operators(E, technique):
     ax["shape_type"] = B
     got, _ = check_against_oracle(E, _kitchen_sink(), ax, 4, technique)
     assert sum(len(g) for g in got) > 64
+
+
+@pytest.mark.parametrize("block_size", [256, 768, 1024])
+def test_workgroup_size_variants(E, block_size):
+    # each workgroup size is its own kernel build (register budget 256/168/128)
+    _note(f"workgroup size variants block_size={block_size}")
+    rng = np.random.default_rng(12)
+    ax = axioms.empty(64)
+    ax["size"] = rng.uniform(0.5, 3.0, size=(64, 3)).astype(np.float32)
+    ax["model"][:, [3, 7, 11]] = rng.uniform(-5, 5, size=(64, 3)).astype(np.float32)
+    ax["seed"] = rng.uniform(0, 1, 64).astype(np.float32)
+    ax["entry_index"] = 0
+    ax["shape_type"] = B
+    for technique in TECHNIQUES:
+        check_against_oracle(E, _kitchen_sink(), ax, 4, technique, block_size=block_size)
+    sx, _ = axioms.stochastic_axioms(200)
+    check_against_oracle(E, grammars.stochastic(), sx, 1, abi.MEGAKERNEL, key=axioms.STOCHASTIC_KEY,
+                         block_size=block_size)
+    lots = axioms.facade_lots(48)
+    check_against_oracle(E, grammars.facade(), lots, 2, abi.MEGAKERNEL, block_size=block_size)
 
 
 def test_error_paths(E):
