@@ -46,7 +46,8 @@ def parse():
     p.add_argument("--only-kernel", action="store_true",
                    help="run the timed batched steps only (for profilers)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                   help="run only device-to-device copies of MIB MiB (scales HBM byte counters)")
+                   help="device-to-device copies of MIB MiB (scale HBM byte counters); alone, or after "
+                        "the steps of --only-kernel")
     return p.parse_args()
 
 
@@ -63,25 +64,30 @@ def make_engine(torch, engine_mod, grammars, axioms, n_lots, key=None):
     eng.set_axioms(dev_axioms)
     eng.bind(0, int(walls.sum()))
     eng.bind(1, int(windows.sum()))
-    return eng, lots, T
+    tune = eng.autotune(repeats=3)
+    return eng, lots, T, tune
 
 
-def timed_steps(torch, eng, steps):
+def timed_steps(torch, eng, steps, technique):
+    from operatorgraph_amd import abi
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        eng.execute_async()
+        if technique == abi.MEGAKERNEL:
+            eng.execute_async()
+        else:
+            eng.execute()     # per-level launches read each level's count on the host
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    eng.synchronize()     # raises on any device error of the last step
+    if technique == abi.MEGAKERNEL:
+        eng.synchronize()     # raises on any device error of the last step
     return (t1 - t0) / steps
 
 
 def kernel_seconds(torch, eng, steps):
     times = []
     for _ in range(steps):
-        eng.execute_async()
-        eng.synchronize()
+        eng.execute()
         times.append(eng.kernel_seconds())
     return float(np.mean(times)), times
 
@@ -115,6 +121,33 @@ def cpu_baseline(lots_fn, seconds, threads=1):
     return float(np.median(rates)), len(done), sum(c for c, _ in done)
 
 
+def copy_calibration(torch, mib, launches):
+    """Device-to-device copies of `mib` MiB: a kernel of known traffic to
+    scale the HBM byte counters of a profiler."""
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(launches):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": launches}), flush=True)
+
+
+def kernel_name(tune):
+    from operatorgraph_amd import abi
+    k = "k_mega" if tune["technique"] == abi.MEGAKERNEL else "k_level"
+    return f"pga::{k}<{tune['block_size']}>"
+
+
+def tune_summary(tune):
+    from operatorgraph_amd import abi
+    tech = "MEGAKERNEL (cooperative persistent launch)" if tune["technique"] == abi.MEGAKERNEL \
+        else "KERNELS (one cooperative launch per level)"
+    return {"technique": tech, "workgroup_size": tune["block_size"], "workgroups": tune["grid_blocks"],
+            "candidates": tune["candidates"], "best_ms": round(tune["seconds"] * 1e3, 4),
+            "untuned_ms": round(tune["initial_seconds"] * 1e3, 4)}
+
+
 def load_traffic(workload):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(path):
@@ -131,35 +164,31 @@ def main():
         sys.exit(2)
     import torch
 
-    if args.copy_calibration:
-        n = args.copy_calibration << 20
-        src = torch.ones(n, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(args.steps):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": args.steps}), flush=True)
+    if args.copy_calibration and not args.only_kernel:
+        copy_calibration(torch, args.copy_calibration, args.steps)
         return
 
     from operatorgraph_amd import axioms, engine, grammars
 
-    eng, lots, T = make_engine(torch, engine, grammars, axioms, args.lots)
+    eng, lots, T, tune = make_engine(torch, engine, grammars, axioms, args.lots)
     A = len(lots)
     for _ in range(args.warmup):
-        eng.execute_async()
-    eng.synchronize()
+        eng.execute()
     assert eng.stats()["terminals"] == T, (eng.stats(), T)
-    sec = timed_steps(torch, eng, args.steps)
+    sec = timed_steps(torch, eng, args.steps, tune["technique"])
     value = T / sec
     if args.only_kernel:
-        print(json.dumps({"terminals_per_step": T, "ms_per_step": sec * 1e3}), flush=True)
+        print(json.dumps({"terminals_per_step": T, "ms_per_step": sec * 1e3, "kernel": kernel_name(tune)}),
+              flush=True)
+        if args.copy_calibration:
+            copy_calibration(torch, args.copy_calibration, args.steps)
         return
     ksec, _ = kernel_seconds(torch, eng, max(5, args.steps // 2))
     alg_bytes = INSTANCE_BYTES * T + AXIOM_BYTES * A
     achieved = alg_bytes / ksec / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(f"facade_lots_{args.lots}"),
-                "kernel": "pga::k_mega", "kernel_ms": round(ksec * 1e3, 4),
+                "kernel": kernel_name(tune), "kernel_ms": round(ksec * 1e3, 4),
                 "algorithmic_bytes_per_launch": alg_bytes}
 
     # plain: one axiom per call (synchronous pga_execute_range per lot)
@@ -176,17 +205,17 @@ def main():
 
     big = None
     if args.big_lots:
-        beng, blots, BT = make_engine(torch, engine, grammars, axioms, args.big_lots)
+        beng, blots, BT, btune = make_engine(torch, engine, grammars, axioms, args.big_lots)
         for _ in range(2):
-            beng.execute_async()
-        beng.synchronize()
-        bsec = timed_steps(torch, beng, max(3, args.steps // 4))
+            beng.execute()
+        bsec = timed_steps(torch, beng, max(3, args.steps // 4), btune["technique"])
         bk, _ = kernel_seconds(torch, beng, 3)
         bbytes = INSTANCE_BYTES * BT + AXIOM_BYTES * len(blots)
         big = {"lots": len(blots), "terminals_per_step": BT, "ms_per_step": round(bsec * 1e3, 4),
                "terminals_per_s": round(BT / bsec), "kernel_ms": round(bk * 1e3, 4),
                "hbm_frac": round(bbytes / bk / 1e9 / HBM_PEAK_GBS, 4),
-               "vs_plain": round((BT / bsec) / plain_rate, 1)}
+               "vs_plain": round((BT / bsec) / plain_rate, 1), "kernel": kernel_name(btune),
+               "autotune": tune_summary(btune)}
         beng.close()
 
     cpu = None
@@ -204,7 +233,7 @@ def main():
         "config": {"workload": f"facade_lots_{args.lots}", "lots": A, "terminals_per_step": T,
                    "grammar": "Lot=CompSplit{Discard|Discard|Facade}; Facade=Repeat(Y,3); "
                               "Floor=Repeat(X,2.5); Tile=IfSizeLess(X,1.6){Wall|SubDiv(X){-1|1.2|-1}}",
-                   "technique": "MEGAKERNEL (cooperative persistent launch)"},
+                   "autotune": tune_summary(tune)},
         "roofline": roofline,
         "plain": {"lots": n_plain, "terminals": plain_T, "terminals_per_s": round(plain_rate),
                   "ms_per_lot": round(plain_sec / n_plain * 1e3, 4)},
