@@ -302,6 +302,29 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
 PGA_API pga_status pga_set_timestamps(pga_context* ctx, uint64_t* device_buffer, uint32_t capacity);
 PGA_API double pga_timestamp_hz(pga_context* ctx);
 
+/* -- linear BVH over terminal AABBs (GPUBVHConstructor.cuh:29-333): 30-bit
+ *    Morton codes of the box centres normalised to a scene box, a sort of the
+ *    unique keys (code << 32 | index), a Karras radix tree and a bottom-up
+ *    refit. Children: >= 0 a leaf (sorted position; keys[leaf] & 0xFFFFFFFF
+ *    is the box index), < 0 an internal node (negated index); node 0 is the
+ *    root. All arrays are device memory: aabbs[n][6] (min xyz, max xyz),
+ *    keys[n], left/right/parent[n-1], leaf_parent[n], node_aabb[n-1][6]. -- */
+typedef struct pga_lbvh pga_lbvh;
+PGA_API pga_status pga_lbvh_create(uint32_t capacity, int32_t device, void* stream, pga_lbvh** out);
+PGA_API void pga_lbvh_destroy(pga_lbvh* bvh);
+PGA_API const char* pga_lbvh_last_error(const pga_lbvh* bvh);
+/* box of each instance: the unit cube (planar = 0) or the unit square in the
+ * z = 0 plane (planar = 1) under the instance's model rows */
+PGA_API pga_status pga_lbvh_instance_aabbs(pga_lbvh* bvh, const pga_instance* instances, uint32_t n,
+                                           int32_t planar, float* aabbs);
+/* union of n boxes, to the host: min xyz, max xyz */
+PGA_API pga_status pga_lbvh_bounds(pga_lbvh* bvh, const float* aabbs, uint32_t n, float bounds[6]);
+PGA_API pga_status pga_lbvh_build(pga_lbvh* bvh, const float* aabbs, uint32_t n, const float scene_min[3],
+                                  const float scene_max[3], uint64_t* keys, int32_t* left, int32_t* right,
+                                  int32_t* parent, int32_t* leaf_parent, float* node_aabb);
+/* device time of the last build (Morton codes to refit) */
+PGA_API pga_status pga_lbvh_seconds(pga_lbvh* bvh, double* seconds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,480 @@
+// pga_lbvh.hip - linear BVH over terminal AABBs on gfx950 (wave64).
+//
+// The construction of GPUBVHConstructor.cuh (Karras 2012) in six steps:
+//   1. k_morton  - AABB centre normalised to the scene box, 10 bits per axis
+//                  interleaved into a 30-bit Morton code (GPUBVHConstructor.cuh:
+//                  44-86); the payload is the AABB index;
+//   2. LSD radix sort of (code, index) pairs, four 8-bit passes, each
+//                  k_hist -> scan (k_scan_reduce, k_scan_top, k_scan_down) ->
+//                  k_scatter; the scatter is stable (per-wave ranks from eight
+//                  ballots, per-workgroup digit cursors in LDS), so sorting the
+//                  codes of indices in ascending order gives the order of the
+//                  reference's unique 64-bit keys (code << 32 | index), which
+//                  thrust::sort orders (:311-312);
+//   3. k_karras  - one thread per internal node: direction, range and split
+//                  by binary search over the common-prefix length of the
+//                  64-bit keys (buildRadixTree, :88-213);
+//   4. k_refit   - one thread per leaf walks up: the first child to reach a
+//                  node stops, the second unites both boxes and goes on
+//                  (propagateBVH's ticket, :215-261). Boxes written and read
+//                  inside the kernel use agent-scope atomic loads and stores,
+//                  which are coherent across the XCDs' L2 caches without a
+//                  full write-back, and each store completes before the
+//                  ticket of the parent is taken.
+// Children are encoded as in the reference: >= 0 a leaf (sorted position),
+// < 0 an internal node (negated index); node 0 is the root.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <string>
+
+#include "../../include/pga.h"
+
+namespace bvhk {
+
+constexpr uint32_t kThreads = 256;
+constexpr uint32_t kItems = 16;
+constexpr uint32_t kTile = kThreads * kItems;   // keys per sort workgroup
+constexpr uint32_t kScanTile = 4096;            // histogram entries per scan workgroup
+
+__device__ __forceinline__ uint32_t f2u_rd(float x) {
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {     // GPUBVHConstructor.cuh:48-54
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t morton30(float x, float y, float z) {   // :56-64
+    uint32_t vx = min(f2u_rd(x * 1023.0f), 1023u);
+    uint32_t vy = min(f2u_rd(y * 1023.0f), 1023u);
+    uint32_t vz = min(f2u_rd(z * 1023.0f), 1023u);
+    return spread10(vx) | (spread10(vy) << 1) | (spread10(vz) << 2);
+}
+
+// AABB of the unit cube (or, planar, the unit square z = 0) under an
+// instance's model rows: centre = translation, half extent of row r =
+// 0.5 (|m_r0| + |m_r1| (+ |m_r2|)).
+__global__ void __launch_bounds__(256) k_instance_aabbs(const pga_instance* __restrict__ inst, uint32_t n,
+                                                        uint32_t planar, float* __restrict__ aabbs) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4* m = reinterpret_cast<const float4*>(inst + i);
+    const float4 r0 = m[0], r1 = m[1], r2 = m[2];
+    float h0 = fabsf(r0.x) + fabsf(r0.y), h1 = fabsf(r1.x) + fabsf(r1.y), h2 = fabsf(r2.x) + fabsf(r2.y);
+    if (!planar) { h0 = h0 + fabsf(r0.z); h1 = h1 + fabsf(r1.z); h2 = h2 + fabsf(r2.z); }
+    h0 = h0 * 0.5f; h1 = h1 * 0.5f; h2 = h2 * 0.5f;
+    float* o = aabbs + (size_t)i * 6u;
+    o[0] = r0.w - h0; o[1] = r1.w - h1; o[2] = r2.w - h2;
+    o[3] = r0.w + h0; o[4] = r1.w + h1; o[5] = r2.w + h2;
+}
+
+// Per-workgroup min/max of the AABB corners (scene box), then one workgroup
+// folds the partials.
+__global__ void __launch_bounds__(256) k_bounds(const float* __restrict__ aabbs, uint32_t n, float* __restrict__ part) {
+    __shared__ float red[6][256];
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float* a = aabbs + (size_t)i * 6u;
+        for (int q = 0; q < 3; ++q) { b[q] = fminf(b[q], a[q]); b[q + 3] = fmaxf(b[q + 3], a[q + 3]); }
+    }
+    for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = b[q];
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 6; ++q)
+                red[q][threadIdx.x] = q < 3 ? fminf(red[q][threadIdx.x], red[q][threadIdx.x + s])
+                                            : fmaxf(red[q][threadIdx.x], red[q][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6u + threadIdx.x] = red[threadIdx.x][0];
+}
+__global__ void __launch_bounds__(64) k_bounds_fold(float* part, uint32_t nparts) {
+    if (threadIdx.x >= 6) return;
+    const uint32_t q = threadIdx.x;
+    float v = part[q];
+    for (uint32_t b = 1; b < nparts; ++b) v = q < 3 ? fminf(v, part[b * 6u + q]) : fmaxf(v, part[b * 6u + q]);
+    part[q] = v;
+}
+
+__global__ void __launch_bounds__(256) k_morton(const float* __restrict__ aabbs, uint32_t n, float mn0, float mn1,
+                                                float mn2, float ex0, float ex1, float ex2,
+                                                uint32_t* __restrict__ code, uint32_t* __restrict__ idx) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* b = aabbs + (size_t)i * 6u;
+    const float cx = (b[0] + b[3]) * 0.5f, cy = (b[1] + b[4]) * 0.5f, cz = (b[2] + b[5]) * 0.5f;
+    code[i] = morton30((cx - mn0) / ex0, (cy - mn1) / ex1, (cz - mn2) / ex2);
+    idx[i] = i;
+}
+
+// ---- radix sort: histogram, scan, stable scatter (8-bit digits) ----
+__global__ void __launch_bounds__(kThreads) k_hist(const uint32_t* __restrict__ code, uint32_t n, uint32_t shift,
+                                                   uint32_t* __restrict__ hist, uint32_t nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kTile;
+    for (uint32_t it = 0; it < kItems; ++it) {
+        const uint32_t i = base + it * kThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(code[i] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];   // digit-major
+}
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
+    // 256 threads, 4 waves: wave inclusive scan by shuffles, then wave offsets
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if ((int)lane >= d) x += y;
+    }
+    if (lane == 63u) sh[w] = x;
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < w; ++k) off += sh[k];
+    *total = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return off + x - v;
+}
+
+__global__ void __launch_bounds__(kThreads) k_scan_reduce(const uint32_t* __restrict__ a, uint32_t n,
+                                                          uint32_t* __restrict__ part) {
+    __shared__ uint32_t sh[4];
+    const uint32_t base = blockIdx.x * kScanTile;
+    uint32_t s = 0;
+    for (uint32_t k = threadIdx.x; k < kScanTile; k += kThreads) {
+        const uint32_t i = base + k;
+        if (i < n) s += a[i];
+    }
+    uint32_t total;
+    (void)block_exclusive_scan(s, sh, &total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kThreads) k_scan_top(uint32_t* part, uint32_t nparts) {
+    __shared__ uint32_t sh[4];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nparts; base += kThreads) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nparts ? part[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, sh, &total);
+        if (i < nparts) part[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_scan_down(uint32_t* __restrict__ a, uint32_t n,
+                                                        const uint32_t* __restrict__ part) {
+    __shared__ uint32_t sh[4];
+    const uint32_t base = blockIdx.x * kScanTile;
+    uint32_t carry = part[blockIdx.x];
+    for (uint32_t k = 0; k < kScanTile; k += kThreads) {
+        const uint32_t i = base + k + threadIdx.x;
+        const uint32_t v = i < n ? a[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, sh, &total);
+        if (i < n) a[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_scatter(const uint32_t* __restrict__ code_in,
+                                                      const uint32_t* __restrict__ idx_in,
+                                                      uint32_t* __restrict__ code_out, uint32_t* __restrict__ idx_out,
+                                                      uint32_t n, uint32_t shift, const uint32_t* __restrict__ offs,
+                                                      uint32_t nblocks) {
+    __shared__ uint32_t base_d[256];
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[4][256];
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    base_d[t] = offs[(size_t)t * nblocks + blockIdx.x];
+    run[t] = 0u;
+    for (int k = 0; k < 4; ++k) wcnt[k][t] = 0u;
+    __syncthreads();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (uint32_t it = 0; it < kItems; ++it) {
+        const uint32_t i = blockIdx.x * kTile + it * kThreads + t;
+        const bool valid = i < n;
+        const uint32_t key = valid ? code_in[i] : 0u;
+        const uint32_t val = valid ? idx_in[i] : 0u;
+        const uint32_t d = (key >> shift) & 0xFFu;
+        // lanes of this wave holding the same digit
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const unsigned long long on = __ballot(valid && ((d >> b) & 1u));
+            peers &= ((d >> b) & 1u) ? on : ~on;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0u) wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = base_d[d] + run[d] + rank;
+            for (uint32_t k = 0; k < w; ++k) off += wcnt[k][d];
+            if (off < n) {            // always true when the histograms are consistent
+                code_out[off] = key;
+                idx_out[off] = val;
+            }
+        }
+        __syncthreads();
+        run[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+        for (int k = 0; k < 4; ++k) wcnt[k][t] = 0u;
+        __syncthreads();
+    }
+}
+
+// ---- Karras build ----
+__device__ __forceinline__ unsigned long long key_at(const uint32_t* code, const uint32_t* idx, int64_t i) {
+    return ((unsigned long long)code[i] << 32) | (unsigned long long)idx[i];
+}
+__device__ __forceinline__ int delta(const uint32_t* code, const uint32_t* idx, int64_t n, int64_t i, int64_t j) {
+    if (j < 0 || j >= n) return -1;
+    return __clzll((long long)(key_at(code, idx, i) ^ key_at(code, idx, j)));
+}
+
+__global__ void __launch_bounds__(256) k_karras(const uint32_t* __restrict__ code, const uint32_t* __restrict__ idx,
+                                                uint32_t n, unsigned long long* __restrict__ keys,
+                                                int32_t* __restrict__ left, int32_t* __restrict__ right,
+                                                int32_t* __restrict__ parent, int32_t* __restrict__ leaf_parent,
+                                                uint32_t* __restrict__ ticket) {
+    const int64_t nn = n;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn) keys[i] = key_at(code, idx, i);
+    if (i >= nn - 1) return;
+    ticket[i] = 0u;
+    if (i == 0) parent[0] = -1;
+    const int dir = (delta(code, idx, nn, i, i + 1) - delta(code, idx, nn, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(code, idx, nn, i, i - dir);
+    int64_t lmax = 2;
+    while (delta(code, idx, nn, i, i + lmax * dir) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (delta(code, idx, nn, i, i + (l + t) * dir) > dmin) l += t;
+    const int64_t j = i + l * dir;
+    const int dnode = delta(code, idx, nn, i, j);
+    int64_t s = 0;
+    for (int64_t t = (l + 1) / 2;; t = (t + 1) / 2) {
+        if (delta(code, idx, nn, i, i + (s + t) * dir) > dnode) s += t;
+        if (t == 1) break;
+    }
+    const int64_t gamma = i + s * dir + (dir < 0 ? -1 : 0);
+    const int32_t lc = (min(i, j) == gamma) ? (int32_t)gamma : -(int32_t)gamma;
+    const int32_t rc = (max(i, j) == gamma + 1) ? (int32_t)(gamma + 1) : -(int32_t)(gamma + 1);
+    left[i] = lc;
+    right[i] = rc;
+    if (lc < 0) parent[-lc] = (int32_t)i; else leaf_parent[lc] = (int32_t)i;
+    if (rc < 0) parent[-rc] = (int32_t)i; else leaf_parent[rc] = (int32_t)i;
+}
+
+// ---- refit ----
+__device__ __forceinline__ void store_box(float* p, const float b[6]) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        __hip_atomic_store(reinterpret_cast<uint32_t*>(p) + q, __float_as_uint(b[q]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void load_box(const float* p, float b[6]) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        b[q] = __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p) + q, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT));
+}
+
+__global__ void __launch_bounds__(256) k_refit(const float* __restrict__ aabbs, const uint32_t* __restrict__ idx,
+                                               uint32_t n, const int32_t* __restrict__ left,
+                                               const int32_t* __restrict__ right, const int32_t* __restrict__ parent,
+                                               const int32_t* __restrict__ leaf_parent, float* node_aabb,
+                                               uint32_t* ticket) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float b[6];
+    const float* src = aabbs + (size_t)idx[k] * 6u;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) b[q] = src[q];
+    int32_t child = (int32_t)k;           // leaf k (>= 0) or internal -c
+    int32_t node = leaf_parent[k];
+    // a radix tree over unique 64-bit keys is at most 65 levels deep
+    for (int guard = 0; node >= 0 && guard < 96; ++guard) {
+        // every box this thread wrote has completed before the ticket is taken
+        __builtin_amdgcn_s_waitcnt(0);
+        const uint32_t t = __hip_atomic_fetch_add(ticket + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t == 0u) return;              // the sibling's thread finishes this node
+        const int32_t l = left[node], r = right[node];
+        const int32_t sib = (l == child) ? r : l;
+        float o[6];
+        if (sib >= 0) {
+            const float* s2 = aabbs + (size_t)idx[sib] * 6u;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) o[q] = s2[q];
+        } else {
+            load_box(node_aabb + (size_t)(-sib) * 6u, o);
+        }
+        // union in child order, left box first: min(L, R) = R < L ? R : L and
+        // max(L, R) = L < R ? R : L, as std::min / std::max fold them
+        const bool mine_left = (l == child);
+        float u[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const float L = mine_left ? b[q] : o[q], R = mine_left ? o[q] : b[q];
+            u[q] = q < 3 ? (R < L ? R : L) : (L < R ? R : L);
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) b[q] = u[q];
+        store_box(node_aabb + (size_t)node * 6u, b);
+        child = -node;
+        node = node == 0 ? -1 : parent[node];
+    }
+}
+
+} // namespace bvhk
+
+// ---- host API ----
+struct pga_lbvh {
+    uint32_t capacity = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t *code[2] = {nullptr, nullptr}, *idx[2] = {nullptr, nullptr};
+    uint32_t* hist = nullptr;       // 256 x sort workgroups
+    uint32_t* part = nullptr;       // scan partials
+    uint32_t* ticket = nullptr;     // internal nodes
+    float* bounds = nullptr;        // bounds partials
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::string error;
+};
+
+namespace {
+
+using namespace bvhk;
+
+pga_status lfail(pga_lbvh* b, pga_status st, const std::string& m) {
+    if (b) b->error = m;
+    return st;
+}
+
+#define LHIP(b, expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) return lfail(b, PGA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+uint32_t sort_blocks(uint32_t n) { return (n + kTile - 1u) / kTile; }
+
+} // namespace
+
+extern "C" {
+
+PGA_API pga_status pga_lbvh_create(uint32_t capacity, int32_t device, void* stream, pga_lbvh** out) {
+    if (!out || capacity < 2u || capacity > 0x7FFFFFFFu) return PGA_ERR_INVALID_ARG;
+    pga_lbvh* b = new pga_lbvh();
+    *out = b;
+    b->capacity = capacity;
+    if (device >= 0) LHIP(b, hipSetDevice(device));
+    LHIP(b, hipGetDevice(&b->device));
+    b->stream = (hipStream_t)stream;
+    for (int k = 0; k < 2; ++k) {
+        LHIP(b, hipMalloc((void**)&b->code[k], (size_t)capacity * 4u));
+        LHIP(b, hipMalloc((void**)&b->idx[k], (size_t)capacity * 4u));
+    }
+    const size_t nh = (size_t)256 * sort_blocks(capacity);
+    LHIP(b, hipMalloc((void**)&b->hist, nh * 4u));
+    LHIP(b, hipMalloc((void**)&b->part, ((nh + kScanTile - 1) / kScanTile + 1) * 4u));
+    LHIP(b, hipMalloc((void**)&b->ticket, (size_t)capacity * 4u));
+    LHIP(b, hipMalloc((void**)&b->bounds, 1024u * 6u * 4u));
+    LHIP(b, hipEventCreate(&b->e0));
+    LHIP(b, hipEventCreate(&b->e1));
+    return PGA_OK;
+}
+
+PGA_API void pga_lbvh_destroy(pga_lbvh* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    else (void)hipDeviceSynchronize();
+    for (int k = 0; k < 2; ++k) { (void)hipFree(b->code[k]); (void)hipFree(b->idx[k]); }
+    (void)hipFree(b->hist); (void)hipFree(b->part); (void)hipFree(b->ticket); (void)hipFree(b->bounds);
+    if (b->e0) (void)hipEventDestroy(b->e0);
+    if (b->e1) (void)hipEventDestroy(b->e1);
+    delete b;
+}
+
+PGA_API const char* pga_lbvh_last_error(const pga_lbvh* b) { return b ? b->error.c_str() : ""; }
+
+PGA_API pga_status pga_lbvh_instance_aabbs(pga_lbvh* b, const pga_instance* instances, uint32_t n, int32_t planar,
+                                           float* aabbs) {
+    if (!b || (n && (!instances || !aabbs))) return lfail(b, PGA_ERR_INVALID_ARG, "null buffer");
+    if (n == 0) return PGA_OK;
+    hipLaunchKernelGGL(k_instance_aabbs, dim3((n + 255u) / 256u), dim3(256), 0, b->stream, instances, n,
+                       planar ? 1u : 0u, aabbs);
+    LHIP(b, hipGetLastError());
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_lbvh_bounds(pga_lbvh* b, const float* aabbs, uint32_t n, float bounds[6]) {
+    if (!b || !aabbs || !bounds || n == 0) return lfail(b, PGA_ERR_INVALID_ARG, "bad bounds arguments");
+    const uint32_t blocks = std::min<uint32_t>(1024u, (n + 255u) / 256u);
+    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, b->stream, aabbs, n, b->bounds);
+    hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(64), 0, b->stream, b->bounds, blocks);
+    LHIP(b, hipGetLastError());
+    LHIP(b, hipMemcpyAsync(bounds, b->bounds, 6u * 4u, hipMemcpyDeviceToHost, b->stream));
+    LHIP(b, hipStreamSynchronize(b->stream));
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_lbvh_build(pga_lbvh* b, const float* aabbs, uint32_t n, const float scene_min[3],
+                                  const float scene_max[3], uint64_t* keys, int32_t* left, int32_t* right,
+                                  int32_t* parent, int32_t* leaf_parent, float* node_aabb) {
+    if (!b || !aabbs || !scene_min || !scene_max || !keys || !left || !right || !parent || !leaf_parent || !node_aabb)
+        return lfail(b, PGA_ERR_INVALID_ARG, "null argument");
+    if (n < 2u || n > b->capacity)
+        return lfail(b, PGA_ERR_INVALID_ARG, "need 2 <= n <= capacity (" + std::to_string(b->capacity) + ")");
+    const float ex0 = scene_max[0] - scene_min[0], ex1 = scene_max[1] - scene_min[1], ex2 = scene_max[2] - scene_min[2];
+    if (!(ex0 > 0.0f) || !(ex1 > 0.0f) || !(ex2 > 0.0f)) return lfail(b, PGA_ERR_INVALID_ARG, "empty scene box");
+    hipStream_t s = b->stream;
+    LHIP(b, hipEventRecord(b->e0, s));
+    const uint32_t g = (n + 255u) / 256u;
+    hipLaunchKernelGGL(k_morton, dim3(g), dim3(256), 0, s, aabbs, n, scene_min[0], scene_min[1], scene_min[2],
+                       ex0, ex1, ex2, b->code[0], b->idx[0]);
+    const uint32_t nb = sort_blocks(n);
+    const uint32_t nh = 256u * nb;
+    const uint32_t np = (nh + kScanTile - 1u) / kScanTile;
+    int cur = 0;
+    for (uint32_t pass = 0; pass < 4; ++pass) {     // 30-bit codes: digits at bits 0, 8, 16, 24
+        const uint32_t shift = 8u * pass;
+        hipLaunchKernelGGL(k_hist, dim3(nb), dim3(kThreads), 0, s, b->code[cur], n, shift, b->hist, nb);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(np), dim3(kThreads), 0, s, b->hist, nh, b->part);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kThreads), 0, s, b->part, np);
+        hipLaunchKernelGGL(k_scan_down, dim3(np), dim3(kThreads), 0, s, b->hist, nh, b->part);
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kThreads), 0, s, b->code[cur], b->idx[cur], b->code[cur ^ 1],
+                           b->idx[cur ^ 1], n, shift, b->hist, nb);
+        cur ^= 1;
+    }
+    hipLaunchKernelGGL(k_karras, dim3(g), dim3(256), 0, s, b->code[cur], b->idx[cur], n,
+                       reinterpret_cast<unsigned long long*>(keys), left, right, parent, leaf_parent, b->ticket);
+    hipLaunchKernelGGL(k_refit, dim3(g), dim3(256), 0, s, aabbs, b->idx[cur], n, left, right, parent, leaf_parent,
+                       node_aabb, b->ticket);
+    LHIP(b, hipGetLastError());
+    LHIP(b, hipEventRecord(b->e1, s));
+    LHIP(b, hipStreamSynchronize(s));
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_lbvh_seconds(pga_lbvh* b, double* seconds) {
+    if (!b || !seconds) return PGA_ERR_INVALID_ARG;
+    float ms = 0.0f;
+    LHIP(b, hipEventElapsedTime(&ms, b->e0, b->e1));
+    *seconds = ms * 1e-3;
+    return PGA_OK;
+}
+
+} // extern "C"

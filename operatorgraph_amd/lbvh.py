@@ -1,0 +1,105 @@
+"""Linear BVH over terminal boxes on the GPU (pga_lbvh_* in include/pga.h).
+
+The reference builds it in GPUBVHConstructor.cuh:29-333 (Morton codes,
+thrust::sort, Karras radix tree, ticket refit); this wrapper drives the
+gfx950 kernels of csrc/pga_lbvh.hip on torch device tensors."""
+import ctypes as C
+
+from . import abi
+from .engine import PGAError, load_library
+
+_SIGS = {
+    "pga_lbvh_create": (C.c_int, [C.c_uint32, C.c_int32, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "pga_lbvh_destroy": (None, [C.c_void_p]),
+    "pga_lbvh_last_error": (C.c_char_p, [C.c_void_p]),
+    "pga_lbvh_instance_aabbs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p]),
+    "pga_lbvh_bounds": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_float * 6)]),
+    "pga_lbvh_build": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_float * 3),
+                                 C.POINTER(C.c_float * 3)] + [C.c_void_p] * 6),
+    "pga_lbvh_seconds": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+}
+
+
+def _lib():
+    L = load_library()
+    if not getattr(L, "_lbvh_bound", False):
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        L._lbvh_bound = True
+    return L
+
+
+class LBVH:
+    """Builder for up to `capacity` boxes; scratch lives on the device."""
+
+    def __init__(self, capacity, device=-1, stream=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise PGAError(abi.ERR_HIP, "no GPU visible: the LBVH has no CPU path")
+        self.torch = torch
+        self.L = _lib()
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        h = C.c_void_p()
+        st = self.L.pga_lbvh_create(int(capacity), device, C.c_void_p(stream), C.byref(h))
+        self._h = h.value
+        self._check(st)
+        self.capacity = int(capacity)
+
+    def _check(self, st):
+        if st != abi.OK:
+            msg = self.L.pga_lbvh_last_error(self._h).decode() if self._h else ""
+            raise PGAError(st, msg)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.pga_lbvh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def instance_aabbs(self, instances, planar):
+        """Boxes of a (n, 16) float32 instance tensor (64-byte records)."""
+        torch = self.torch
+        n = instances.shape[0]
+        out = torch.empty((n, 6), dtype=torch.float32, device=instances.device)
+        self._check(self.L.pga_lbvh_instance_aabbs(self._h, C.c_void_p(instances.data_ptr()), n,
+                                                   1 if planar else 0, C.c_void_p(out.data_ptr())))
+        return out
+
+    def bounds(self, aabbs):
+        out = (C.c_float * 6)()
+        self._check(self.L.pga_lbvh_bounds(self._h, C.c_void_p(aabbs.data_ptr()), aabbs.shape[0], C.byref(out)))
+        return list(out)
+
+    def build(self, aabbs, scene_min, scene_max):
+        """Returns a dict of device tensors: keys (uint64 as int64), left, right,
+        parent (n-1), leaf_parent (n), node_aabb (n-1, 6)."""
+        torch = self.torch
+        n = aabbs.shape[0]
+        dev = aabbs.device
+        keys = torch.empty(n, dtype=torch.int64, device=dev)
+        left = torch.empty(max(n - 1, 1), dtype=torch.int32, device=dev)
+        right = torch.empty_like(left)
+        parent = torch.empty_like(left)
+        leaf_parent = torch.empty(n, dtype=torch.int32, device=dev)
+        node_aabb = torch.empty((max(n - 1, 1), 6), dtype=torch.float32, device=dev)
+        smin = (C.c_float * 3)(*[float(v) for v in scene_min])
+        smax = (C.c_float * 3)(*[float(v) for v in scene_max])
+        ptr = lambda t: C.c_void_p(t.data_ptr())
+        self._check(self.L.pga_lbvh_build(self._h, ptr(aabbs.contiguous()), n, C.byref(smin), C.byref(smax),
+                                          ptr(keys), ptr(left), ptr(right), ptr(parent), ptr(leaf_parent),
+                                          ptr(node_aabb)))
+        return dict(keys=keys, left=left, right=right, parent=parent, leaf_parent=leaf_parent,
+                    node_aabb=node_aabb)
+
+    def seconds(self):
+        s = C.c_double(0.0)
+        self._check(self.L.pga_lbvh_seconds(self._h, C.byref(s)))
+        return s.value
