@@ -41,6 +41,7 @@ def parse():
     p.add_argument("--lots", type=int, default=1024)
     p.add_argument("--big-lots", type=int, default=16384)
     p.add_argument("--plain-sample", type=int, default=256)
+    p.add_argument("--stochastic-axioms", type=int, default=65536)
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--only-kernel", action="store_true",
@@ -133,6 +134,48 @@ def copy_calibration(torch, mib, launches):
     print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": launches}), flush=True)
 
 
+LBVH_BYTES_PER_LEAF = 204   # 24 B box read + 8 B key write + 4 sort passes x 16 B + Karras 24 + 12 + refit 72
+
+
+def stochastic_workload(torch, engine, grammars, axioms, n, steps):
+    """65,536-axiom stochastic batch (per-shape Philox rule choice and split
+    ratios, lognormal terminal counts), autotuned, then the LBVH over all
+    terminal boxes."""
+    from operatorgraph_amd.lbvh import LBVH
+    ax, _ = axioms.stochastic_axioms(n)
+    eng = engine.Engine(queue_capacity=1 << 23)
+    eng.load(grammars.stochastic()).set_axioms(ax)
+    eng.set_philox_key(axioms.STOCHASTIC_KEY)
+    eng.bind(0, 0)                           # count-only run sizes the buffer
+    eng.execute()
+    T = int(eng.counts(1)[0])
+    buf = eng.bind(0, T)
+    tune = eng.autotune(repeats=3)
+    eng.execute()
+    sec = timed_steps(torch, eng, steps, tune["technique"])
+    ksec, _ = kernel_seconds(torch, eng, 3)
+    lb = LBVH(T)
+    boxes = lb.instance_aabbs(buf[:T], planar=False)
+    bounds = lb.bounds(boxes)
+    smin, smax = [v - 1.0 for v in bounds[:3]], [v + 1.0 for v in bounds[3:]]
+    lb.build(boxes, smin, smax)
+    times = []
+    for _ in range(max(3, steps)):
+        lb.build(boxes, smin, smax)
+        times.append(lb.seconds())
+    lsec = float(np.median(times))
+    out = {"axioms": n, "terminals_per_step": T, "ms_per_step": round(sec * 1e3, 4),
+           "terminals_per_s": round(T / sec), "kernel_ms": round(ksec * 1e3, 4),
+           "hbm_frac": round((INSTANCE_BYTES * T + AXIOM_BYTES * n) / ksec / 1e9 / HBM_PEAK_GBS, 4),
+           "kernel": kernel_name(tune), "autotune": tune_summary(tune),
+           "lbvh": {"leaves": T, "ms": round(lsec * 1e3, 4), "leaves_per_s": round(T / lsec),
+                    "algorithmic_bytes": LBVH_BYTES_PER_LEAF * T,
+                    "hbm_frac": round(LBVH_BYTES_PER_LEAF * T / lsec / 1e9 / HBM_PEAK_GBS, 4)}}
+    lb.close()
+    eng.close()
+    return out
+
+
 def kernel_name(tune):
     from operatorgraph_amd import abi
     k = "k_mega" if tune["technique"] == abi.MEGAKERNEL else "k_level"
@@ -218,6 +261,11 @@ def main():
                "autotune": tune_summary(btune)}
         beng.close()
 
+    stoch = None
+    if args.stochastic_axioms:
+        stoch = stochastic_workload(torch, engine, grammars, axioms, args.stochastic_axioms,
+                                    max(3, args.steps // 4))
+
     cpu = None
     if not args.no_cpu:
         rate, runs, total = cpu_baseline(lambda: axioms.facade_lots(args.lots), args.cpu_seconds, 1)
@@ -239,6 +287,7 @@ def main():
                   "ms_per_lot": round(plain_sec / n_plain * 1e3, 4)},
         "batched_vs_plain": round(value / plain_rate, 1),
         "batched_16384": big,
+        "stochastic_65536_lbvh": stoch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
