@@ -302,6 +302,21 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
 PGA_API pga_status pga_set_timestamps(pga_context* ctx, uint64_t* device_buffer, uint32_t capacity);
 PGA_API double pga_timestamp_hz(pga_context* ctx);
 
+/* -- reverse mode (no counterpart in the reference: the derivation tape and
+ *    adjoint of a differentiable build). pga_tape_reserve sizes the tape:
+ *    parent records of all levels, and children of all levels counting the
+ *    axioms. pga_forward_tape derives the bound axioms [0, n) recording the
+ *    tape (every fan-out materialised, the topology frozen) into the bound
+ *    instance buffers; pga_backward takes a host array of PGA_MAX_TERMINALS
+ *    device pointers (NULL: no gradient) to dL/dinstance in the instance
+ *    layout (16 floats per instance, the 12 model entries are used) and
+ *    writes dL/dparams (device, n x params_per_axiom floats) for the
+ *    PGA_PT_AXIOM_PARAM parameters. -- */
+PGA_API pga_status pga_tape_reserve(pga_context* ctx, uint32_t max_records, uint32_t max_children);
+PGA_API pga_status pga_forward_tape(pga_context* ctx, uint32_t num_axioms, double* seconds);
+PGA_API pga_status pga_backward(pga_context* ctx, const float* const* dl_dinstances, float* dl_dparams,
+                                double* seconds);
+
 /* -- linear BVH over terminal AABBs (GPUBVHConstructor.cuh:29-333): 30-bit
  *    Morton codes of the box centres normalised to a scene box, a sort of the
  *    unique keys (code << 32 | index), a Karras radix tree and a bottom-up

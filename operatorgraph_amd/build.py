@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpga_amd.so")
-SOURCES = ["pga_kernels.hip", "pga_lbvh.hip", "pga_api.cpp"]
+SOURCES = ["pga_kernels.hip", "pga_adjoint.hip", "pga_lbvh.hip", "pga_api.cpp"]
 HEADERS = ["pga_device.h", "pga_ops.h", "pga_kernels.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -83,6 +83,19 @@ struct pga_context {
     uint32_t mega_blocks = 0;
     uint64_t max_level_parents = 0, max_level_children = 0;   // of the last execute
 
+    // derivation tape (reverse mode)
+    pga::Rec* d_tape_par = nullptr;
+    pga::Desc* d_tape_desc = nullptr;
+    uint4* d_tape_out = nullptr;
+    float* d_adj_slots = nullptr;     // per child: the adjoint it leaves for its parent
+    const float** d_dl = nullptr;     // [PGA_MAX_TERMINALS] device table of dL/dinstance buffers
+    double* d_grad64 = nullptr;       // FP64 accumulators of dL/dparams
+    size_t grad64_cap = 0;
+    uint32_t tape_par_cap = 0, tape_child_cap = 0, tape_blocks = 0;
+    bool tape_valid = false;
+    uint32_t tape_axioms = 0;
+    std::vector<uint32_t> tape_pc, tape_dc;   // parents / descriptors written by each level
+
     pga_stats stats{};
 };
 
@@ -615,6 +628,8 @@ PGA_API void pga_destroy(pga_context* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     hfree(ctx->d_image);
+    hfree(ctx->d_tape_par); hfree(ctx->d_tape_desc); hfree(ctx->d_tape_out); hfree(ctx->d_adj_slots);
+    hfree(ctx->d_dl); hfree(ctx->d_grad64);
     hfree(ctx->d_axioms_owned); hfree(ctx->d_aparams);
     hfree(ctx->d_inst_ptr); hfree(ctx->d_inst_cap);
     for (int b = 0; b < 2; ++b) { hfree(ctx->d_par[b]); hfree(ctx->d_desc[b]); }
@@ -915,6 +930,139 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
     st = pga_execute_range(ctx, 0, num_axioms, 0, nullptr);   // the buffers hold the chosen run
     *out = res;
     return st;
+}
+
+// ---- reverse mode ----
+
+PGA_API pga_status pga_tape_reserve(pga_context* ctx, uint32_t max_records, uint32_t max_children) {
+    if (!ctx) return PGA_ERR_INVALID_ARG;
+    if (max_records == 0 || max_children == 0) return fail(ctx, PGA_ERR_INVALID_ARG, "empty tape");
+    if (ctx->stream) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    hfree(ctx->d_tape_par); hfree(ctx->d_tape_desc); hfree(ctx->d_tape_out); hfree(ctx->d_adj_slots);
+    ctx->tape_valid = false;
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_tape_par, (size_t)max_records * sizeof(pga::Rec)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_tape_desc, (size_t)max_children * sizeof(pga::Desc)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_tape_out, (size_t)max_children * sizeof(uint4)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_adj_slots, (size_t)max_children * pga::kSlotFloats * sizeof(float)));
+    if (!ctx->d_dl) HIP_TRY(ctx, hipMalloc((void**)&ctx->d_dl, PGA_MAX_TERMINALS * sizeof(float*)));
+    ctx->tape_par_cap = max_records;
+    ctx->tape_child_cap = max_children;
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_forward_tape(pga_context* ctx, uint32_t num_axioms, double* seconds) {
+    pga_status st = check_launch(ctx, 0, num_axioms);
+    if (st) return st;
+    if (!ctx->d_tape_par) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no tape: call pga_tape_reserve first");
+    if (num_axioms > ctx->tape_child_cap) return fail(ctx, PGA_ERR_OVERFLOW_QUEUE, "tape smaller than the axioms");
+    if (ctx->tape_blocks == 0) {
+        int per_cu = 0;
+        HIP_TRY(ctx, pga_tape_occupancy(ctx->lds_bytes, &per_cu));
+        if (per_cu < 1) return fail(ctx, PGA_ERR_UNSUPPORTED, "tape kernel does not fit on a compute unit");
+        ctx->tape_blocks = std::min((uint32_t)per_cu, kMaxBlocksPerCu) * (uint32_t)ctx->num_cus;
+    }
+    ctx->tape_valid = false;
+    pga::KArgs a = make_args(ctx, 0, num_axioms);
+    a.tape = 1;
+    a.tape_par = ctx->d_tape_par;
+    a.tape_desc = ctx->d_tape_desc;
+    a.tape_out = ctx->d_tape_out;
+    a.tape_out_cap = ctx->tape_child_cap;
+    a.cap_par = ctx->tape_par_cap;
+    a.cap_desc = ctx->tape_child_cap - num_axioms;   // children after the axioms
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, pga_launch_reset(a, 1, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->evk0, ctx->stream));
+    if (num_axioms > 0) HIP_TRY(ctx, pga_launch_mega_tape(a, ctx->tape_blocks, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->evk1, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.seconds = ms * 1e-3;
+    if (seconds) *seconds = ms * 1e-3;
+    // per-level counts of the tape (the stats' queue check uses the ping-pong
+    // capacity, the tape its own)
+    uint32_t err = 0;
+    HIP_TRY(ctx, hipMemcpy(&err, ctx->d_err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> lc(ctx->cfg.max_levels + 2), pc(ctx->cfg.max_levels + 2);
+    HIP_TRY(ctx, hipMemcpy(lc.data(), ctx->d_desc_counts, lc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(pc.data(), ctx->d_par_counts, pc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    unsigned long long tc[PGA_MAX_TERMINALS];
+    HIP_TRY(ctx, hipMemcpy(tc, ctx->d_term_counts, sizeof(tc), hipMemcpyDeviceToHost));
+    const uint32_t levels = num_axioms ? lc[ctx->cfg.max_levels + 1] : 0u;
+    ctx->stats.levels = levels;
+    ctx->stats.records = 0;
+    ctx->stats.terminals = 0;
+    for (int t = 0; t < PGA_MAX_TERMINALS; ++t) ctx->stats.terminals += tc[t];
+    ctx->tape_pc.assign(pc.begin(), pc.begin() + levels);
+    ctx->tape_dc.assign(lc.begin(), lc.begin() + levels);
+    uint64_t sp = 0, sd = 0;
+    for (uint32_t l = 0; l < levels; ++l) { sp += pc[l]; sd += lc[l]; }
+    ctx->stats.records = sp;
+    if (sp > ctx->tape_par_cap || (uint64_t)num_axioms + sd > ctx->tape_child_cap) err |= 1u << PGA_ERR_OVERFLOW_QUEUE;
+    st = status_from_err(ctx, err);
+    if (st == PGA_OK) {
+        ctx->tape_valid = true;
+        ctx->tape_axioms = num_axioms;
+    }
+    return st;
+}
+
+PGA_API pga_status pga_backward(pga_context* ctx, const float* const* dl_dinstances, float* dl_dparams,
+                                double* seconds) {
+    if (!ctx || !dl_dinstances || !dl_dparams) return fail(ctx, PGA_ERR_INVALID_ARG, "null argument");
+    if (!ctx->tape_valid) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no recorded tape: run pga_forward_tape");
+    if (ctx->aparams_per == 0) return fail(ctx, PGA_ERR_INVALID_ARG, "no per-axiom parameters bound");
+    HIP_TRY(ctx, hipMemcpyAsync((void*)ctx->d_dl, (const void*)dl_dinstances, PGA_MAX_TERMINALS * sizeof(float*),
+                                hipMemcpyHostToDevice, ctx->stream));
+    const uint32_t n = ctx->tape_axioms;
+    pga::KArgs a = make_args(ctx, 0, n);
+    a.tape = 1;
+    a.tape_par = ctx->d_tape_par;
+    a.tape_desc = ctx->d_tape_desc;
+    a.tape_out = ctx->d_tape_out;
+    a.tape_out_cap = ctx->tape_child_cap;
+    a.cap_par = ctx->tape_par_cap;
+    a.cap_desc = ctx->tape_child_cap - n;
+    const uint32_t levels = (uint32_t)ctx->tape_dc.size();
+    std::vector<uint32_t> pbase(levels + 1, 0), dbase(levels + 1, 0);
+    for (uint32_t l = 0; l < levels; ++l) {
+        pbase[l + 1] = pbase[l] + ctx->tape_pc[l];
+        dbase[l + 1] = dbase[l] + ctx->tape_dc[l];
+    }
+    const size_t ng = (size_t)n * ctx->aparams_per;
+    if (ng > ctx->grad64_cap) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        hfree(ctx->d_grad64);
+        HIP_TRY(ctx, hipMalloc((void**)&ctx->d_grad64, ng * sizeof(double)));
+        ctx->grad64_cap = ng;
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->d_grad64, 0, ng * sizeof(double), ctx->stream));
+    for (uint32_t L = levels; L-- > 0;) {
+        pga::AdjArgs g{};
+        g.dl = ctx->d_dl;
+        g.slots = ctx->d_adj_slots;
+        g.grad = ctx->d_grad64;
+        g.level = L;
+        g.n_children = L == 0 ? n : ctx->tape_dc[L - 1];
+        g.child_base = L == 0 ? 0u : n + dbase[L - 1];
+        g.next_child_base = n + dbase[L];
+        g.par_in_base = L == 0 ? 0u : pbase[L - 1];
+        g.desc_in_base = L == 0 ? 0u : dbase[L - 1];
+        g.slots_cap = ctx->tape_child_cap;
+        HIP_TRY(ctx, pga_launch_adjoint(a, g, ctx->stream));
+    }
+    HIP_TRY(ctx, pga_launch_grad_out(ctx->d_grad64, dl_dparams, (uint32_t)ng, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (seconds) *seconds = ms * 1e-3;
+    uint32_t err = 0;
+    HIP_TRY(ctx, hipMemcpy(&err, ctx->d_err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return status_from_err(ctx, err);
 }
 
 } // extern "C"

@@ -114,6 +114,7 @@ struct Ctx {
     uint32_t max_chain;
     uint32_t geom;                      // 0: skip model-matrix arithmetic (counting only)
     uint32_t seed;                      // 0: no parameter reads a seed, the seed chain is skipped
+    uint32_t tape;                      // 1: recording a tape (no in-place leaf expansion)
 };
 
 __device__ __forceinline__ void raise(const Ctx& c, int status) {

@@ -43,7 +43,35 @@ struct KArgs {
     uint32_t max_chain;
     uint32_t count_geom;                // 1: some decision reads the model matrix
     uint32_t use_seed;                  // 1: some parameter reads a shape seed
+    // derivation tape (reverse mode): every level's parent records and
+    // descriptors kept (cap_par / cap_desc = their capacities), every
+    // fan-out materialised, and one outcome per child
+    uint32_t tape;
+    Rec* tape_par;
+    Desc* tape_desc;
+    uint4* tape_out;                    // per child: {kind | slot << 8, index, dslot, nd}
+    uint32_t tape_out_cap;
 };
+
+// Outcome of a child in the tape.
+enum TapeKind : uint32_t { T_DROP = 0, T_TERM = 1, T_FAN = 2, T_SELF = 3 };
+
+// One level of the reverse pass.
+struct AdjArgs {
+    const float* const* dl;       // [PGA_MAX_TERMINALS] dL/dinstance (16 floats per instance) or null
+    float* slots;                 // per child: {dparent.m[12], dparent.s[3], d offset, d size, 0, 0, 0}
+    double* grad;                 // [axioms][params per axiom], FP64 accumulators
+    uint32_t level;
+    uint32_t n_children;          // children of this level
+    uint32_t child_base;          // first child of this level in the child numbering
+    uint32_t next_child_base;     // first child of the next level
+    uint32_t par_in_base;         // parent records read by this level (level - 1 outputs)
+    uint32_t desc_in_base;        // descriptors read by this level
+    uint32_t slots_cap;           // children in the slot array
+};
+
+constexpr uint32_t kTapeBlock = 256;    // workgroup size of the tape-recording build
+constexpr uint32_t kSlotFloats = 20;    // adjoint slot of a child (80 bytes)
 
 } // namespace pga
 
@@ -53,3 +81,7 @@ hipError_t pga_launch_level(const pga::KArgs& a, uint32_t level, uint32_t blocks
                             hipStream_t s);
 hipError_t pga_mega_occupancy(uint32_t block_size, uint32_t lds_bytes, int* blocks_per_cu);
 hipError_t pga_launch_mega(const pga::KArgs& a, uint32_t blocks, uint32_t block_size, hipStream_t s);
+hipError_t pga_tape_occupancy(uint32_t lds_bytes, int* blocks_per_cu);
+hipError_t pga_launch_mega_tape(const pga::KArgs& a, uint32_t blocks, hipStream_t s);
+hipError_t pga_launch_adjoint(const pga::KArgs& a, const pga::AdjArgs& g, hipStream_t s);
+hipError_t pga_launch_grad_out(const double* acc, float* out, uint32_t n, hipStream_t s);

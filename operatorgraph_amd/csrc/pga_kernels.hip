@@ -43,6 +43,8 @@ struct LevelLDS {
     uint32_t level_total[kCounters];   // totals of the level over the grid
     unsigned long long term_base[PGA_MAX_TERMINALS];  // running instance count per slot
     uint32_t next_n;                   // children of the next level (all blocks agree)
+    uint32_t tp_prev, tp_cur;          // tape: parent-record base of the previous / this level
+    uint32_t td_prev, td_cur;          // tape: descriptor base of the previous / this level
 };
 
 enum Mode : uint32_t { M_DONE = 0, M_SETTLE = 1, M_FAN = 2 };
@@ -94,9 +96,9 @@ __device__ __forceinline__ uint32_t claim(LevelLDS& L, bool wr, uint32_t k, unsi
 // 155-189): instance = model4 * scale(size), custom attribute. A full buffer
 // is an error (the reference wraps modulo capacity); capacity 0 counts only.
 __device__ __forceinline__ void emit_terminal(const Ctx& c, LevelLDS& L, bool wr, bool has, uint32_t slot,
-                                              const Rec& r) {
+                                              const Rec& r, unsigned long long& idx) {
     unsigned long long pending = __ballot(has);
-    unsigned long long idx = 0;
+    idx = 0;
     while (pending) {                          // one claim per distinct slot in the wave
         const int leader = __ffsll((long long)pending) - 1;
         const uint32_t sl = __shfl(slot, leader);
@@ -188,7 +190,8 @@ __device__ __forceinline__ void stage_program(const KArgs& a) {
     __syncthreads();
 }
 
-__device__ __forceinline__ Ctx make_ctx(const KArgs& a, uint32_t level, bool wr) {
+template <bool TAPE>
+__device__ __forceinline__ Ctx make_ctx(const KArgs& a, const LevelLDS& L, uint32_t level, bool wr) {
     Ctx c;
     c.prog.entries = reinterpret_cast<const DevEntry*>(pga_smem);
     c.prog.params = reinterpret_cast<const DevParam*>(pga_smem + a.off_params);
@@ -212,6 +215,13 @@ __device__ __forceinline__ Ctx make_ctx(const KArgs& a, uint32_t level, bool wr)
     c.desc_count = a.desc_counts + level;
     c.par_cap = a.cap_par;
     c.desc_cap = a.cap_desc;
+    c.tape = TAPE ? 1u : 0u;
+    if (TAPE) {        // this level's outputs follow the earlier levels' in the tape
+        c.par_out = a.tape_par + L.tp_cur;
+        c.desc_out = a.tape_desc + L.td_cur;
+        c.par_cap = a.cap_par > L.tp_cur ? a.cap_par - L.tp_cur : 0u;
+        c.desc_cap = a.cap_desc > L.td_cur ? a.cap_desc - L.td_cur : 0u;
+    }
     c.err = a.err;
     c.max_chain = a.max_chain;
     c.geom = wr ? 1u : a.count_geom;
@@ -246,8 +256,8 @@ __device__ __forceinline__ Rec par_load(const float4* PW) {
 
 // Child i of the level: an axiom (level 0) or the child that a descriptor
 // of the previous level names. Returns false when there is none.
-__device__ __forceinline__ bool load_child(const KArgs& a, const Ctx& c, bool wr, uint32_t level, uint32_t i,
-                                           Rec& r) {
+__device__ __forceinline__ bool load_child(const KArgs& a, const Ctx& c, const Rec* par_in, const Desc* desc_in,
+                                           bool wr, uint32_t level, uint32_t i, Rec& r) {
     if (level == 0u) {
         const pga_axiom& ax = a.axioms[a.first_axiom + i];
 #pragma unroll
@@ -267,8 +277,6 @@ __device__ __forceinline__ bool load_child(const KArgs& a, const Ctx& c, bool wr
         r.ed = (uint32_t)ent;
         return true;
     }
-    const Rec* par_in = (level & 1u) ? a.par0 : a.par1;      // written by level - 1
-    const Desc* desc_in = (level & 1u) ? a.desc0 : a.desc1;
     const uint4 dv = *reinterpret_cast<const uint4*>(desc_in + i);
     const uint32_t parent = dv.x, jx = dv.y;
     if (parent >= a.cap_par) return false;       // parent was dropped on overflow
@@ -283,13 +291,19 @@ __device__ __forceinline__ bool load_child(const KArgs& a, const Ctx& c, bool wr
 
 // One pass over the children of a level; waves take 64 consecutive
 // children at a time, in a fixed order, so both passes see the same lanes.
+template <bool TAPE>
 __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS& L, float4* PW, bool wr,
                                          uint32_t level, uint32_t n, uint32_t wave, uint32_t nwaves) {
+    // inputs: written by level - 1 (ping-pong buffers, or that level's part of the tape)
+    const Rec* par_in = TAPE ? a.tape_par + L.tp_prev : ((level & 1u) ? a.par0 : a.par1);
+    const Desc* desc_in = TAPE ? a.tape_desc + L.td_prev : ((level & 1u) ? a.desc0 : a.desc1);
+    const uint32_t child_base = level == 0u ? 0u : a.n_axioms + L.td_prev;   // tape outcomes
 #pragma unroll 1
     for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u) {
         const uint32_t i = base + lane_id();
         Rec cur;
-        const bool valid = i < n && load_child(a, c, wr, level, i, cur);
+        const bool valid = i < n && load_child(a, c, par_in, desc_in, wr, level, i, cur);
+        uint32_t out_kind = T_DROP, out_a = 0, out_b = 0, out_c = 0;   // tape outcome
         uint32_t mode = valid ? M_SETTLE : M_DONE;
         bool leaf = false, self = false;
         uint32_t steps = 0, j = 0, dslot = 0, pidx = ~0u;
@@ -354,7 +368,7 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
                         par_store(PW, cur);
                         fan_init(c, cur, e, f);
                         j = 0;
-                        leaf = (e.flags() & EF_LEAF) != 0;
+                        leaf = !TAPE && (e.flags() & EF_LEAF) != 0;
                         start = !leaf;
                         const uint32_t sk = entry_aux(c, e, 0);
                         if (k == K_REPEAT) {
@@ -364,7 +378,7 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
                             skipm = sk;
                             nd = f.n - (uint32_t)__popc(skipm & (f.n < 32u ? (1u << f.n) - 1u : 0xFFFFFFFFu));
                         }
-                        if (!wr && (e.flags() & EF_DIRECT)) {
+                        if (!wr && leaf && (e.flags() & EF_DIRECT)) {
                             count_direct(c, L, e, f);
                             mode = M_DONE;
                         } else {
@@ -394,8 +408,21 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
                     ++steps;
                 }
             }
-            if (__ballot(emit) != 0ull) emit_terminal(c, L, wr, emit, slot, cur);
-            if (__ballot(start) != 0ull) start_fanout(c, L, wr, start, par_load(PW), nd, pidx, dslot);
+            if (__ballot(emit) != 0ull) {
+                unsigned long long idx;
+                emit_terminal(c, L, wr, emit, slot, cur, idx);
+                if (TAPE && emit) { out_kind = T_TERM | (slot << 8); out_a = (uint32_t)idx; }
+            }
+            if (__ballot(start) != 0ull) {
+                start_fanout(c, L, wr, start, par_load(PW), nd, pidx, dslot);
+                if (TAPE && start) { out_kind = self ? T_SELF : T_FAN; out_a = pidx; out_b = dslot; out_c = nd; }
+            }
+        }
+        if (TAPE && wr && i < n) {
+            if (child_base + i < a.tape_out_cap)
+                a.tape_out[child_base + i] = make_uint4(out_kind, out_a, out_b, out_c);
+            else
+                raise(c, PGA_ERR_OVERFLOW_QUEUE);
         }
     }
 }
@@ -490,6 +517,7 @@ __device__ __forceinline__ void scan_counts(const KArgs& a, LevelLDS& L, uint32_
 // One derivation level: count pass, scan, write pass. Leaves L.next_n =
 // children of the next level; block 0 records the level's parent and
 // descriptor counts.
+template <bool TAPE>
 __device__ __forceinline__ void run_level(const KArgs& a, LevelLDS& L, float4* PW, uint32_t level, uint32_t n) {
     const uint32_t nslots = a.prog.nslots, ncnt = nslots + 2u;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -500,8 +528,8 @@ __device__ __forceinline__ void run_level(const KArgs& a, LevelLDS& L, float4* P
 #pragma unroll 1
     for (uint32_t pass = 0; pass < 2u; ++pass) {
         const bool wr = pass != 0u;
-        const Ctx c = make_ctx(a, level, wr);
-        run_pass(a, c, L, PW, wr, level, n, wave, nwaves);
+        const Ctx c = make_ctx<TAPE>(a, L, level, wr);
+        run_pass<TAPE>(a, c, L, PW, wr, level, n, wave, nwaves);
         __syncthreads();
         if (!wr) scan_counts(a, L, level);
     }
@@ -514,9 +542,11 @@ __device__ __forceinline__ void run_level(const KArgs& a, LevelLDS& L, float4* P
 }
 
 __device__ __forceinline__ void init_bases(const KArgs& a, LevelLDS& L) {
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         for (uint32_t k = 0; k < a.prog.nslots; ++k)
             L.term_base[k] = a.term_counts[pga_smem[a.off_slots + PGA_MAX_TERMINALS + k]];
+        L.tp_prev = L.tp_cur = L.td_prev = L.td_cur = 0u;
+    }
     __syncthreads();
 }
 
@@ -541,13 +571,14 @@ __global__ void __launch_bounds__(BS) k_level(KArgs a, uint32_t level) {
     stage_program(a);
     init_bases(a, L);
     const uint32_t n = first_level_size(a, level);
-    if (n != 0u) run_level(a, L, PW, level, n);
+    if (n != 0u) run_level<false>(a, L, PW, level, n);
     // every block has read the old bases before the barrier inside run_level
     store_bases(a, L);
 }
 
-// MEGAKERNEL technique: all levels in one persistent launch.
-template <int BS>
+// MEGAKERNEL technique: all levels in one persistent launch. TAPE: record
+// the derivation tape (reverse mode), one build only.
+template <int BS, bool TAPE>
 __global__ void __launch_bounds__(BS) k_mega(KArgs a) {
     __shared__ LevelLDS L;
     __shared__ float4 PW[5 * BS];
@@ -556,8 +587,15 @@ __global__ void __launch_bounds__(BS) k_mega(KArgs a) {
     uint32_t level = 0;
     uint32_t n = a.n_axioms;
     for (; level < a.max_levels && n != 0u; ++level) {
-        run_level(a, L, PW, level, n);
-        n = min(L.next_n, a.cap_desc);
+        run_level<TAPE>(a, L, PW, level, n);
+        n = min(L.next_n, TAPE ? (a.cap_desc > L.td_cur ? a.cap_desc - L.td_cur : 0u) : a.cap_desc);
+        if (TAPE && threadIdx.x == 0) {       // the next level's inputs are this level's outputs
+            const uint32_t ns = a.prog.nslots;
+            L.tp_prev = L.tp_cur;
+            L.tp_cur += L.level_total[ns];
+            L.td_prev = L.td_cur;
+            L.td_cur += L.level_total[ns + 1u];
+        }
         grid_barrier(a.bar, gridDim.x);       // next level reads this level's records
     }
     store_bases(a, L);
@@ -594,10 +632,10 @@ namespace {
 template <class F>
 bool pick_variant(uint32_t block_size, F&& f) {
     switch (block_size) {
-    case 256: f(pga::k_mega<256>, pga::k_level<256>); return true;
-    case 512: f(pga::k_mega<512>, pga::k_level<512>); return true;
-    case 768: f(pga::k_mega<768>, pga::k_level<768>); return true;
-    case 1024: f(pga::k_mega<1024>, pga::k_level<1024>); return true;
+    case 256: f(pga::k_mega<256, false>, pga::k_level<256>); return true;
+    case 512: f(pga::k_mega<512, false>, pga::k_level<512>); return true;
+    case 768: f(pga::k_mega<768, false>, pga::k_level<768>); return true;
+    case 1024: f(pga::k_mega<1024, false>, pga::k_level<1024>); return true;
     default: return false;
     }
 }
@@ -636,4 +674,16 @@ hipError_t pga_launch_mega(const pga::KArgs& a, uint32_t blocks, uint32_t block_
     if (!pick_variant(block_size, [&](auto mega, auto lvl) { (void)lvl; fn = (const void*)mega; }))
         return hipErrorInvalidValue;
     return hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(block_size), args, a.lds_bytes, s);
+}
+
+hipError_t pga_tape_occupancy(uint32_t lds_bytes, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, pga::k_mega<pga::kTapeBlock, true>,
+                                                         (int)pga::kTapeBlock, lds_bytes);
+}
+
+hipError_t pga_launch_mega_tape(const pga::KArgs& a, uint32_t blocks, hipStream_t s) {
+    pga::KArgs copy = a;
+    void* args[] = {&copy};
+    return hipLaunchCooperativeKernel((const void*)pga::k_mega<pga::kTapeBlock, true>, dim3(blocks),
+                                      dim3(pga::kTapeBlock), args, a.lds_bytes, s);
 }

@@ -337,7 +337,7 @@ __device__ __forceinline__ void axis_vec(uint32_t axis, float o, float& tx, floa
 // 146-197). Returns false when the child is not dispatched (a zero factor of
 // the static Subdivide form).
 __device__ __forceinline__ bool fan_geom(const Ctx& c, const Rec& r, const EntryV& e, Fan& f, uint32_t j, float& o,
-                                         float& ns) {
+                                         float& ns, float* fac_out = nullptr) {
     const uint32_t kind = e.kind();
     o = 0.0f;
     ns = 0.0f;
@@ -360,6 +360,7 @@ __device__ __forceinline__ bool fan_geom(const Ctx& c, const Rec& r, const Entry
         } else {
             fac = eval_param(c, p, r, r.axiom, r.path, rec_depth(r));
         }
+        if (fac_out) *fac_out = fac;
         if (is_static && fac == 0.0f) return false;
         ns = (((float)(fac < 0.0f) * f.r2a) * fac) * -1.0f + ((float)(fac > 0.0f) * fac);
         o = f.off + (ns * 0.5f);

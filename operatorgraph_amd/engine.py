@@ -61,6 +61,9 @@ def load_library():
         "pga_set_timestamps": (C.c_int, [P, P, C.c_uint32]),
         "pga_timestamp_hz": (C.c_double, [P]),
         "pga_autotune": (C.c_int, [P, C.c_uint32, C.c_uint32, C.POINTER(abi.TuneResult)]),
+        "pga_tape_reserve": (C.c_int, [P, C.c_uint32, C.c_uint32]),
+        "pga_forward_tape": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_double)]),
+        "pga_backward": (C.c_int, [P, C.POINTER(C.c_void_p * abi.MAX_TERMINALS), P, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -90,6 +93,7 @@ class Engine:
         self._check(st)
         self._keep = {}
         self.n_axioms = 0
+        self.params_per_axiom = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -134,8 +138,11 @@ class Engine:
         return self
 
     def set_axiom_params(self, params):
+        if hasattr(params, "detach"):
+            params = params.detach().cpu().numpy()
         p = np.ascontiguousarray(params, dtype=np.float32)
         self._check(self.L.pga_set_axiom_params(self._h, p.ctypes.data, p.shape[0], p.shape[1]))
+        self.params_per_axiom = p.shape[1]
         return self
 
     def set_philox_key(self, key):
@@ -205,6 +212,38 @@ class Engine:
         r = abi.TuneResult()
         self._check(self.L.pga_autotune(self._h, self.n_axioms if n is None else n, repeats, C.byref(r)))
         return {name: getattr(r, name) for name, _ in abi.TuneResult._fields_}
+
+    # -- reverse mode --
+    def tape_reserve(self, max_records, max_children):
+        """Sizes the derivation tape: parent records of all levels, children
+        of all levels counting the axioms."""
+        self._check(self.L.pga_tape_reserve(self._h, int(max_records), int(max_children)))
+        return self
+
+    def forward_tape(self, n=None):
+        """Derivation of axioms [0, n) recording the tape; returns seconds."""
+        sec = C.c_double(0.0)
+        self._check(self.L.pga_forward_tape(self._h, self.n_axioms if n is None else n, C.byref(sec)))
+        return sec.value
+
+    def backward(self, dl_dinstances, n=None):
+        """dl_dinstances: {terminal: (count, 16) float32 cuda tensor}.
+        Returns dL/dparams as an (n, params_per_axiom) cuda tensor."""
+        torch = self.torch
+        n = self.n_axioms if n is None else n
+        ptrs = (C.c_void_p * abi.MAX_TERMINALS)()
+        keep = []
+        for t, g in dl_dinstances.items():
+            if g is None:
+                continue
+            g = g.detach().contiguous().to(torch.float32)
+            keep.append(g)
+            ptrs[t] = g.data_ptr()
+        grad = torch.empty((n, self.params_per_axiom), dtype=torch.float32, device="cuda")
+        sec = C.c_double(0.0)
+        self._check(self.L.pga_backward(self._h, C.byref(ptrs), C.c_void_p(grad.data_ptr()), C.byref(sec)))
+        self.last_backward_seconds = sec.value
+        return grad
 
     def enable_timestamps(self, capacity=4 * 64):
         """Phase timestamps of the derivation kernel (see pga_set_timestamps)."""

@@ -1,0 +1,215 @@
+"""Reverse mode on the GPU (pga_forward_tape / pga_backward and the
+torch.autograd binding) against the oracle's FP64 adjoint twin: dL/dparams
+within 1e-4 relative, the tape's derivation identical to the plain one, and
+central finite differences of the FP64 twin (decisions frozen)."""
+import numpy as np
+import pytest
+
+from operatorgraph_amd import abi, axioms, grammars
+from operatorgraph_amd.grammars import op
+from operatorgraph_amd.program import AxiomParam, DispatchTable, Entry, Program, Scalar
+from tests.helpers import assert_same_multiset
+
+pytestmark = pytest.mark.gpu
+
+B, Q = abi.BOX, abi.QUAD
+
+
+def _note(msg):
+    print(msg, flush=True)
+
+
+def _weights(inst):
+    """A fixed pseudo-random dL/dmodel row per instance, keyed by (axiom, path)
+    so that the GPU and oracle orders get the same rows."""
+    key = inst["axiom"].astype(np.float64) * 7919.0 + (inst["path"] % 100003).astype(np.float64)
+    k = np.arange(12, dtype=np.float64)
+    return np.sin(key[:, None] * 0.013 + k[None, :] * 1.7)
+
+
+def _diff_grammar():
+    """Every differentiable operator with per-axiom parameters:
+        A   = Rotate(p0, p1, p2) S;  S = Scale(-1, p3, -1) C;
+        C   = CompSplit() { Top | Bot | Side };
+        Side = Repeat(X, p4, ANCHOR_TO_START) T;  T = Translate(0, 0, p5) G0;
+        Top = SubDiv(Y) { p6: G1 | -1: G1 | -1.5: G0 };
+        Bot = Extrude(Z, p7) G2;  G0/G1 = Generate(0/1) (Quad), G2 = Generate(2) (Box)."""
+    A, S, C, SIDE, T, TOP, BOT, G0, G1, G2 = range(10)
+    table = DispatchTable([
+        Entry(op(abi.ROTATE, B), [AxiomParam(0), AxiomParam(1), AxiomParam(2)], [(S, 0)]),
+        Entry(op(abi.SCALE, B), [Scalar(-1.0), AxiomParam(3), Scalar(-1.0)], [(C, 0)]),
+        Entry(op(abi.COMPSPLIT, B), [], [(TOP, 0), (BOT, 0), (SIDE, 0)]),
+        Entry(op(abi.REPEAT, Q), [Scalar(abi.X), AxiomParam(4), Scalar(abi.ANCHOR_TO_START)], [(T, 0)]),
+        Entry(op(abi.TRANSLATE, Q), [Scalar(0.0), Scalar(0.0), AxiomParam(5)], [(G0, 0)]),
+        Entry(op(abi.SUBDIV, Q), [Scalar(abi.Y), AxiomParam(6), Scalar(-1.0), Scalar(-1.5)],
+              [(G1, 0), (G1, 0), (G0, 0)]),
+        Entry(op(abi.EXTRUDE, Q), [Scalar(abi.Z), AxiomParam(7)], [(G2, 0)]),
+        Entry(op(abi.GENERATE, Q, 1), [Scalar(0)], []),
+        Entry(op(abi.GENERATE, Q, 1), [Scalar(1)], []),
+        Entry(op(abi.GENERATE, B, 1), [Scalar(2)], []),
+    ])
+    return Program(grammars.INTERPRETER_PROCEDURES, table, "diff_kitchen_sink")
+
+
+def _diff_axioms(n, seed=21):
+    rng = np.random.default_rng(seed)
+    ax = axioms.empty(n)
+    ax["size"] = rng.uniform(4.0, 9.0, size=(n, 3)).astype(np.float32)
+    ax["model"][:, [3, 7, 11]] = rng.uniform(-20, 20, size=(n, 3)).astype(np.float32)
+    ax["entry_index"] = 0
+    ax["shape_type"] = B
+    params = np.stack([rng.uniform(-30, 30, n), rng.uniform(-30, 30, n), rng.uniform(-30, 30, n),
+                       rng.uniform(-1.2, -0.6, n), rng.uniform(0.9, 1.7, n), rng.uniform(-0.3, 0.3, n),
+                       rng.uniform(0.5, 1.5, n), rng.uniform(0.5, 2.0, n)], axis=1).astype(np.float32)
+    return ax, params
+
+
+def _gpu_run(program, ax, params, n_terms, tape=True):
+    import torch
+    from operatorgraph_amd import engine
+    eng = engine.Engine(queue_capacity=1 << 20)
+    eng.load(program).set_axioms(ax).set_axiom_params(params)
+    for t in range(n_terms):
+        eng.bind(t, 1 << 18)
+    if tape:
+        eng.tape_reserve(1 << 20, 1 << 21)
+        eng.forward_tape()
+    else:
+        eng.execute()
+    return eng
+
+
+def _adjoint_pair(program, ax, params, n_terms):
+    import torch
+    from oracle import oracle as O
+    eng = _gpu_run(program, ax, params, n_terms)
+    got = [eng.instances(t) for t in range(n_terms)]
+    dl = {}
+    for t in range(n_terms):
+        g = np.zeros((len(got[t]), 16), dtype=np.float32)
+        g[:, :12] = _weights(got[t])
+        dl[t] = torch.from_numpy(g).cuda()
+    grad = eng.backward(dl).cpu().numpy().astype(np.float64)
+    r = O.derive(program, ax, params=params, twins=O.TWIN_F64 | O.TWIN_ADJ)
+    ref_inst = [r.instances(t) for t in range(n_terms)]
+    for t in range(n_terms):
+        assert_same_multiset(got[t], ref_inst[t], f"terminal {t}")
+    ref = r.backward([_weights(ref_inst[t]) for t in range(n_terms)])
+    eng.close()
+    return grad, ref, r
+
+
+def _assert_close(grad, ref, rel=1e-4):
+    scale = max(1e-30, np.abs(ref).max())
+    err = np.abs(grad - ref) / (np.abs(ref) + 1e-3 * scale)
+    assert err.max() <= rel, (err.max(), np.unravel_index(err.argmax(), err.shape), grad, ref)
+
+
+def test_tape_forward_matches_plain_forward():
+    _note("tape forward vs plain forward")
+    ax, params = axioms.param_lots(24, perturb=0.05)
+    prog = grammars.facade_params()
+    a = _gpu_run(prog, ax, params, 2, tape=True)
+    b = _gpu_run(prog, ax, params, 2, tape=False)
+    for t in range(2):
+        assert_same_multiset(a.instances(t), b.instances(t), f"terminal {t}")
+    a.close()
+    b.close()
+
+
+def test_adjoint_facade_params_vs_fp64():
+    _note("adjoint facade_params vs FP64 adjoint")
+    ax, params = axioms.param_lots(16, perturb=0.05)
+    grad, ref, _ = _adjoint_pair(grammars.facade_params(), ax, params, 2)
+    # repetition counts are frozen: the repeat extents get no gradient
+    assert np.all(grad[:, 2] == 0.0) and np.all(grad[:, 3] == 0.0)
+    _assert_close(grad, ref)
+
+
+def test_adjoint_every_operator_vs_fp64():
+    _note("adjoint rotate/scale/compsplit/repeat/translate/subdiv/extrude vs FP64 adjoint")
+    ax, params = _diff_axioms(12)
+    grad, ref, _ = _adjoint_pair(_diff_grammar(), ax, params, 3)
+    assert np.all(np.abs(ref[:, [0, 1, 2, 3, 4, 5, 6, 7]]).max(axis=0) > 0.0)
+    _assert_close(grad, ref)
+
+
+def test_adjoint_against_central_differences():
+    """dL/dparams of the GPU adjoint against central differences (h = 1e-3)
+    of the oracle's FP64 twin with the FP32 pass's decisions frozen."""
+    from oracle import oracle as O
+    _note("adjoint vs central differences")
+    ax, params = axioms.param_lots(3, perturb=0.05)
+    prog = grammars.facade_params()
+    grad, _, r = _adjoint_pair(prog, ax, params, 2)
+    g = [_weights(r.instances(t)) for t in range(2)]
+
+    def loss(p):
+        rr = O.derive(prog, ax, params=p, twins=O.TWIN_F64)
+        assert rr.count(0) == r.count(0) and rr.count(1) == r.count(1)
+        return sum(float(np.sum(rr.instances_f64(t)[:, :12] * g[t])) for t in range(2))
+
+    for lot in range(3):
+        for k in (0, 1, 4, 5, 6, 7):
+            pp, pm = params.copy(), params.copy()
+            pp[lot, k] += 1e-3
+            pm[lot, k] -= 1e-3
+            hh = (float(pp[lot, k]) - float(pm[lot, k])) / 2.0
+            fd = (loss(pp) - loss(pm)) / (2 * hh)
+            assert abs(fd - grad[lot, k]) <= 2e-3 * max(1.0, abs(fd)), (lot, k, fd, grad[lot, k])
+
+
+def _keys(inst):
+    """(axiom << 32) | path of a (count, 16) instance tensor, as int64."""
+    import torch
+    ax = inst[:, 13].contiguous().view(torch.int32).long() & 0xFFFFFFFF
+    pa = inst[:, 14].contiguous().view(torch.int32).long() & 0xFFFFFFFF
+    return (ax << 32) | pa
+
+
+def test_autograd_function_matches_fp64_adjoint():
+    """C4-style loss through torch.autograd: MSE of instance centres and
+    sizes against targets derived with perturbed split parameters (same
+    topology), checked against the FP64 adjoint of the same dL/dinstance."""
+    import torch
+    from oracle import oracle as O
+    from operatorgraph_amd import engine
+    from operatorgraph_amd.autograd import centers_and_sizes, derive_instances
+    _note("autograd function")
+    ax, params = axioms.param_lots(32)
+    rng = np.random.default_rng(4)
+    tparams = params.copy()
+    tparams[:, [0, 1, 4, 5, 6, 7]] += rng.normal(0, 0.05, size=(32, 6)).astype(np.float32)
+    prog = grammars.facade_params()
+    tgt_eng = _gpu_run(prog, ax, tparams, 2, tape=False)
+    targets = [tgt_eng.buffer(t)[: int(tgt_eng.counts(2)[t])].clone() for t in range(2)]
+    tgt_eng.close()
+    eng = engine.Engine(queue_capacity=1 << 20)
+    eng.load(prog).set_axioms(ax).set_axiom_params(params)
+    eng.bind(0, 1 << 19)
+    eng.bind(1, 1 << 19)
+    eng.tape_reserve(1 << 20, 1 << 21)
+    p = torch.from_numpy(params).cuda().requires_grad_(True)
+    outs = derive_instances(p, eng, (0, 1))
+    loss = 0.0
+    for inst, tgt in zip(outs, targets):
+        assert inst.shape == tgt.shape
+        oi, ot = torch.argsort(_keys(inst)), torch.argsort(_keys(tgt))
+        assert torch.equal(_keys(inst)[oi], _keys(tgt)[ot])
+        c, sz = centers_and_sizes(inst[oi])
+        ct, st = centers_and_sizes(tgt[ot])
+        loss = loss + ((c - ct) ** 2).sum(1).mean() + ((sz - st) ** 2).sum(1).mean()
+    g_inst = torch.autograd.grad(loss, outs, retain_graph=True)
+    loss.backward()
+    r = O.derive(prog, ax, params=params, twins=O.TWIN_F64 | O.TWIN_ADJ)
+    dl = []
+    for t in range(2):
+        keys = _keys(outs[t]).cpu().numpy()
+        row = {int(k): i for i, k in enumerate(keys)}
+        ri = r.instances(t)
+        rk = (ri["axiom"].astype(np.int64) << 32) | ri["path"].astype(np.int64)
+        g = g_inst[t][:, :12].double().cpu().numpy()
+        dl.append(np.stack([g[row[int(k)]] for k in rk]))
+    ref = r.backward(dl)
+    _assert_close(p.grad.double().cpu().numpy(), ref)
+    eng.close()
