@@ -42,6 +42,7 @@ def parse():
     p.add_argument("--big-lots", type=int, default=16384)
     p.add_argument("--plain-sample", type=int, default=256)
     p.add_argument("--stochastic-axioms", type=int, default=65536)
+    p.add_argument("--gradient-lots", type=int, default=256)
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--only-kernel", action="store_true",
@@ -176,6 +177,60 @@ def stochastic_workload(torch, engine, grammars, axioms, n, steps):
     return out
 
 
+def gradient_workload(torch, engine, grammars, axioms, n, steps):
+    """256 parameterised facade lots x 8 parameters: forward with tape, MSE of
+    instance centres and sizes against targets from perturbed split
+    parameters (same topology), backward to dL/dparams."""
+    from operatorgraph_amd.autograd import centers_and_sizes, derive_instances
+    ax, params = axioms.param_lots(n)
+    rng = np.random.default_rng(4)
+    tparams = params.copy()
+    tparams[:, [0, 1, 4, 5, 6, 7]] += rng.normal(0, 0.05, size=(n, 6)).astype(np.float32)
+    prog = grammars.facade_params()
+    teng = engine.Engine(queue_capacity=1 << 21)
+    teng.load(prog).set_axioms(ax).set_axiom_params(tparams)
+    teng.bind(0, 1 << 21)
+    teng.bind(1, 1 << 21)
+    teng.execute()
+    counts = teng.counts(2)
+    targets = [centers_and_sizes(teng.buffer(t)[: int(counts[t])].clone()) for t in range(2)]
+    teng.close()
+    eng = engine.Engine(queue_capacity=1 << 21)
+    eng.load(prog).set_axioms(ax).set_axiom_params(params)
+    eng.bind(0, 1 << 21)
+    eng.bind(1, 1 << 21)
+    eng.tape_reserve(1 << 21, 1 << 22)
+    p = torch.from_numpy(params).cuda()
+
+    def step():
+        q = p.clone().requires_grad_(True)
+        outs = derive_instances(q, eng, (0, 1))
+        loss = 0.0
+        for inst, (ct, st) in zip(outs, targets):
+            c, sz = centers_and_sizes(inst)
+            loss = loss + ((c - ct) ** 2).sum(1).mean() + ((sz - st) ** 2).sum(1).mean()
+        loss.backward()
+        return q.grad
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fwd = bwd = 0.0
+    for _ in range(steps):
+        step()
+        fwd += eng.stats()["seconds"]
+        bwd += eng.last_backward_seconds
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    T = int(eng.stats()["terminals"])
+    out = {"lots": n, "params": n * params.shape[1], "terminals": T, "ms_per_step": round(sec * 1e3, 4),
+           "grads_per_s": round(n * params.shape[1] / sec),
+           "forward_tape_ms": round(fwd / steps * 1e3, 4), "backward_ms": round(bwd / steps * 1e3, 4)}
+    eng.close()
+    return out
+
+
 def kernel_name(tune):
     from operatorgraph_amd import abi
     k = "k_mega" if tune["technique"] == abi.MEGAKERNEL else "k_level"
@@ -266,6 +321,10 @@ def main():
         stoch = stochastic_workload(torch, engine, grammars, axioms, args.stochastic_axioms,
                                     max(3, args.steps // 4))
 
+    grad = None
+    if args.gradient_lots:
+        grad = gradient_workload(torch, engine, grammars, axioms, args.gradient_lots, max(5, args.steps // 2))
+
     cpu = None
     if not args.no_cpu:
         rate, runs, total = cpu_baseline(lambda: axioms.facade_lots(args.lots), args.cpu_seconds, 1)
@@ -288,6 +347,7 @@ def main():
         "batched_vs_plain": round(value / plain_rate, 1),
         "batched_16384": big,
         "stochastic_65536_lbvh": stoch,
+        "fwd_bwd_gradient_256": grad,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
