@@ -13,6 +13,15 @@ Prints ONE JSON line. Beside the headline (batched terminals/s) it reports:
   plain         - the one-axiom-per-call path (pga_execute per lot), the
                   reference's executePGA usage, on a sample of lots;
   batched_16384 - the same grammar over 16,384 lots in one launch;
+  graph         - the headline derivation recorded once in a HIP graph and
+                  replayed every step;
+  stochastic_65536_lbvh - per-shape Philox rule choice over 65,536 axioms,
+                  then the LBVH over the terminal boxes;
+  fwd_bwd_gradient_256 - tape forward, torch loss and adjoint over 256
+                  parameterised lots (dL/dparams per second);
+  mixed_menger5_16384_lbvh - a Menger sponge of depth 5 and 16,384 lots in
+                  one launch plus the LBVH over all terminals from the device
+                  counts, one HIP graph per step;
   cpu_baseline  - the CPU restatement (oracle) on the host, one thread.
 """
 import argparse
@@ -43,6 +52,7 @@ def parse():
     p.add_argument("--plain-sample", type=int, default=256)
     p.add_argument("--stochastic-axioms", type=int, default=65536)
     p.add_argument("--gradient-lots", type=int, default=256)
+    p.add_argument("--mixed-lots", type=int, default=16384)
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--only-kernel", action="store_true",
@@ -84,6 +94,26 @@ def timed_steps(torch, eng, steps, technique):
     if technique == abi.MEGAKERNEL:
         eng.synchronize()     # raises on any device error of the last step
     return (t1 - t0) / steps
+
+
+def graph_steps(torch, eng, steps):
+    """The derivation recorded once in a HIP graph (pga_capture_begin) and
+    replayed `steps` times back to back; returns (wall seconds per step,
+    device seconds of the last replay)."""
+    with eng.capture() as g:
+        eng.execute_async()
+    g.launch()
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.launch()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    eng.synchronize()
+    dev = eng.kernel_seconds()
+    g.close()
+    return (t1 - t0) / steps, dev
 
 
 def kernel_seconds(torch, eng, steps):
@@ -177,6 +207,74 @@ def stochastic_workload(torch, engine, grammars, axioms, n, steps):
     return out
 
 
+MENGER5_CUBES = 20 ** 5
+
+
+def mixed_workload(torch, engine, grammars, axioms, n_lots, steps):
+    """A Menger sponge of depth 5 (one Box axiom of size 81, IfSizeLess(X, 1):
+    3.2M cubes) and n_lots facade lots derived by one persistent launch
+    (grammars.combine), then the boxes of all terminals gathered from the
+    device counts and the LBVH built over them with the scene box computed
+    on the device; the whole iteration is one HIP graph, replayed per step."""
+    from operatorgraph_amd import abi
+    from operatorgraph_amd.lbvh import LBVH
+    prog, starts = grammars.combine((grammars.facade(), 0), (grammars.menger_interpreter(1.0), 2))
+    lots = axioms.facade_lots(n_lots)
+    walls, windows = axioms.facade_lot_terminals(lots)
+    both = np.concatenate([axioms.menger_axiom(81.0, starts[1]), lots])
+    caps = [int(walls.sum()), int(windows.sum()), MENGER5_CUBES]
+    T = sum(caps)
+    eng = engine.Engine(technique=abi.MEGAKERNEL,
+                        queue_capacity=int(windows.sum() * 1.05) + MENGER5_CUBES + (1 << 16))
+    eng.load(prog)
+    eng.set_axioms(torch.from_numpy(both.view(np.uint8).copy()).cuda())
+    for t in range(3):
+        eng.bind(t, caps[t])
+    tune = eng.autotune(repeats=2)
+    eng.execute()
+    got = [int(c) for c in eng.counts(3)]
+    assert got == caps, (got, caps)
+    dsec, _ = kernel_seconds(torch, eng, 3)
+    bvh = LBVH(T, stream=eng.stream())
+    out = bvh.alloc()
+    bvh.gather_engine(eng, 3, 0b011, out)
+    bvh.build_device(out)
+    times = []
+    for _ in range(3):
+        bvh.build_device(out)
+        bvh.leaf_count()
+        times.append(bvh.seconds())
+    lsec = float(np.median(times))
+    with eng.capture() as g:
+        eng.execute_async()
+        bvh.gather_engine(eng, 3, 0b011, out)
+        bvh.build_device(out)
+    g.launch()
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.launch()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    eng.synchronize()
+    gsec = eng.kernel_seconds()
+    n, overflow = bvh.leaf_count()
+    assert n == T and not overflow, (n, T, overflow)
+    assert [int(c) for c in eng.counts(3)] == caps
+    alg = (INSTANCE_BYTES + LBVH_BYTES_PER_LEAF) * T + AXIOM_BYTES * len(both)
+    res = {"lots": n_lots, "menger_cubes": MENGER5_CUBES, "terminals_per_step": T,
+           "ms_per_step": round(sec * 1e3, 4), "terminals_per_s": round(T / sec),
+           "graph_device_ms": round(gsec * 1e3, 4), "derive_kernel_ms": round(dsec * 1e3, 4),
+           "lbvh_ms": round(lsec * 1e3, 4),
+           "hbm_frac": round(alg / gsec / 1e9 / HBM_PEAK_GBS, 4), "kernel": kernel_name(tune),
+           "autotune": tune_summary(tune)}
+    g.close()
+    bvh.close()
+    eng.close()
+    return res
+
+
 def gradient_workload(torch, engine, grammars, axioms, n, steps):
     """256 parameterised facade lots x 8 parameters: forward with tape, MSE of
     instance centres and sizes against targets from perturbed split
@@ -266,7 +364,7 @@ def main():
         copy_calibration(torch, args.copy_calibration, args.steps)
         return
 
-    from operatorgraph_amd import axioms, engine, grammars
+    from operatorgraph_amd import abi, axioms, engine, grammars
 
     eng, lots, T, tune = make_engine(torch, engine, grammars, axioms, args.lots)
     A = len(lots)
@@ -282,6 +380,11 @@ def main():
             copy_calibration(torch, args.copy_calibration, args.steps)
         return
     ksec, _ = kernel_seconds(torch, eng, max(5, args.steps // 2))
+    graph = None
+    if tune["technique"] == abi.MEGAKERNEL:
+        gsec, gdev = graph_steps(torch, eng, args.steps)
+        graph = {"ms_per_step": round(gsec * 1e3, 4), "terminals_per_s": round(T / gsec),
+                 "device_ms": round(gdev * 1e3, 4)}
     alg_bytes = INSTANCE_BYTES * T + AXIOM_BYTES * A
     achieved = alg_bytes / ksec / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -321,6 +424,10 @@ def main():
         stoch = stochastic_workload(torch, engine, grammars, axioms, args.stochastic_axioms,
                                     max(3, args.steps // 4))
 
+    mixed = None
+    if args.mixed_lots:
+        mixed = mixed_workload(torch, engine, grammars, axioms, args.mixed_lots, max(3, args.steps // 4))
+
     grad = None
     if args.gradient_lots:
         grad = gradient_workload(torch, engine, grammars, axioms, args.gradient_lots, max(5, args.steps // 2))
@@ -345,9 +452,11 @@ def main():
         "plain": {"lots": n_plain, "terminals": plain_T, "terminals_per_s": round(plain_rate),
                   "ms_per_lot": round(plain_sec / n_plain * 1e3, 4)},
         "batched_vs_plain": round(value / plain_rate, 1),
+        "graph": graph,
         "batched_16384": big,
         "stochastic_65536_lbvh": stoch,
         "fwd_bwd_gradient_256": grad,
+        "mixed_menger5_16384_lbvh": mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

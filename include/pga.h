@@ -340,6 +340,52 @@ PGA_API pga_status pga_lbvh_build(pga_lbvh* bvh, const float* aabbs, uint32_t n,
 /* device time of the last build (Morton codes to refit) */
 PGA_API pga_status pga_lbvh_seconds(pga_lbvh* bvh, double* seconds);
 
+/* -- device-count form (no counterpart in the reference, whose build takes a
+ *    host count, GPUBVHConstructor.cuh:275-333): the leaf count stays in
+ *    device memory, so a derivation, the gather of its terminals and the
+ *    build run without a host round trip and can be recorded in one HIP
+ *    graph (pga_capture_begin) on the stream the builder was created with.
+ *    pga_lbvh_gather_instances writes the boxes of the instances of terminal
+ *    types 0..n_types-1 (buffers/capacities as bound to the context; bit t of
+ *    planar_mask: type t is planar) into aabbs (room for `capacity` boxes),
+ *    concatenated in type order, reading the counts from device_counts
+ *    (pga_device_counts). The leaf count is min(total, capacity); more is an
+ *    overflow, reported by pga_lbvh_leaf_count. pga_lbvh_build_device builds
+ *    over that count, sized for `capacity`, with the given scene box or, when
+ *    both corners are NULL, the union of the boxes computed on the device. -- */
+PGA_API pga_status pga_lbvh_gather_instances(pga_lbvh* bvh, const pga_instance* const* buffers,
+                                             const uint64_t* capacities, uint32_t n_types,
+                                             const uint64_t* device_counts, uint32_t planar_mask, float* aabbs);
+PGA_API pga_status pga_lbvh_build_device(pga_lbvh* bvh, const float* aabbs, const float* scene_min,
+                                         const float* scene_max, uint64_t* keys, int32_t* left, int32_t* right,
+                                         int32_t* parent, int32_t* leaf_parent, float* node_aabb);
+/* waits for the stream; the leaf count of the last gather and its overflow flag */
+PGA_API pga_status pga_lbvh_leaf_count(pga_lbvh* bvh, uint32_t* n, int32_t* overflow);
+
+/* -- HIP graphs of repeated derivations (no counterpart in the reference,
+ *    which launches every derivation from the host, GPUEvaluator.cuh:508-521).
+ *    Between pga_capture_begin and pga_capture_end, pga_execute_async calls
+ *    (and any other work enqueued on the context stream, such as the
+ *    device-count LBVH build) are recorded instead of run. pga_graph_launch
+ *    replays the recording on the context stream without waiting;
+ *    pga_synchronize then waits and reports errors and stats as for
+ *    pga_execute_async. The graph holds the context's buffers, launch
+ *    configuration and argument values as they were at capture: values in
+ *    those buffers (axioms, axiom parameters, instance memory) may change
+ *    between replays, but a replay after a change of bindings, program
+ *    layout, queue capacity or configuration is refused (capture again).
+ *    pga_device_counts gives the device address of the 64-bit terminal
+ *    counters (PGA_MAX_TERMINALS), valid for the context's lifetime. -- */
+typedef struct pga_graph pga_graph;
+PGA_API pga_status pga_capture_begin(pga_context* ctx);
+PGA_API pga_status pga_capture_end(pga_context* ctx, pga_graph** graph);
+PGA_API pga_status pga_graph_launch(pga_context* ctx, pga_graph* graph);
+PGA_API void pga_graph_destroy(pga_graph* graph);
+/* kernel nodes of the graph and how many of them launch cooperatively */
+PGA_API pga_status pga_graph_info(const pga_graph* graph, uint32_t* kernel_nodes, uint32_t* cooperative_nodes);
+PGA_API pga_status pga_device_counts(pga_context* ctx, const uint64_t** device_counts);
+PGA_API pga_status pga_get_stream(pga_context* ctx, void** stream);
+
 #ifdef __cplusplus
 }
 #endif

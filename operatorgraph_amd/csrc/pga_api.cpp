@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -18,6 +19,14 @@
 
 using pga::DevEntry;
 using pga::DevParam;
+
+// One derivation recorded into a HIP graph: the argument block and launch
+// shape it was captured with (a replay is valid while the context would
+// build the same ones).
+struct CapturedLaunch {
+    pga::KArgs args;
+    uint32_t blocks, block_size, first, count;
+};
 
 struct pga_context {
     pga_config cfg{};
@@ -96,7 +105,27 @@ struct pga_context {
     uint32_t tape_axioms = 0;
     std::vector<uint32_t> tape_pc, tape_dc;   // parents / descriptors written by each level
 
+    // graph capture
+    bool capturing = false;
+    bool capture_failed = false;
+    std::string capture_error;
+    std::vector<CapturedLaunch> captured;
+
     pga_stats stats{};
+};
+
+struct pga_graph {
+    pga_context* ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<CapturedLaunch> launches;
+    uint32_t cap = 0;
+    uint32_t last_count = 0;
+    uint32_t kernel_nodes = 0, cooperative_nodes = 0;
+    ~pga_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
 };
 
 namespace {
@@ -122,6 +151,13 @@ pga_status hip_fail(pga_context* ctx, hipError_t e, const char* what) {
         hipError_t e_ = (expr);                              \
         if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
     } while (0)
+
+// Calls that wait for the device or copy synchronously cannot be recorded.
+pga_status refuse_in_capture(pga_context* ctx, const char* what) {
+    return fail(ctx, PGA_ERR_INVALID_ARG, std::string(what) + " cannot run while a graph capture is open");
+}
+#define NO_CAPTURE(ctx, what) \
+    do { if ((ctx) && (ctx)->capturing) return refuse_in_capture(ctx, what); } while (0)
 
 template <class T> void hfree(T*& p) {
     if (p) (void)hipFree(p);
@@ -466,8 +502,12 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
     return PGA_OK;
 }
 
+uint32_t queue_capacity(const pga_context* ctx) {
+    return ctx->cfg.queue_capacity ? ctx->cfg.queue_capacity : kDefaultCapacity;
+}
+
 pga_status ensure_buffers(pga_context* ctx) {
-    uint32_t cap = ctx->cfg.queue_capacity ? ctx->cfg.queue_capacity : kDefaultCapacity;
+    uint32_t cap = queue_capacity(ctx);
     if (ctx->cap != cap) {
         for (int b = 0; b < 2; ++b) {
             hfree(ctx->d_par[b]);
@@ -648,12 +688,14 @@ PGA_API const char* pga_last_error(const pga_context* ctx) { return ctx ? ctx->e
 
 PGA_API pga_status pga_initialize(pga_context* ctx, const pga_procedure* procedures, uint32_t num_procedures,
                                   const pga_dispatch_entry* entries, uint32_t num_entries) {
+    NO_CAPTURE(ctx, "pga_initialize");
     if (!ctx) return PGA_ERR_INVALID_ARG;
     ctx->initialized = false;
     return build_program(ctx, procedures, num_procedures, entries, num_entries);
 }
 
 PGA_API pga_status pga_set_axioms(pga_context* ctx, const pga_axiom* axioms, uint32_t n) {
+    NO_CAPTURE(ctx, "pga_set_axioms");
     if (!ctx || (!axioms && n)) return fail(ctx, PGA_ERR_INVALID_ARG, "null axioms");
     size_t bytes = (size_t)std::max<uint32_t>(n, 1) * sizeof(pga_axiom);
     if (bytes > ctx->axioms_owned_cap) {
@@ -668,6 +710,7 @@ PGA_API pga_status pga_set_axioms(pga_context* ctx, const pga_axiom* axioms, uin
 }
 
 PGA_API pga_status pga_set_axioms_device(pga_context* ctx, const pga_axiom* device_axioms, uint32_t n) {
+    NO_CAPTURE(ctx, "pga_set_axioms_device");
     if (!ctx || (!device_axioms && n)) return fail(ctx, PGA_ERR_INVALID_ARG, "null axioms");
     ctx->axioms = device_axioms;
     ctx->n_axioms = n;
@@ -675,6 +718,7 @@ PGA_API pga_status pga_set_axioms_device(pga_context* ctx, const pga_axiom* devi
 }
 
 PGA_API pga_status pga_set_seeds(pga_context* ctx, const float* seeds, uint32_t n) {
+    NO_CAPTURE(ctx, "pga_set_seeds");
     if (!ctx || (!seeds && n)) return fail(ctx, PGA_ERR_INVALID_ARG, "null seeds");
     uint32_t k = n % (PGA_MAX_NUM_AXIOM_SEEDS + 1);      // GlobalVariables.cuh:28-32
     if (k) std::memcpy(ctx->seeds, seeds, k * sizeof(float));
@@ -689,6 +733,7 @@ PGA_API pga_status pga_get_seeds(const pga_context* ctx, float* seeds32) {
 
 PGA_API pga_status pga_set_axiom_params(pga_context* ctx, const float* params, uint32_t num_axioms,
                                         uint32_t per_axiom) {
+    NO_CAPTURE(ctx, "pga_set_axiom_params");
     if (!ctx || (!params && num_axioms && per_axiom)) return fail(ctx, PGA_ERR_INVALID_ARG, "null parameters");
     size_t bytes = (size_t)num_axioms * per_axiom * sizeof(float);
     if (bytes > ctx->aparams_cap) {
@@ -720,6 +765,7 @@ PGA_API pga_status pga_bind_instances(pga_context* ctx, uint32_t terminal, pga_i
 
 PGA_API pga_status pga_get_counts(pga_context* ctx, uint64_t* counts, uint32_t n) {
     if (!ctx || !counts || n > PGA_MAX_TERMINALS) return fail(ctx, PGA_ERR_INVALID_ARG, "bad counts request");
+    NO_CAPTURE(ctx, "pga_get_counts");
     unsigned long long tmp[PGA_MAX_TERMINALS];
     HIP_TRY(ctx, hipMemcpyAsync(tmp, ctx->d_term_counts, sizeof(tmp), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -735,6 +781,11 @@ static pga_status check_launch(pga_context* ctx, uint32_t first, uint32_t count)
     if (ctx->max_param_slot >= 0 &&
         ((uint32_t)ctx->max_param_slot >= ctx->aparams_per || ctx->aparams_axioms < ctx->n_axioms))
         return fail(ctx, PGA_ERR_INVALID_ARG, "program reads axiom parameters that are not set");
+    if (ctx->capturing) {   // nothing may be allocated or copied while the stream records
+        if (ctx->bind_dirty || ctx->cap != queue_capacity(ctx))
+            return fail(ctx, PGA_ERR_INVALID_ARG, "bindings or queue capacity changed during the capture");
+        return PGA_OK;
+    }
     return ensure_buffers(ctx);
 }
 
@@ -764,22 +815,34 @@ static pga_status finish(pga_context* ctx, uint32_t count, uint32_t host_levels)
     return status_from_err(ctx, err);
 }
 
-PGA_API pga_status pga_execute_async(pga_context* ctx, uint32_t first, uint32_t count, int32_t append) {
+static pga_status execute_async(pga_context* ctx, uint32_t first, uint32_t count, int32_t append) {
     pga_status st = check_launch(ctx, first, count);
     if (st) return st;
     if (ctx->cfg.technique != PGA_TECHNIQUE_MEGAKERNEL)
         return fail(ctx, PGA_ERR_UNSUPPORTED, "asynchronous execution needs the MEGAKERNEL technique");
     pga::KArgs a = make_args(ctx, first, count);
+    const bool timed = !ctx->capturing;     // the events time eager launches only
     HIP_TRY(ctx, pga_launch_reset(a, append ? 0 : 1, ctx->stream));
-    HIP_TRY(ctx, hipEventRecord(ctx->evk0, ctx->stream));
+    if (timed) HIP_TRY(ctx, hipEventRecord(ctx->evk0, ctx->stream));
     if (count > 0) HIP_TRY(ctx, pga_launch_mega(a, ctx->mega_blocks, ctx->cfg.block_size, ctx->stream));
-    HIP_TRY(ctx, hipEventRecord(ctx->evk1, ctx->stream));
+    if (timed) HIP_TRY(ctx, hipEventRecord(ctx->evk1, ctx->stream));
+    if (ctx->capturing) ctx->captured.push_back({a, ctx->mega_blocks, ctx->cfg.block_size, first, count});
     ctx->pending_count = count;
     return PGA_OK;
 }
 
+PGA_API pga_status pga_execute_async(pga_context* ctx, uint32_t first, uint32_t count, int32_t append) {
+    pga_status st = execute_async(ctx, first, count, append);
+    if (st && ctx && ctx->capturing && !ctx->capture_failed) {
+        ctx->capture_failed = true;
+        ctx->capture_error = ctx->error;
+    }
+    return st;
+}
+
 PGA_API pga_status pga_synchronize(pga_context* ctx) {
     if (!ctx) return PGA_ERR_INVALID_ARG;
+    NO_CAPTURE(ctx, "pga_synchronize");
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.0f;
     HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
@@ -797,6 +860,7 @@ PGA_API pga_status pga_kernel_seconds(pga_context* ctx, double* seconds) {
 
 PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first, uint32_t count, int32_t append,
                                      double* seconds) {
+    NO_CAPTURE(ctx, "pga_execute");
     pga_status st = check_launch(ctx, first, count);
     if (st) return st;
     pga::KArgs a = make_args(ctx, first, count);
@@ -866,6 +930,7 @@ static pga_status resident_blocks(pga_context* ctx, uint32_t bs, uint32_t* block
 }
 
 PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t repeats, pga_tune_result* out) {
+    NO_CAPTURE(ctx, "pga_autotune");
     if (!ctx || !out) return fail(ctx, PGA_ERR_INVALID_ARG, "null argument");
     if (repeats == 0) repeats = 3;
     const pga_config saved = ctx->cfg;
@@ -935,6 +1000,7 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
 // ---- reverse mode ----
 
 PGA_API pga_status pga_tape_reserve(pga_context* ctx, uint32_t max_records, uint32_t max_children) {
+    NO_CAPTURE(ctx, "pga_tape_reserve");
     if (!ctx) return PGA_ERR_INVALID_ARG;
     if (max_records == 0 || max_children == 0) return fail(ctx, PGA_ERR_INVALID_ARG, "empty tape");
     if (ctx->stream) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -951,6 +1017,7 @@ PGA_API pga_status pga_tape_reserve(pga_context* ctx, uint32_t max_records, uint
 }
 
 PGA_API pga_status pga_forward_tape(pga_context* ctx, uint32_t num_axioms, double* seconds) {
+    NO_CAPTURE(ctx, "pga_forward_tape");
     pga_status st = check_launch(ctx, 0, num_axioms);
     if (st) return st;
     if (!ctx->d_tape_par) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no tape: call pga_tape_reserve first");
@@ -1011,6 +1078,7 @@ PGA_API pga_status pga_forward_tape(pga_context* ctx, uint32_t num_axioms, doubl
 
 PGA_API pga_status pga_backward(pga_context* ctx, const float* const* dl_dinstances, float* dl_dparams,
                                 double* seconds) {
+    NO_CAPTURE(ctx, "pga_backward");
     if (!ctx || !dl_dinstances || !dl_dparams) return fail(ctx, PGA_ERR_INVALID_ARG, "null argument");
     if (!ctx->tape_valid) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no recorded tape: run pga_forward_tape");
     if (ctx->aparams_per == 0) return fail(ctx, PGA_ERR_INVALID_ARG, "no per-axiom parameters bound");
@@ -1063,6 +1131,109 @@ PGA_API pga_status pga_backward(pga_context* ctx, const float* const* dl_dinstan
     uint32_t err = 0;
     HIP_TRY(ctx, hipMemcpy(&err, ctx->d_err, sizeof(uint32_t), hipMemcpyDeviceToHost));
     return status_from_err(ctx, err);
+}
+
+PGA_API pga_status pga_capture_begin(pga_context* ctx) {
+    if (!ctx) return PGA_ERR_INVALID_ARG;
+    NO_CAPTURE(ctx, "pga_capture_begin");
+    if (!ctx->initialized) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no program loaded");
+    if (ctx->cfg.technique != PGA_TECHNIQUE_MEGAKERNEL)
+        return fail(ctx, PGA_ERR_UNSUPPORTED, "graphs need the MEGAKERNEL technique");
+    pga_status st = ensure_buffers(ctx);        // bindings copied before the stream records
+    if (st) return st;
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    ctx->capture_failed = false;
+    ctx->capture_error.clear();
+    ctx->captured.clear();
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_capture_end(pga_context* ctx, pga_graph** out) {
+    if (!ctx || !out) return fail(ctx, PGA_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (!ctx->capturing) return fail(ctx, PGA_ERR_INVALID_ARG, "no graph capture is open");
+    ctx->capturing = false;
+    std::unique_ptr<pga_graph> g(new pga_graph());
+    g->ctx = ctx;
+    g->launches.swap(ctx->captured);
+    g->cap = ctx->cap;
+    g->last_count = g->launches.empty() ? 0u : g->launches.back().count;
+    HIP_TRY(ctx, hipStreamEndCapture(ctx->stream, &g->graph));
+    if (ctx->capture_failed)
+        return fail(ctx, PGA_ERR_INVALID_ARG, "a call failed during the capture: " + ctx->capture_error);
+    size_t n = 0;
+    HIP_TRY(ctx, hipGraphGetNodes(g->graph, nullptr, &n));
+    std::vector<hipGraphNode_t> nodes(n);
+    if (n) HIP_TRY(ctx, hipGraphGetNodes(g->graph, nodes.data(), &n));
+    for (hipGraphNode_t node : nodes) {
+        hipGraphNodeType type;
+        HIP_TRY(ctx, hipGraphNodeGetType(node, &type));
+        if (type != hipGraphNodeTypeKernel) continue;
+        ++g->kernel_nodes;
+        hipKernelNodeParams kp{};
+        HIP_TRY(ctx, hipGraphKernelNodeGetParams(node, &kp));
+        hipKernelNodeAttrValue v{};
+        HIP_TRY(ctx, hipGraphKernelNodeGetAttribute(node, hipKernelNodeAttributeCooperative, &v));
+        if (!v.cooperative && pga_needs_coresidency(kp.func)) {   // the grid barrier needs every workgroup resident
+            v.cooperative = 1;
+            HIP_TRY(ctx, hipGraphKernelNodeSetAttribute(node, hipKernelNodeAttributeCooperative, &v));
+            HIP_TRY(ctx, hipGraphKernelNodeGetAttribute(node, hipKernelNodeAttributeCooperative, &v));
+            if (!v.cooperative) return fail(ctx, PGA_ERR_UNSUPPORTED, "graph node cannot launch cooperatively");
+        }
+        if (v.cooperative) ++g->cooperative_nodes;
+    }
+    HIP_TRY(ctx, hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g.release();
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_graph_launch(pga_context* ctx, pga_graph* g) {
+    if (!ctx || !g) return fail(ctx, PGA_ERR_INVALID_ARG, "null argument");
+    NO_CAPTURE(ctx, "pga_graph_launch");
+    if (g->ctx != ctx) return fail(ctx, PGA_ERR_INVALID_ARG, "the graph was captured on another context");
+    if (ctx->bind_dirty || ctx->cap != g->cap || queue_capacity(ctx) != g->cap)
+        return fail(ctx, PGA_ERR_INVALID_ARG, "bindings or queue capacity changed since the capture; capture again");
+    for (const CapturedLaunch& l : g->launches) {
+        pga_status st = check_launch(ctx, l.first, l.count);
+        if (st) return st;
+        const pga::KArgs a = make_args(ctx, l.first, l.count);
+        if (std::memcmp(&a, &l.args, sizeof(a)) != 0 || l.blocks != ctx->mega_blocks ||
+            l.block_size != ctx->cfg.block_size)
+            return fail(ctx, PGA_ERR_INVALID_ARG,
+                        "the context changed since the capture (bindings, program layout or configuration); "
+                        "capture again");
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->evk0, ctx->stream));
+    HIP_TRY(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->evk1, ctx->stream));
+    ctx->pending_count = g->last_count;
+    return PGA_OK;
+}
+
+PGA_API void pga_graph_destroy(pga_graph* g) {
+    if (!g) return;
+    if (g->ctx && g->ctx->stream) (void)hipStreamSynchronize(g->ctx->stream);
+    delete g;
+}
+
+PGA_API pga_status pga_graph_info(const pga_graph* g, uint32_t* kernel_nodes, uint32_t* cooperative_nodes) {
+    if (!g) return PGA_ERR_INVALID_ARG;
+    if (kernel_nodes) *kernel_nodes = g->kernel_nodes;
+    if (cooperative_nodes) *cooperative_nodes = g->cooperative_nodes;
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_device_counts(pga_context* ctx, const uint64_t** device_counts) {
+    if (!ctx || !device_counts) return fail(ctx, PGA_ERR_INVALID_ARG, "null argument");
+    *device_counts = reinterpret_cast<const uint64_t*>(ctx->d_term_counts);
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_get_stream(pga_context* ctx, void** stream) {
+    if (!ctx || !stream) return fail(ctx, PGA_ERR_INVALID_ARG, "null argument");
+    *stream = (void*)ctx->stream;
+    return PGA_OK;
 }
 
 } // extern "C"

@@ -81,6 +81,9 @@ hipError_t pga_launch_level(const pga::KArgs& a, uint32_t level, uint32_t blocks
                             hipStream_t s);
 hipError_t pga_mega_occupancy(uint32_t block_size, uint32_t lds_bytes, int* blocks_per_cu);
 hipError_t pga_launch_mega(const pga::KArgs& a, uint32_t blocks, uint32_t block_size, hipStream_t s);
+// true for the kernels whose workgroups wait on one another (grid barrier):
+// a graph must launch them cooperatively
+bool pga_needs_coresidency(const void* fn);
 hipError_t pga_tape_occupancy(uint32_t lds_bytes, int* blocks_per_cu);
 hipError_t pga_launch_mega_tape(const pga::KArgs& a, uint32_t blocks, hipStream_t s);
 hipError_t pga_launch_adjoint(const pga::KArgs& a, const pga::AdjArgs& g, hipStream_t s);

@@ -646,6 +646,13 @@ bool pga_block_size_supported(uint32_t block_size) {
     return pick_variant(block_size, [](auto, auto) {});
 }
 
+bool pga_needs_coresidency(const void* fn) {
+    bool hit = fn == (const void*)pga::k_mega<pga::kTapeBlock, true>;
+    for (uint32_t bs : {256u, 512u, 768u, 1024u})
+        pick_variant(bs, [&](auto mega, auto lvl) { hit = hit || fn == (const void*)mega || fn == (const void*)lvl; });
+    return hit;
+}
+
 hipError_t pga_launch_level(const pga::KArgs& a, uint32_t level, uint32_t blocks, uint32_t block_size,
                             hipStream_t s) {
     pga::KArgs copy = a;

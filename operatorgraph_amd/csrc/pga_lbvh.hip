@@ -23,6 +23,12 @@
 //                  ticket of the parent is taken.
 // Children are encoded as in the reference: >= 0 a leaf (sorted position),
 // < 0 an internal node (negated index); node 0 is the root.
+//
+// Device-count mode: every kernel that reads the leaf count n takes the host
+// bound n (grid size) and an optional device word nd; with nd the count is
+// min(*nd, n), so a gather of the derived terminals, the scene box and the
+// whole build run from device state without a host round trip, and can be
+// recorded in a HIP graph and replayed while the terminal counts change.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -37,6 +43,12 @@ constexpr uint32_t kThreads = 256;
 constexpr uint32_t kItems = 16;
 constexpr uint32_t kTile = kThreads * kItems;   // keys per sort workgroup
 constexpr uint32_t kScanTile = 4096;            // histogram entries per scan workgroup
+constexpr uint32_t kGatherBlocks = 8192;         // grid-stride workgroups of the terminal gather
+
+// leaf count: the host bound, or the device word when there is one
+__device__ __forceinline__ uint32_t leaf_count(uint32_t n, const uint32_t* nd) {
+    return nd ? min(*nd, n) : n;
+}
 
 __device__ __forceinline__ uint32_t f2u_rd(float x) {
     if (!(x > 0.0f)) return 0u;
@@ -76,10 +88,57 @@ __global__ void __launch_bounds__(256) k_instance_aabbs(const pga_instance* __re
     o[3] = r0.w + h0; o[4] = r1.w + h1; o[5] = r2.w + h2;
 }
 
+// Boxes of the instances of terminal types 0..ntypes-1, concatenated in type
+// order (each buffer in its own order): counts come from the derivation's
+// device counters, each clamped to its buffer's capacity. Writes the leaf
+// count min(total, capacity) and an overflow flag.
+struct GatherArgs {
+    const pga_instance* buf[PGA_MAX_TERMINALS];
+    unsigned long long cap[PGA_MAX_TERMINALS];
+    uint32_t ntypes, planar_mask, capacity;
+};
+
+__global__ void __launch_bounds__(256) k_gather_boxes(GatherArgs g, const unsigned long long* __restrict__ counts,
+                                                      float* __restrict__ aabbs, uint32_t* __restrict__ nd,
+                                                      uint32_t* __restrict__ overflow) {
+    __shared__ unsigned long long start[PGA_MAX_TERMINALS + 1];
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (uint32_t t = 0; t < g.ntypes; ++t) {
+            start[t] = s;
+            const unsigned long long c = counts[t];
+            s += c < g.cap[t] ? c : g.cap[t];
+        }
+        start[g.ntypes] = s;
+        if (blockIdx.x == 0) {
+            *nd = (uint32_t)(s < g.capacity ? s : g.capacity);
+            *overflow = s > g.capacity ? 1u : 0u;
+        }
+    }
+    __syncthreads();
+    const unsigned long long total = start[g.ntypes] < g.capacity ? start[g.ntypes] : g.capacity;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        uint32_t t = 0;                       // the last type whose range starts at or before i
+        for (uint32_t step = 32; step > 0; step >>= 1)
+            if (t + step < g.ntypes && start[t + step] <= i) t += step;
+        const float4* m = reinterpret_cast<const float4*>(g.buf[t] + (i - start[t]));
+        const float4 r0 = m[0], r1 = m[1], r2 = m[2];
+        float h0 = fabsf(r0.x) + fabsf(r0.y), h1 = fabsf(r1.x) + fabsf(r1.y), h2 = fabsf(r2.x) + fabsf(r2.y);
+        if (!((g.planar_mask >> t) & 1u)) { h0 = h0 + fabsf(r0.z); h1 = h1 + fabsf(r1.z); h2 = h2 + fabsf(r2.z); }
+        h0 = h0 * 0.5f; h1 = h1 * 0.5f; h2 = h2 * 0.5f;
+        float* o = aabbs + i * 6u;
+        o[0] = r0.w - h0; o[1] = r1.w - h1; o[2] = r2.w - h2;
+        o[3] = r0.w + h0; o[4] = r1.w + h1; o[5] = r2.w + h2;
+    }
+}
+
 // Per-workgroup min/max of the AABB corners (scene box), then one workgroup
 // folds the partials.
-__global__ void __launch_bounds__(256) k_bounds(const float* __restrict__ aabbs, uint32_t n, float* __restrict__ part) {
+__global__ void __launch_bounds__(256) k_bounds(const float* __restrict__ aabbs, uint32_t n, const uint32_t* nd,
+                                                float* __restrict__ part) {
     __shared__ float red[6][256];
+    n = leaf_count(n, nd);
     float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float* a = aabbs + (size_t)i * 6u;
@@ -104,11 +163,19 @@ __global__ void __launch_bounds__(64) k_bounds_fold(float* part, uint32_t nparts
     part[q] = v;
 }
 
-__global__ void __launch_bounds__(256) k_morton(const float* __restrict__ aabbs, uint32_t n, float mn0, float mn1,
-                                                float mn2, float ex0, float ex1, float ex2,
+// scene box from the host (mn, ex) or, with `box`, from the device: min xyz,
+// max xyz, an empty extent taken as 1
+__global__ void __launch_bounds__(256) k_morton(const float* __restrict__ aabbs, uint32_t n, const uint32_t* nd,
+                                                float mn0, float mn1, float mn2, float ex0, float ex1, float ex2,
+                                                const float* __restrict__ box,
                                                 uint32_t* __restrict__ code, uint32_t* __restrict__ idx) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= leaf_count(n, nd)) return;
+    if (box) {
+        mn0 = box[0]; mn1 = box[1]; mn2 = box[2];
+        ex0 = box[3] - mn0; ex1 = box[4] - mn1; ex2 = box[5] - mn2;
+        ex0 = ex0 > 0.0f ? ex0 : 1.0f; ex1 = ex1 > 0.0f ? ex1 : 1.0f; ex2 = ex2 > 0.0f ? ex2 : 1.0f;
+    }
     const float* b = aabbs + (size_t)i * 6u;
     const float cx = (b[0] + b[3]) * 0.5f, cy = (b[1] + b[4]) * 0.5f, cz = (b[2] + b[5]) * 0.5f;
     code[i] = morton30((cx - mn0) / ex0, (cy - mn1) / ex1, (cz - mn2) / ex2);
@@ -116,12 +183,17 @@ __global__ void __launch_bounds__(256) k_morton(const float* __restrict__ aabbs,
 }
 
 // ---- radix sort: histogram, scan, stable scatter (8-bit digits) ----
-__global__ void __launch_bounds__(kThreads) k_hist(const uint32_t* __restrict__ code, uint32_t n, uint32_t shift,
-                                                   uint32_t* __restrict__ hist, uint32_t nblocks) {
+__global__ void __launch_bounds__(kThreads) k_hist(const uint32_t* __restrict__ code, uint32_t n, const uint32_t* nd,
+                                                   uint32_t shift, uint32_t* __restrict__ hist, uint32_t nblocks) {
     __shared__ uint32_t h[256];
+    n = leaf_count(n, nd);
+    const uint32_t base = blockIdx.x * kTile;
+    if (base >= n) {                          // tile past the leaves: an empty histogram
+        hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = 0u;
+        return;
+    }
     h[threadIdx.x] = 0u;
     __syncthreads();
-    const uint32_t base = blockIdx.x * kTile;
     for (uint32_t it = 0; it < kItems; ++it) {
         const uint32_t i = base + it * kThreads + threadIdx.x;
         if (i < n) atomicAdd(&h[(code[i] >> shift) & 0xFFu], 1u);
@@ -193,11 +265,13 @@ __global__ void __launch_bounds__(kThreads) k_scan_down(uint32_t* __restrict__ a
 __global__ void __launch_bounds__(kThreads) k_scatter(const uint32_t* __restrict__ code_in,
                                                       const uint32_t* __restrict__ idx_in,
                                                       uint32_t* __restrict__ code_out, uint32_t* __restrict__ idx_out,
-                                                      uint32_t n, uint32_t shift, const uint32_t* __restrict__ offs,
-                                                      uint32_t nblocks) {
+                                                      uint32_t n, const uint32_t* nd, uint32_t shift,
+                                                      const uint32_t* __restrict__ offs, uint32_t nblocks) {
     __shared__ uint32_t base_d[256];
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[4][256];
+    n = leaf_count(n, nd);
+    if (blockIdx.x * kTile >= n) return;      // whole workgroup: no leaves in this tile
     const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
     base_d[t] = offs[(size_t)t * nblocks + blockIdx.x];
     run[t] = 0u;
@@ -245,11 +319,11 @@ __device__ __forceinline__ int delta(const uint32_t* code, const uint32_t* idx, 
 }
 
 __global__ void __launch_bounds__(256) k_karras(const uint32_t* __restrict__ code, const uint32_t* __restrict__ idx,
-                                                uint32_t n, unsigned long long* __restrict__ keys,
+                                                uint32_t n, const uint32_t* nd, unsigned long long* __restrict__ keys,
                                                 int32_t* __restrict__ left, int32_t* __restrict__ right,
                                                 int32_t* __restrict__ parent, int32_t* __restrict__ leaf_parent,
                                                 uint32_t* __restrict__ ticket) {
-    const int64_t nn = n;
+    const int64_t nn = leaf_count(n, nd);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nn) keys[i] = key_at(code, idx, i);
     if (i >= nn - 1) return;
@@ -293,12 +367,13 @@ __device__ __forceinline__ void load_box(const float* p, float b[6]) {
 }
 
 __global__ void __launch_bounds__(256) k_refit(const float* __restrict__ aabbs, const uint32_t* __restrict__ idx,
-                                               uint32_t n, const int32_t* __restrict__ left,
+                                               uint32_t n, const uint32_t* nd, const int32_t* __restrict__ left,
                                                const int32_t* __restrict__ right, const int32_t* __restrict__ parent,
                                                const int32_t* __restrict__ leaf_parent, float* node_aabb,
                                                uint32_t* ticket) {
+    n = leaf_count(n, nd);
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    if (k >= n || n < 2u) return;             // one leaf: no internal node, no parent links
     float b[6];
     const float* src = aabbs + (size_t)idx[k] * 6u;
 #pragma unroll
@@ -349,7 +424,8 @@ struct pga_lbvh {
     uint32_t* hist = nullptr;       // 256 x sort workgroups
     uint32_t* part = nullptr;       // scan partials
     uint32_t* ticket = nullptr;     // internal nodes
-    float* bounds = nullptr;        // bounds partials
+    float* bounds = nullptr;        // bounds partials; [0, 6) the folded scene box
+    uint32_t* dev_words = nullptr;  // [0] leaf count of the last gather, [1] its overflow flag
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::string error;
 };
@@ -370,6 +446,36 @@ pga_status lfail(pga_lbvh* b, pga_status st, const std::string& m) {
     } while (0)
 
 uint32_t sort_blocks(uint32_t n) { return (n + kTile - 1u) / kTile; }
+
+// Morton codes to refit for n leaves (nd: the device count, n its bound),
+// with a host scene box (mn, ex) or the device box `box`.
+hipError_t enqueue_build(pga_lbvh* b, const float* aabbs, uint32_t n, const uint32_t* nd, const float mn[3],
+                         const float ex[3], const float* box, uint64_t* keys, int32_t* left, int32_t* right,
+                         int32_t* parent, int32_t* leaf_parent, float* node_aabb) {
+    hipStream_t s = b->stream;
+    const uint32_t g = (n + 255u) / 256u;
+    hipLaunchKernelGGL(k_morton, dim3(g), dim3(256), 0, s, aabbs, n, nd, mn[0], mn[1], mn[2], ex[0], ex[1], ex[2],
+                       box, b->code[0], b->idx[0]);
+    const uint32_t nb = sort_blocks(n);
+    const uint32_t nh = 256u * nb;
+    const uint32_t np = (nh + kScanTile - 1u) / kScanTile;
+    int cur = 0;
+    for (uint32_t pass = 0; pass < 4; ++pass) {     // 30-bit codes: digits at bits 0, 8, 16, 24
+        const uint32_t shift = 8u * pass;
+        hipLaunchKernelGGL(k_hist, dim3(nb), dim3(kThreads), 0, s, b->code[cur], n, nd, shift, b->hist, nb);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(np), dim3(kThreads), 0, s, b->hist, nh, b->part);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kThreads), 0, s, b->part, np);
+        hipLaunchKernelGGL(k_scan_down, dim3(np), dim3(kThreads), 0, s, b->hist, nh, b->part);
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kThreads), 0, s, b->code[cur], b->idx[cur], b->code[cur ^ 1],
+                           b->idx[cur ^ 1], n, nd, shift, b->hist, nb);
+        cur ^= 1;
+    }
+    hipLaunchKernelGGL(k_karras, dim3(g), dim3(256), 0, s, b->code[cur], b->idx[cur], n, nd,
+                       reinterpret_cast<unsigned long long*>(keys), left, right, parent, leaf_parent, b->ticket);
+    hipLaunchKernelGGL(k_refit, dim3(g), dim3(256), 0, s, aabbs, b->idx[cur], n, nd, left, right, parent,
+                       leaf_parent, node_aabb, b->ticket);
+    return hipGetLastError();
+}
 
 } // namespace
 
@@ -392,6 +498,8 @@ PGA_API pga_status pga_lbvh_create(uint32_t capacity, int32_t device, void* stre
     LHIP(b, hipMalloc((void**)&b->part, ((nh + kScanTile - 1) / kScanTile + 1) * 4u));
     LHIP(b, hipMalloc((void**)&b->ticket, (size_t)capacity * 4u));
     LHIP(b, hipMalloc((void**)&b->bounds, 1024u * 6u * 4u));
+    LHIP(b, hipMalloc((void**)&b->dev_words, 64u));
+    LHIP(b, hipMemset(b->dev_words, 0, 64u));
     LHIP(b, hipEventCreate(&b->e0));
     LHIP(b, hipEventCreate(&b->e1));
     return PGA_OK;
@@ -403,6 +511,7 @@ PGA_API void pga_lbvh_destroy(pga_lbvh* b) {
     else (void)hipDeviceSynchronize();
     for (int k = 0; k < 2; ++k) { (void)hipFree(b->code[k]); (void)hipFree(b->idx[k]); }
     (void)hipFree(b->hist); (void)hipFree(b->part); (void)hipFree(b->ticket); (void)hipFree(b->bounds);
+    (void)hipFree(b->dev_words);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
     delete b;
@@ -423,7 +532,7 @@ PGA_API pga_status pga_lbvh_instance_aabbs(pga_lbvh* b, const pga_instance* inst
 PGA_API pga_status pga_lbvh_bounds(pga_lbvh* b, const float* aabbs, uint32_t n, float bounds[6]) {
     if (!b || !aabbs || !bounds || n == 0) return lfail(b, PGA_ERR_INVALID_ARG, "bad bounds arguments");
     const uint32_t blocks = std::min<uint32_t>(1024u, (n + 255u) / 256u);
-    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, b->stream, aabbs, n, b->bounds);
+    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, b->stream, aabbs, n, nullptr, b->bounds);
     hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(64), 0, b->stream, b->bounds, blocks);
     LHIP(b, hipGetLastError());
     LHIP(b, hipMemcpyAsync(bounds, b->bounds, 6u * 4u, hipMemcpyDeviceToHost, b->stream));
@@ -438,34 +547,75 @@ PGA_API pga_status pga_lbvh_build(pga_lbvh* b, const float* aabbs, uint32_t n, c
         return lfail(b, PGA_ERR_INVALID_ARG, "null argument");
     if (n < 2u || n > b->capacity)
         return lfail(b, PGA_ERR_INVALID_ARG, "need 2 <= n <= capacity (" + std::to_string(b->capacity) + ")");
-    const float ex0 = scene_max[0] - scene_min[0], ex1 = scene_max[1] - scene_min[1], ex2 = scene_max[2] - scene_min[2];
-    if (!(ex0 > 0.0f) || !(ex1 > 0.0f) || !(ex2 > 0.0f)) return lfail(b, PGA_ERR_INVALID_ARG, "empty scene box");
-    hipStream_t s = b->stream;
-    LHIP(b, hipEventRecord(b->e0, s));
-    const uint32_t g = (n + 255u) / 256u;
-    hipLaunchKernelGGL(k_morton, dim3(g), dim3(256), 0, s, aabbs, n, scene_min[0], scene_min[1], scene_min[2],
-                       ex0, ex1, ex2, b->code[0], b->idx[0]);
-    const uint32_t nb = sort_blocks(n);
-    const uint32_t nh = 256u * nb;
-    const uint32_t np = (nh + kScanTile - 1u) / kScanTile;
-    int cur = 0;
-    for (uint32_t pass = 0; pass < 4; ++pass) {     // 30-bit codes: digits at bits 0, 8, 16, 24
-        const uint32_t shift = 8u * pass;
-        hipLaunchKernelGGL(k_hist, dim3(nb), dim3(kThreads), 0, s, b->code[cur], n, shift, b->hist, nb);
-        hipLaunchKernelGGL(k_scan_reduce, dim3(np), dim3(kThreads), 0, s, b->hist, nh, b->part);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kThreads), 0, s, b->part, np);
-        hipLaunchKernelGGL(k_scan_down, dim3(np), dim3(kThreads), 0, s, b->hist, nh, b->part);
-        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kThreads), 0, s, b->code[cur], b->idx[cur], b->code[cur ^ 1],
-                           b->idx[cur ^ 1], n, shift, b->hist, nb);
-        cur ^= 1;
+    const float ex[3] = {scene_max[0] - scene_min[0], scene_max[1] - scene_min[1], scene_max[2] - scene_min[2]};
+    if (!(ex[0] > 0.0f) || !(ex[1] > 0.0f) || !(ex[2] > 0.0f)) return lfail(b, PGA_ERR_INVALID_ARG, "empty scene box");
+    LHIP(b, hipEventRecord(b->e0, b->stream));
+    LHIP(b, enqueue_build(b, aabbs, n, nullptr, scene_min, ex, nullptr, keys, left, right, parent, leaf_parent,
+                          node_aabb));
+    LHIP(b, hipEventRecord(b->e1, b->stream));
+    LHIP(b, hipStreamSynchronize(b->stream));
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_lbvh_gather_instances(pga_lbvh* b, const pga_instance* const* buffers,
+                                             const uint64_t* capacities, uint32_t n_types,
+                                             const uint64_t* device_counts, uint32_t planar_mask, float* aabbs) {
+    if (!b || !buffers || !capacities || !device_counts || !aabbs || n_types == 0 || n_types > PGA_MAX_TERMINALS)
+        return lfail(b, PGA_ERR_INVALID_ARG, "bad gather arguments");
+    GatherArgs g{};
+    for (uint32_t t = 0; t < n_types; ++t) {
+        if (!buffers[t] && capacities[t]) return lfail(b, PGA_ERR_INVALID_ARG, "null buffer with nonzero capacity");
+        g.buf[t] = buffers[t];
+        g.cap[t] = capacities[t];
     }
-    hipLaunchKernelGGL(k_karras, dim3(g), dim3(256), 0, s, b->code[cur], b->idx[cur], n,
-                       reinterpret_cast<unsigned long long*>(keys), left, right, parent, leaf_parent, b->ticket);
-    hipLaunchKernelGGL(k_refit, dim3(g), dim3(256), 0, s, aabbs, b->idx[cur], n, left, right, parent, leaf_parent,
-                       node_aabb, b->ticket);
+    g.ntypes = n_types;
+    g.planar_mask = planar_mask;
+    g.capacity = b->capacity;
+    const uint32_t blocks = std::min<uint32_t>(kGatherBlocks, (b->capacity + 255u) / 256u);
+    hipLaunchKernelGGL(k_gather_boxes, dim3(blocks), dim3(256), 0, b->stream, g,
+                       reinterpret_cast<const unsigned long long*>(device_counts), aabbs, b->dev_words,
+                       b->dev_words + 1);
     LHIP(b, hipGetLastError());
-    LHIP(b, hipEventRecord(b->e1, s));
-    LHIP(b, hipStreamSynchronize(s));
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_lbvh_build_device(pga_lbvh* b, const float* aabbs, const float* scene_min,
+                                         const float* scene_max, uint64_t* keys, int32_t* left, int32_t* right,
+                                         int32_t* parent, int32_t* leaf_parent, float* node_aabb) {
+    if (!b || !aabbs || !keys || !left || !right || !parent || !leaf_parent || !node_aabb)
+        return lfail(b, PGA_ERR_INVALID_ARG, "null argument");
+    if (!scene_min != !scene_max) return lfail(b, PGA_ERR_INVALID_ARG, "give both scene corners or neither");
+    const uint32_t n = b->capacity;
+    const uint32_t* nd = b->dev_words;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    LHIP(b, hipStreamIsCapturing(b->stream, &cs));
+    const bool timed = cs == hipStreamCaptureStatusNone;
+    if (timed) LHIP(b, hipEventRecord(b->e0, b->stream));
+    if (scene_min) {
+        const float ex[3] = {scene_max[0] - scene_min[0], scene_max[1] - scene_min[1], scene_max[2] - scene_min[2]};
+        if (!(ex[0] > 0.0f) || !(ex[1] > 0.0f) || !(ex[2] > 0.0f))
+            return lfail(b, PGA_ERR_INVALID_ARG, "empty scene box");
+        LHIP(b, enqueue_build(b, aabbs, n, nd, scene_min, ex, nullptr, keys, left, right, parent, leaf_parent,
+                              node_aabb));
+    } else {
+        const uint32_t blocks = std::min<uint32_t>(1024u, (n + 255u) / 256u);
+        hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, b->stream, aabbs, n, nd, b->bounds);
+        hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(64), 0, b->stream, b->bounds, blocks);
+        const float zero[3] = {0.0f, 0.0f, 0.0f}, one[3] = {1.0f, 1.0f, 1.0f};
+        LHIP(b, enqueue_build(b, aabbs, n, nd, zero, one, b->bounds, keys, left, right, parent, leaf_parent,
+                              node_aabb));
+    }
+    if (timed) LHIP(b, hipEventRecord(b->e1, b->stream));
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_lbvh_leaf_count(pga_lbvh* b, uint32_t* n, int32_t* overflow) {
+    if (!b || !n) return lfail(b, PGA_ERR_INVALID_ARG, "null argument");
+    uint32_t w[2] = {0u, 0u};
+    LHIP(b, hipMemcpyAsync(w, b->dev_words, sizeof(w), hipMemcpyDeviceToHost, b->stream));
+    LHIP(b, hipStreamSynchronize(b->stream));
+    *n = w[0];
+    if (overflow) *overflow = w[1] ? 1 : 0;
     return PGA_OK;
 }
 

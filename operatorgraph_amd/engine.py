@@ -7,6 +7,7 @@ terminal buffers, execute. Device memory comes from torch (plumbing only);
 every derivation runs in the HIP kernels of libpga_amd.so. There is no CPU
 fallback: without the built library, or without a GPU, construction raises.
 """
+import contextlib
 import ctypes as C
 import os
 
@@ -64,6 +65,13 @@ def load_library():
         "pga_tape_reserve": (C.c_int, [P, C.c_uint32, C.c_uint32]),
         "pga_forward_tape": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_double)]),
         "pga_backward": (C.c_int, [P, C.POINTER(C.c_void_p * abi.MAX_TERMINALS), P, C.POINTER(C.c_double)]),
+        "pga_capture_begin": (C.c_int, [P]),
+        "pga_capture_end": (C.c_int, [P, C.POINTER(P)]),
+        "pga_graph_launch": (C.c_int, [P, P]),
+        "pga_graph_destroy": (None, [P]),
+        "pga_graph_info": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "pga_device_counts": (C.c_int, [P, C.POINTER(P)]),
+        "pga_get_stream": (C.c_int, [P, C.POINTER(P)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -206,6 +214,47 @@ class Engine:
         self._check(self.L.pga_kernel_seconds(self._h, C.byref(sec)))
         return sec.value
 
+    # -- HIP graphs --
+    def stream(self):
+        """The context's HIP stream (pga_get_stream) as an integer handle:
+        create an LBVH on it to record its device-count build in a graph."""
+        h = C.c_void_p()
+        self._check(self.L.pga_get_stream(self._h, C.byref(h)))
+        return h.value or 0
+
+    def device_counts(self):
+        """Device address of the 64-bit terminal counters (pga_device_counts)."""
+        h = C.c_void_p()
+        self._check(self.L.pga_device_counts(self._h, C.byref(h)))
+        return h.value
+
+    @contextlib.contextmanager
+    def capture(self):
+        """Records the work enqueued on the context stream inside the block
+        (execute_async, device-count LBVH calls on self.stream()) into a
+        Graph instead of running it:
+
+            with eng.capture() as g:
+                eng.execute_async()
+            g.launch(); eng.synchronize()
+
+        Values in the bound buffers may change between replays (update them
+        while no replay runs: after synchronize()); a change of bindings,
+        program layout or configuration makes launch() refuse the graph."""
+        g = Graph(self)
+        self._check(self.L.pga_capture_begin(self._h))
+        try:
+            yield g
+        except BaseException:
+            h = C.c_void_p()
+            self.L.pga_capture_end(self._h, C.byref(h))
+            if h.value:
+                self.L.pga_graph_destroy(h.value)
+            raise
+        h = C.c_void_p()
+        self._check(self.L.pga_capture_end(self._h, C.byref(h)))
+        g._h = h.value
+
     def autotune(self, n=None, repeats=3):
         """Times every launch configuration on the bound axioms and keeps the
         fastest (pga_autotune); returns the chosen configuration."""
@@ -302,3 +351,34 @@ def derive(program, axioms, capacities, technique=abi.MEGAKERNEL, params=None, k
     st = eng.stats()
     eng.close()
     return out, st
+
+
+class Graph:
+    """A recorded sequence of derivations (pga_graph); see Engine.capture."""
+
+    def __init__(self, engine):
+        self.engine = engine
+        self._h = None
+
+    def launch(self):
+        """Replays the recording on the context stream without waiting;
+        Engine.synchronize() waits and reports errors and stats."""
+        e = self.engine
+        e._check(e.L.pga_graph_launch(e._h, self._h))
+
+    def info(self):
+        """(kernel nodes, cooperative kernel nodes) of the graph."""
+        k, c = C.c_uint32(0), C.c_uint32(0)
+        self.engine._check(self.engine.L.pga_graph_info(self._h, C.byref(k), C.byref(c)))
+        return k.value, c.value
+
+    def close(self):
+        if self._h:
+            self.engine.L.pga_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -166,6 +166,30 @@ def stochastic():
     return Program(INTERPRETER_PROCEDURES, table, "stochastic")
 
 
+def combine(*parts, name="combined"):
+    """One dispatch table holding several grammars over the interpreter's
+    procedure list, so that axioms of all of them derive in one launch (an
+    axiom's entry_index selects its grammar). parts: (program,
+    terminal_offset) pairs. Successor indices move with their grammar's
+    first entry; Generate's terminal index (parameter 0) moves by
+    terminal_offset. Returns (program, first entry index of each part)."""
+    entries, starts = [], []
+    for prog, toff in parts:
+        if [p.key() for p in prog.procedures.procedures] != \
+                [p.key() for p in INTERPRETER_PROCEDURES.procedures]:
+            raise ValueError(f"{prog.name}: not over the interpreter procedure list")
+        base = len(entries)
+        starts.append(base)
+        for e in prog.table.entries:
+            params = [Parameter(p.type, list(p.values)) for p in e.parameters]
+            if INTERPRETER_PROCEDURES.procedures[e.operator_code].op_type == abi.GENERATE and toff:
+                params[0] = Parameter(abi.PT_SCALAR, [params[0].values[0] + toff])
+            succ = [(s.entry_index + base if s.entry_index >= 0 else s.entry_index, s.phase_index)
+                    for s in e.successors]
+            entries.append(Entry(e.operator_code, params, succ, e.edge_indices, e.sub_graph_index))
+    return Program(INTERPRETER_PROCEDURES, DispatchTable(entries), name), starts
+
+
 # per-lot parameter slots of facade_params()
 FACADE_PARAM_NAMES = ("left_weight", "right_weight", "floor_height", "tile_width",
                       "window_width", "window_recess", "sill_height", "lintel_height")

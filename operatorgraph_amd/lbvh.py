@@ -17,6 +17,12 @@ _SIGS = {
     "pga_lbvh_build": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_float * 3),
                                  C.POINTER(C.c_float * 3)] + [C.c_void_p] * 6),
     "pga_lbvh_seconds": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "pga_lbvh_gather_instances": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p * abi.MAX_TERMINALS),
+                                            C.POINTER(C.c_uint64 * abi.MAX_TERMINALS), C.c_uint32, C.c_void_p,
+                                            C.c_uint32, C.c_void_p]),
+    "pga_lbvh_build_device": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float * 3),
+                                        C.POINTER(C.c_float * 3)] + [C.c_void_p] * 6),
+    "pga_lbvh_leaf_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]),
 }
 
 
@@ -98,6 +104,53 @@ class LBVH:
                                           ptr(node_aabb)))
         return dict(keys=keys, left=left, right=right, parent=parent, leaf_parent=leaf_parent,
                     node_aabb=node_aabb)
+
+    # -- device-count form: the leaf count stays on the device --
+    def alloc(self):
+        """Device arrays for a build of up to `capacity` leaves: aabbs and the
+        build_device outputs (a graph records their addresses)."""
+        torch = self.torch
+        n = self.capacity
+        dev = "cuda"
+        return dict(aabbs=torch.empty((n, 6), dtype=torch.float32, device=dev),
+                    keys=torch.empty(n, dtype=torch.int64, device=dev),
+                    left=torch.empty(n - 1, dtype=torch.int32, device=dev),
+                    right=torch.empty(n - 1, dtype=torch.int32, device=dev),
+                    parent=torch.empty(n - 1, dtype=torch.int32, device=dev),
+                    leaf_parent=torch.empty(n, dtype=torch.int32, device=dev),
+                    node_aabb=torch.empty((n - 1, 6), dtype=torch.float32, device=dev))
+
+    def gather_engine(self, engine, n_types, planar_mask, out):
+        """Boxes of the instances of terminal types 0..n_types-1 bound to
+        `engine`, counted on the device (pga_lbvh_gather_instances)."""
+        bufs = (C.c_void_p * abi.MAX_TERMINALS)()
+        caps = (C.c_uint64 * abi.MAX_TERMINALS)()
+        for t in range(n_types):
+            b = engine.buffer(t)
+            if b is not None:
+                bufs[t] = b.data_ptr()
+                caps[t] = b.shape[0]
+        self._check(self.L.pga_lbvh_gather_instances(self._h, C.byref(bufs), C.byref(caps), n_types,
+                                                     C.c_void_p(engine.device_counts()), planar_mask,
+                                                     C.c_void_p(out["aabbs"].data_ptr())))
+
+    def build_device(self, out, scene_min=None, scene_max=None):
+        """Builds over the leaf count of the last gather into `out` (from
+        alloc()); the scene box is the union of the boxes when not given."""
+        ptr = lambda k: C.c_void_p(out[k].data_ptr())
+        smin = smax = None
+        if scene_min is not None:
+            smin = C.byref((C.c_float * 3)(*[float(v) for v in scene_min]))
+            smax = C.byref((C.c_float * 3)(*[float(v) for v in scene_max]))
+        self._check(self.L.pga_lbvh_build_device(self._h, ptr("aabbs"), smin, smax, ptr("keys"), ptr("left"),
+                                                 ptr("right"), ptr("parent"), ptr("leaf_parent"),
+                                                 ptr("node_aabb")))
+
+    def leaf_count(self):
+        """Waits for the builder's stream; (leaf count, overflowed)."""
+        n, o = C.c_uint32(0), C.c_int32(0)
+        self._check(self.L.pga_lbvh_leaf_count(self._h, C.byref(n), C.byref(o)))
+        return n.value, bool(o.value)
 
     def seconds(self):
         s = C.c_double(0.0)
