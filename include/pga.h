@@ -386,6 +386,14 @@ PGA_API pga_status pga_graph_info(const pga_graph* graph, uint32_t* kernel_nodes
 PGA_API pga_status pga_device_counts(pga_context* ctx, const uint64_t** device_counts);
 PGA_API pga_status pga_get_stream(pga_context* ctx, void** stream);
 
+/* -- device memory for callers that link no HIP runtime themselves (the
+ *    C++ front end in pga/core/Core.h allocates terminal buffers with
+ *    these; the reference's evaluator called cudaMalloc in its headers,
+ *    techniqueMegakernel.cuh:607). -- */
+PGA_API pga_status pga_device_alloc(uint64_t bytes, void** device_ptr);
+PGA_API pga_status pga_device_free(void* device_ptr);
+PGA_API pga_status pga_copy_to_host(void* host_dst, const void* device_src, uint64_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,158 @@
+/*
+ * pga/facade/Interpreter.h - the DSL interpreter's entry points
+ * (src/interpreter/PGAFacade.h:17-21, PGAFacade.cu:354-923) on the C ABI:
+ * getProcedureList(), initializePGA(dispatch table), executePGA(),
+ * releasePGA(), destroyPGA(), and setAxioms for the interpreter's axioms
+ * (Device::setAxioms, src/interpreter/GlobalVariables.cuh:47-59; each axiom
+ * is a Box or Quad at the origin of size 1 whose seed and custom attribute
+ * are Seeds[index % 32], PGAFacade.cu:684-729).
+ *
+ * The procedure list is the interpreter's for Box and Quad, in its order
+ * (PGAFacade.cu:354-392), so dispatch tables compiled against the
+ * interpreter's getProcedureList() index it unchanged. Header-only: the
+ * evaluator lives in a function-local static shared by all translation
+ * units.
+ */
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "../core/Core.h"
+
+namespace PGA {
+namespace Compiler {
+
+// SingleOperatorProcedure / ProcedureList (ProcedureList.h:18-69)
+struct SingleOperatorProcedure {
+    int opType;
+    int shapeType;
+    int genFuncIdx;
+};
+
+struct ProcedureList {
+    std::vector<SingleOperatorProcedure> procedures;
+    int indexOf(int opType, int shapeType, int genFuncIdx = 0) const {
+        for (size_t i = 0; i < procedures.size(); ++i)
+            if (procedures[i].opType == opType && procedures[i].shapeType == shapeType &&
+                procedures[i].genFuncIdx == genFuncIdx)
+                return (int)i;
+        return -1;
+    }
+    size_t size() const { return procedures.size(); }
+};
+
+} // namespace Compiler
+
+namespace Interpreter {
+
+using namespace PGA::Operators;
+using namespace PGA::Parameters;
+using PGA::Shapes::Box;
+using PGA::Shapes::Quad;
+
+// Quad: Discard .. Scale, four SetAs, Subdivide, SwapSize, Translate;
+// Box: ComponentSplit first, no Extrude (PGAFacade.cu:354-392)
+struct ProcList
+    : T::List<
+          Proc<Quad, Discard>, Proc<Quad, Extrude<DynParam<0>, DynParam<1>, DCall<0>>>,
+          Proc<Quad, Generate<false, 0, DynParam<0>>>, Proc<Quad, Generate<false, 1, DynParam<0>>>,
+          Proc<Quad, If<DynParam<0>, DCall<0>, DCall<1>>>,
+          Proc<Quad, IfSizeLess<DynParam<0>, DynParam<1>, DCall<0>, DCall<1>>>, Proc<Quad, RandomRule<DynParams>>,
+          Proc<Quad, Repeat<false, DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>, Proc<Quad, Replicate<DynParams>>,
+          Proc<Quad, Rotate<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
+          Proc<Quad, Scale<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
+          Proc<Quad, SetAsDynamicConvexPolygon<DCall<0>>>, Proc<Quad, SetAsDynamicConvexRightPrism<DCall<0>>>,
+          Proc<Quad, SetAsDynamicConcavePolygon<DCall<0>>>, Proc<Quad, SetAsDynamicConcaveRightPrism<DCall<0>>>,
+          Proc<Quad, Subdivide<DynParams>>, Proc<Quad, SwapSize<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
+          Proc<Quad, Translate<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
+          Proc<Box, ComponentSplit<false, DCall<0>, DCall<1>, DCall<2>>>, Proc<Box, Discard>,
+          Proc<Box, Generate<true, 0, DynParam<0>>, 16>, Proc<Box, Generate<false, 1, DynParam<0>>>,
+          Proc<Box, If<DynParam<0>, DCall<0>, DCall<1>>>,
+          Proc<Box, IfSizeLess<DynParam<0>, DynParam<1>, DCall<0>, DCall<1>>>, Proc<Box, RandomRule<DynParams>>,
+          Proc<Box, Repeat<false, DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>, Proc<Box, Replicate<DynParams>>,
+          Proc<Box, Rotate<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
+          Proc<Box, Scale<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
+          Proc<Box, SetAsDynamicConvexPolygon<DCall<0>>>, Proc<Box, SetAsDynamicConvexRightPrism<DCall<0>>>,
+          Proc<Box, SetAsDynamicConcavePolygon<DCall<0>>>, Proc<Box, SetAsDynamicConcaveRightPrism<DCall<0>>>,
+          Proc<Box, Subdivide<DynParams>>, Proc<Box, SwapSize<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
+          Proc<Box, Translate<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>> {};
+
+struct Configuration {
+    static const PGA::GPU::Technique Technique = PGA::GPU::MEGAKERNEL;
+    static const unsigned int QueueSize = 1024 * 32;   // PGAFacade.cu:883-893
+    static const unsigned int MaxSharedMemory = 0;
+};
+
+// The interpreter's axiom record (src/interpreter/Axiom.h:8-15), Box and
+// Quad only.
+struct Axiom {
+    PGA::Compiler::ShapeType shapeType;
+    int entryIndex;
+};
+
+struct AxiomGenerator : PGA::ListedAxioms {};
+
+typedef PGA::SinglePhaseEvaluator<ProcList, AxiomGenerator, PGA::Rendering::GenFuncFilter, false, 0, 0, Configuration>
+    Evaluator;
+typedef std::unique_ptr<Evaluator, PGA::ReleaseCallback> EvaluatorPtr;
+
+inline EvaluatorPtr& evaluator() {
+    static EvaluatorPtr e;
+    return e;
+}
+inline std::vector<uint64_t>& capacities() {
+    static std::vector<uint64_t> c;
+    return c;
+}
+
+} // namespace Interpreter
+} // namespace PGA
+
+// the interpreter's axioms: a Box or Quad at the origin, size 1, seeded
+// from the seed table by index
+inline void setAxioms(const PGA::Interpreter::Axiom* axioms, unsigned int n) {
+    std::vector<pga_axiom> a(n);
+    for (unsigned int i = 0; i < n; ++i) {
+        std::memset(&a[i], 0, sizeof(pga_axiom));
+        a[i].model[0] = a[i].model[5] = a[i].model[10] = 1.0f;
+        a[i].size[0] = a[i].size[1] = a[i].size[2] = 1.0f;
+        a[i].seed = a[i].custom = PGA::GlobalVars::getSeed((int)i);
+        a[i].shape_type = axioms[i].shapeType;
+        a[i].entry_index = axioms[i].entryIndex;
+    }
+    PGA::Interpreter::AxiomGenerator::set(a.data(), n);
+}
+
+// capacity of the instance buffer of terminal type `terminal`
+inline void setMaxNumInstances(unsigned int terminal, uint64_t capacity) {
+    auto& c = PGA::Interpreter::capacities();
+    if (c.size() <= terminal) c.resize(terminal + 1, 0);
+    c[terminal] = capacity;
+}
+
+inline PGA::Compiler::ProcedureList getProcedureList() {
+    PGA::Compiler::ProcedureList l;
+    for (const auto& t : PGA::Lowering::lower<PGA::Interpreter::ProcList>()) {
+        const auto& n = t.nodes[0];
+        l.procedures.push_back({n.op_type, n.shape_type, n.gen_func});
+    }
+    return l;
+}
+
+inline void initializePGA(const std::unique_ptr<PGA::DispatchTable>& dispatchTable) {
+    auto& e = PGA::Interpreter::evaluator();
+    if (!e) e = PGA::Interpreter::EvaluatorPtr(new PGA::Interpreter::Evaluator());
+    const auto& c = PGA::Interpreter::capacities();
+    for (unsigned int t = 0; t < c.size(); ++t)
+        if (c[t]) e->setMaxNumInstances(t, c[t]);
+    e->initialize(dispatchTable->toDispatchTableEntriesPtr().get(), dispatchTable->entries.size());
+}
+
+inline double executePGA() { return PGA::Interpreter::evaluator()->execute(); }
+
+// as the reference: gives the evaluator up without releasing it
+// (PGAFacade.cu:913-916); destroyPGA releases it
+inline void releasePGA() { (void)PGA::Interpreter::evaluator().release(); }
+
+inline void destroyPGA() { PGA::Interpreter::evaluator().reset(); }

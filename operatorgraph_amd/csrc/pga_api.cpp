@@ -1230,6 +1230,24 @@ PGA_API pga_status pga_device_counts(pga_context* ctx, const uint64_t** device_c
     return PGA_OK;
 }
 
+PGA_API pga_status pga_device_alloc(uint64_t bytes, void** ptr) {
+    if (!ptr) return PGA_ERR_INVALID_ARG;
+    *ptr = nullptr;
+    if (bytes == 0) return PGA_OK;
+    return hipMalloc(ptr, bytes) == hipSuccess ? PGA_OK : PGA_ERR_HIP;
+}
+
+PGA_API pga_status pga_device_free(void* ptr) {
+    if (!ptr) return PGA_OK;
+    return hipFree(ptr) == hipSuccess ? PGA_OK : PGA_ERR_HIP;
+}
+
+PGA_API pga_status pga_copy_to_host(void* dst, const void* src, uint64_t bytes) {
+    if ((!dst || !src) && bytes) return PGA_ERR_INVALID_ARG;
+    if (!bytes) return PGA_OK;
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? PGA_OK : PGA_ERR_HIP;
+}
+
 PGA_API pga_status pga_get_stream(pga_context* ctx, void** stream) {
     if (!ctx || !stream) return fail(ctx, PGA_ERR_INVALID_ARG, "null argument");
     *stream = (void*)ctx->stream;
