@@ -10,17 +10,22 @@
 //                  ballots, per-workgroup digit cursors in LDS), so sorting the
 //                  codes of indices in ascending order gives the order of the
 //                  reference's unique 64-bit keys (code << 32 | index), which
-//                  thrust::sort orders (:311-312);
+//                  thrust::sort orders (:311-312); the last pass writes those
+//                  keys directly;
 //   3. k_karras  - one thread per internal node: direction, range and split
 //                  by binary search over the common-prefix length of the
-//                  64-bit keys (buildRadixTree, :88-213);
-//   4. k_refit   - one thread per leaf walks up: the first child to reach a
-//                  node stops, the second unites both boxes and goes on
-//                  (propagateBVH's ticket, :215-261). Boxes written and read
-//                  inside the kernel use agent-scope atomic loads and stores,
-//                  which are coherent across the XCDs' L2 caches without a
-//                  full write-back, and each store completes before the
-//                  ticket of the parent is taken.
+//                  64-bit keys (buildRadixTree, :88-213); it also records the
+//                  far end of each node's leaf range;
+//   4. refit     - climbs from the leaves: the first child to reach a node
+//                  stops, the second unites both boxes and goes on
+//                  (propagateBVH's ticket, :215-261). k_refit_local completes
+//                  the nodes whose leaf range lies inside one workgroup's tile
+//                  of sorted leaves, through LDS; k_refit_global the nodes
+//                  spanning tiles, through global memory, where boxes are
+//                  written and read with agent-scope atomic loads and stores,
+//                  coherent across the XCDs' L2 caches without a full
+//                  write-back, and each store completes before the ticket of
+//                  the parent is taken.
 // Children are encoded as in the reference: >= 0 a leaf (sorted position),
 // < 0 an internal node (negated index); node 0 is the root.
 //
@@ -44,6 +49,7 @@ constexpr uint32_t kItems = 16;
 constexpr uint32_t kTile = kThreads * kItems;   // keys per sort workgroup
 constexpr uint32_t kScanTile = 4096;            // histogram entries per scan workgroup
 constexpr uint32_t kGatherBlocks = 8192;         // grid-stride workgroups of the terminal gather
+constexpr uint32_t kRefitTile = 1024;            // sorted leaves per refit workgroup (256 threads)
 
 // leaf count: the host bound, or the device word when there is one
 __device__ __forceinline__ uint32_t leaf_count(uint32_t n, const uint32_t* nd) {
@@ -155,12 +161,21 @@ __global__ void __launch_bounds__(256) k_bounds(const float* __restrict__ aabbs,
     }
     if (threadIdx.x < 6) part[blockIdx.x * 6u + threadIdx.x] = red[threadIdx.x][0];
 }
-__global__ void __launch_bounds__(64) k_bounds_fold(float* part, uint32_t nparts) {
-    if (threadIdx.x >= 6) return;
-    const uint32_t q = threadIdx.x;
-    float v = part[q];
-    for (uint32_t b = 1; b < nparts; ++b) v = q < 3 ? fminf(v, part[b * 6u + q]) : fmaxf(v, part[b * 6u + q]);
-    part[q] = v;
+__global__ void __launch_bounds__(256) k_bounds_fold(float* part, uint32_t nparts) {
+    __shared__ float red[6][256];
+    float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x)
+        for (int q = 0; q < 6; ++q) v[q] = q < 3 ? fminf(v[q], part[b * 6u + q]) : fmaxf(v[q], part[b * 6u + q]);
+    for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int q = 0; q < 6; ++q)
+                red[q][threadIdx.x] = q < 3 ? fminf(red[q][threadIdx.x], red[q][threadIdx.x + h])
+                                            : fmaxf(red[q][threadIdx.x], red[q][threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[threadIdx.x] = red[threadIdx.x][0];
 }
 
 // scene box from the host (mn, ex) or, with `box`, from the device: min xyz,
@@ -262,9 +277,12 @@ __global__ void __launch_bounds__(kThreads) k_scan_down(uint32_t* __restrict__ a
     }
 }
 
+// keys_out (the last pass): the sorted 64-bit keys (code << 32 | index)
+// instead of the code and index arrays
 __global__ void __launch_bounds__(kThreads) k_scatter(const uint32_t* __restrict__ code_in,
                                                       const uint32_t* __restrict__ idx_in,
                                                       uint32_t* __restrict__ code_out, uint32_t* __restrict__ idx_out,
+                                                      unsigned long long* __restrict__ keys_out,
                                                       uint32_t n, const uint32_t* nd, uint32_t shift,
                                                       const uint32_t* __restrict__ offs, uint32_t nblocks) {
     __shared__ uint32_t base_d[256];
@@ -298,8 +316,12 @@ __global__ void __launch_bounds__(kThreads) k_scatter(const uint32_t* __restrict
             uint32_t off = base_d[d] + run[d] + rank;
             for (uint32_t k = 0; k < w; ++k) off += wcnt[k][d];
             if (off < n) {            // always true when the histograms are consistent
-                code_out[off] = key;
-                idx_out[off] = val;
+                if (keys_out) {
+                    keys_out[off] = ((unsigned long long)key << 32) | (unsigned long long)val;
+                } else {
+                    code_out[off] = key;
+                    idx_out[off] = val;
+                }
             }
         }
         __syncthreads();
@@ -310,37 +332,35 @@ __global__ void __launch_bounds__(kThreads) k_scatter(const uint32_t* __restrict
 }
 
 // ---- Karras build ----
-__device__ __forceinline__ unsigned long long key_at(const uint32_t* code, const uint32_t* idx, int64_t i) {
-    return ((unsigned long long)code[i] << 32) | (unsigned long long)idx[i];
-}
-__device__ __forceinline__ int delta(const uint32_t* code, const uint32_t* idx, int64_t n, int64_t i, int64_t j) {
+__device__ __forceinline__ int delta(const unsigned long long* keys, int64_t n, int64_t i, int64_t j) {
     if (j < 0 || j >= n) return -1;
-    return __clzll((long long)(key_at(code, idx, i) ^ key_at(code, idx, j)));
+    return __clzll((long long)(keys[i] ^ keys[j]));
 }
 
-__global__ void __launch_bounds__(256) k_karras(const uint32_t* __restrict__ code, const uint32_t* __restrict__ idx,
-                                                uint32_t n, const uint32_t* nd, unsigned long long* __restrict__ keys,
-                                                int32_t* __restrict__ left, int32_t* __restrict__ right,
-                                                int32_t* __restrict__ parent, int32_t* __restrict__ leaf_parent,
+// other_end[i]: the end of node i's leaf range other than i (the range is
+// [min(i, j), max(i, j)])
+__global__ void __launch_bounds__(256) k_karras(const unsigned long long* __restrict__ keys, uint32_t n,
+                                                const uint32_t* nd, int32_t* __restrict__ left,
+                                                int32_t* __restrict__ right, int32_t* __restrict__ parent,
+                                                int32_t* __restrict__ leaf_parent, int32_t* __restrict__ other_end,
                                                 uint32_t* __restrict__ ticket) {
     const int64_t nn = leaf_count(n, nd);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nn) keys[i] = key_at(code, idx, i);
     if (i >= nn - 1) return;
     ticket[i] = 0u;
     if (i == 0) parent[0] = -1;
-    const int dir = (delta(code, idx, nn, i, i + 1) - delta(code, idx, nn, i, i - 1)) >= 0 ? 1 : -1;
-    const int dmin = delta(code, idx, nn, i, i - dir);
+    const int dir = (delta(keys, nn, i, i + 1) - delta(keys, nn, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, nn, i, i - dir);
     int64_t lmax = 2;
-    while (delta(code, idx, nn, i, i + lmax * dir) > dmin) lmax *= 2;
+    while (delta(keys, nn, i, i + lmax * dir) > dmin) lmax *= 2;
     int64_t l = 0;
     for (int64_t t = lmax / 2; t >= 1; t /= 2)
-        if (delta(code, idx, nn, i, i + (l + t) * dir) > dmin) l += t;
+        if (delta(keys, nn, i, i + (l + t) * dir) > dmin) l += t;
     const int64_t j = i + l * dir;
-    const int dnode = delta(code, idx, nn, i, j);
+    const int dnode = delta(keys, nn, i, j);
     int64_t s = 0;
     for (int64_t t = (l + 1) / 2;; t = (t + 1) / 2) {
-        if (delta(code, idx, nn, i, i + (s + t) * dir) > dnode) s += t;
+        if (delta(keys, nn, i, i + (s + t) * dir) > dnode) s += t;
         if (t == 1) break;
     }
     const int64_t gamma = i + s * dir + (dir < 0 ? -1 : 0);
@@ -348,6 +368,7 @@ __global__ void __launch_bounds__(256) k_karras(const uint32_t* __restrict__ cod
     const int32_t rc = (max(i, j) == gamma + 1) ? (int32_t)(gamma + 1) : -(int32_t)(gamma + 1);
     left[i] = lc;
     right[i] = rc;
+    other_end[i] = (int32_t)j;
     if (lc < 0) parent[-lc] = (int32_t)i; else leaf_parent[lc] = (int32_t)i;
     if (rc < 0) parent[-rc] = (int32_t)i; else leaf_parent[rc] = (int32_t)i;
 }
@@ -366,50 +387,208 @@ __device__ __forceinline__ void load_box(const float* p, float b[6]) {
                                                  __HIP_MEMORY_SCOPE_AGENT));
 }
 
-__global__ void __launch_bounds__(256) k_refit(const float* __restrict__ aabbs, const uint32_t* __restrict__ idx,
-                                               uint32_t n, const uint32_t* nd, const int32_t* __restrict__ left,
-                                               const int32_t* __restrict__ right, const int32_t* __restrict__ parent,
-                                               const int32_t* __restrict__ leaf_parent, float* node_aabb,
-                                               uint32_t* ticket) {
-    n = leaf_count(n, nd);
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n || n < 2u) return;             // one leaf: no internal node, no parent links
-    float b[6];
-    const float* src = aabbs + (size_t)idx[k] * 6u;
+// The union is taken in child order, left box first: min(L, R) = R < L ? R : L
+// and max(L, R) = L < R ? R : L, as std::min / std::max fold them.
+__device__ __forceinline__ void unite(const float b[6], const float o[6], bool mine_left, float u[6]) {
 #pragma unroll
-    for (int q = 0; q < 6; ++q) b[q] = src[q];
-    int32_t child = (int32_t)k;           // leaf k (>= 0) or internal -c
-    int32_t node = leaf_parent[k];
-    // a radix tree over unique 64-bit keys is at most 65 levels deep
+    for (int q = 0; q < 6; ++q) {
+        const float L = mine_left ? b[q] : o[q], R = mine_left ? o[q] : b[q];
+        u[q] = q < 3 ? (R < L ? R : L) : (L < R ? R : L);
+    }
+}
+
+// a node is local to the tile [s, e) of sorted leaves when its leaf range
+// [min(i, j), max(i, j)] lies inside it; both its children then do too
+__device__ __forceinline__ bool is_local(int32_t i, int32_t j, uint32_t s, uint32_t e) {
+    return (uint32_t)min(i, j) >= s && (uint32_t)max(i, j) < e;
+}
+__device__ __forceinline__ uint32_t tile_of(uint32_t leaf) { return leaf / kRefitTile; }
+
+// Refit, first kernel: one workgroup per tile of kRefitTile sorted leaves
+// completes the tile's local nodes. A local node's index lies in its own
+// range, so in the tile: the tile's topology (children, parent, range end,
+// leaf parents) and leaf boxes are staged in LDS with coalesced loads, and
+// both climbing threads of a local node belong to the workgroup, so its ticket
+// and box live in LDS too. A climb stops where it
+// would enter a node spanning tiles; the second kernel starts there. Every
+// completed box goes to node_aabb with plain stores, which the next kernel
+// sees.
+__global__ void __launch_bounds__(256) k_refit_local(const float* __restrict__ aabbs,
+                                                     const unsigned long long* __restrict__ keys, uint32_t n,
+                                                     const uint32_t* nd, const int32_t* __restrict__ left,
+                                                     const int32_t* __restrict__ right,
+                                                     const int32_t* __restrict__ parent,
+                                                     const int32_t* __restrict__ leaf_parent,
+                                                     const int32_t* __restrict__ other_end,
+                                                     float* __restrict__ node_aabb) {
+    __shared__ __align__(16) float lbox[kRefitTile * 6];   // the tile's leaf boxes, sorted order
+    __shared__ float nbox[kRefitTile * 6];    // boxes of the tile's local nodes (index - s)
+    __shared__ uint32_t ltk[kRefitTile];      // tickets of the tile's local nodes
+    __shared__ int32_t tl[kRefitTile], tr[kRefitTile], tp[kRefitTile], te[kRefitTile], tlp[kRefitTile];
+    n = leaf_count(n, nd);
+    const uint32_t s = blockIdx.x * kRefitTile;
+    if (n < 2u || s >= n) return;             // one leaf: no internal node, no parent links
+    const uint32_t e = min(s + kRefitTile, n);
+    // staging: every load of a thread's slots is issued before any is waited
+    // for. Indices are clamped into range so that the loads need no branch;
+    // validity only selects what is stored. Leaf boxes are gathered through
+    // the sorted keys, 8 bytes at a time (boxes are 24-byte records).
+    constexpr uint32_t kSlots = kRefitTile / 256u;
+    const uint32_t last_leaf = e - 1u, last_node = n - 2u;
+    uint32_t box_of[kSlots];
+#pragma unroll
+    for (uint32_t u = 0; u < kSlots; ++u)
+        box_of[u] = (uint32_t)(keys[min(s + threadIdx.x + u * 256u, last_leaf)] & 0xFFFFFFFFull);
+    float2 gb[kSlots][3];
+    int32_t vlp[kSlots], vl[kSlots], vr[kSlots], vp[kSlots], ve[kSlots];
+#pragma unroll
+    for (uint32_t u = 0; u < kSlots; ++u) {
+        const uint32_t i = s + threadIdx.x + u * 256u;
+        const float2* src = reinterpret_cast<const float2*>(aabbs + (size_t)box_of[u] * 6u);
+        gb[u][0] = src[0];
+        gb[u][1] = src[1];
+        gb[u][2] = src[2];
+        const uint32_t li = min(i, last_leaf), ni = min(i, last_node);
+        vlp[u] = leaf_parent[li];
+        vl[u] = left[ni];
+        vr[u] = right[ni];
+        vp[u] = parent[ni];
+        ve[u] = other_end[ni];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kSlots; ++u) {
+        const uint32_t k = threadIdx.x + u * 256u, i = s + k;
+        const bool internal = i < e && i <= last_node;   // internal nodes 0 .. n-2
+        ltk[k] = 0u;
+        tlp[k] = vlp[u];
+        tl[k] = vl[u];
+        tr[k] = vr[u];
+        tp[k] = vp[u];
+        te[k] = internal ? ve[u] : -1;                   // -1: never local
+        float2* d = reinterpret_cast<float2*>(&lbox[k * 6u]);
+        d[0] = gb[u][0];
+        d[1] = gb[u][1];
+        d[2] = gb[u][2];
+    }
+    __syncthreads();
+    for (uint32_t k = s + threadIdx.x; k < e; k += blockDim.x) {
+        float b[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) b[q] = lbox[(k - s) * 6u + q];
+        int32_t cur = (int32_t)k;             // leaf k, or the local node just completed
+        bool cur_leaf = true;
+        int32_t node = tlp[k - s];
+        // a radix tree over unique 64-bit keys is at most 65 levels deep
+        for (int guard = 0; node >= 0 && guard < 96; ++guard) {
+            if ((uint32_t)node < s || (uint32_t)node >= e) break;   // outside the tile: spanning
+            const uint32_t x = (uint32_t)node - s;
+            const int32_t j = te[x];
+            if (j < 0 || !is_local(node, j, s, e)) break;
+            // only LDS needs ordering here: this thread's LDS box writes are
+            // complete before its ticket, and the sibling's box is read after
+            // it (a release at workgroup scope would also wait for the global
+            // stores of node_aabb)
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            const uint32_t t = __hip_atomic_fetch_add(&ltk[x], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            asm volatile("" ::: "memory");
+            if (t == 0u) break;               // the sibling's thread finishes this node
+            const int32_t l = tl[x], r = tr[x];
+            const bool mine_left = cur_leaf ? (l == cur) : (l == -cur);
+            const int32_t sib = mine_left ? r : l;
+            const float* src = sib >= 0 ? &lbox[(uint32_t)(sib - (int32_t)s) * 6u]
+                                        : &nbox[(uint32_t)(-sib - (int32_t)s) * 6u];
+            float o[6], u[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) o[q] = src[q];
+            unite(b, o, mine_left, u);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                b[q] = u[q];
+                nbox[x * 6u + q] = u[q];
+                node_aabb[(size_t)node * 6u + q] = u[q];
+            }
+            cur = node;
+            cur_leaf = false;
+            if (node == 0) break;
+            node = tp[x];
+        }
+    }
+}
+
+// the box of child c of a node spanning tiles: a leaf's from the input, a
+// local node's as the first kernel left it
+__device__ __forceinline__ void child_box(const float* aabbs, const unsigned long long* keys, const float* node_aabb,
+                                          int32_t c, float o[6]) {
+    const float* src = c >= 0 ? aabbs + (size_t)(keys[c] & 0xFFFFFFFFull) * 6u : node_aabb + (size_t)(-c) * 6u;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) o[q] = src[q];
+}
+
+// Refit, second kernel: the nodes spanning tiles. One thread per internal
+// node; a spanning node with a child that is a leaf or a local node (both
+// complete) starts a climb there - with both children complete it unites them
+// at once, with one it takes the node's ticket - and climbs as the reference
+// does: the first arriver stops, the second unites and goes on. Boxes that
+// climbs exchange are written and read with agent-scope atomics, each store
+// complete before the next ticket.
+__global__ void __launch_bounds__(256) k_refit_global(const float* __restrict__ aabbs,
+                                                      const unsigned long long* __restrict__ keys, uint32_t n,
+                                                      const uint32_t* nd, const int32_t* __restrict__ left,
+                                                      const int32_t* __restrict__ right,
+                                                      const int32_t* __restrict__ parent,
+                                                      const int32_t* __restrict__ other_end, float* node_aabb,
+                                                      uint32_t* ticket) {
+    n = leaf_count(n, nd);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < 2u || i >= n - 1u) return;
+    const int32_t node0 = (int32_t)i, j = other_end[i];
+    const uint32_t lo = (uint32_t)min(node0, j), hi = (uint32_t)max(node0, j);
+    if (tile_of(lo) == tile_of(hi)) return;   // local: done by the first kernel
+    const int32_t l = left[i], r = right[i];
+    // a child is complete when it is a leaf or a node inside one tile
+    auto complete = [&](int32_t c) {
+        if (c >= 0) return true;
+        const int32_t jj = other_end[-c];
+        return tile_of((uint32_t)min(-c, jj)) == tile_of((uint32_t)max(-c, jj));
+    };
+    const bool lc = complete(l), rc = complete(r);
+    if (!lc && !rc) return;                   // both children arrive by climbs
+    float b[6];
+    int32_t cur, node = node0;
+    if (lc && rc) {                           // both known: this node is complete now
+        float lb[6], rb[6];
+        child_box(aabbs, keys, node_aabb, l, lb);
+        child_box(aabbs, keys, node_aabb, r, rb);
+        unite(lb, rb, true, b);
+        store_box(node_aabb + (size_t)node * 6u, b);
+        if (node == 0) return;
+        cur = -node;
+        node = parent[node];
+    } else {
+        cur = lc ? l : r;                     // arrive at this node from the complete child
+        child_box(aabbs, keys, node_aabb, cur, b);
+    }
     for (int guard = 0; node >= 0 && guard < 96; ++guard) {
         // every box this thread wrote has completed before the ticket is taken
         __builtin_amdgcn_s_waitcnt(0);
         const uint32_t t = __hip_atomic_fetch_add(ticket + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (t == 0u) return;              // the sibling's thread finishes this node
-        const int32_t l = left[node], r = right[node];
-        const int32_t sib = (l == child) ? r : l;
+        if (t == 0u) return;                  // the sibling's thread finishes this node
+        const int32_t nl = left[node], nr = right[node];
+        const bool mine_left = (nl == cur);
+        const int32_t sib = mine_left ? nr : nl;
+        // a leaf's box from the input; a node's, written by the first kernel
+        // or by a climb, with agent-scope loads (current for both)
         float o[6];
-        if (sib >= 0) {
-            const float* s2 = aabbs + (size_t)idx[sib] * 6u;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) o[q] = s2[q];
-        } else {
-            load_box(node_aabb + (size_t)(-sib) * 6u, o);
-        }
-        // union in child order, left box first: min(L, R) = R < L ? R : L and
-        // max(L, R) = L < R ? R : L, as std::min / std::max fold them
-        const bool mine_left = (l == child);
+        if (sib >= 0) child_box(aabbs, keys, node_aabb, sib, o);
+        else load_box(node_aabb + (size_t)(-sib) * 6u, o);
         float u[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const float L = mine_left ? b[q] : o[q], R = mine_left ? o[q] : b[q];
-            u[q] = q < 3 ? (R < L ? R : L) : (L < R ? R : L);
-        }
+        unite(b, o, mine_left, u);
 #pragma unroll
         for (int q = 0; q < 6; ++q) b[q] = u[q];
         store_box(node_aabb + (size_t)node * 6u, b);
-        child = -node;
-        node = node == 0 ? -1 : parent[node];
+        if (node == 0) return;
+        cur = -node;
+        node = parent[node];
     }
 }
 
@@ -424,6 +603,7 @@ struct pga_lbvh {
     uint32_t* hist = nullptr;       // 256 x sort workgroups
     uint32_t* part = nullptr;       // scan partials
     uint32_t* ticket = nullptr;     // internal nodes
+    int32_t* other_end = nullptr;   // internal nodes: the far end of each node's leaf range
     float* bounds = nullptr;        // bounds partials; [0, 6) the folded scene box
     uint32_t* dev_words = nullptr;  // [0] leaf count of the last gather, [1] its overflow flag
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -466,14 +646,18 @@ hipError_t enqueue_build(pga_lbvh* b, const float* aabbs, uint32_t n, const uint
         hipLaunchKernelGGL(k_scan_reduce, dim3(np), dim3(kThreads), 0, s, b->hist, nh, b->part);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kThreads), 0, s, b->part, np);
         hipLaunchKernelGGL(k_scan_down, dim3(np), dim3(kThreads), 0, s, b->hist, nh, b->part);
+        unsigned long long* keys_out = pass == 3 ? reinterpret_cast<unsigned long long*>(keys) : nullptr;
         hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kThreads), 0, s, b->code[cur], b->idx[cur], b->code[cur ^ 1],
-                           b->idx[cur ^ 1], n, nd, shift, b->hist, nb);
+                           b->idx[cur ^ 1], keys_out, n, nd, shift, b->hist, nb);
         cur ^= 1;
     }
-    hipLaunchKernelGGL(k_karras, dim3(g), dim3(256), 0, s, b->code[cur], b->idx[cur], n, nd,
-                       reinterpret_cast<unsigned long long*>(keys), left, right, parent, leaf_parent, b->ticket);
-    hipLaunchKernelGGL(k_refit, dim3(g), dim3(256), 0, s, aabbs, b->idx[cur], n, nd, left, right, parent,
-                       leaf_parent, node_aabb, b->ticket);
+    const unsigned long long* k64 = reinterpret_cast<const unsigned long long*>(keys);
+    hipLaunchKernelGGL(k_karras, dim3(g), dim3(256), 0, s, k64, n, nd, left, right, parent, leaf_parent,
+                       b->other_end, b->ticket);
+    hipLaunchKernelGGL(k_refit_local, dim3((n + kRefitTile - 1u) / kRefitTile), dim3(256), 0, s, aabbs, k64, n, nd,
+                       left, right, parent, leaf_parent, b->other_end, node_aabb);
+    hipLaunchKernelGGL(k_refit_global, dim3(g), dim3(256), 0, s, aabbs, k64, n, nd, left, right, parent,
+                       b->other_end, node_aabb, b->ticket);
     return hipGetLastError();
 }
 
@@ -497,6 +681,7 @@ PGA_API pga_status pga_lbvh_create(uint32_t capacity, int32_t device, void* stre
     LHIP(b, hipMalloc((void**)&b->hist, nh * 4u));
     LHIP(b, hipMalloc((void**)&b->part, ((nh + kScanTile - 1) / kScanTile + 1) * 4u));
     LHIP(b, hipMalloc((void**)&b->ticket, (size_t)capacity * 4u));
+    LHIP(b, hipMalloc((void**)&b->other_end, (size_t)capacity * 4u));
     LHIP(b, hipMalloc((void**)&b->bounds, 1024u * 6u * 4u));
     LHIP(b, hipMalloc((void**)&b->dev_words, 64u));
     LHIP(b, hipMemset(b->dev_words, 0, 64u));
@@ -511,7 +696,7 @@ PGA_API void pga_lbvh_destroy(pga_lbvh* b) {
     else (void)hipDeviceSynchronize();
     for (int k = 0; k < 2; ++k) { (void)hipFree(b->code[k]); (void)hipFree(b->idx[k]); }
     (void)hipFree(b->hist); (void)hipFree(b->part); (void)hipFree(b->ticket); (void)hipFree(b->bounds);
-    (void)hipFree(b->dev_words);
+    (void)hipFree(b->dev_words); (void)hipFree(b->other_end);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
     delete b;
@@ -533,7 +718,7 @@ PGA_API pga_status pga_lbvh_bounds(pga_lbvh* b, const float* aabbs, uint32_t n, 
     if (!b || !aabbs || !bounds || n == 0) return lfail(b, PGA_ERR_INVALID_ARG, "bad bounds arguments");
     const uint32_t blocks = std::min<uint32_t>(1024u, (n + 255u) / 256u);
     hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, b->stream, aabbs, n, nullptr, b->bounds);
-    hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(64), 0, b->stream, b->bounds, blocks);
+    hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(256), 0, b->stream, b->bounds, blocks);
     LHIP(b, hipGetLastError());
     LHIP(b, hipMemcpyAsync(bounds, b->bounds, 6u * 4u, hipMemcpyDeviceToHost, b->stream));
     LHIP(b, hipStreamSynchronize(b->stream));
@@ -600,7 +785,7 @@ PGA_API pga_status pga_lbvh_build_device(pga_lbvh* b, const float* aabbs, const 
     } else {
         const uint32_t blocks = std::min<uint32_t>(1024u, (n + 255u) / 256u);
         hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, b->stream, aabbs, n, nd, b->bounds);
-        hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(64), 0, b->stream, b->bounds, blocks);
+        hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(256), 0, b->stream, b->bounds, blocks);
         const float zero[3] = {0.0f, 0.0f, 0.0f}, one[3] = {1.0f, 1.0f, 1.0f};
         LHIP(b, enqueue_build(b, aabbs, n, nd, zero, one, b->bounds, keys, left, right, parent, leaf_parent,
                               node_aabb));

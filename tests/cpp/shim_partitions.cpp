@@ -3,6 +3,7 @@
 // CodeGenerator::fromPartition writes them, CodeGenerator.cpp:1691-1722)
 // compiled against pga/core/Core.h and run through the C ABI.
 //
+// Built as a program and as a shared library exporting shim_main:
 //   shim_partitions lower <scene> <dir>        lowered program -> <dir>/<scene>.procs/.entries
 //   shim_partitions run <scene> <dir>          derive (GPU); <dir>/<scene>.axioms in (listed
 //                                              scenes), <dir>/<scene>.t<k> instances out
@@ -162,7 +163,10 @@ static int run_scene(const char* mode, const std::string& scene, const std::stri
     return 0;
 }
 
-int main(int argc, char** argv) {
+// The test program's entry point, also exported from the shared-library
+// build (SHIM_LIBRARY) so that a process already using the GPU can run it
+// in-process: cooperative launches of two processes do not share a device.
+extern "C" __attribute__((visibility("default"))) int shim_main(int argc, char** argv) {
     if (argc < 3) {
         std::fprintf(stderr, "usage: %s lower|run <scene> <dir> | interp <dir>\n", argv[0]);
         return 2;
@@ -230,3 +234,7 @@ int main(int argc, char** argv) {
         return 1;
     }
 }
+
+#ifndef SHIM_LIBRARY
+int main(int argc, char** argv) { return shim_main(argc, argv); }
+#endif

@@ -1,7 +1,8 @@
 """The C++ front end (include/pga/core/Core.h, include/pga/facade/Interpreter.h):
 grammars written as the reference's partition code (Proc<Shape, operator
 tree> lists, DispatchTable initialisers, Grid axiom generators) compiled
-with g++ in tests/cpp/shim_partitions.cpp.
+with g++ in tests/cpp/shim_partitions.cpp (loaded in-process as a shared
+library).
 
 CPU: the lowered programs are checked against what the reference's forms
 mean - the Menger partition lowers to the compiled test-runner table
@@ -26,24 +27,43 @@ from tests.helpers import assert_same_multiset
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "tests", "cpp", "shim_partitions.cpp")
-BIN = os.path.join(ROOT, "tests", "cpp", "_build", "shim_partitions")
+SO = os.path.join(ROOT, "tests", "cpp", "_build", "libshim_partitions.so")
 HEADERS = [os.path.join(ROOT, "include", p) for p in
            ("pga.h", "pga/core/Core.h", "pga/facade/Interpreter.h")]
+_shim = None
 
 
-def binary():
-    """The test program, built with g++ against the in-tree library when it
-    is missing or older than its sources."""
+def shim():
+    """The test program as a shared library (shim_main), built with g++
+    against the in-tree library when missing or older than its sources, and
+    loaded into this process: cooperative launches of a second process on
+    the GPU would be refused while this one holds the device."""
+    global _shim
+    if _shim is not None:
+        return _shim
     lib = os.path.join(ROOT, "operatorgraph_amd", "libpga_amd.so")
     if not os.path.exists(lib):
         pytest.skip("libpga_amd.so is not built")
     srcs = [SRC, lib] + HEADERS
-    if not os.path.exists(BIN) or os.path.getmtime(BIN) < max(os.path.getmtime(s) for s in srcs):
-        os.makedirs(os.path.dirname(BIN), exist_ok=True)
-        subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
-                        SRC, "-L", os.path.join(ROOT, "operatorgraph_amd"), "-lpga_amd",
-                        "-Wl,-rpath," + os.path.join(ROOT, "operatorgraph_amd"), "-o", BIN], check=True)
-    return BIN
+    if not os.path.exists(SO) or os.path.getmtime(SO) < max(os.path.getmtime(s) for s in srcs):
+        os.makedirs(os.path.dirname(SO), exist_ok=True)
+        subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-fPIC", "-shared",
+                        "-DSHIM_LIBRARY", "-I", os.path.join(ROOT, "include"), SRC, "-L",
+                        os.path.join(ROOT, "operatorgraph_amd"), "-lpga_amd",
+                        "-Wl,-rpath," + os.path.join(ROOT, "operatorgraph_amd"), "-o", SO], check=True)
+    from operatorgraph_amd import engine
+    engine.load_library()                    # the same libpga_amd.so instance as the engine's
+    _shim = C.CDLL(SO)
+    _shim.shim_main.restype = C.c_int
+    _shim.shim_main.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
+    return _shim
+
+
+def run_shim(*args):
+    argv = [b"shim_partitions"] + [str(a).encode() for a in args]
+    arr = (C.c_char_p * len(argv))(*argv)
+    rc = shim().shim_main(len(argv), arr)
+    assert rc == 0, f"shim_main{args} returned {rc}"
 
 
 class Lowered:
@@ -65,7 +85,7 @@ class Lowered:
 
 
 def lower(scene, tmp_path):
-    subprocess.run([binary(), "lower", scene, str(tmp_path)], check=True, capture_output=True)
+    run_shim("lower", scene, tmp_path)
     return Lowered(os.path.join(str(tmp_path), scene))
 
 
@@ -165,9 +185,7 @@ def test_cpp_evaluator_menger(tmp_path):
     _note("C++ SinglePhaseEvaluator: Menger partition")
     low = lower("menger", tmp_path)
     ax = gen_axioms(tmp_path, "menger")
-    out = subprocess.run([binary(), "run", "menger", str(tmp_path)], check=True, capture_output=True, text=True,
-                         timeout=120)
-    _note(out.stdout)
+    run_shim("run", "menger", tmp_path)
     got = _read_instances(os.path.join(str(tmp_path), "menger.t0"))
     assert len(got) == 8000
     assert_same_multiset(got, O.derive(low, ax).instances(0), "menger")
@@ -193,9 +211,7 @@ def test_cpp_evaluator_partitions(tmp_path, scene, n):
     base = os.path.join(str(tmp_path), scene)
     open(base + ".axioms", "wb").write(np.ascontiguousarray(ax).tobytes())
     open(base + ".caps", "wb").write(caps.tobytes())
-    out = subprocess.run([binary(), "run", scene, str(tmp_path)], check=True, capture_output=True, text=True,
-                         timeout=120)
-    _note(out.stdout)
+    run_shim("run", scene, tmp_path)
     for t in range(nt):
         got = _read_instances(f"{base}.t{t}")
         assert_same_multiset(got, r.instances(t), f"{scene} terminal {t}")
@@ -213,8 +229,7 @@ def test_interpreter_facade_runs_an_interpreter_table(tmp_path):
     open(os.path.join(d, "interp.table"), "wb").write(bytes(entries))
     open(os.path.join(d, "interp.axioms"), "wb").write(np.array([abi.BOX, 0, abi.BOX, -1], dtype=np.int32).tobytes())
     open(os.path.join(d, "interp.caps"), "wb").write(np.array([8000], dtype=np.uint64).tobytes())
-    out = subprocess.run([binary(), "interp", d], check=True, capture_output=True, text=True, timeout=120)
-    _note(out.stdout)
+    run_shim("interp", d)
     got = _read_instances(os.path.join(d, "interp.t0"))
     ax = axioms.menger_axiom(1.0, 0, seed=0.0)   # the interpreter's axiom: origin, size 1, seed table zero
     ref = O.derive(prog, ax).instances(0)
