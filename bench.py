@@ -22,6 +22,9 @@ Prints ONE JSON line. Beside the headline (batched terminals/s) it reports:
   mixed_menger5_16384_lbvh - a Menger sponge of depth 5 and 16,384 lots in
                   one launch plus the LBVH over all terminals from the device
                   counts, one HIP graph per step;
+  mixed_fwd_bwd_menger5_16384_lbvh - the same batch with parameterised lots:
+                  tape forward, torch loss, adjoint and the LBVH rebuild per
+                  step;
   cpu_baseline  - the CPU restatement (oracle) on the host, one thread.
 """
 import argparse
@@ -53,6 +56,7 @@ def parse():
     p.add_argument("--stochastic-axioms", type=int, default=65536)
     p.add_argument("--gradient-lots", type=int, default=256)
     p.add_argument("--mixed-lots", type=int, default=16384)
+    p.add_argument("--mixed-grad-lots", type=int, default=16384)
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--only-kernel", action="store_true",
@@ -275,6 +279,77 @@ def mixed_workload(torch, engine, grammars, axioms, n_lots, steps):
     return res
 
 
+def mixed_gradient_workload(torch, engine, grammars, axioms, n_lots, steps):
+    """The mixed batch forward and backward: a Menger sponge of depth 5 and
+    n_lots parameterised facade lots (8 parameters each) in one tape forward,
+    the MSE of instance centres and sizes against targets from perturbed
+    split parameters, the adjoint back to dL/dparams, and the LBVH over all
+    terminals rebuilt from the device counts, every iteration."""
+    from operatorgraph_amd import abi
+    from operatorgraph_amd.autograd import centers_and_sizes, derive_instances
+    from operatorgraph_amd.lbvh import LBVH
+    prog, starts = grammars.combine((grammars.facade_params(), 0), (grammars.menger_interpreter(1.0), 2))
+    lots, lparams = axioms.param_lots(n_lots)
+    both = np.concatenate([axioms.menger_axiom(81.0, starts[1]), lots])
+    params = np.zeros((len(both), lparams.shape[1]), dtype=np.float32)
+    params[1:] = lparams                     # the sponge reads no parameter
+    rng = np.random.default_rng(4)
+    tparams = params.copy()
+    tparams[1:, [0, 1, 4, 5, 6, 7]] += rng.normal(0, 0.05, size=(n_lots, 6)).astype(np.float32)
+    # the widest level: three children per tile (about 19M for 16,384 lots)
+    eng = engine.Engine(technique=abi.MEGAKERNEL, queue_capacity=1 << 25)
+    eng.load(prog).set_axioms(torch.from_numpy(both.view(np.uint8).copy()).cuda()).set_axiom_params(tparams)
+    eng.bind(0, 0)                           # count-only run sizes the buffers
+    eng.bind(1, 0)
+    eng.bind(2, 0)
+    eng.execute()
+    caps = [int(c) for c in eng.counts(3)]
+    assert caps[2] == MENGER5_CUBES, caps
+    for t in range(3):
+        eng.bind(t, caps[t])
+    eng.tape_reserve(1 << 25, 1 << 26)      # every fan-out's record; every child of every level
+    eng.forward_tape()
+    targets = [centers_and_sizes(eng.buffer(t)[:caps[t]].clone()) for t in range(3)]
+    T = sum(caps)
+    bvh = LBVH(T, stream=eng.stream())
+    out = bvh.alloc()
+    p = torch.from_numpy(params).cuda()
+
+    def step():
+        q = p.clone().requires_grad_(True)
+        outs = derive_instances(q, eng, (0, 1, 2))
+        loss = 0.0
+        for inst, (ct, st) in zip(outs, targets):
+            c, sz = centers_and_sizes(inst)
+            loss = loss + ((c - ct) ** 2).sum(1).mean() + ((sz - st) ** 2).sum(1).mean()
+        loss.backward()
+        torch.cuda.synchronize()
+        bvh.gather_engine(eng, 3, 0b011, out)
+        bvh.build_device(out)
+        return q.grad
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fwd = bwd = lb = 0.0
+    for _ in range(steps):
+        step()
+        fwd += eng.stats()["seconds"]
+        bwd += eng.last_backward_seconds
+        n, overflow = bvh.leaf_count()
+        lb += bvh.seconds()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    assert n == T and not overflow, (n, T)
+    res = {"lots": n_lots, "menger_cubes": MENGER5_CUBES, "params": n_lots * lparams.shape[1], "terminals": T,
+           "ms_per_step": round(sec * 1e3, 4), "grads_per_s": round(n_lots * lparams.shape[1] / sec),
+           "terminals_per_s": round(T / sec), "forward_tape_ms": round(fwd / steps * 1e3, 4),
+           "backward_ms": round(bwd / steps * 1e3, 4), "lbvh_ms": round(lb / steps * 1e3, 4)}
+    bvh.close()
+    eng.close()
+    return res
+
+
 def gradient_workload(torch, engine, grammars, axioms, n, steps):
     """256 parameterised facade lots x 8 parameters: forward with tape, MSE of
     instance centres and sizes against targets from perturbed split
@@ -285,19 +360,16 @@ def gradient_workload(torch, engine, grammars, axioms, n, steps):
     tparams = params.copy()
     tparams[:, [0, 1, 4, 5, 6, 7]] += rng.normal(0, 0.05, size=(n, 6)).astype(np.float32)
     prog = grammars.facade_params()
-    teng = engine.Engine(queue_capacity=1 << 21)
-    teng.load(prog).set_axioms(ax).set_axiom_params(tparams)
-    teng.bind(0, 1 << 21)
-    teng.bind(1, 1 << 21)
-    teng.execute()
-    counts = teng.counts(2)
-    targets = [centers_and_sizes(teng.buffer(t)[: int(counts[t])].clone()) for t in range(2)]
-    teng.close()
     eng = engine.Engine(queue_capacity=1 << 21)
-    eng.load(prog).set_axioms(ax).set_axiom_params(params)
+    eng.load(prog).set_axioms(ax).set_axiom_params(tparams)
     eng.bind(0, 1 << 21)
     eng.bind(1, 1 << 21)
     eng.tape_reserve(1 << 21, 1 << 22)
+    # targets: the tape forward with the perturbed parameters (the same
+    # topology, so the same instance order as the fitted run)
+    eng.forward_tape()
+    counts = eng.counts(2)
+    targets = [centers_and_sizes(eng.buffer(t)[: int(counts[t])].clone()) for t in range(2)]
     p = torch.from_numpy(params).cuda()
 
     def step():
@@ -428,6 +500,11 @@ def main():
     if args.mixed_lots:
         mixed = mixed_workload(torch, engine, grammars, axioms, args.mixed_lots, max(3, args.steps // 4))
 
+    mixed_grad = None
+    if args.mixed_grad_lots:
+        mixed_grad = mixed_gradient_workload(torch, engine, grammars, axioms, args.mixed_grad_lots,
+                                             max(3, args.steps // 4))
+
     grad = None
     if args.gradient_lots:
         grad = gradient_workload(torch, engine, grammars, axioms, args.gradient_lots, max(5, args.steps // 2))
@@ -457,6 +534,7 @@ def main():
         "stochastic_65536_lbvh": stoch,
         "fwd_bwd_gradient_256": grad,
         "mixed_menger5_16384_lbvh": mixed,
+        "mixed_fwd_bwd_menger5_16384_lbvh": mixed_grad,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

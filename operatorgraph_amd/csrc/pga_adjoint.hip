@@ -37,15 +37,63 @@ __device__ __forceinline__ void adj_zero(Adj& d) {
     d.s[0] = d.s[1] = d.s[2] = 0.0f;
 }
 
+// dL/dparams of one thread. A thread replays one child, so every
+// contribution is to the same axiom: up to kGradSlots per-axiom parameters
+// accumulate in registers (FP64) and leave once, at the end (flush_grads).
+constexpr uint32_t kGradSlots = 8;
+struct GradAcc {
+    double v[kGradSlots];
+    uint32_t axiom;   // ~0u: no contribution yet
+};
+
+__device__ __forceinline__ void grad_zero(GradAcc& ga) {
+#pragma unroll
+    for (uint32_t q = 0; q < kGradSlots; ++q) ga.v[q] = 0.0;
+    ga.axiom = ~0u;
+}
+
 // dL/dparam k of entry e for the shape of `axiom`: only per-axiom parameters
 // are differentiable.
-__device__ __forceinline__ void grad_param(const Ctx& c, const AdjArgs& g, const EntryV& e, uint32_t k,
+__device__ __forceinline__ void grad_param(const Ctx& c, const AdjArgs& g, GradAcc& ga, const EntryV& e, uint32_t k,
                                            uint32_t axiom, float v) {
     if (v == 0.0f || k >= e.nparams()) return;
     if (k < 3u && ((e.inl() >> k) & 1u)) return;
     const DevParam p = c.prog.params[e.pbase + k];
     if (p.type != PGA_PT_AXIOM_PARAM || c.aparams_per == 0u) return;
-    atomicAdd(g.grad + (size_t)axiom * c.aparams_per + (uint32_t)p.a, (double)v);
+    const uint32_t slot = (uint32_t)p.a;
+    if (c.aparams_per <= kGradSlots && slot < kGradSlots && (ga.axiom == ~0u || ga.axiom == axiom)) {
+        ga.axiom = axiom;
+#pragma unroll
+        for (uint32_t q = 0; q < kGradSlots; ++q) ga.v[q] += q == slot ? (double)v : 0.0;
+    } else {
+        atomicAdd(g.grad + (size_t)axiom * c.aparams_per + slot, (double)v);
+    }
+}
+
+// The accumulated contributions to the FP64 gradient: a full wave whose
+// contributions all go to one axiom sums each parameter across its lanes
+// first (one atomic per parameter instead of one per lane and operator);
+// other waves add lane by lane.
+__device__ __forceinline__ void flush_grads(const Ctx& c, const AdjArgs& g, const GradAcc& ga) {
+    const uint32_t per = c.aparams_per;
+    if (per == 0u || per > kGradSlots) return;
+    const bool has = ga.axiom != ~0u;
+    const unsigned long long act = __ballot(1), hm = __ballot(has);
+    if (hm == 0ull) return;
+    const int lead = __ffsll((long long)hm) - 1;
+    const uint32_t ax0 = __shfl(ga.axiom, lead);
+    const bool uniform = act == ~0ull && __ballot(has && ga.axiom != ax0) == 0ull;
+    for (uint32_t q = 0; q < per; ++q) {
+        if (__ballot(ga.v[q] != 0.0) == 0ull) continue;
+        if (uniform) {
+            double x = ga.v[q];
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d);
+            if ((int)(threadIdx.x & 63u) == lead) atomicAdd(g.grad + (size_t)ax0 * per + q, x);
+        } else if (ga.v[q] != 0.0) {
+            atomicAdd(g.grad + (size_t)ga.axiom * per + q, ga.v[q]);
+        }
+    }
 }
 
 // c = a * translate(t): da, dt from dc.
@@ -93,7 +141,8 @@ __device__ __forceinline__ void emit_adj(const Rec& r, const float* di, Adj& d) 
 // Reverse of one single-child operator applied to the state `r` (before
 // the operator): d holds the adjoint after it and is turned into the
 // adjoint before it.
-__device__ __forceinline__ void single_adj(const Ctx& c, const AdjArgs& g, const EntryV& e, const Rec& r, Adj& d) {
+__device__ __forceinline__ void single_adj(const Ctx& c, const AdjArgs& g, GradAcc& ga, const EntryV& e, const Rec& r,
+                                           Adj& d) {
     const uint32_t k = e.kind();
     if (k == K_IFSIZELESS || k == K_IF || k == K_RANDOM) return;
     float v0, v1, v2;
@@ -104,9 +153,9 @@ __device__ __forceinline__ void single_adj(const Ctx& c, const AdjArgs& g, const
         translate_adj(r.m, t, d.m, dm, dt);
 #pragma unroll
         for (int i = 0; i < 12; ++i) d.m[i] = dm[i];
-        grad_param(c, g, e, 0, r.axiom, dt[0]);
-        grad_param(c, g, e, 1, r.axiom, dt[1]);
-        grad_param(c, g, e, 2, r.axiom, dt[2]);
+        grad_param(c, g, ga, e, 0, r.axiom, dt[0]);
+        grad_param(c, g, ga, e, 1, r.axiom, dt[1]);
+        grad_param(c, g, ga, e, 2, r.axiom, dt[2]);
         return;
     }
     if (k == K_ROTATE) {                       // m' = m * R(alpha, beta, gamma)
@@ -139,16 +188,16 @@ __device__ __forceinline__ void single_adj(const Ctx& c, const AdjArgs& g, const
         const float Rg[9] = {-cb * sg, -sg * sa * sb - ca * cg, -ca * sg * sb + sa * cg,
                              cb * cg, -ca * sg + sa * sb * cg, sg * sa + ca * sb * cg,
                              0.0f, 0.0f, 0.0f};
-        float ga = 0.0f, gb = 0.0f, gg = 0.0f;
+        float dal = 0.0f, dbe = 0.0f, dgm = 0.0f;   // dL/d(alpha, beta, gamma) in radians
 #pragma unroll
-        for (int q = 0; q < 9; ++q) { ga += dR[q] * Ra[q]; gb += dR[q] * Rb[q]; gg += dR[q] * Rg[q]; }
+        for (int q = 0; q < 9; ++q) { dal += dR[q] * Ra[q]; dbe += dR[q] * Rb[q]; dgm += dR[q] * Rg[q]; }
         float dm[12];
         mul_adj_a(R, d.m, dm);
 #pragma unroll
         for (int i = 0; i < 12; ++i) d.m[i] = dm[i];
-        grad_param(c, g, e, 0, r.axiom, ga * kd);
-        grad_param(c, g, e, 1, r.axiom, gb * kd);
-        grad_param(c, g, e, 2, r.axiom, gg * kd);
+        grad_param(c, g, ga, e, 0, r.axiom, dal * kd);
+        grad_param(c, g, ga, e, 1, r.axiom, dbe * kd);
+        grad_param(c, g, ga, e, 2, r.axiom, dgm * kd);
         return;
     }
     if (k == K_SCALE) {                        // s'_i = v_i > 0 ? v_i : (v_i < 0 ? -s_i v_i : 0)
@@ -158,10 +207,10 @@ __device__ __forceinline__ void single_adj(const Ctx& c, const AdjArgs& g, const
             const float ds = d.s[i];
             if (v[i] < 0.0f) {
                 d.s[i] = ds * -v[i];
-                grad_param(c, g, e, (uint32_t)i, r.axiom, ds * -r.s[i]);
+                grad_param(c, g, ga, e, (uint32_t)i, r.axiom, ds * -r.s[i]);
             } else {
                 d.s[i] = 0.0f;
-                if (v[i] > 0.0f) grad_param(c, g, e, (uint32_t)i, r.axiom, ds);
+                if (v[i] > 0.0f) grad_param(c, g, ga, e, (uint32_t)i, r.axiom, ds);
             }
         }
         return;
@@ -186,7 +235,7 @@ __device__ __forceinline__ void single_adj(const Ctx& c, const AdjArgs& g, const
     d.s[0] = ds0;
     d.s[1] = ds2;
     d.s[2] = 0.0f;
-    grad_param(c, g, e, 1, r.axiom, ds1);
+    grad_param(c, g, ga, e, 1, r.axiom, ds1);
 }
 
 // Reverse of build_child: the child state adjoint dc becomes the parent
@@ -248,7 +297,7 @@ __device__ __forceinline__ const float* slot_at(const AdjArgs& g, uint32_t child
 
 // Reverse of a fan-out at state `r` (entry e), whose children's slots start
 // at child index `first` of the next level.
-__device__ __forceinline__ void fan_adj(const Ctx& c, const AdjArgs& g, const EntryV& e, const Rec& r, uint32_t first,
+__device__ __forceinline__ void fan_adj(const Ctx& c, const AdjArgs& g, GradAcc& ga, const EntryV& e, const Rec& r, uint32_t first,
                                         uint32_t nd, bool self, Adj& d) {
     // parent-state adjoints the children left
     for (uint32_t q = 0; q < nd; ++q) {
@@ -286,10 +335,10 @@ __device__ __forceinline__ void fan_adj(const Ctx& c, const AdjArgs& g, const En
         const float dn = d_nsj + 0.5f * d_oj + (total_o - prefix);
         if (kind == K_SUBDIV) {
             if (fac > 0.0f) {
-                grad_param(c, g, e, j + 1u, r.axiom, dn);
+                grad_param(c, g, ga, e, j + 1u, r.axiom, dn);
             } else if (fac < 0.0f) {
                 d_r2a += dn * -fac;
-                grad_param(c, g, e, j + 1u, r.axiom, dn * -f.r2a);
+                grad_param(c, g, ga, e, j + 1u, r.axiom, dn * -f.r2a);
             }
         } else {
             if (j == 0u) d_e1 += dn;
@@ -312,8 +361,8 @@ __device__ __forceinline__ void fan_adj(const Ctx& c, const AdjArgs& g, const En
             const float d_abs = -d_r2a / rel_sum, d_rel = -d_r2a * f.r2a / rel_sum;
             for (uint32_t i = 1; i < e.nparams(); ++i) {
                 const float fac = P(c, e, i, r, ax, pa, de);
-                if (fac > 0.0f) grad_param(c, g, e, i, r.axiom, d_abs);
-                else if (fac < 0.0f) grad_param(c, g, e, i, r.axiom, -d_rel);
+                if (fac > 0.0f) grad_param(c, g, ga, e, i, r.axiom, d_abs);
+                else if (fac < 0.0f) grad_param(c, g, ga, e, i, r.axiom, -d_rel);
             }
         }
     } else {
@@ -341,10 +390,10 @@ __device__ __forceinline__ void fan_adj(const Ctx& c, const AdjArgs& g, const En
             }
         }
         if (ex > 0.0f) {
-            grad_param(c, g, e, 1, r.axiom, d_ext);
+            grad_param(c, g, ga, e, 1, r.axiom, d_ext);
         } else if (ex < 0.0f) {
             d_sz += d_ext * -ex;
-            grad_param(c, g, e, 1, r.axiom, d_ext * -sz);
+            grad_param(c, g, ga, e, 1, r.axiom, d_ext * -sz);
         }
         d_total += d_sz;
     }
@@ -423,6 +472,8 @@ __global__ void __launch_bounds__(256) k_adjoint(KArgs a, AdjArgs g) {
     }
     Adj d;
     adj_zero(d);
+    GradAcc ga;
+    grad_zero(ga);
     if (kind != T_DROP) {
         // replay the chain of single-child operators to its end
         Rec r = s0;
@@ -443,7 +494,7 @@ __global__ void __launch_bounds__(256) k_adjoint(KArgs a, AdjArgs g) {
         } else {   // T_FAN / T_SELF: the children of the fan-out are in the next level
             const uint32_t first = g.next_child_base + out.z;
             const uint32_t nd = first <= g.slots_cap && out.w <= g.slots_cap - first ? out.w : 0u;
-            fan_adj(c, g, load_entry(c, rec_entry(r)), r, first, nd, kind == T_SELF, d);
+            fan_adj(c, g, ga, load_entry(c, rec_entry(r)), r, first, nd, kind == T_SELF, d);
         }
         // back through the chain, recomputing each operator's input state
         for (uint32_t t = steps; t-- > 0;) {
@@ -454,9 +505,10 @@ __global__ void __launch_bounds__(256) k_adjoint(KArgs a, AdjArgs g) {
                 const int32_t next = apply_single(c, e, q, &ord);
                 advance(c, q, next, ord);
             }
-            single_adj(c, g, load_entry(c, rec_entry(q)), q, d);
+            single_adj(c, g, ga, load_entry(c, rec_entry(q)), q, d);
         }
     }
+    flush_grads(c, g, ga);
     if (g.level == 0u) return;
     Adj dp;
     float d_o = 0.0f, d_ns = 0.0f;
