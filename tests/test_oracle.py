@@ -414,6 +414,23 @@ def test_lbvh_matches_brute_force(n):
                 assert res["leaf_parent"][ch] == i
     assert np.allclose(res["node_aabb"][0, :3], aabb[:, :3].min(0))
     assert np.allclose(res["node_aabb"][0, 3:], aabb[:, 3:].max(0))
+    # refit: every node's box is the min / max over the boxes of the leaves
+    # below it (collected from the brute-force topology)
+    order = (keys & 0xFFFFFFFF).astype(np.int64)
+    below = {}
+
+    def leaves(node):
+        if node not in below:
+            out = []
+            for ch in (left[node], right[node]):
+                out += [int(ch)] if ch >= 0 else leaves(int(-ch))
+            below[node] = out
+        return below[node]
+
+    for i in range(n - 1):
+        b = aabb[order[leaves(i)]]
+        assert np.array_equal(res["node_aabb"][i, :3], b[:, :3].min(0)), i
+        assert np.array_equal(res["node_aabb"][i, 3:], b[:, 3:].max(0)), i
     # Morton code of the first leaf from its centre
     c = (aabb[keys[0] & 0xFFFFFFFF, :3] + aabb[keys[0] & 0xFFFFFFFF, 3:]) * f32(0.5) / f32(100)
     assert int(keys[0]) >> 32 == O.morton30(*[float(v) for v in c])
