@@ -394,6 +394,32 @@ PGA_API pga_status pga_device_alloc(uint64_t bytes, void** device_ptr);
 PGA_API pga_status pga_device_free(void* device_ptr);
 PGA_API pga_status pga_copy_to_host(void* host_dst, const void* device_src, uint64_t bytes);
 
+/* -- grammar front end: the PGA grammar language (*.pga) compiled on the
+ *    host into a dispatch table over the interpreter's procedure list.
+ *    Replaces PGA::Compiler::Parser::parse + DispatchTableGenerator::fromRuleSet
+ *    (src/compiler/Parser.cpp:494-540, src/compiler/DispatchTableGenerator.cpp:555-609)
+ *    as driven by the interpreter (src/interpreter/PGAFacade.cu:600-660).
+ *    pga_interpreter_procedures copies the interpreter's procedure list
+ *    (src/interpreter/PGAFacade.cu:24-62) to out (capacity entries; out may
+ *    be NULL to query the count). pga_compile_grammar always sets *out to a
+ *    grammar object that the caller destroys; on error it holds no entries
+ *    and pga_grammar_error gives the message (with the line number). The
+ *    entries and axioms stay valid until pga_grammar_destroy. An axiom whose
+ *    symbol has no rule gets entry_index -1 (skipped by the axiom loader). -- */
+typedef struct pga_grammar pga_grammar;
+typedef struct pga_grammar_axiom {
+    int32_t entry_index;
+    int32_t shape_type; /* pga_shape_type */
+} pga_grammar_axiom;
+PGA_API pga_status pga_interpreter_procedures(pga_procedure* out, uint32_t capacity, uint32_t* count);
+PGA_API pga_status pga_compile_grammar(const char* source, pga_grammar** out);
+PGA_API const char* pga_grammar_error(const pga_grammar* grammar);
+PGA_API pga_status pga_grammar_entries(const pga_grammar* grammar, const pga_dispatch_entry** entries,
+                                       uint32_t* count);
+PGA_API pga_status pga_grammar_axioms(const pga_grammar* grammar, const pga_grammar_axiom** axioms,
+                                      uint32_t* count);
+PGA_API void pga_grammar_destroy(pga_grammar* grammar);
+
 #ifdef __cplusplus
 }
 #endif

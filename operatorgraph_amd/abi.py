@@ -98,6 +98,10 @@ class Axiom(C.Structure):
                 ("reserved", C.c_uint32)]
 
 
+class GrammarAxiom(C.Structure):
+    _fields_ = [("entry_index", C.c_int32), ("shape_type", C.c_int32)]
+
+
 class Instance(C.Structure):
     _fields_ = [("model", C.c_float * 12), ("custom", C.c_float), ("axiom", C.c_uint32),
                 ("path", C.c_uint32), ("meta", C.c_uint32)]

@@ -17,9 +17,11 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpga_amd.so")
 SOURCES = ["pga_kernels.hip", "pga_adjoint.hip", "pga_lbvh.hip", "pga_api.cpp"]
+HOST_SOURCES = ["pga_compiler.cpp"]   # no device code: built with the host compiler
 HEADERS = ["pga_device.h", "pga_ops.h", "pga_kernels.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
 ARCH = os.environ.get("PGA_OFFLOAD_ARCH", "gfx950")
 
 COMMON_FLAGS = [
@@ -34,7 +36,7 @@ def _run(cmd):
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed: " + " ".join(cmd))
+        raise RuntimeError("build step failed: " + " ".join(cmd))
     return res
 
 
@@ -50,6 +52,14 @@ def build(verbose=False):
         if src.endswith(".cpp"):
             cmd.insert(1, "-x")
             cmd.insert(2, "hip")
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = os.path.join(build_dir, src + ".o")
+        cmd = [CXX, "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-fvisibility=hidden", "-Wall", "-Wextra", "-Werror",
+               "-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)

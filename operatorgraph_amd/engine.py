@@ -72,6 +72,12 @@ def load_library():
         "pga_graph_info": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "pga_device_counts": (C.c_int, [P, C.POINTER(P)]),
         "pga_get_stream": (C.c_int, [P, C.POINTER(P)]),
+        "pga_interpreter_procedures": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_uint32)]),
+        "pga_compile_grammar": (C.c_int, [C.c_char_p, C.POINTER(P)]),
+        "pga_grammar_error": (C.c_char_p, [P]),
+        "pga_grammar_entries": (C.c_int, [P, C.POINTER(C.POINTER(abi.DispatchEntry)), C.POINTER(C.c_uint32)]),
+        "pga_grammar_axioms": (C.c_int, [P, C.POINTER(C.POINTER(abi.GrammarAxiom)), C.POINTER(C.c_uint32)]),
+        "pga_grammar_destroy": (None, [P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
