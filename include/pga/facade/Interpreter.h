@@ -156,3 +156,53 @@ inline double executePGA() { return PGA::Interpreter::evaluator()->execute(); }
 inline void releasePGA() { (void)PGA::Interpreter::evaluator().release(); }
 
 inline void destroyPGA() { PGA::Interpreter::evaluator().reset(); }
+
+namespace PGA {
+namespace Interpreter {
+
+// a library-written entry back in the DispatchTable form
+inline PGA::DispatchTable::Entry toEntry(const pga_dispatch_entry& e) {
+    PGA::DispatchTable::Entry x;
+    x.operatorCode = (size_t)e.operator_code;
+    for (uint32_t k = 0; k < e.num_parameters && k < PGA_MAX_NUM_PARAMETERS; ++k) {
+        const pga_dispatch_parameter& p = e.parameters[k];
+        size_t n = PGA_MAX_NUM_PARAMETER_VALUES;
+        while (n > 0 && p.value[n - 1] == 0.0f && !std::signbit(p.value[n - 1])) --n;
+        x.parameters.push_back(
+            PGA::DispatchTable::Parameter((PGA::ParameterType)p.type, std::vector<double>(p.value, p.value + n)));
+    }
+    for (uint32_t k = 0; k < e.num_successors && k < PGA_MAX_NUM_SUCCESSORS; ++k)
+        x.successors.push_back({e.successors[k].entry_index, e.successors[k].phase_index});
+    for (uint32_t k = 0; k < e.num_edge_indices && k < PGA_MAX_NUM_SUCCESSORS; ++k)
+        x.edgeIndices.push_back(e.edge_indices[k]);
+    x.subGraphIndex = e.sub_graph_index;
+    return x;
+}
+
+} // namespace Interpreter
+} // namespace PGA
+
+// InterpreterApp::parse (src/interpreter/InterpreterApp.cpp:175-231) without
+// the rendering side: the grammar text compiled by the library's front end
+// (pga_compile_grammar) into a dispatch table over getProcedureList(), and
+// the declared axioms (an axiom whose symbol has no rule gets entry -1 and
+// is skipped). Throws std::runtime_error carrying the compiler's message,
+// which names the line, where the reference prints its parser log and throws
+// "parsing error".
+inline std::unique_ptr<PGA::DispatchTable> compileGrammar(const std::string& source,
+                                                          std::vector<PGA::Interpreter::Axiom>& axioms) {
+    pga_grammar* g = nullptr;
+    const pga_status st = pga_compile_grammar(source.c_str(), &g);
+    std::unique_ptr<pga_grammar, void (*)(pga_grammar*)> guard(g, pga_grammar_destroy);
+    if (st != PGA_OK) throw std::runtime_error(std::string("parsing error: ") + pga_grammar_error(g));
+    const pga_dispatch_entry* entries = nullptr;
+    const pga_grammar_axiom* ax = nullptr;
+    uint32_t n = 0, m = 0;
+    if (pga_grammar_entries(g, &entries, &n) != PGA_OK || pga_grammar_axioms(g, &ax, &m) != PGA_OK)
+        throw std::runtime_error("parsing error: no compiled grammar");
+    std::unique_ptr<PGA::DispatchTable> table(new PGA::DispatchTable());
+    for (uint32_t i = 0; i < n; ++i) table->entries.push_back(PGA::Interpreter::toEntry(entries[i]));
+    axioms.clear();
+    for (uint32_t i = 0; i < m; ++i) axioms.push_back({(PGA::Compiler::ShapeType)ax[i].shape_type, ax[i].entry_index});
+    return table;
+}

@@ -168,7 +168,7 @@ static int run_scene(const char* mode, const std::string& scene, const std::stri
 // in-process: cooperative launches of two processes do not share a device.
 extern "C" __attribute__((visibility("default"))) int shim_main(int argc, char** argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s lower|run <scene> <dir> | interp <dir>\n", argv[0]);
+        std::fprintf(stderr, "usage: %s lower|run <scene> <dir> | interp|grammar|grammar-run <dir>\n", argv[0]);
         return 2;
     }
     const std::string mode = argv[1];
@@ -179,17 +179,7 @@ extern "C" __attribute__((visibility("default"))) int shim_main(int argc, char**
             const auto ax = read_all<int32_t>(dir + "/interp.axioms");
             const auto caps = read_all<uint64_t>(dir + "/interp.caps");
             std::unique_ptr<DispatchTable> table(new DispatchTable());
-            for (const auto& e : raw) {   // back to the DispatchTable form a DSL front end builds
-                DispatchTable::Entry x;
-                x.operatorCode = (size_t)e.operator_code;
-                for (uint32_t k = 0; k < e.num_parameters; ++k) {
-                    std::vector<double> v(e.parameters[k].value, e.parameters[k].value + PGA_MAX_NUM_PARAMETER_VALUES);
-                    x.parameters.push_back(DispatchTable::Parameter((ParameterType)e.parameters[k].type, v));
-                }
-                for (uint32_t k = 0; k < e.num_successors; ++k)
-                    x.successors.push_back({e.successors[k].entry_index, e.successors[k].phase_index});
-                table->entries.push_back(x);
-            }
+            for (const auto& e : raw) table->entries.push_back(PGA::Interpreter::toEntry(e));
             std::vector<PGA::Interpreter::Axiom> axioms;
             for (size_t i = 0; i + 1 < ax.size(); i += 2)
                 axioms.push_back({(PGA::Compiler::ShapeType)ax[i], ax[i + 1]});
@@ -207,6 +197,35 @@ extern "C" __attribute__((visibility("default"))) int shim_main(int argc, char**
                 std::printf("interp: terminal %u: %zu instances\n", t, inst.size());
             }
             std::printf("interp: %.6f s\n", seconds);
+            destroyPGA();
+            return 0;
+        }
+        if (mode == "grammar" || mode == "grammar-run") {
+            // the interpreter's flow: grammar text -> compileGrammar -> (run)
+            const std::string dir = argv[2];
+            const auto text = read_all<char>(dir + "/grammar.pga");
+            std::vector<PGA::Interpreter::Axiom> axioms;
+            const auto table = compileGrammar(std::string(text.begin(), text.end()), axioms);
+            const auto entries = table->toDispatchTableEntriesPtr();
+            write_all(dir + "/grammar.table",
+                      std::vector<pga_dispatch_entry>(reinterpret_cast<const pga_dispatch_entry*>(entries.get()),
+                                                      reinterpret_cast<const pga_dispatch_entry*>(entries.get()) +
+                                                          table->entries.size()));
+            std::printf("grammar: %zu entries, %zu axioms, first at entry %d\n", table->entries.size(), axioms.size(),
+                        axioms.empty() ? -1 : axioms[0].entryIndex);
+            if (mode == "grammar") return 0;
+            const auto caps = read_all<uint64_t>(dir + "/grammar.caps");
+            setAxioms(axioms.data(), (unsigned int)axioms.size());
+            for (unsigned int t = 0; t < caps.size(); ++t) setMaxNumInstances(t, caps[t]);
+            initializePGA(table);
+            const double seconds = executePGA();
+            auto& ev = PGA::Interpreter::evaluator();
+            for (unsigned int t = 0; t < caps.size(); ++t) {
+                const auto inst = ev->getInstances(t);
+                write_all(dir + "/grammar.t" + std::to_string(t), inst);
+                std::printf("grammar: terminal %u: %zu instances\n", t, inst.size());
+            }
+            std::printf("grammar: %.6f s\n", seconds);
             destroyPGA();
             return 0;
         }

@@ -235,3 +235,57 @@ def test_interpreter_facade_runs_an_interpreter_table(tmp_path):
     ref = O.derive(prog, ax).instances(0)
     assert len(got) == 8000
     assert_same_multiset(got, ref, "interpreter Menger")
+
+
+FACADE_TEXT = """
+// the batched facade in the grammar language
+axiom Box Lot;
+terminal Wall (1, 0);
+terminal Window (1, 1);
+Lot    = CompSplit() { Discard() | Discard() | Facade };
+Facade = Repeat(Y, 3, ADJUST_TO_FILL) Floor;
+Floor  = Repeat(X, 2.5, ADJUST_TO_FILL) Tile;
+Tile   = IfSizeLess(X, 1.6) { Wall | SubDiv(X) { -1: Wall | 1.2: Window | -1: Wall } };
+"""
+
+
+def test_compile_grammar_from_cpp(tmp_path):
+    """compileGrammar (the interpreter's parse step in C++) gives, through
+    DispatchTable::toDispatchTableEntriesPtr, the same entry bytes as the C
+    ABI's compile, and the axiom at entry 0."""
+    from operatorgraph_amd.compiler import compile_grammar
+    d = str(tmp_path)
+    open(os.path.join(d, "grammar.pga"), "w").write(FACADE_TEXT)
+    run_shim("grammar", d)
+    got = open(os.path.join(d, "grammar.table"), "rb").read()
+    assert got == compile_grammar(FACADE_TEXT).raw
+
+
+def test_compile_grammar_from_cpp_reports_the_line(tmp_path, capfd):
+    d = str(tmp_path)
+    open(os.path.join(d, "grammar.pga"), "w").write("axiom Box A;\nA = Translate(0, 0, 0)\n")
+    argv = [b"shim_partitions", b"grammar", d.encode()]
+    rc = shim().shim_main(len(argv), (C.c_char_p * len(argv))(*argv))
+    assert rc == 1
+    assert "parsing error: line 3" in capfd.readouterr().err
+
+
+@pytest.mark.gpu
+def test_interpreter_flow_from_grammar_text(tmp_path):
+    """Grammar text -> compileGrammar -> setAxioms/initializePGA/executePGA,
+    all in C++: the interpreter's Box axiom (origin, size 1, seed table
+    zero) becomes one facade lot, checked against the oracle."""
+    from operatorgraph_amd.compiler import compile_grammar
+    _note("C++ interpreter flow: facade grammar text")
+    d = str(tmp_path)
+    open(os.path.join(d, "grammar.pga"), "w").write(FACADE_TEXT.replace("1.6", "0.2").replace("2.5", "0.25")
+                                                    .replace("(Y, 3,", "(Y, 0.2,").replace("1.2:", "0.1:"))
+    open(os.path.join(d, "grammar.caps"), "wb").write(np.array([4096, 4096], dtype=np.uint64).tobytes())
+    run_shim("grammar-run", d)
+    g = compile_grammar(open(os.path.join(d, "grammar.pga")).read())
+    ax = g.axiom_records([0.0])
+    r = O.derive(g.program, ax)
+    for t in range(2):
+        got = _read_instances(os.path.join(d, f"grammar.t{t}"))
+        assert len(got) > 0
+        assert_same_multiset(got, r.instances(t), f"interpreter flow terminal {t}")
