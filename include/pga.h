@@ -309,7 +309,8 @@ PGA_API double pga_timestamp_hz(pga_context* ctx);
  *    tape (every fan-out materialised, the topology frozen) into the bound
  *    instance buffers; pga_backward takes a host array of PGA_MAX_TERMINALS
  *    device pointers (NULL: no gradient) to dL/dinstance in the instance
- *    layout (16 floats per instance, the 12 model entries are used) and
+ *    layout (16 floats per instance, 16-byte aligned, the 12 model entries
+ *    are used) and
  *    writes dL/dparams (device, n x params_per_axiom floats) for the
  *    PGA_PT_AXIOM_PARAM parameters. -- */
 PGA_API pga_status pga_tape_reserve(pga_context* ctx, uint32_t max_records, uint32_t max_children);

@@ -83,8 +83,9 @@ __device__ __forceinline__ void flush_grads(const Ctx& c, const AdjArgs& g, cons
     const int lead = __ffsll((long long)hm) - 1;
     const uint32_t ax0 = __shfl(ga.axiom, lead);
     const bool uniform = act == ~0ull && __ballot(has && ga.axiom != ax0) == 0ull;
-    for (uint32_t q = 0; q < per; ++q) {
-        if (__ballot(ga.v[q] != 0.0) == 0ull) continue;
+#pragma unroll
+    for (uint32_t q = 0; q < kGradSlots; ++q) {   // unrolled: ga.v stays in registers
+        if (q >= per || __ballot(ga.v[q] != 0.0) == 0ull) continue;
         if (uniform) {
             double x = ga.v[q];
 #pragma unroll
@@ -126,7 +127,14 @@ __device__ __forceinline__ void mul_adj_a(const float b[16], const float dc[12],
 }
 
 // Instance = model4 * scale(size) (GenerationFunctions.cuh:155-189).
-__device__ __forceinline__ void emit_adj(const Rec& r, const float* di, Adj& d) {
+__device__ __forceinline__ void emit_adj(const Rec& r, const float* dl, Adj& d) {
+    float di[12];   // rows 0..2 of dL/dinstance, three 16-byte loads
+    const float4* d4 = reinterpret_cast<const float4*>(dl);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float4 v = d4[i];
+        di[i * 4] = v.x; di[i * 4 + 1] = v.y; di[i * 4 + 2] = v.z; di[i * 4 + 3] = v.w;
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
 #pragma unroll
@@ -218,10 +226,15 @@ __device__ __forceinline__ void single_adj(const Ctx& c, const AdjArgs& g, GradA
     if (k == K_SWAPSIZE) {                     // s'_k = s[a_k]
         const uint32_t a0 = min(f2u_rd(v0), 2u), a1 = min(f2u_rd(v1), 2u), a2 = min(f2u_rd(v2), 2u);
         const float d0 = d.s[0], d1 = d.s[1], d2 = d.s[2];
-        d.s[0] = d.s[1] = d.s[2] = 0.0f;
-        d.s[a0] += d0;
-        d.s[a1] += d1;
-        d.s[a2] += d2;
+        // d.s[a_k] += d_k in k order, written with selects (no indexed local)
+#pragma unroll
+        for (uint32_t q = 0; q < 3; ++q) {
+            float x = 0.0f;
+            x = a0 == q ? x + d0 : x;
+            x = a1 == q ? x + d1 : x;
+            x = a2 == q ? x + d2 : x;
+            d.s[q] = x;
+        }
         return;
     }
     // K_EXTRUDE: m' = m * B, s' = (s0, v1, s1)
@@ -251,10 +264,11 @@ __device__ __forceinline__ void build_adj(const Rec& p, const EntryV& pe, uint32
         axis_vec(axis, o, t[0], t[1], t[2]);
         float dt[3];
         translate_adj(p.m, t, dc.m, dp.m, dt);
-        d_o = dt[axis];
+        // selects, not indexing by `axis`: an indexed local lives in scratch
+        d_o = axis == 0u ? dt[0] : (axis == 1u ? dt[1] : dt[2]);
 #pragma unroll
         for (uint32_t q = 0; q < 3; ++q) dp.s[q] = q == axis ? 0.0f : dc.s[q];
-        d_ns = dc.s[axis];
+        d_ns = axis == 0u ? dc.s[0] : (axis == 1u ? dc.s[1] : dc.s[2]);
         return;
     }
     if (kind == K_COMPSPLIT) {                 // m_c = (m * translate(t_j)) * B_j
@@ -299,12 +313,17 @@ __device__ __forceinline__ const float* slot_at(const AdjArgs& g, uint32_t child
 // at child index `first` of the next level.
 __device__ __forceinline__ void fan_adj(const Ctx& c, const AdjArgs& g, GradAcc& ga, const EntryV& e, const Rec& r, uint32_t first,
                                         uint32_t nd, bool self, Adj& d) {
-    // parent-state adjoints the children left
+    // parent-state adjoints the children left (four 16-byte loads per slot),
+    // and the sum of their offset adjoints in child order
+    float total_o = 0.0f;
     for (uint32_t q = 0; q < nd; ++q) {
-        const float* sl = slot_at(g, first + q);
-#pragma unroll
-        for (int i = 0; i < 12; ++i) d.m[i] += sl[i];
-        d.s[0] += sl[12]; d.s[1] += sl[13]; d.s[2] += sl[14];
+        const float4* sl = reinterpret_cast<const float4*>(slot_at(g, first + q));
+        const float4 a = sl[0], b = sl[1], cc = sl[2], dd = sl[3];
+        d.m[0] += a.x; d.m[1] += a.y; d.m[2] += a.z; d.m[3] += a.w;
+        d.m[4] += b.x; d.m[5] += b.y; d.m[6] += b.z; d.m[7] += b.w;
+        d.m[8] += cc.x; d.m[9] += cc.y; d.m[10] += cc.z; d.m[11] += cc.w;
+        d.s[0] += dd.x; d.s[1] += dd.y; d.s[2] += dd.z;
+        total_o += dd.w;
     }
     if (self) return;
     const uint32_t kind = e.kind();
@@ -316,8 +335,6 @@ __device__ __forceinline__ void fan_adj(const Ctx& c, const AdjArgs& g, GradAcc&
     if (kind == K_REPEAT) skipm = (skipm & 1u) ? 0xFFFFFFFFu : 0u;
     // o_j = off_j + ns_j / 2, off_{j+1} = off_j + ns_j, off_0 = -total / 2:
     //   dL/doff_0 = sum_j do_j, dL/dns_j = dns_j + do_j / 2 + sum_{k > j} do_k
-    float total_o = 0.0f;
-    for (uint32_t q = 0; q < nd; ++q) total_o += slot_at(g, first + q)[15];
     float prefix = 0.0f, d_r2a = 0.0f, d_e1 = 0.0f, d_e2 = 0.0f, d_e3 = 0.0f;
     uint32_t q = 0;
     Fan ff = f;
@@ -397,7 +414,8 @@ __device__ __forceinline__ void fan_adj(const Ctx& c, const AdjArgs& g, GradAcc&
         }
         d_total += d_sz;
     }
-    d.s[f.axis] += d_total;
+#pragma unroll
+    for (uint32_t q = 0; q < 3; ++q) d.s[q] = q == f.axis ? d.s[q] + d_total : d.s[q];
 }
 
 __device__ __forceinline__ Ctx adj_ctx(const KArgs& a) {
@@ -425,7 +443,7 @@ __device__ __forceinline__ Ctx adj_ctx(const KArgs& a) {
     return c;
 }
 
-__global__ void __launch_bounds__(256) k_adjoint(KArgs a, AdjArgs g) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_adjoint(KArgs a, AdjArgs g) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.n_children) return;
     const Ctx c = adj_ctx(a);

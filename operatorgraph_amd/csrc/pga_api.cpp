@@ -1082,6 +1082,9 @@ PGA_API pga_status pga_backward(pga_context* ctx, const float* const* dl_dinstan
     if (!ctx || !dl_dinstances || !dl_dparams) return fail(ctx, PGA_ERR_INVALID_ARG, "null argument");
     if (!ctx->tape_valid) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no recorded tape: run pga_forward_tape");
     if (ctx->aparams_per == 0) return fail(ctx, PGA_ERR_INVALID_ARG, "no per-axiom parameters bound");
+    for (uint32_t t = 0; t < PGA_MAX_TERMINALS; ++t)   // read as 16-byte rows
+        if (((uintptr_t)dl_dinstances[t] & 15u) != 0u)
+            return fail(ctx, PGA_ERR_INVALID_ARG, "dL/dinstance buffers must be 16-byte aligned");
     HIP_TRY(ctx, hipMemcpyAsync((void*)ctx->d_dl, (const void*)dl_dinstances, PGA_MAX_TERMINALS * sizeof(float*),
                                 hipMemcpyHostToDevice, ctx->stream));
     const uint32_t n = ctx->tape_axioms;
