@@ -17,13 +17,14 @@ Prints ONE JSON line. Beside the headline (batched terminals/s) it reports:
                   replayed every step;
   stochastic_65536_lbvh - per-shape Philox rule choice over 65,536 axioms,
                   then the LBVH over the terminal boxes;
-  fwd_bwd_gradient_256 - tape forward, torch loss and adjoint over 256
-                  parameterised lots (dL/dparams per second);
+  fwd_bwd_gradient_256 - tape forward, the fused centre/size loss kernel
+                  and the adjoint over 256 parameterised lots (dL/dparams
+                  per second);
   mixed_menger5_16384_lbvh - a Menger sponge of depth 5 and 16,384 lots in
                   one launch plus the LBVH over all terminals from the device
                   counts, one HIP graph per step;
   mixed_fwd_bwd_menger5_16384_lbvh - the same batch with parameterised lots:
-                  tape forward, torch loss, adjoint and the LBVH rebuild per
+                  tape forward, fused loss, adjoint and the LBVH rebuild per
                   step;
   cpu_baseline  - the CPU restatement (oracle) on the host, one thread.
 """
@@ -286,7 +287,7 @@ def mixed_gradient_workload(torch, engine, grammars, axioms, n_lots, steps):
     split parameters, the adjoint back to dL/dparams, and the LBVH over all
     terminals rebuilt from the device counts, every iteration."""
     from operatorgraph_amd import abi
-    from operatorgraph_amd.autograd import centers_and_sizes, derive_instances
+    from operatorgraph_amd.autograd import center_size_mse, centers_and_sizes, derive_instances
     from operatorgraph_amd.lbvh import LBVH
     prog, starts = grammars.combine((grammars.facade_params(), 0), (grammars.menger_interpreter(1.0), 2))
     lots, lparams = axioms.param_lots(n_lots)
@@ -309,7 +310,7 @@ def mixed_gradient_workload(torch, engine, grammars, axioms, n_lots, steps):
         eng.bind(t, caps[t])
     eng.tape_reserve(1 << 25, 1 << 26)      # every fan-out's record; every child of every level
     eng.forward_tape()
-    targets = [centers_and_sizes(eng.buffer(t)[:caps[t]].clone()) for t in range(3)]
+    targets = [tuple(x.contiguous() for x in centers_and_sizes(eng.buffer(t)[:caps[t]].clone())) for t in range(3)]
     T = sum(caps)
     bvh = LBVH(T, stream=eng.stream())
     out = bvh.alloc()
@@ -320,8 +321,7 @@ def mixed_gradient_workload(torch, engine, grammars, axioms, n_lots, steps):
         outs = derive_instances(q, eng, (0, 1, 2))
         loss = 0.0
         for inst, (ct, st) in zip(outs, targets):
-            c, sz = centers_and_sizes(inst)
-            loss = loss + ((c - ct) ** 2).sum(1).mean() + ((sz - st) ** 2).sum(1).mean()
+            loss = loss + center_size_mse(inst, ct, st)   # one kernel: loss and dL/dinstance
         loss.backward()
         torch.cuda.synchronize()
         bvh.gather_engine(eng, 3, 0b011, out)
@@ -354,7 +354,7 @@ def gradient_workload(torch, engine, grammars, axioms, n, steps):
     """256 parameterised facade lots x 8 parameters: forward with tape, MSE of
     instance centres and sizes against targets from perturbed split
     parameters (same topology), backward to dL/dparams."""
-    from operatorgraph_amd.autograd import centers_and_sizes, derive_instances
+    from operatorgraph_amd.autograd import center_size_mse, centers_and_sizes, derive_instances
     ax, params = axioms.param_lots(n)
     rng = np.random.default_rng(4)
     tparams = params.copy()
@@ -369,7 +369,8 @@ def gradient_workload(torch, engine, grammars, axioms, n, steps):
     # topology, so the same instance order as the fitted run)
     eng.forward_tape()
     counts = eng.counts(2)
-    targets = [centers_and_sizes(eng.buffer(t)[: int(counts[t])].clone()) for t in range(2)]
+    targets = [tuple(x.contiguous() for x in centers_and_sizes(eng.buffer(t)[: int(counts[t])].clone()))
+               for t in range(2)]
     p = torch.from_numpy(params).cuda()
 
     def step():
@@ -377,8 +378,7 @@ def gradient_workload(torch, engine, grammars, axioms, n, steps):
         outs = derive_instances(q, eng, (0, 1))
         loss = 0.0
         for inst, (ct, st) in zip(outs, targets):
-            c, sz = centers_and_sizes(inst)
-            loss = loss + ((c - ct) ** 2).sum(1).mean() + ((sz - st) ** 2).sum(1).mean()
+            loss = loss + center_size_mse(inst, ct, st)   # one kernel: loss and dL/dinstance
         loss.backward()
         return q.grad
 

@@ -317,6 +317,17 @@ PGA_API pga_status pga_tape_reserve(pga_context* ctx, uint32_t max_records, uint
 PGA_API pga_status pga_forward_tape(pga_context* ctx, uint32_t num_axioms, double* seconds);
 PGA_API pga_status pga_backward(pga_context* ctx, const float* const* dl_dinstances, float* dl_dparams,
                                 double* seconds);
+/* The loss of the gradient fits: wc * sum |c_i - tc_i|^2 + ws * sum |s_i - ts_i|^2
+ * over `count` instances (device, 16 floats each, 16-byte aligned), where
+ * c_i is the translation column and s_i the column norms of the 3x3 part;
+ * targets are device arrays of count x 3 floats. Writes the gradient rows
+ * dL/dinstance (count x 16 floats, the layout pga_backward reads; columns
+ * 12..15 zero) and adds the loss to *loss (one device double). One kernel
+ * on `stream` (a hipStream_t, NULL: the default stream); no synchronisation.
+ * No counterpart in the reference (it has no reverse mode). */
+PGA_API pga_status pga_center_size_mse(const float* instances, const float* target_centers,
+                                       const float* target_sizes, uint64_t count, float weight_centers,
+                                       float weight_sizes, float* dl_dinstances, double* loss, void* stream);
 
 /* -- linear BVH over terminal AABBs (GPUBVHConstructor.cuh:29-333): 30-bit
  *    Morton codes of the box centres normalised to a scene box, a sort of the

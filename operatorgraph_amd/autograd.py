@@ -38,6 +38,49 @@ def derive_instances(params, engine, terminals):
     return _Derive.apply(params, engine, tuple(terminals))
 
 
+class _CenterSizeMSE(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, inst, target_centers, target_sizes, wc, ws):
+        import ctypes as C
+
+        from .engine import PGAError, load_library
+        n = inst.shape[0]
+        inst = inst.contiguous()
+        tc = target_centers.contiguous()
+        ts = target_sizes.contiguous()
+        grad = torch.empty_like(inst)
+        loss = torch.zeros(1, dtype=torch.float64, device=inst.device)
+        st = load_library().pga_center_size_mse(
+            C.c_void_p(inst.data_ptr()), C.c_void_p(tc.data_ptr()), C.c_void_p(ts.data_ptr()), n, wc, ws,
+            C.c_void_p(grad.data_ptr()), C.c_void_p(loss.data_ptr()),
+            C.c_void_p(torch.cuda.current_stream(inst.device).cuda_stream))
+        if st != 0:
+            raise PGAError(st, "pga_center_size_mse")
+        ctx.save_for_backward(grad)
+        return loss.to(torch.float32).reshape(())
+
+    @staticmethod
+    def backward(ctx, g):
+        (grad,) = ctx.saved_tensors
+        if g.numel() == 1 and float(g) == 1.0:   # loss.backward() on a sum of losses
+            return grad, None, None, None, None
+        return grad * g, None, None, None, None
+
+
+def center_size_mse(inst, target_centers, target_sizes, mean=True):
+    """The centre/size fitting loss in one kernel (pga_center_size_mse):
+    sum over instances of |c - tc|^2 and |s - ts|^2 (c the translation
+    column, s the column norms of the 3x3 part), each averaged over the
+    instances when `mean`; the gradient rows come out of the same pass.
+    Equal to ((c - tc) ** 2).sum(1).mean() + ((s - ts) ** 2).sum(1).mean()
+    on centers_and_sizes(inst) up to FP32 rounding."""
+    n = inst.shape[0]
+    if n == 0:
+        return inst.sum() * 0.0
+    w = 1.0 / n if mean else 1.0
+    return _CenterSizeMSE.apply(inst, target_centers, target_sizes, w, w)
+
+
 def centers_and_sizes(inst):
     """Centre (translation column) and size (column norms of the 3x3 part)
     of each instance of a (count, 16) record tensor."""

@@ -78,6 +78,7 @@ def load_library():
         "pga_grammar_entries": (C.c_int, [P, C.POINTER(C.POINTER(abi.DispatchEntry)), C.POINTER(C.c_uint32)]),
         "pga_grammar_axioms": (C.c_int, [P, C.POINTER(C.POINTER(abi.GrammarAxiom)), C.POINTER(C.c_uint32)]),
         "pga_grammar_destroy": (None, [P]),
+        "pga_center_size_mse": (C.c_int, [P, P, P, C.c_uint64, C.c_float, C.c_float, P, P, P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -106,6 +107,8 @@ class Engine:
         self._h = h.value
         self._check(st)
         self._keep = {}
+        self._ext = None
+        self._capturing = False
         self.n_axioms = 0
         self.params_per_axiom = 0
 
@@ -119,6 +122,32 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+    # -- ordering against torch --
+    # With no stream given, the library runs the context on a stream of its
+    # own, which does not wait for torch's streams: device inputs that torch
+    # wrote (dL/dinstance, device axioms) need an explicit order, and so do
+    # results that torch reads after an asynchronous call.
+    def _torch_stream_pair(self):
+        cur = self.torch.cuda.current_stream()
+        s = self.stream()
+        if self._capturing or cur.cuda_stream == s:
+            return None, None
+        if self._ext is None:
+            self._ext = self.torch.cuda.ExternalStream(s)
+        return cur, self._ext
+
+    def _after_torch(self):
+        """The context stream waits for the work enqueued on torch's current stream."""
+        cur, ext = self._torch_stream_pair()
+        if ext is not None:
+            ext.wait_stream(cur)
+
+    def _before_torch(self):
+        """Torch's current stream waits for the work enqueued on the context stream."""
+        cur, ext = self._torch_stream_pair()
+        if ext is not None:
+            cur.wait_stream(ext)
 
     def _check(self, st):
         if st != abi.OK:
@@ -210,7 +239,9 @@ class Engine:
         """Enqueues the derivation on the engine's stream without waiting."""
         if n is None:
             n = self.n_axioms - first
+        self._after_torch()
         self._check(self.L.pga_execute_async(self._h, first, n, 1 if append else 0))
+        self._before_torch()
 
     def synchronize(self):
         self._check(self.L.pga_synchronize(self._h))
@@ -248,15 +279,19 @@ class Engine:
         while no replay runs: after synchronize()); a change of bindings,
         program layout or configuration makes launch() refuse the graph."""
         g = Graph(self)
+        self._after_torch()
         self._check(self.L.pga_capture_begin(self._h))
+        self._capturing = True
         try:
             yield g
         except BaseException:
+            self._capturing = False
             h = C.c_void_p()
             self.L.pga_capture_end(self._h, C.byref(h))
             if h.value:
                 self.L.pga_graph_destroy(h.value)
             raise
+        self._capturing = False
         h = C.c_void_p()
         self._check(self.L.pga_capture_end(self._h, C.byref(h)))
         g._h = h.value
@@ -265,6 +300,7 @@ class Engine:
         """Times every launch configuration on the bound axioms and keeps the
         fastest (pga_autotune); returns the chosen configuration."""
         r = abi.TuneResult()
+        self._after_torch()
         self._check(self.L.pga_autotune(self._h, self.n_axioms if n is None else n, repeats, C.byref(r)))
         return {name: getattr(r, name) for name, _ in abi.TuneResult._fields_}
 
@@ -278,6 +314,7 @@ class Engine:
     def forward_tape(self, n=None):
         """Derivation of axioms [0, n) recording the tape; returns seconds."""
         sec = C.c_double(0.0)
+        self._after_torch()
         self._check(self.L.pga_forward_tape(self._h, self.n_axioms if n is None else n, C.byref(sec)))
         return sec.value
 
@@ -296,6 +333,7 @@ class Engine:
             ptrs[t] = g.data_ptr()
         grad = torch.empty((n, self.params_per_axiom), dtype=torch.float32, device="cuda")
         sec = C.c_double(0.0)
+        self._after_torch()   # dL/dinstance comes from torch's stream
         self._check(self.L.pga_backward(self._h, C.byref(ptrs), C.c_void_p(grad.data_ptr()), C.byref(sec)))
         self.last_backward_seconds = sec.value
         return grad
@@ -370,7 +408,9 @@ class Graph:
         """Replays the recording on the context stream without waiting;
         Engine.synchronize() waits and reports errors and stats."""
         e = self.engine
+        e._after_torch()
         e._check(e.L.pga_graph_launch(e._h, self._h))
+        e._before_torch()
 
     def info(self):
         """(kernel nodes, cooperative kernel nodes) of the graph."""

@@ -167,15 +167,17 @@ def _keys(inst):
     return (ax << 32) | pa
 
 
-def test_autograd_function_matches_fp64_adjoint():
+@pytest.mark.parametrize("fused", [False, True])
+def test_autograd_function_matches_fp64_adjoint(fused):
     """C4-style loss through torch.autograd: MSE of instance centres and
     sizes against targets derived with perturbed split parameters (same
-    topology), checked against the FP64 adjoint of the same dL/dinstance."""
+    topology), written with torch ops or as the fused loss kernel, checked
+    against the FP64 adjoint of the same dL/dinstance."""
     import torch
     from oracle import oracle as O
     from operatorgraph_amd import engine
-    from operatorgraph_amd.autograd import centers_and_sizes, derive_instances
-    _note("autograd function")
+    from operatorgraph_amd.autograd import center_size_mse, centers_and_sizes, derive_instances
+    _note(f"autograd function, fused loss={fused}")
     ax, params = axioms.param_lots(32)
     rng = np.random.default_rng(4)
     tparams = params.copy()
@@ -196,9 +198,12 @@ def test_autograd_function_matches_fp64_adjoint():
         assert inst.shape == tgt.shape
         oi, ot = torch.argsort(_keys(inst)), torch.argsort(_keys(tgt))
         assert torch.equal(_keys(inst)[oi], _keys(tgt)[ot])
-        c, sz = centers_and_sizes(inst[oi])
         ct, st = centers_and_sizes(tgt[ot])
-        loss = loss + ((c - ct) ** 2).sum(1).mean() + ((sz - st) ** 2).sum(1).mean()
+        if fused:
+            loss = loss + center_size_mse(inst[oi], ct, st)
+        else:
+            c, sz = centers_and_sizes(inst[oi])
+            loss = loss + ((c - ct) ** 2).sum(1).mean() + ((sz - st) ** 2).sum(1).mean()
     g_inst = torch.autograd.grad(loss, outs, retain_graph=True)
     loss.backward()
     r = O.derive(prog, ax, params=params, twins=O.TWIN_F64 | O.TWIN_ADJ)
@@ -212,4 +217,63 @@ def test_autograd_function_matches_fp64_adjoint():
         dl.append(np.stack([g[row[int(k)]] for k in rk]))
     ref = r.backward(dl)
     _assert_close(p.grad.double().cpu().numpy(), ref)
+    eng.close()
+
+
+def test_fused_center_size_loss_matches_torch():
+    """pga_center_size_mse (one kernel: loss and gradient rows) against the
+    same loss written with torch ops and differentiated by autograd, FP32
+    tolerance 1e-5; zero-norm columns get a zero gradient, as torch's norm
+    backward gives."""
+    import torch
+    from operatorgraph_amd.autograd import center_size_mse, centers_and_sizes
+    _note("fused centre/size loss")
+    gen = torch.Generator().manual_seed(3)
+    n = 100_003
+    inst = torch.randn(n, 16, generator=gen)
+    inst[:7, 0] = inst[:7, 4] = inst[:7, 8] = 0.0
+    inst = inst.cuda()
+    tc = torch.randn(n, 3, generator=gen).cuda()
+    ts = torch.rand(n, 3, generator=gen).cuda()
+    for mean in (True, False):
+        x = inst.clone().requires_grad_(True)
+        c, s = centers_and_sizes(x)
+        ref = ((c - tc) ** 2).sum(1) + ((s - ts) ** 2).sum(1)
+        ref = ref.mean() if mean else ref.sum()
+        (3.0 * ref).backward()
+        y = inst.clone().requires_grad_(True)
+        got = center_size_mse(y, tc, ts, mean=mean)
+        (3.0 * got).backward()
+        assert abs(got.item() - ref.item()) <= 1e-5 * abs(ref.item())
+        gx, gy = x.grad.double(), y.grad.double()
+        assert torch.all(gy[:, 12:] == 0)
+        assert torch.all(gy[:7, [0, 4, 8]] == 0)
+        err = (gx - gy).abs().max().item()
+        assert err <= 1e-5 * gx.abs().max().item(), err
+
+
+def test_backward_waits_for_torch_stream():
+    """dL/dinstance written on torch's stream behind a busy kernel reaches
+    the adjoint: the engine's own stream is ordered after torch's current
+    stream (regression test: the engine stream does not wait for torch's
+    default stream by itself)."""
+    import torch
+    _note("backward ordered after torch's stream")
+    ax, params = axioms.param_lots(16)
+    eng = _gpu_run(grammars.facade_params(), ax, params, 2)
+    got = [eng.instances(t) for t in range(2)]
+    want = {}
+    for t in range(2):
+        g = np.zeros((len(got[t]), 16), dtype=np.float32)
+        g[:, :12] = _weights(got[t])
+        want[t] = torch.from_numpy(g).cuda()
+    ref = eng.backward(want).cpu().numpy()
+    dl = {t: torch.zeros_like(want[t]) for t in range(2)}
+    torch.cuda.synchronize()
+    torch.cuda._sleep(50_000_000)          # a bounded spin on torch's stream
+    for t in range(2):
+        dl[t].copy_(want[t])               # queued behind the spin
+    grad = eng.backward(dl).cpu().numpy()
+    assert np.abs(ref).max() > 0
+    np.testing.assert_array_equal(grad, ref)
     eng.close()
