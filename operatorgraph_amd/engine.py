@@ -126,28 +126,20 @@ class Engine:
     # -- ordering against torch --
     # With no stream given, the library runs the context on a stream of its
     # own, which does not wait for torch's streams: device inputs that torch
-    # wrote (dL/dinstance, device axioms) need an explicit order, and so do
-    # results that torch reads after an asynchronous call.
-    def _torch_stream_pair(self):
+    # wrote (dL/dinstance, device axioms, buffers) need an explicit order.
+    # Results of the asynchronous calls are read after synchronize().
+    def _after_torch(self):
+        """The context stream waits for the work enqueued on torch's current
+        stream, unless that stream is already idle."""
+        if self._capturing:
+            return
         cur = self.torch.cuda.current_stream()
         s = self.stream()
-        if self._capturing or cur.cuda_stream == s:
-            return None, None
+        if cur.cuda_stream == s or cur.query():
+            return
         if self._ext is None:
             self._ext = self.torch.cuda.ExternalStream(s)
-        return cur, self._ext
-
-    def _after_torch(self):
-        """The context stream waits for the work enqueued on torch's current stream."""
-        cur, ext = self._torch_stream_pair()
-        if ext is not None:
-            ext.wait_stream(cur)
-
-    def _before_torch(self):
-        """Torch's current stream waits for the work enqueued on the context stream."""
-        cur, ext = self._torch_stream_pair()
-        if ext is not None:
-            cur.wait_stream(ext)
+        self._ext.wait_stream(cur)
 
     def _check(self, st):
         if st != abi.OK:
@@ -236,12 +228,13 @@ class Engine:
         return sec.value
 
     def execute_async(self, n=None, first=0, append=False):
-        """Enqueues the derivation on the engine's stream without waiting."""
+        """Enqueues the derivation on the engine's stream without waiting
+        (after the work torch has enqueued); read the results after
+        synchronize()."""
         if n is None:
             n = self.n_axioms - first
         self._after_torch()
         self._check(self.L.pga_execute_async(self._h, first, n, 1 if append else 0))
-        self._before_torch()
 
     def synchronize(self):
         self._check(self.L.pga_synchronize(self._h))
@@ -410,7 +403,6 @@ class Graph:
         e = self.engine
         e._after_torch()
         e._check(e.L.pga_graph_launch(e._h, self._h))
-        e._before_torch()
 
     def info(self):
         """(kernel nodes, cooperative kernel nodes) of the graph."""
