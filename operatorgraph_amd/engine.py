@@ -36,6 +36,10 @@ def load_library():
     if not os.path.exists(LIB_PATH):
         raise PGAError(abi.ERR_NOT_INITIALIZED,
                        f"{LIB_PATH} is missing: build it with `python -m operatorgraph_amd.build`")
+    # torch first: the library then binds to the HIP runtime torch loaded,
+    # so both see the same devices in one process (loaded the other way
+    # round, a later pga_create found no device)
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     P = C.c_void_p
     sigs = {
