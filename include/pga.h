@@ -432,6 +432,24 @@ PGA_API pga_status pga_grammar_axioms(const pga_grammar* grammar, const pga_gram
                                       uint32_t* count);
 PGA_API void pga_grammar_destroy(pga_grammar* grammar);
 
+/* -- partition code (SURVEY's operator-graph codegen): the grammar's
+ *    operator graph, as Context builds it from the first axiom
+ *    (src/compiler/Context.cpp:11-253; this build numbers the edges in
+ *    depth-first pre-order), cut by `partition` (one '0' keep / '1' cut per
+ *    edge; NULL cuts every edge, as the interpreter does,
+ *    src/interpreter/InterpreterApp.cpp:214-223), written as C++ in the
+ *    form CodeGenerator::fromPartition emits (src/compiler/CodeGenerator.cpp:
+ *    1691-1722): Proc<Shape, operator tree, 1> structs P<k>, the procedure
+ *    list, a DispatchTable, in `namespace name`. Uncut edges are inlined;
+ *    cut edges become FSCall<procedure, 0, -1, entry>. Edges that close a
+ *    cycle must be cut. Host code only. The result object holds the code
+ *    (pga_grammar_code), the edges' cycle flags (pga_grammar_edges) and an
+ *    error message; destroy it with pga_grammar_destroy. -- */
+PGA_API pga_status pga_generate_partition(const char* source, const char* partition, const char* name,
+                                          pga_grammar** out);
+PGA_API pga_status pga_grammar_code(const pga_grammar* grammar, const char** code, uint64_t* length);
+PGA_API pga_status pga_grammar_edges(const pga_grammar* grammar, uint32_t* num_edges, const uint8_t** cycle_flags);
+
 #ifdef __cplusplus
 }
 #endif

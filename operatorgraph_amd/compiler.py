@@ -15,6 +15,12 @@ seed and custom attribute are the axiom's seed, PGAFacade.cu:697-729).
     g.axiom_records(seeds)          # pga_axiom records for Engine.set_axioms
 
 Errors in the source raise `GrammarError` with the line number.
+
+`generate_partition(text, partition, name)` writes the grammar's operator
+graph, cut by a partition, as C++ partition code (Proc<Shape, operator
+tree> structs, the procedure list and a DispatchTable) for the C++ front
+end (include/pga/core/Core.h); `operator_graph_edges(text)` gives the
+graph's edges with their cycle flags (cycle edges must be cut).
 """
 import ctypes as C
 
@@ -124,3 +130,37 @@ def compile_file(path, name=None):
     with open(path, "r", encoding="utf-8") as f:
         text = f.read()
     return compile_grammar(text, name or os.path.basename(os.path.dirname(os.path.abspath(path))))
+
+
+def _partition(source, partition, name):
+    L = _lib()
+    if isinstance(source, str):
+        source = source.encode("utf-8")
+    h = C.c_void_p()
+    st = L.pga_generate_partition(source, None if partition is None else partition.encode(), name.encode(),
+                                  C.byref(h))
+    try:
+        if st != abi.OK:
+            msg = L.pga_grammar_error(h) if h.value else b""
+            raise GrammarError(f"{name}: {msg.decode() if msg else abi.STATUS_NAMES.get(st, st)}")
+        code, n = C.c_char_p(), C.c_uint64()
+        L.pga_grammar_code(h, C.byref(code), C.byref(n))
+        ne, flags = C.c_uint32(), C.POINTER(C.c_uint8)()
+        L.pga_grammar_edges(h, C.byref(ne), C.byref(flags))
+        cycles = [bool(flags[i]) for i in range(ne.value)]
+        return C.string_at(code, n.value).decode(), cycles
+    finally:
+        if h.value:
+            L.pga_grammar_destroy(h)
+
+
+def operator_graph_edges(source):
+    """Cycle flags of the operator graph's edges, in edge order."""
+    return _partition(source, None, "G")[1]
+
+
+def generate_partition(source, partition=None, name="G"):
+    """C++ partition code of the grammar in `namespace name`. `partition` is
+    one '0' (keep) or '1' (cut) per operator-graph edge; None cuts every
+    edge."""
+    return _partition(source, partition, name)[0]

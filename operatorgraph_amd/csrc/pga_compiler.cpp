@@ -798,11 +798,349 @@ private:
     }
 };
 
+// ---- operator graph and partition code ----
+// The graph of the operators reachable from the first axiom
+// (Context::Context, src/compiler/Context.cpp:11-253, in this build's own
+// numbering). It has one vertex per rule body and per inline operator
+// occurrence, one Generate vertex per terminal symbol (or per inline
+// Generate()), and one per Discard(). Its edges are the successor links,
+// numbered in depth-first pre-order. An edge that closes a cycle is marked:
+// a partition must cut it.
+struct GVertex {
+    const Successor* node = nullptr;   // the operator (nullptr: a terminal symbol's Generate)
+    std::string terminal;              // terminal symbol (Generate vertices)
+    std::string rule;                  // enclosing rule (an inline Generate() takes its declaration)
+    int shape = 0;
+    std::vector<int> out;              // edges, in successor order
+};
+struct GEdge {
+    int from = -1, to = -1;
+    bool cycle = false;
+};
+
+class GraphBuilder {
+public:
+    explicit GraphBuilder(const Parser& parser) : g_(parser) {}
+
+    std::vector<GVertex> V;
+    std::vector<GEdge> E;
+
+    void run() {
+        if (g_.axioms.empty()) throw CompileError("no axiom");
+        const Axiom& a = g_.axioms[0];
+        symbol_vertex(a.symbol, a.shape, 0);
+    }
+
+private:
+    const Parser& g_;
+    std::map<std::string, int> named_;   // rule or terminal symbol -> vertex
+    std::vector<char> on_stack_;
+
+    std::string resolve(const std::string& name) const {
+        std::string s = name;
+        for (size_t guard = 0; guard <= g_.rules.size(); ++guard) {
+            auto it = g_.rules.find(s);
+            if (it == g_.rules.end() || !it->second->is_symbol) return s;
+            s = it->second->symbol;
+        }
+        throw CompileError("the rules aliasing '" + name + "' form a cycle");
+    }
+    int add_vertex(const Successor* node, const std::string& terminal, const std::string& rule, int shape) {
+        GVertex v;
+        v.node = node;
+        v.terminal = terminal;
+        v.rule = rule;
+        v.shape = shape;
+        V.push_back(v);
+        on_stack_.push_back(0);
+        return (int)V.size() - 1;
+    }
+    static int next_shape(int op, int shape) {
+        if (op == PGA_OP_COMPSPLIT && shape == PGA_SHAPE_BOX) return PGA_SHAPE_QUAD;
+        if (op == PGA_OP_EXTRUDE && shape == PGA_SHAPE_QUAD) return PGA_SHAPE_BOX;
+        return shape;
+    }
+    int symbol_vertex(const std::string& name, int shape, int line) {
+        const std::string s = resolve(name);
+        auto it = named_.find(s);
+        if (it != named_.end()) {
+            if (V[it->second].shape != shape)
+                throw CompileError(at_line(line) + "'" + s + "' expects shapes of type " +
+                                   shape_name(V[it->second].shape) + " but also gets shapes of type " +
+                                   shape_name(shape));
+            return it->second;
+        }
+        auto r = g_.rules.find(s);
+        if (r == g_.rules.end()) {
+            const int v = add_vertex(nullptr, s, s, shape);
+            named_[s] = v;
+            return v;
+        }
+        const int v = add_vertex(r->second.get(), "", s, shape);
+        named_[s] = v;
+        expand(v);
+        return v;
+    }
+    void expand(int v) {
+        on_stack_[v] = 1;
+        const Successor* s = V[v].node;
+        const std::string rule = V[v].rule;
+        const int shape = V[v].shape;
+        for (size_t i = 0; i < s->next.size(); ++i) {
+            const SuccessorP& n = s->next[i];
+            const int ns = next_shape(s->op, shape);
+            const int e = (int)E.size();
+            E.push_back(GEdge());
+            E[e].from = v;
+            V[v].out.push_back(e);
+            int w;
+            if (n->is_symbol) {
+                w = symbol_vertex(n->symbol, ns, n->line);
+            } else {
+                w = add_vertex(n.get(), "", rule, ns);
+                if (n->op != PGA_OP_DISCARD && n->op != PGA_OP_GENERATE) expand(w);
+            }
+            E[e].to = w;
+            E[e].cycle = on_stack_[w] != 0;
+        }
+        on_stack_[v] = 0;
+    }
+};
+
+// Partition code in the form CodeGenerator::fromPartition writes
+// (src/compiler/CodeGenerator.cpp:1691-1722; e.g. src/test_runner/*.cuh): one
+// `struct P<k> : Proc<Shape, operator tree, 1>` per subgraph, the procedure
+// list and a DispatchTable with one entry per procedure. A subgraph is a
+// vertex reached by a cut edge (or the axiom's) and every vertex below it
+// through uncut edges, written inline. A cut edge becomes
+// FSCall<procedure, 0, -1, entry>, a call to that subgraph's own entry.
+// Every operand is a static template parameter: Scalar<V, N> with
+// V / 10^N equal in FP32 to the grammar's value, AxisParam, RepeatModeParam,
+// Rand, the shape attributes, Vec2 and Exp<A, B, Op>.
+class CodeGen {
+public:
+    CodeGen(const Parser& p, const GraphBuilder& gb, const std::string& cuts)
+        : g_(p), V_(gb.V), E_(gb.E), cut_(gb.E.size(), 0) {
+        if (!cuts.empty() && cuts.size() != E_.size())
+            throw CompileError("the partition names " + std::to_string(cuts.size()) + " edges; the graph has " +
+                               std::to_string(E_.size()));
+        for (size_t e = 0; e < E_.size(); ++e) {
+            const bool c = cuts.empty() ? true : cuts[e] == '1';
+            if (!cuts.empty() && cuts[e] != '0' && cuts[e] != '1')
+                throw CompileError("a partition is a string of '0' (keep) and '1' (cut), one per edge");
+            if (E_[e].cycle && !c)
+                throw CompileError("edge " + std::to_string(e) + " closes a cycle and must be cut");
+            cut_[e] = c;
+        }
+        proc_of_.assign(V_.size(), -1);
+        roots_.push_back(0);
+        proc_of_[0] = 0;
+        for (size_t e = 0; e < E_.size(); ++e) {
+            const int w = E_[e].to;
+            if (cut_[e] && proc_of_[w] < 0) {
+                proc_of_[w] = (int)roots_.size();
+                roots_.push_back(w);
+            }
+        }
+    }
+
+    std::string run(const std::string& name) {
+        std::string o;
+        o += "// Partition code: " + std::to_string(roots_.size()) + " procedures over " +
+             std::to_string(V_.size()) + " operators and " + std::to_string(E_.size()) + " edges; partition ";
+        for (size_t e = 0; e < E_.size(); ++e) o += cut_[e] ? '1' : '0';
+        o += "\nnamespace " + name + " {\n\n";
+        for (size_t k = 0; k < roots_.size(); ++k)
+            o += "struct P" + std::to_string(k) + " : public Proc<" + shape_type(V_[roots_[k]].shape) + ", " +
+                 tree(roots_[k]) + ", 1> {};\n";
+        o += "\nstruct ProcedureList : T::List<";
+        for (size_t k = 0; k < roots_.size(); ++k) o += (k ? ", P" : "P") + std::to_string(k);
+        o += "> {};\n\nDispatchTable dispatchTable = {\n";
+        for (size_t k = 0; k < roots_.size(); ++k)
+            o += "    /* entries[" + std::to_string(k) + "]= */ {" + std::to_string(k) + ", {}, {}},\n";
+        o += "};\n\nconst unsigned int AxiomEntry = 0;\nconst unsigned int AxiomShape = " +
+             std::to_string(V_[0].shape) + ";\n\n} // namespace " + name + "\n";
+        return o;
+    }
+
+private:
+    const Parser& g_;
+    const std::vector<GVertex>& V_;
+    const std::vector<GEdge>& E_;
+    std::vector<char> cut_;
+    std::vector<int> proc_of_;
+    std::vector<int> roots_;
+
+    static std::string shape_type(int s) {
+        if (s == PGA_SHAPE_BOX) return "Box";
+        if (s == PGA_SHAPE_QUAD) return "Quad";
+        throw CompileError("only Box and Quad procedures are generated");
+    }
+    static const char* axis_name(int a) { return a == 0 ? "X" : (a == 1 ? "Y" : "Z"); }
+    // Scalar<V, N>: the smallest N with V / 10^N (FP32, as the template
+    // evaluates it) equal to the FP32 value
+    static std::string fixed(double x, std::string* nspart = nullptr) {
+        const float want = (float)x;
+        for (int n = 0; n <= 9; ++n) {
+            const double scaled = std::nearbyint((double)want * std::pow(10.0, n));
+            if (std::fabs(scaled) > 2147483647.0) break;
+            const int v = (int)scaled;
+            float p = 1.0f;
+            for (int k = 0; k < n; ++k) p *= 10.0f;
+            if (v / p == want) {
+                if (nspart) *nspart = std::to_string(n);
+                return std::to_string(v);
+            }
+        }
+        throw CompileError("the value " + std::to_string(x) + " has no Scalar<V, N> form");
+    }
+    static std::string scalar(double x) {
+        std::string n;
+        const std::string v = fixed(x, &n);
+        return "Scalar<" + v + ", " + n + ">";
+    }
+    static std::string operation(int op) {
+        static const char* names[] = {"Add", "Sub", "Div", "Multi", "Eq", "Neq", "Lt", "Gt", "Leq", "Geq", "And", "Or"};
+        if (op < 0 || op > 11) throw CompileError("unknown operation");
+        return names[op];
+    }
+    static std::string operand(const Operand& o, bool in_exp, int line) {
+        switch (o.kind) {
+        case OD_NUMBER: return scalar(o.a);
+        case OD_AXIS: return std::string("AxisParam<") + axis_name((int)o.a) + ">";
+        case OD_MODE: {
+            static const char* m[] = {"ANCHOR_TO_START", "ANCHOR_TO_END", "ADJUST_TO_FILL"};
+            return std::string("RepeatModeParam<") + m[(int)o.a] + ">";
+        }
+        case OD_RAND: {
+            std::string na, nb;
+            const std::string a = fixed(o.a, &na), b = fixed(o.b, &nb);
+            return "Rand<" + a + ", " + b + ", " + na + ", " + nb + ">";
+        }
+        case OD_VEC2: {
+            std::string nx, ny;
+            const std::string x = fixed(o.a, &nx), y = fixed(o.b, &ny);
+            return "Vec2<" + x + ", " + y + ", " + nx + ", " + ny + ">";
+        }
+        case OD_ATTR:
+            switch (o.attr) {
+            case AT_POS: return std::string("ShapePos<") + axis_name(o.axis) + ">";
+            case AT_SIZE: return std::string("ShapeSize<") + axis_name(o.axis) + ">";
+            case AT_SEED: return "ShapeSeed";
+            case AT_CUSTOM: return "ShapeCustomAttribute";
+            case AT_ROTATION:
+                if (in_exp) return std::string("ShapeRotation<") + axis_name(o.axis) + ", " + axis_name(o.component) + ">";
+                break;
+            case AT_NORMAL:
+                if (in_exp) return std::string("ShapeNormal<") + axis_name(o.axis) + ">";
+                break;
+            default: break;
+            }
+            throw CompileError(at_line(line) + "ShapeRotation and ShapeNormal are only valid inside Exp(...)");
+        case OD_EXP:
+            return "Exp<" + operand(*o.left, true, line) + ", " + operand(*o.right, true, line) + ", " +
+                   operation(o.op) + ">";
+        }
+        throw CompileError("unknown operand");
+    }
+    const std::vector<double>* terminal_of(const std::string& name) const {
+        auto it = g_.terminals.find(name);
+        return it == g_.terminals.end() ? nullptr : &it->second;
+    }
+    std::string generate(const std::vector<double>* attrs, int line) const {
+        int gen = 0;
+        std::vector<double> rest;
+        if (attrs && !attrs->empty()) {
+            gen = (int)(*attrs)[0];
+            rest.assign(attrs->begin() + 1, attrs->end());
+        }
+        if (rest.empty()) rest.push_back(0.0);
+        if (rest.size() > 6) throw CompileError(at_line(line) + "too many terminal attributes");
+        std::string o = "Generate<false, " + std::to_string(gen);
+        for (double x : rest) o += ", " + scalar(x);
+        return o + ">";
+    }
+    std::string succ(int v, size_t i) {
+        const int e = V_[v].out[i];
+        const int w = E_[e].to;
+        if (cut_[e]) {
+            const std::string k = std::to_string(proc_of_[w]);
+            return "FSCall<" + k + ", 0, -1, " + k + ">";
+        }
+        return tree(w);
+    }
+    std::string tree(int v) {
+        const GVertex& x = V_[v];
+        if (!x.node) return generate(terminal_of(x.terminal), 0);
+        const Successor* s = x.node;
+        const int ln = s->line;
+        auto P = [&](size_t k) {
+            if (k >= s->params.size())
+                throw CompileError(at_line(ln) + operator_name(s->op) + " requires at least " + std::to_string(k + 1) +
+                                   " parameters");
+            return operand(s->params[k], false, ln);
+        };
+        auto need_succ = [&](size_t n) {
+            if (s->next.size() != n)
+                throw CompileError(at_line(ln) + operator_name(s->op) + " requires " + std::to_string(n) +
+                                   " successors");
+        };
+        switch (s->op) {
+        case PGA_OP_DISCARD: return "Discard";
+        case PGA_OP_GENERATE: return generate(terminal_of(x.rule), ln);
+        case PGA_OP_TRANSLATE: case PGA_OP_ROTATE: case PGA_OP_SCALE: case PGA_OP_SWAPSIZE: {
+            need_succ(1);
+            static const std::map<int, const char*> names = {{PGA_OP_TRANSLATE, "Translate"}, {PGA_OP_ROTATE, "Rotate"},
+                                                             {PGA_OP_SCALE, "Scale"}, {PGA_OP_SWAPSIZE, "SwapSize"}};
+            return std::string(names.at(s->op)) + "<" + P(0) + ", " + P(1) + ", " + P(2) + ", " + succ(v, 0) + ">";
+        }
+        case PGA_OP_EXTRUDE:
+            need_succ(1);
+            return "Extrude<AxisParam<Z>, " + P(0) + ", " + succ(v, 0) + ">";
+        case PGA_OP_COMPSPLIT:
+            need_succ(3);
+            return "ComponentSplit<false, " + succ(v, 0) + ", " + succ(v, 1) + ", " + succ(v, 2) + ">";
+        case PGA_OP_SUBDIV: {
+            if (s->factors.size() != s->next.size() || s->next.empty())
+                throw CompileError(at_line(ln) + "SubDiv requires a factor before each successor");
+            std::string o = "Subdivide<" + P(0);
+            for (size_t i = 0; i < s->next.size(); ++i) o += ", T::Pair<" + scalar(s->factors[i]) + ", " + succ(v, i) + ">";
+            return o + ">";
+        }
+        case PGA_OP_REPEAT:
+            need_succ(1);
+            return "Repeat<false, " + P(0) + ", " + P(1) + ", " + P(2) + ", " + succ(v, 0) + ">";
+        case PGA_OP_IF:
+            need_succ(2);
+            return "If<" + P(0) + ", " + succ(v, 0) + ", " + succ(v, 1) + ">";
+        case PGA_OP_IFSIZELESS:
+            need_succ(2);
+            return "IfSizeLess<" + P(0) + ", " + P(1) + ", " + succ(v, 0) + ", " + succ(v, 1) + ">";
+        case PGA_OP_STOCHASTIC: {
+            if (s->factors.size() != s->next.size() || s->next.empty())
+                throw CompileError(at_line(ln) + "RandomRule requires a factor before each successor");
+            std::string o = "RandomRule<" + P(0);
+            for (size_t i = 0; i < s->next.size(); ++i) o += ", T::Pair<" + scalar(s->factors[i]) + ", " + succ(v, i) + ">";
+            return o + ">";
+        }
+        case PGA_OP_REPLICATE: {
+            std::string o = "Replicate<";
+            for (size_t i = 0; i < s->next.size(); ++i) o += (i ? ", " : "") + succ(v, i);
+            return o + ">";
+        }
+        default:
+            throw CompileError(at_line(ln) + operator_name(s->op) + " is not supported");
+        }
+    }
+};
+
 } // namespace
 
 struct pga_grammar {
     std::vector<pga_dispatch_entry> entries;
     std::vector<pga_grammar_axiom> axioms;
+    std::string code;                 // partition code (pga_generate_partition)
+    std::vector<uint8_t> cycle;       // per operator-graph edge: 1 = closes a cycle
     std::string error;
 };
 
@@ -860,5 +1198,52 @@ PGA_API pga_status pga_grammar_axioms(const pga_grammar* g, const pga_grammar_ax
 }
 
 PGA_API void pga_grammar_destroy(pga_grammar* g) { delete g; }
+
+PGA_API pga_status pga_generate_partition(const char* source, const char* partition, const char* name,
+                                          pga_grammar** out) {
+    if (!out) return PGA_ERR_INVALID_ARG;
+    *out = nullptr;
+    pga_grammar* g = new (std::nothrow) pga_grammar();
+    if (!g) return PGA_ERR_INVALID_ARG;
+    *out = g;
+    if (!source || !name || !*name) {
+        g->error = "null source or name";
+        return PGA_ERR_INVALID_ARG;
+    }
+    for (const char* c = name; *c; ++c)
+        if (!(std::isalnum((unsigned char)*c) || *c == '_') || std::isdigit((unsigned char)name[0])) {
+            g->error = "the name must be a C++ identifier";
+            return PGA_ERR_INVALID_ARG;
+        }
+    try {
+        const std::string src(source);
+        Parser p(src);
+        p.parse();
+        GraphBuilder gb(p);
+        gb.run();
+        for (const GEdge& e : gb.E) g->cycle.push_back(e.cycle ? 1u : 0u);
+        CodeGen cg(p, gb, partition ? std::string(partition) : std::string());
+        g->code = cg.run(name);
+    } catch (const std::exception& e) {
+        g->error = e.what();
+        g->code.clear();
+        return PGA_ERR_BAD_PROGRAM;
+    }
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_grammar_code(const pga_grammar* g, const char** code, uint64_t* length) {
+    if (!g || !code || !length) return PGA_ERR_INVALID_ARG;
+    *code = g->code.c_str();
+    *length = g->code.size();
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_grammar_edges(const pga_grammar* g, uint32_t* num_edges, const uint8_t** cycle_flags) {
+    if (!g || !num_edges || !cycle_flags) return PGA_ERR_INVALID_ARG;
+    *num_edges = (uint32_t)g->cycle.size();
+    *cycle_flags = g->cycle.data();
+    return PGA_OK;
+}
 
 } // extern "C"

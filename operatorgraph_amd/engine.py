@@ -83,6 +83,9 @@ def load_library():
         "pga_grammar_axioms": (C.c_int, [P, C.POINTER(C.POINTER(abi.GrammarAxiom)), C.POINTER(C.c_uint32)]),
         "pga_grammar_destroy": (None, [P]),
         "pga_center_size_mse": (C.c_int, [P, P, P, C.c_uint64, C.c_float, C.c_float, P, P, P]),
+        "pga_generate_partition": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(P)]),
+        "pga_grammar_code": (C.c_int, [P, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64)]),
+        "pga_grammar_edges": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.POINTER(C.c_uint8))]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
