@@ -406,6 +406,28 @@ PGA_API pga_status pga_device_alloc(uint64_t bytes, void** device_ptr);
 PGA_API pga_status pga_device_free(void* device_ptr);
 PGA_API pga_status pga_copy_to_host(void* host_dst, const void* device_src, uint64_t bytes);
 
+/* -- generation function 0 (triangle soup): terminals whose procedure uses
+ *    generation function 0 are derived into instance buffers like
+ *    instanced ones; pga_tessellate then writes the reference's geometry
+ *    for `count` instances of one shape type (ShapeGenerator<Box, false>::run,
+ *    include/pga/core/ShapeGenerator.cuh:376-511: 24 vertices, 36 indices
+ *    per box; ShapeGenerator<Quad, false>::run, :670-708: 4 and 6 per quad;
+ *    the default winding, PGA_CW undefined). Vertex records are
+ *    TriangleMeshData::VertexAttributes (include/pga/rendering/TriangleMeshData.h:12-17).
+ *    Instance i's V vertices are written at vertices[i * V] and its
+ *    1.5 V indices at indices[i * 1.5 V]; an index holds
+ *    base_vertex + i * V + corner, so calls that fill one mesh share its
+ *    index space. Instances keep their buffer order. All pointers are
+ *    device memory; `instances` 16-byte aligned. One kernel on `stream`,
+ *    no synchronisation. -- */
+typedef struct pga_vertex {
+    float position[4];
+    float normal[3];
+    float uv[2];
+} pga_vertex; /* 36 bytes */
+PGA_API pga_status pga_tessellate(const pga_instance* instances, uint64_t count, int32_t shape_type,
+                                  pga_vertex* vertices, uint32_t* indices, uint32_t base_vertex, void* stream);
+
 /* -- grammar front end: the PGA grammar language (*.pga) compiled on the
  *    host into a dispatch table over the interpreter's procedure list.
  *    Replaces PGA::Compiler::Parser::parse + DispatchTableGenerator::fromRuleSet

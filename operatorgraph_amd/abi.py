@@ -107,6 +107,10 @@ class Instance(C.Structure):
                 ("path", C.c_uint32), ("meta", C.c_uint32)]
 
 
+class Vertex(C.Structure):
+    _fields_ = [("position", C.c_float * 4), ("normal", C.c_float * 3), ("uv", C.c_float * 2)]
+
+
 class Config(C.Structure):
     _fields_ = [("technique", C.c_int32), ("queue_capacity", C.c_uint32),
                 ("block_size", C.c_uint32), ("max_levels", C.c_uint32),

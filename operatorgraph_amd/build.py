@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpga_amd.so")
-SOURCES = ["pga_kernels.hip", "pga_adjoint.hip", "pga_lbvh.hip", "pga_loss.hip", "pga_api.cpp"]
+SOURCES = ["pga_kernels.hip", "pga_adjoint.hip", "pga_lbvh.hip", "pga_loss.hip", "pga_mesh.hip", "pga_api.cpp"]
 HOST_SOURCES = ["pga_compiler.cpp"]   # no device code: built with the host compiler
 HEADERS = ["pga_device.h", "pga_ops.h", "pga_kernels.h"]
 

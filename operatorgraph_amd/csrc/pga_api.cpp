@@ -252,8 +252,10 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
             break;
         case PGA_OP_REPLICATE: d.kind = pga::K_REPLICATE; break;
         case PGA_OP_GENERATE:
-            if (p.gen_func_idx != 1)
-                return fail(ctx, PGA_ERR_UNSUPPORTED, at + "only instanced generation (GenFunc 1) is supported");
+            // generation function 0 (triangle soup) is derived into instance
+            // records as well; pga_tessellate turns them into its geometry
+            if (p.gen_func_idx != 0 && p.gen_func_idx != 1)
+                return fail(ctx, PGA_ERR_UNSUPPORTED, at + "only generation functions 0 and 1 are supported");
             d.kind = pga::K_GENERATE; need_p = 1; break;
         default:
             return fail(ctx, PGA_ERR_UNSUPPORTED,

@@ -1,0 +1,114 @@
+// pga_mesh.hip - generation function 0 (triangle soup) over instance
+// records (gfx950).
+//
+// The reference's GenerationFunction 0 writes, per Box, 24 vertices
+// (four per face: position, face normal, uv) and 36 indices, and per Quad
+// 4 vertices and 6 indices (ShapeGenerator<Box, false>::run,
+// include/pga/core/ShapeGenerator.cuh:376-511; <Quad, false>:670-708, the
+// default winding with PGA_CW undefined). A corner is model4 * (+-size/2, 1).
+// The instance record holds I = model4 * scale(size) (rows 0..2), and
+//   m_ij * (s_j * 0.5) == (m_ij * s_j) * 0.5
+// in FP32 (a power-of-two scale commutes with rounding), so a corner is
+// I * (+-0.5, 1), with the same operation order as the reference's
+// matrix-vector product: ((a x + b y) + c z) + d w. The result equals the
+// reference's vertex values; the only exception is the sign of an exact zero.
+//
+// One thread per vertex: it rebuilds its face's corners 0..2 (for the
+// normal) and its own corner, and writes one 36-byte vertex, so consecutive
+// threads write consecutive vertices. The same thread writes indices k and
+// k + V of its shape (V vertices, 1.5 V indices per shape). Shapes keep
+// the order of the instance buffer; the reference's atomics leave it
+// unspecified.
+#include <hip/hip_runtime.h>
+
+#include "../../include/pga.h"
+
+namespace pgamesh {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ void corner(const float4& r0, const float4& r1, const float4& r2, float x, float y, float z,
+                                       float o[3]) {
+    o[0] = ((r0.x * x + r0.y * y) + r0.z * z) + r0.w * 1.0f;
+    o[1] = ((r1.x * x + r1.y * y) + r1.z * z) + r1.w * 1.0f;
+    o[2] = ((r2.x * x + r2.y * y) + r2.z * z) + r2.w * 1.0f;
+}
+
+// corner signs (x, y, z) of corner c of face f (0..5: top, bottom, left,
+// right, front, back; 6: a quad, z = 0), in half-size units
+__constant__ int8_t kCorners[7][4][3] = {
+    {{-1, 1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, 1, 1}},       // top
+    {{-1, -1, 1}, {1, -1, 1}, {1, -1, -1}, {-1, -1, -1}},   // bottom
+    {{-1, 1, -1}, {-1, 1, 1}, {-1, -1, 1}, {-1, -1, -1}},   // left
+    {{1, 1, 1}, {1, 1, -1}, {1, -1, -1}, {1, -1, 1}},       // right
+    {{-1, 1, 1}, {1, 1, 1}, {1, -1, 1}, {-1, -1, 1}},       // front
+    {{1, 1, -1}, {-1, 1, -1}, {-1, -1, -1}, {1, -1, -1}},   // back
+    {{-1, 1, 0}, {1, 1, 0}, {1, -1, 0}, {-1, -1, 0}}};      // quad
+
+__device__ __forceinline__ void face_corner(uint32_t f, uint32_t c, float& x, float& y, float& z) {
+    x = 0.5f * (float)kCorners[f][c][0];
+    y = 0.5f * (float)kCorners[f][c][1];
+    z = 0.5f * (float)kCorners[f][c][2];
+}
+
+__global__ void __launch_bounds__(kBlock) k_tessellate(const float4* __restrict__ inst, uint64_t count, uint32_t quad,
+                                                       float* __restrict__ vert, uint32_t* __restrict__ idx,
+                                                       uint32_t base_vertex) {
+    const uint32_t V = quad ? 4u : 24u;
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count * V) return;
+    const uint64_t s = t / V;
+    const uint32_t k = (uint32_t)(t - s * V);
+    const uint32_t f = quad ? 6u : k / 4u, c = k & 3u;
+    const float4 r0 = inst[s * 4u], r1 = inst[s * 4u + 1u], r2 = inst[s * 4u + 2u];
+    float v[3][3], p[3], x, y, z;
+#pragma unroll
+    for (uint32_t q = 0; q < 3; ++q) {
+        face_corner(f, q, x, y, z);
+        corner(r0, r1, r2, x, y, z, v[q]);
+    }
+    face_corner(f, c, x, y, z);
+    corner(r0, r1, r2, x, y, z, p);
+    const float a[3] = {v[0][0] - v[1][0], v[0][1] - v[1][1], v[0][2] - v[1][2]};
+    const float b[3] = {v[2][0] - v[1][0], v[2][1] - v[1][1], v[2][2] - v[1][2]};
+    const float n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const float len = sqrtf((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
+    const float inv = 1.0f / len;
+    const float u = (c == 0u || c == 3u) ? 1.0f : 0.0f, w = c < 2u ? 1.0f : 0.0f;
+    float* o = vert + t * 9u;
+    o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; o[3] = ((0.0f * x + 0.0f * y) + 0.0f * z) + 1.0f * 1.0f;
+    o[4] = n[0] * inv; o[5] = n[1] * inv; o[6] = n[2] * inv;
+    o[7] = u; o[8] = w;
+    // indices: (0, 1, 2), (0, 2, 3) per face of four vertices
+    const uint32_t NI = V + V / 2u;
+    const uint32_t first = base_vertex + (uint32_t)(s * V);
+    for (uint32_t j = k; j < NI; j += V) {
+        const uint32_t face = j / 6u, r = j - face * 6u;
+        const uint32_t cc = r == 0u || r == 3u ? 0u : (r == 1u ? 1u : (r == 2u || r == 4u ? 2u : 3u));
+        idx[s * NI + j] = first + face * 4u + cc;
+    }
+}
+
+} // namespace pgamesh
+
+extern "C" {
+
+// Declared in include/pga.h.
+PGA_API pga_status pga_tessellate(const pga_instance* instances, uint64_t count, int32_t shape_type,
+                                  pga_vertex* vertices, uint32_t* indices, uint32_t base_vertex, void* stream) {
+    if (count == 0u) return PGA_OK;
+    if (!instances || !vertices || !indices) return PGA_ERR_INVALID_ARG;
+    if (shape_type != PGA_SHAPE_BOX && shape_type != PGA_SHAPE_QUAD) return PGA_ERR_UNSUPPORTED;
+    if (((uintptr_t)instances & 15u) != 0u) return PGA_ERR_INVALID_ARG;
+    const uint64_t V = shape_type == PGA_SHAPE_QUAD ? 4u : 24u;
+    if (count > (0xFFFFFFFFull - base_vertex) / V) return PGA_ERR_INVALID_ARG;   // 32-bit vertex indices
+    const uint64_t threads = count * V;
+    const uint64_t blocks = (threads + pgamesh::kBlock - 1u) / pgamesh::kBlock;
+    if (blocks > 0x7FFFFFFFull) return PGA_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(pgamesh::k_tessellate, dim3((uint32_t)blocks), dim3(pgamesh::kBlock), 0, (hipStream_t)stream,
+                       reinterpret_cast<const float4*>(instances), count, shape_type == PGA_SHAPE_QUAD ? 1u : 0u,
+                       reinterpret_cast<float*>(vertices), indices, base_vertex);
+    return hipGetLastError() == hipSuccess ? PGA_OK : PGA_ERR_HIP;
+}
+
+} // extern "C"

@@ -47,6 +47,9 @@ def lib():
         L.pgao_executions.argtypes = [P]
         L.pgao_copy_instances.argtypes = [P, C.c_uint32, P]
         L.pgao_copy_instances_f64.argtypes = [P, C.c_uint32, P]
+        L.pgao_mesh_floats.restype = C.c_uint64
+        L.pgao_mesh_floats.argtypes = [P, C.c_uint32]
+        L.pgao_copy_mesh.argtypes = [P, C.c_uint32, P]
         L.pgao_backward.argtypes = [P, P, P]
         L.pgao_next_seed.restype = C.c_float
         L.pgao_next_seed.argtypes = [C.c_float, C.c_uint32]
@@ -109,6 +112,15 @@ class Result:
         if n:
             lib().pgao_copy_instances(self._h, t, out.ctypes.data)
         return out
+
+    def mesh(self, t):
+        """Generation-function-0 vertices of terminal t: (n_vertices, 9)
+        float32 (position xyzw, normal xyz, uv), in derivation order."""
+        n = lib().pgao_mesh_floats(self._h, t)
+        out = np.zeros(n, dtype=np.float32)
+        if n:
+            lib().pgao_copy_mesh(self._h, t, out.ctypes.data)
+        return out.reshape(-1, 9)
 
     def instances_f64(self, t):
         n = self.count(t)
@@ -218,3 +230,62 @@ def lbvh(aabbs, scene_min, scene_max):
         raise OracleError(st, "lbvh failed")
     return dict(keys=keys, left=left, right=right, parent=parent, leaf_parent=leaf_parent,
                 node_aabb=node_aabb)
+
+
+# Corner signs (x, y, z) of the reference's generation-function-0 faces, in
+# half-size units, default winding (PGA_CW undefined): top, bottom, left,
+# right, front, back of ShapeGenerator<Box, false>::run
+# (include/pga/core/ShapeGenerator.cuh:376-511) and the one face of
+# ShapeGenerator<Quad, false>::run (:670-708).
+BOX_FACES = np.array([
+    [[-1, 1, -1], [1, 1, -1], [1, 1, 1], [-1, 1, 1]],
+    [[-1, -1, 1], [1, -1, 1], [1, -1, -1], [-1, -1, -1]],
+    [[-1, 1, -1], [-1, 1, 1], [-1, -1, 1], [-1, -1, -1]],
+    [[1, 1, 1], [1, 1, -1], [1, -1, -1], [1, -1, 1]],
+    [[-1, 1, 1], [1, 1, 1], [1, -1, 1], [-1, -1, 1]],
+    [[1, 1, -1], [-1, 1, -1], [-1, -1, -1], [1, -1, -1]]], dtype=np.float32)
+QUAD_FACE = np.array([[[-1, 1, 0], [1, 1, 0], [1, -1, 0], [-1, -1, 0]]], dtype=np.float32)
+FACE_UV = np.array([[1, 1], [0, 1], [0, 0], [1, 0]], dtype=np.float32)   # uv0..uv3
+
+
+def mesh_from_instances(models, shape_type):
+    """Generation-function-0 vertices of instance records, vectorised in
+    numpy float32 for sizes the C oracle is slow at: (n, 12) row-major
+    I = model4 * scale(size) -> (n * V, 9). A corner is I * (+-0.5, 1)
+    summed as ((a x + b y) + c z) + d w (math::float4x4 * float4,
+    include/pga/math/matrix.h:793-799); the normal is
+    normalize(cross(v0 - v1, v2 - v1)) (vector.h:510-515, 566-574). Each
+    numpy float32 operation rounds once, with no fused multiply-add, so
+    the values are those of the C oracle's shape_mesh up to the sign of a
+    zero (the C oracle scales the model by size / 2 instead)."""
+    faces = QUAD_FACE if shape_type == abi.QUAD else BOX_FACES
+    m = np.ascontiguousarray(models, dtype=np.float32).reshape(-1, 3, 4)
+    half = faces * np.float32(0.5)                                     # (F, 4, 3)
+    x, y, z = half[..., 0], half[..., 1], half[..., 2]
+    one = np.float32(1.0)
+    p = np.empty((len(m), len(faces), 4, 4), dtype=np.float32)
+    for r in range(3):
+        a, b, c, d = (m[:, r, k][:, None, None] for k in range(4))
+        p[..., r] = ((a * x + b * y) + c * z) + d * one
+    p[..., 3] = ((np.float32(0) * x + np.float32(0) * y) + np.float32(0) * z) + one * one
+    e1 = p[:, :, 0, :3] - p[:, :, 1, :3]
+    e2 = p[:, :, 2, :3] - p[:, :, 1, :3]
+    n = np.stack([e1[..., 1] * e2[..., 2] - e1[..., 2] * e2[..., 1],
+                  e1[..., 2] * e2[..., 0] - e1[..., 0] * e2[..., 2],
+                  e1[..., 0] * e2[..., 1] - e1[..., 1] * e2[..., 0]], -1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = one / np.sqrt((n[..., 0] * n[..., 0] + n[..., 1] * n[..., 1]) + n[..., 2] * n[..., 2])
+        n = n * inv[..., None]
+    out = np.empty((len(m), len(faces), 4, 9), dtype=np.float32)
+    out[..., 0:4] = p
+    out[..., 4:7] = n[:, :, None, :]
+    out[..., 7:9] = FACE_UV
+    return out.reshape(-1, 9)
+
+
+def mesh_indices(n, shape_type, base_vertex=0):
+    """Index buffer of n shapes: (0, 1, 2), (0, 2, 3) per face of four
+    vertices (ShapeGenerator.cuh:505-509, 703-704), offset by base_vertex."""
+    faces = 1 if shape_type == abi.QUAD else 6
+    q = (np.arange(n * faces, dtype=np.int64) * 4)[:, None] + np.array([0, 1, 2, 0, 2, 3])
+    return (q.reshape(-1) + base_vertex).astype(np.uint32)

@@ -767,11 +767,61 @@ void run_op(const Ctx& c, const Meta& m, const Shape<T>& sh, Dec& d, std::vector
     case PGA_OP_REPLICATE: op_replicate(m, sh, e, out); break;
     case PGA_OP_DISCARD: break;                               // Discard.cuh:15-31
     case PGA_OP_GENERATE:
-        if (p.gen_func_idx != 1) throw OpErr(PGA_ERR_UNSUPPORTED, "only instanced generation (GenFunc 1)");
+        if (p.gen_func_idx != 0 && p.gen_func_idx != 1)
+            throw OpErr(PGA_ERR_UNSUPPORTED, "generation functions 0 and 1 only");
         op_generate(c, m, sh, e, d, out);
         break;
     default: throw OpErr(PGA_ERR_UNSUPPORTED, "unsupported operator type " + std::to_string(p.op_type));
     }
+}
+
+// GenerationFunction 0 (triangle soup) for Box and Quad, counter-clockwise
+// (the default, PGA_CW off): ShapeGenerator<Box, false>::run
+// (include/pga/core/ShapeGenerator.cuh:376-511) and ShapeGenerator<Quad,
+// false>::run (:670-708). Vertex k of a shape: position = model4 * corner
+// (matrix.h:793-799: ((m0 x + m1 y) + m2 z) + m3 w), normal =
+// normalize(cross(v0 - v1, v2 - v1)) of its face (vector.h:510-515, 566-574:
+// sqrt of ((x x + y y) + z z), times 1 / length), uv per corner. Nine floats
+// per vertex: position xyzw, normal xyz, uv.
+void xform4(const float m[12], float x, float y, float z, float o[4]) {
+    for (int r = 0; r < 3; ++r) o[r] = ((m[r * 4] * x + m[r * 4 + 1] * y) + m[r * 4 + 2] * z) + m[r * 4 + 3] * 1.0f;
+    o[3] = ((0.0f * x + 0.0f * y) + 0.0f * z) + 1.0f * 1.0f;
+}
+
+void face_vertices(const float m[12], const float c[4][3], std::vector<float>& out) {
+    float v[4][4];
+    for (int k = 0; k < 4; ++k) xform4(m, c[k][0], c[k][1], c[k][2], v[k]);
+    const float a[3] = {v[0][0] - v[1][0], v[0][1] - v[1][1], v[0][2] - v[1][2]};
+    const float b[3] = {v[2][0] - v[1][0], v[2][1] - v[1][1], v[2][2] - v[1][2]};
+    const float n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const float len = std::sqrt((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
+    const float inv = 1.0f / len;
+    static const float uv[4][2] = {{1.0f, 1.0f}, {0.0f, 1.0f}, {0.0f, 0.0f}, {1.0f, 0.0f}};
+    for (int k = 0; k < 4; ++k) {
+        out.insert(out.end(), v[k], v[k] + 4);
+        out.push_back(n[0] * inv);
+        out.push_back(n[1] * inv);
+        out.push_back(n[2] * inv);
+        out.push_back(uv[k][0]);
+        out.push_back(uv[k][1]);
+    }
+}
+
+void shape_mesh(const Shape<float>& s, int32_t shape_type, std::vector<float>& out) {
+    const float hx = s.size[0] * 0.5f, hy = s.size[1] * 0.5f, hz = s.size[2] * 0.5f;   // Shape.cuh:70-73
+    if (shape_type == PGA_SHAPE_QUAD) {
+        const float q[4][3] = {{-hx, hy, 0.0f}, {hx, hy, 0.0f}, {hx, -hy, 0.0f}, {-hx, -hy, 0.0f}};
+        face_vertices(s.m, q, out);
+        return;
+    }
+    const float f[6][4][3] = {
+        {{-hx, hy, -hz}, {hx, hy, -hz}, {hx, hy, hz}, {-hx, hy, hz}},        // top
+        {{-hx, -hy, hz}, {hx, -hy, hz}, {hx, -hy, -hz}, {-hx, -hy, -hz}},    // bottom
+        {{-hx, hy, -hz}, {-hx, hy, hz}, {-hx, -hy, hz}, {-hx, -hy, -hz}},    // left
+        {{hx, hy, hz}, {hx, hy, -hz}, {hx, -hy, -hz}, {hx, -hy, hz}},        // right
+        {{-hx, hy, hz}, {hx, hy, hz}, {hx, -hy, hz}, {-hx, -hy, hz}},        // front
+        {{hx, hy, -hz}, {-hx, hy, -hz}, {-hx, -hy, -hz}, {hx, -hy, -hz}}};   // back
+    for (int k = 0; k < 6; ++k) face_vertices(s.m, f[k], out);
 }
 
 struct Item {
@@ -789,6 +839,7 @@ struct pgao_result {
     uint64_t steps = 0, executions = 0;
     uint32_t nparams = 0;
     std::vector<std::vector<pga_instance>> inst;
+    std::vector<std::vector<float>> mesh;         // generation function 0: 9 floats per vertex
     std::vector<std::vector<double>> inst64;
     std::vector<std::vector<int32_t>> inst_ids;   // 12 tape ids per instance
     Tape tape;
@@ -807,6 +858,7 @@ int pgao_derive(const pga_procedure* procs, uint32_t num_procs,
     *out = res;
     res->twins = twins;
     res->inst.resize(PGA_MAX_TERMINALS);
+    res->mesh.resize(PGA_MAX_TERMINALS);
     res->inst64.resize(PGA_MAX_TERMINALS);
     res->inst_ids.resize(PGA_MAX_TERMINALS);
     Ctx c;
@@ -888,6 +940,8 @@ int pgao_derive(const pga_procedure* procs, uint32_t num_procs,
                             inst.path = it.meta.path;
                             inst.meta = (it.meta.depth << 16) | ((uint32_t)it.meta.entry & 0xFFFFu);
                             res->inst[o.terminal].push_back(inst);
+                            if (procs[entries[it.meta.entry].operator_code].gen_func_idx == 0)
+                                shape_mesh(it.f, it.meta.shape_type, res->mesh[o.terminal]);
                             if (twins & PGAO_TWIN_F64) {
                                 for (int i = 0; i < 12; ++i) res->inst64[o.terminal].push_back(od[k].s.m[i]);
                                 res->inst64[o.terminal].push_back(od[k].s.custom);
@@ -945,6 +999,14 @@ uint64_t pgao_executions(const pgao_result* r) { return r->executions; }
 int pgao_copy_instances(const pgao_result* r, uint32_t t, pga_instance* dst) {
     if (t >= PGA_MAX_TERMINALS) return PGA_ERR_INVALID_ARG;
     if (!r->inst[t].empty()) std::memcpy(dst, r->inst[t].data(), r->inst[t].size() * sizeof(pga_instance));
+    return PGA_OK;
+}
+uint64_t pgao_mesh_floats(const pgao_result* r, uint32_t t) {
+    return t < PGA_MAX_TERMINALS ? (uint64_t)r->mesh[t].size() : 0u;
+}
+int pgao_copy_mesh(const pgao_result* r, uint32_t t, float* dst) {
+    if (t >= PGA_MAX_TERMINALS) return PGA_ERR_INVALID_ARG;
+    if (!r->mesh[t].empty()) std::memcpy(dst, r->mesh[t].data(), r->mesh[t].size() * sizeof(float));
     return PGA_OK;
 }
 int pgao_copy_instances_f64(const pgao_result* r, uint32_t t, double* dst) {

@@ -46,6 +46,13 @@ uint64_t pgao_steps(const pgao_result* r);
 uint64_t pgao_executions(const pgao_result* r);
 /* FP32 instances in derivation order (the reference CPU order) */
 int pgao_copy_instances(const pgao_result* r, uint32_t terminal, pga_instance* dst);
+/* Generation function 0 (triangle soup, ShapeGenerator.cuh:376-511 Box,
+ * 670-708 Quad): the vertices of each such terminal's shapes, in derivation
+ * order, 9 floats per vertex (position xyzw, normal xyz, uv), 24 vertices
+ * per box and 4 per quad; the index pattern is implied (two triangles
+ * (0,1,2) (0,2,3) per face of four vertices). */
+uint64_t pgao_mesh_floats(const pgao_result* r, uint32_t terminal);
+int pgao_copy_mesh(const pgao_result* r, uint32_t terminal, float* dst);
 /* FP64 twin: 13 doubles per instance (model[12], custom) */
 int pgao_copy_instances_f64(const pgao_result* r, uint32_t terminal, double* dst);
 /* Reverse sweep of the adj twin: dl_dmodel holds 12 doubles per instance of

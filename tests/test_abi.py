@@ -29,7 +29,7 @@ int main(void) {
   O(pga_axiom, size) O(pga_axiom, seed) O(pga_axiom, custom) O(pga_axiom, entry_index)
   O(pga_axiom, shape_type) O(pga_instance, custom) O(pga_instance, axiom) O(pga_instance, path)
   O(pga_instance, meta) O(pga_config, stream) O(pga_tune_result, seconds)
-  O(pga_tune_result, max_level_children)
+  O(pga_tune_result, max_level_children) S(pga_vertex) O(pga_vertex, normal) O(pga_vertex, uv)
   return 0;
 }
 """
@@ -38,7 +38,7 @@ MIRRORS = {
     "pga_dispatch_parameter": abi.DispatchParameter, "pga_dispatch_successor": abi.DispatchSuccessor,
     "pga_dispatch_entry": abi.DispatchEntry, "pga_procedure": abi.Procedure, "pga_axiom": abi.Axiom,
     "pga_instance": abi.Instance, "pga_config": abi.Config, "pga_stats": abi.Stats,
-    "pga_tune_result": abi.TuneResult,
+    "pga_tune_result": abi.TuneResult, "pga_vertex": abi.Vertex,
 }
 
 

@@ -1,0 +1,172 @@
+"""GPU parity of generation function 0 (triangle soup): grammars whose
+terminals use generation function 0 are derived by the HIP engine, their
+instance buffers tessellated by pga_tessellate through the C ABI, and the
+vertices compared value for value with the oracle's mesh of the same
+shapes (canonical instance order; zeros may differ in sign), the indices
+with the reference's (0, 1, 2), (0, 2, 3) per face. At full size
+(2^20 random boxes) the check is the numpy restatement on a sample plus
+the whole index buffer."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from operatorgraph_amd import abi
+from operatorgraph_amd.compiler import compile_grammar
+from tests.helpers import assert_same_multiset, canon_order
+from tests.test_mesh import BOX_ONE, MENGER0, QUAD_ONE, random_lots
+
+pytestmark = pytest.mark.gpu
+
+FACADE0 = """
+axiom Box Lot;
+terminal Wall (0, 0);
+terminal Window (0, 1);
+Lot    = CompSplit() { Discard() | Discard() | Facade };
+Facade = Repeat(Y, 3, ADJUST_TO_FILL) Floor;
+Floor  = Repeat(X, 2.5, ADJUST_TO_FILL) Tile;
+Tile   = IfSizeLess(X, 1.6) { Wall | SubDiv(X) { -1: Wall | 1.2: Window | -1: Wall } };
+"""
+
+# (grammar, lots, terminal shape types)
+CASES = {
+    "box_one": (BOX_ONE, 256, [abi.BOX]),
+    "quad_one": (QUAD_ONE, 256, [abi.QUAD]),
+    "menger3": (MENGER0, 4, [abi.BOX]),
+    "facade": (FACADE0, 24, [abi.QUAD, abi.QUAD]),
+}
+
+
+def _note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def facade_lots(g, n, seed):
+    rng = np.random.default_rng(seed)
+    sizes = np.stack([rng.uniform(8, 30, n), rng.uniform(9, 40, n), rng.uniform(8, 30, n)], 1)
+    pos = np.stack([np.arange(n) * 40.0, np.zeros(n), np.zeros(n)], 1)
+    return g.axiom_records(rng.random(n).astype(np.float32), sizes=sizes, positions=pos)
+
+
+def derived(case, technique):
+    """(oracle result, engine) after deriving the case on both."""
+    from oracle import oracle as O
+    from operatorgraph_amd import engine
+    src, n, shapes = CASES[case]
+    g = compile_grammar(src, case)
+    ax = facade_lots(g, n, 11) if case == "facade" else random_lots(g, n, 11)
+    r = O.derive(g.program, ax)
+    eng = engine.Engine(technique=technique)
+    eng.load(g.program).set_axioms(ax)
+    for t in range(len(shapes)):
+        eng.bind(t, r.count(t) + 64)
+    eng.execute()
+    for t in range(len(shapes)):
+        assert r.count(t) > 0
+        assert_same_multiset(eng.instances(t), r.instances(t), f"{case} terminal {t}")
+    return r, eng, shapes
+
+
+@pytest.mark.parametrize("technique", [abi.KERNELS, abi.MEGAKERNEL])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_tessellate_matches_oracle(case, technique):
+    import torch
+
+    from oracle.oracle import mesh_from_instances, mesh_indices
+    from operatorgraph_amd.mesh import tessellate
+    _note(f"tessellate {case}, technique={technique}")
+    r, eng, shapes = derived(case, technique)
+    for t, shape in enumerate(shapes):
+        n = r.count(t)
+        V = 4 if shape == abi.QUAD else 24
+        verts, idx = tessellate(eng.buffer(t)[:n], shape)
+        torch.cuda.synchronize()
+        gv = verts.cpu().numpy().reshape(n, V, 9)
+        gi = idx.cpu().numpy().view(np.uint32)
+        got, ref = eng.instances(t), r.instances(t)
+        assert np.array_equal(gv[canon_order(got)], r.mesh(t).reshape(n, V, 9)[canon_order(ref)])
+        assert np.array_equal(gv.reshape(-1, 9), mesh_from_instances(got["model"], shape))
+        assert np.array_equal(gi, mesh_indices(n, shape))
+    eng.close()
+
+
+def test_two_calls_share_one_index_space(tmp_path):
+    """Walls then windows into one vertex/index space (base_vertex), written
+    as one OBJ file and read back."""
+    import torch
+
+    from oracle.oracle import mesh_indices
+    from operatorgraph_amd.mesh import read_obj, tessellate, write_obj
+    _note("tessellate facade walls + windows, one index space")
+    r, eng, _ = derived("facade", abi.MEGAKERNEL)
+    nw, nq = r.count(0), r.count(1)
+    v0, i0 = tessellate(eng.buffer(0)[:nw], abi.QUAD)
+    v1, i1 = tessellate(eng.buffer(1)[:nq], abi.QUAD, base_vertex=4 * nw)
+    verts, idx = torch.cat([v0, v1]), torch.cat([i0, i1])
+    torch.cuda.synchronize()
+    want = np.concatenate([mesh_indices(nw, abi.QUAD), mesh_indices(nq, abi.QUAD, 4 * nw)])
+    assert np.array_equal(idx.cpu().numpy().view(np.uint32), want)
+    path = tmp_path / "facade.obj"
+    write_obj(str(path), verts, idx.cpu().numpy().view(np.uint32))
+    v, vn, vt, tri = read_obj(str(path))
+    host = verts.cpu().numpy()
+    assert np.array_equal(v, host[:, :3]) and np.array_equal(vn, host[:, 4:7]) and np.array_equal(vt, host[:, 7:9])
+    assert np.array_equal(tri.reshape(-1), want.astype(np.int64))
+    eng.close()
+
+
+def test_full_size_random_boxes():
+    """2^20 random instance records (1.3 GB of output): a sample of 4096
+    shapes equals the numpy restatement value for value, including records
+    with a zero column (degenerate faces: NaN normals on both sides); the
+    whole index buffer is checked on the GPU."""
+    import torch
+
+    from oracle.oracle import mesh_from_instances
+    from operatorgraph_amd.mesh import tessellate
+    n = 1 << 20
+    _note(f"tessellate {n} random boxes")
+    g = torch.Generator(device="cuda").manual_seed(5)
+    inst = torch.randn((n, 16), device="cuda", generator=g) * 10.0
+    inst[::97, 0:12:4] = 0.0                       # zero first column: degenerate faces
+    verts, idx = tessellate(inst, abi.BOX)
+    torch.cuda.synchronize()
+    sample = np.random.default_rng(6).choice(n, 4096, replace=False)
+    sample[:8] = np.arange(0, 8 * 97, 97)
+    s = torch.from_numpy(sample).cuda()
+    gv = verts.view(n, 24, 9)[s].cpu().numpy()
+    ref = mesh_from_instances(inst[s, :12].cpu().numpy(), abi.BOX).reshape(-1, 24, 9)
+    assert np.array_equal(gv, ref, equal_nan=True)
+    deg = sample % 97 == 0
+    assert np.isnan(gv[deg]).any(axis=(1, 2)).all() and not np.isnan(gv[~deg]).any()
+    j = torch.arange(n * 36, device="cuda", dtype=torch.int64)
+    corner = torch.tensor([0, 1, 2, 0, 2, 3], device="cuda")[j % 6]
+    want = (j // 36) * 24 + ((j % 36) // 6) * 4 + corner
+    assert torch.equal(idx.to(torch.int64) & 0xFFFFFFFF, want)
+
+
+def test_tessellate_argument_errors():
+    import torch
+
+    from operatorgraph_amd.engine import load_library
+    from operatorgraph_amd.mesh import tessellate
+    _note("tessellate argument errors")
+    L = load_library()
+    inst = torch.zeros((8, 16), device="cuda")
+    verts = torch.empty((8 * 24, 9), device="cuda")
+    idx = torch.empty((8 * 36,), dtype=torch.int32, device="cuda")
+    p = [C.c_void_p(x.data_ptr()) for x in (inst, verts, idx)]
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert L.pga_tessellate(p[0], 8, abi.TRIANGLE, p[1], p[2], 0, s) == abi.ERR_UNSUPPORTED
+    assert L.pga_tessellate(None, 8, abi.BOX, p[1], p[2], 0, s) == abi.ERR_INVALID_ARG
+    assert L.pga_tessellate(C.c_void_p(inst.data_ptr() + 4), 7, abi.BOX, p[1], p[2], 0, s) == abi.ERR_INVALID_ARG
+    # 32-bit vertex indices: more than 2^32 vertices is refused before any launch
+    assert L.pga_tessellate(p[0], (1 << 32) // 24 + 1, abi.BOX, p[1], p[2], 0, s) == abi.ERR_INVALID_ARG
+    assert L.pga_tessellate(p[0], 8, abi.BOX, p[1], p[2], 0xFFFFFFFF - 100, s) == abi.ERR_INVALID_ARG
+    assert L.pga_tessellate(None, 0, abi.BOX, None, None, 0, s) == abi.OK
+    v, i = tessellate(inst[:0], abi.BOX)
+    assert v.shape == (0, 9) and i.shape == (0,)
+    with pytest.raises(ValueError):
+        tessellate(inst[:, :12], abi.BOX)
