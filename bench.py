@@ -26,6 +26,10 @@ Prints ONE JSON line. Beside the headline (batched terminals/s) it reports:
   mixed_fwd_bwd_menger5_16384_lbvh - the same batch with parameterised lots:
                   tape forward, fused loss, adjoint and the LBVH rebuild per
                   step;
+  triangle_soup_16384 - the facade grammar with triangle-soup terminals
+                  (generation function 0) over 16,384 lots: derivation plus
+                  one tessellation launch (4 vertices, 6 indices per quad)
+                  per step, with the tessellation kernel's HBM roofline;
   cpu_baseline  - the CPU restatement (oracle) on the host, one thread.
 """
 import argparse
@@ -58,6 +62,7 @@ def parse():
     p.add_argument("--gradient-lots", type=int, default=256)
     p.add_argument("--mixed-lots", type=int, default=16384)
     p.add_argument("--mixed-grad-lots", type=int, default=16384)
+    p.add_argument("--mesh-lots", type=int, default=16384)
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--only-kernel", action="store_true",
@@ -276,6 +281,76 @@ def mixed_workload(torch, engine, grammars, axioms, n_lots, steps):
            "autotune": tune_summary(tune)}
     g.close()
     bvh.close()
+    eng.close()
+    return res
+
+
+FACADE_MESH = """
+axiom Box Lot;
+terminal Wall (0, 0);
+terminal Window (0, 1);
+Lot    = CompSplit() { Discard() | Discard() | Facade };
+Facade = Repeat(Y, 3, ADJUST_TO_FILL) Floor;
+Floor  = Repeat(X, 2.5, ADJUST_TO_FILL) Tile;
+Tile   = IfSizeLess(X, 1.6) { Wall | SubDiv(X) { -1: Wall | 1.2: Window | -1: Wall } };
+"""
+QUAD_MESH_BYTES = 48 + 4 * 36 + 6 * 4   # instance rows read; 4 vertices, 6 indices written
+
+
+def mesh_workload(torch, engine, axioms, n_lots, steps):
+    """The facade grammar with triangle-soup terminals (generation function
+    0, compiled from grammar text) over n_lots lots. One step derives the
+    lots into one instance buffer (walls, then windows) and tessellates it
+    with one pga_tessellate launch into one vertex/index buffer, all on the
+    engine's stream with no host synchronisation between steps."""
+    from operatorgraph_amd import abi
+    from operatorgraph_amd.compiler import compile_grammar
+    from operatorgraph_amd.mesh import tessellate
+    g = compile_grammar(FACADE_MESH, "facade_mesh")
+    lots = axioms.facade_lots(n_lots)
+    lots["entry_index"] = g.axioms[0][0]
+    walls, windows = axioms.facade_lot_terminals(lots)
+    nw, nq = int(walls.sum()), int(windows.sum())
+    n = nw + nq
+    eng = engine.Engine(technique=abi.MEGAKERNEL, queue_capacity=int(nq * 1.05) + (1 << 16))
+    eng.load(g.program)
+    eng.set_axioms(torch.from_numpy(lots.view(np.uint8).copy()).cuda())
+    inst = torch.empty((n, 16), dtype=torch.float32, device="cuda")
+    eng.bind(0, nw, buffer=inst[:nw])
+    eng.bind(1, nq, buffer=inst[nw:])
+    tune = eng.autotune(repeats=2)
+    eng.execute()
+    assert [int(c) for c in eng.counts(2)] == [nw, nq]
+    verts = torch.empty((4 * n, 9), dtype=torch.float32, device="cuda")
+    idx = torch.empty((6 * n,), dtype=torch.int32, device="cuda")
+    ext = torch.cuda.ExternalStream(eng.stream())
+    with torch.cuda.stream(ext):
+        tessellate(inst, abi.QUAD, out=(verts, idx))
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        reps = max(3, steps)
+        e[0].record(ext)
+        for _ in range(reps):
+            tessellate(inst, abi.QUAD, out=(verts, idx))
+        e[1].record(ext)
+        torch.cuda.synchronize()
+        tsec = e[0].elapsed_time(e[1]) / 1e3 / reps
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.execute_async()
+            tessellate(inst, abi.QUAD, out=(verts, idx))
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+    eng.synchronize()
+    assert [int(c) for c in eng.counts(2)] == [nw, nq]
+    assert int(idx[-1]) == 4 * n - 1
+    alg = QUAD_MESH_BYTES * n
+    res = {"lots": n_lots, "quads_per_step": n, "triangles_per_step": 2 * n, "ms_per_step": round(sec * 1e3, 4),
+           "triangles_per_s": round(2 * n / sec),
+           "tessellate": {"kernel": "k_tessellate", "kernel_ms": round(tsec * 1e3, 4),
+                          "algorithmic_bytes_per_launch": alg, "bound": "hbm",
+                          "achieved": round(alg / tsec / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(alg / tsec / 1e9 / HBM_PEAK_GBS, 4)},
+           "kernel": kernel_name(tune)}
     eng.close()
     return res
 
@@ -505,6 +580,10 @@ def main():
         mixed_grad = mixed_gradient_workload(torch, engine, grammars, axioms, args.mixed_grad_lots,
                                              max(3, args.steps // 4))
 
+    mesh = None
+    if args.mesh_lots:
+        mesh = mesh_workload(torch, engine, axioms, args.mesh_lots, max(3, args.steps // 4))
+
     grad = None
     if args.gradient_lots:
         grad = gradient_workload(torch, engine, grammars, axioms, args.gradient_lots, max(5, args.steps // 2))
@@ -535,6 +614,7 @@ def main():
         "fwd_bwd_gradient_256": grad,
         "mixed_menger5_16384_lbvh": mixed,
         "mixed_fwd_bwd_menger5_16384_lbvh": mixed_grad,
+        "triangle_soup_16384": mesh,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

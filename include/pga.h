@@ -418,8 +418,9 @@ PGA_API pga_status pga_copy_to_host(void* host_dst, const void* device_src, uint
  *    1.5 V indices at indices[i * 1.5 V]; an index holds
  *    base_vertex + i * V + corner, so calls that fill one mesh share its
  *    index space. Instances keep their buffer order. All pointers are
- *    device memory; `instances` 16-byte aligned. One kernel on `stream`,
- *    no synchronisation. -- */
+ *    device memory; `instances` 16-byte aligned, `vertices` and `indices`
+ *    4-byte aligned (vertices on a 16-byte boundary are written with
+ *    16-byte stores). One kernel on `stream`, no synchronisation. -- */
 typedef struct pga_vertex {
     float position[4];
     float normal[3];

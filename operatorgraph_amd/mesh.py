@@ -19,9 +19,10 @@ import numpy as np
 from . import abi
 
 
-def tessellate(instances, shape_type, base_vertex=0):
+def tessellate(instances, shape_type, base_vertex=0, out=None):
     """Runs pga_tessellate on the current torch stream; returns
-    (vertices (n * V, 9) float32, indices (n * 1.5 V,) int32) cuda tensors."""
+    (vertices (n * V, 9) float32, indices (n * 1.5 V,) int32) cuda tensors,
+    written into `out` = (vertices, indices) when given."""
     import torch
 
     from .engine import PGAError, load_library
@@ -30,8 +31,14 @@ def tessellate(instances, shape_type, base_vertex=0):
         raise ValueError("instances must be an (n, 16) float32 tensor")
     n = inst.shape[0]
     v = 4 if shape_type == abi.QUAD else 24
-    verts = torch.empty((n * v, 9), dtype=torch.float32, device=inst.device)
-    idx = torch.empty((n * (v + v // 2),), dtype=torch.int32, device=inst.device)
+    if out is None:
+        verts = torch.empty((n * v, 9), dtype=torch.float32, device=inst.device)
+        idx = torch.empty((n * (v + v // 2),), dtype=torch.int32, device=inst.device)
+    else:
+        verts, idx = out
+        if (verts.dtype != torch.float32 or tuple(verts.shape) != (n * v, 9) or not verts.is_contiguous()
+                or idx.dtype != torch.int32 or tuple(idx.shape) != (n * (v + v // 2),) or not idx.is_contiguous()):
+            raise ValueError(f"out must be contiguous ({n * v}, 9) float32 and ({n * (v + v // 2)},) int32 tensors")
     if n == 0:
         return verts, idx
     st = load_library().pga_tessellate(C.c_void_p(inst.data_ptr()), n, int(shape_type), C.c_void_p(verts.data_ptr()),

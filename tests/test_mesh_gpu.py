@@ -170,3 +170,29 @@ def test_tessellate_argument_errors():
     assert v.shape == (0, 9) and i.shape == (0,)
     with pytest.raises(ValueError):
         tessellate(inst[:, :12], abi.BOX)
+
+
+def test_vertex_buffer_off_a_16_byte_boundary():
+    """A vertex buffer 4 bytes past a 16-byte boundary takes the dword copy
+    of the tile; its vertices equal those of the aligned buffer, and a
+    ragged last tile (count not a multiple of the tile) stays in bounds."""
+    import torch
+
+    from operatorgraph_amd.engine import load_library
+    from operatorgraph_amd.mesh import tessellate
+    _note("tessellate into an unaligned vertex buffer")
+    L = load_library()
+    for shape, n in ((abi.BOX, 1000), (abi.QUAD, 1001)):
+        V = 4 if shape == abi.QUAD else 24
+        inst = torch.randn((n, 16), device="cuda")
+        want, want_i = tessellate(inst, shape)
+        flat = torch.full((n * V * 9 + 4 + 64,), 7.0, device="cuda")
+        idx = torch.empty((n * V * 3 // 2,), dtype=torch.int32, device="cuda")
+        s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        assert L.pga_tessellate(C.c_void_p(inst.data_ptr()), n, shape, C.c_void_p(flat.data_ptr() + 4),
+                                C.c_void_p(idx.data_ptr()), 0, s) == abi.OK
+        torch.cuda.synchronize()
+        assert torch.equal(flat[1:1 + n * V * 9].view(-1, 9), want)
+        assert torch.equal(flat[0], torch.tensor(7.0, device="cuda"))
+        assert bool((flat[1 + n * V * 9:] == 7.0).all())
+        assert torch.equal(idx, want_i)
