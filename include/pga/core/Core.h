@@ -1033,6 +1033,39 @@ public:
         if (n) check(pga_copy_to_host(out.data(), buffers_[b], n * sizeof(pga_instance)));
         return out;
     }
+    // generation function 0: the triangle soup of terminal type `terminal`
+    // (shapes of `shapeType`, Box or Quad), copied to the host in the
+    // reference's TriangleMeshData layout: 24 vertices and 36 indices per
+    // box, 4 and 6 per quad, indices counted from baseVertex
+    // (ShapeGenerator<Box|Quad, false>::run, ShapeGenerator.cuh:376-511, 670-708)
+    void getTriangleMesh(unsigned int terminal, int shapeType, std::vector<pga_vertex>& vertices,
+                         std::vector<uint32_t>& indices, uint32_t baseVertex = 0) {
+        vertices.clear();
+        indices.clear();
+        if (terminal >= caps_.size() || !caps_[terminal]) return;
+        const uint64_t n = std::min<uint64_t>(getNumInstances(terminal), caps_[terminal]);
+        if (!n) return;
+        const uint64_t V = shapeType == PGA_SHAPE_QUAD ? 4u : 24u;
+        size_t b = 0;
+        for (unsigned int t = 0; t < terminal; ++t) b += caps_[t] ? 1 : 0;
+        void* dv = nullptr;
+        void* di = nullptr;
+        pga_status st = pga_device_alloc(n * V * sizeof(pga_vertex), &dv);
+        if (st == PGA_OK) st = pga_device_alloc(n * (V + V / 2) * sizeof(uint32_t), &di);
+        // the null stream: execute() has finished, and the copies below wait for it
+        if (st == PGA_OK)
+            st = pga_tessellate(static_cast<const pga_instance*>(buffers_[b]), n, shapeType,
+                                static_cast<pga_vertex*>(dv), static_cast<uint32_t*>(di), baseVertex, nullptr);
+        if (st == PGA_OK) {
+            vertices.resize(n * V);
+            indices.resize(n * (V + V / 2));
+            st = pga_copy_to_host(vertices.data(), dv, vertices.size() * sizeof(pga_vertex));
+            if (st == PGA_OK) st = pga_copy_to_host(indices.data(), di, indices.size() * sizeof(uint32_t));
+        }
+        pga_device_free(dv);
+        pga_device_free(di);
+        check(st);
+    }
     const Lowering::Program& program() const { return program_; }
     pga_context* context() { return ctx_; }
 

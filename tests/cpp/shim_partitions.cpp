@@ -225,6 +225,19 @@ extern "C" __attribute__((visibility("default"))) int shim_main(int argc, char**
                 write_all(dir + "/grammar.t" + std::to_string(t), inst);
                 std::printf("grammar: terminal %u: %zu instances\n", t, inst.size());
             }
+            // generation function 0: one shape type per terminal in grammar.mesh
+            std::ifstream has_mesh(dir + "/grammar.mesh");
+            if (has_mesh.good()) {
+                const auto shapes = read_all<int32_t>(dir + "/grammar.mesh");
+                for (unsigned int t = 0; t < shapes.size(); ++t) {
+                    std::vector<pga_vertex> v;
+                    std::vector<uint32_t> idx;
+                    ev->getTriangleMesh(t, shapes[t], v, idx);
+                    write_all(dir + "/grammar.t" + std::to_string(t) + ".vtx", v);
+                    write_all(dir + "/grammar.t" + std::to_string(t) + ".idx", idx);
+                    std::printf("grammar: terminal %u: %zu vertices, %zu indices\n", t, v.size(), idx.size());
+                }
+            }
             std::printf("grammar: %.6f s\n", seconds);
             destroyPGA();
             return 0;

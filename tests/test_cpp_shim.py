@@ -289,3 +289,33 @@ def test_interpreter_flow_from_grammar_text(tmp_path):
         got = _read_instances(os.path.join(d, f"grammar.t{t}"))
         assert len(got) > 0
         assert_same_multiset(got, r.instances(t), f"interpreter flow terminal {t}")
+
+
+@pytest.mark.gpu
+def test_interpreter_flow_triangle_soup(tmp_path):
+    """The same flow with generation-function-0 terminals: the evaluator's
+    getTriangleMesh gives, per terminal, the oracle's mesh of its shapes
+    (canonical instance order) and the reference's index pattern."""
+    from oracle.oracle import mesh_indices
+    from operatorgraph_amd.compiler import compile_grammar
+    from tests.helpers import canon_order
+    _note("C++ interpreter flow: facade grammar text, triangle soup")
+    d = str(tmp_path)
+    text = (FACADE_TEXT.replace("1.6", "0.2").replace("2.5", "0.25").replace("(Y, 3,", "(Y, 0.2,")
+            .replace("1.2:", "0.1:").replace("(1, 0)", "(0, 0)").replace("(1, 1)", "(0, 1)"))
+    open(os.path.join(d, "grammar.pga"), "w").write(text)
+    open(os.path.join(d, "grammar.caps"), "wb").write(np.array([4096, 4096], dtype=np.uint64).tobytes())
+    open(os.path.join(d, "grammar.mesh"), "wb").write(np.array([abi.QUAD, abi.QUAD], dtype=np.int32).tobytes())
+    run_shim("grammar-run", d)
+    g = compile_grammar(text)
+    r = O.derive(g.program, g.axiom_records([0.0]))
+    for t in range(2):
+        got = _read_instances(os.path.join(d, f"grammar.t{t}"))
+        n = len(got)
+        assert n > 0
+        assert_same_multiset(got, r.instances(t), f"triangle soup terminal {t}")
+        v = np.fromfile(os.path.join(d, f"grammar.t{t}.vtx"), dtype=np.float32).reshape(n, 4, 9)
+        idx = np.fromfile(os.path.join(d, f"grammar.t{t}.idx"), dtype=np.uint32)
+        ref = r.mesh(t).reshape(n, 4, 9)
+        assert np.array_equal(v[canon_order(got)], ref[canon_order(r.instances(t))])
+        assert np.array_equal(idx, mesh_indices(n, abi.QUAD))
