@@ -95,8 +95,25 @@ __device__ __forceinline__ uint32_t claim(LevelLDS& L, bool wr, uint32_t k, unsi
 // GenerationFunction<GenFuncFilter, 1>::perShape (GenerationFunctions.cuh:
 // 155-189): instance = model4 * scale(size), custom attribute. A full buffer
 // is an error (the reference wraps modulo capacity); capacity 0 counts only.
-__device__ __forceinline__ void emit_terminal(const Ctx& c, LevelLDS& L, bool wr, bool has, uint32_t slot,
+// The lanes of one claim get consecutive records, so the write pass stages
+// them in the wave's part of LDS (ST, 64 records) in rank order and writes
+// them out as consecutive 16-byte nontemporal stores: a lane storing its own
+// record would leave every store instruction a 64-byte stride, and partly
+// written lines cost more than the coalesced stream (measured 0.91 -> 0.80 ms
+// on 16,384 facade lots).
+__device__ __forceinline__ void emit_terminal(const Ctx& c, LevelLDS& L, f4* ST, bool wr, bool has, uint32_t slot,
                                               const Rec& r, unsigned long long& idx) {
+    f4 q0 = {}, q1 = {}, q2 = {}, q3 = {};
+    if (wr && has) {
+        const float b[16] = {r.s[0], 0.0f, 0.0f, 0.0f, 0.0f, r.s[1], 0.0f, 0.0f,
+                             0.0f, 0.0f, r.s[2], 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+        float mm[12];
+        mul34x44(r.m, b, mm);
+        q0 = f4{mm[0], mm[1], mm[2], mm[3]};
+        q1 = f4{mm[4], mm[5], mm[6], mm[7]};
+        q2 = f4{mm[8], mm[9], mm[10], mm[11]};
+        q3 = f4{r.custom, __uint_as_float(r.axiom), __uint_as_float(r.path), __uint_as_float(r.ed)};
+    }
     unsigned long long pending = __ballot(has);
     idx = 0;
     while (pending) {                          // one claim per distinct slot in the wave
@@ -105,25 +122,34 @@ __device__ __forceinline__ void emit_terminal(const Ctx& c, LevelLDS& L, bool wr
         const bool mine = has && slot == sl;
         const unsigned long long m = __ballot(mine);
         const uint32_t base = claim(L, wr, sl, m, leader);
-        if (mine) idx = L.term_base[sl] + base + mbcnt(m);
-        pending &= ~m;
-    }
-    if (wr && has) {
-        const uint32_t t = c.prog.slot_term[slot];
-        const unsigned long long cap = c.inst_cap[t];
-        if (idx < cap) {
-            const float b[16] = {r.s[0], 0.0f, 0.0f, 0.0f, 0.0f, r.s[1], 0.0f, 0.0f,
-                                 0.0f, 0.0f, r.s[2], 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
-            float mm[12];
-            mul34x44(r.m, b, mm);
-            f4* d = reinterpret_cast<f4*>(c.inst[t] + idx);
-            d[0] = f4{mm[0], mm[1], mm[2], mm[3]};
-            d[1] = f4{mm[4], mm[5], mm[6], mm[7]};
-            d[2] = f4{mm[8], mm[9], mm[10], mm[11]};
-            d[3] = f4{r.custom, __uint_as_float(r.axiom), __uint_as_float(r.path), __uint_as_float(r.ed)};
-        } else if (cap != 0ull) {
-            raise(c, PGA_ERR_OVERFLOW_INSTANCES);
+        const uint32_t rank = mbcnt(m);
+        const unsigned long long first = L.term_base[sl] + base;
+        if (mine) idx = first + rank;
+        if (wr) {
+            // the group's records are consecutive: stage them in rank order
+            // and write them out as consecutive 16-byte stores
+            if (mine) {
+                ST[rank * 4u + 0u] = q0;
+                ST[rank * 4u + 1u] = q1;
+                ST[rank * 4u + 2u] = q2;
+                ST[rank * 4u + 3u] = q3;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t t = c.prog.slot_term[sl];
+            const unsigned long long cap = c.inst_cap[t];
+            const uint32_t n4 = (uint32_t)__popcll(m) * 4u;
+            f4* dst = reinterpret_cast<f4*>(c.inst[t] + first);
+            for (uint32_t q = lane_id(); q < n4; q += 64u) {
+                if (first + (q >> 2) < cap) __builtin_nontemporal_store(ST[q], dst + q);
+            }
+            if (mine && idx >= cap && cap != 0ull) raise(c, PGA_ERR_OVERFLOW_INSTANCES);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
+        pending &= ~m;
     }
 }
 
@@ -410,7 +436,8 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
             }
             if (__ballot(emit) != 0ull) {
                 unsigned long long idx;
-                emit_terminal(c, L, wr, emit, slot, cur, idx);
+                emit_terminal(c, L, reinterpret_cast<f4*>(PW + 5u * blockDim.x) + wave_in_block() * 256u, wr, emit, slot,
+                              cur, idx);
                 if (TAPE && emit) { out_kind = T_TERM | (slot << 8); out_a = (uint32_t)idx; }
             }
             if (__ballot(start) != 0ull) {
@@ -567,7 +594,7 @@ __device__ __forceinline__ uint32_t first_level_size(const KArgs& a, uint32_t le
 template <int BS>
 __global__ void __launch_bounds__(BS) k_level(KArgs a, uint32_t level) {
     __shared__ LevelLDS L;
-    __shared__ float4 PW[5 * BS];
+    __shared__ float4 PW[9 * BS];   // parent records (5 words per lane), then instance staging (4)
     stage_program(a);
     init_bases(a, L);
     const uint32_t n = first_level_size(a, level);
@@ -581,7 +608,7 @@ __global__ void __launch_bounds__(BS) k_level(KArgs a, uint32_t level) {
 template <int BS, bool TAPE>
 __global__ void __launch_bounds__(BS) k_mega(KArgs a) {
     __shared__ LevelLDS L;
-    __shared__ float4 PW[5 * BS];
+    __shared__ float4 PW[9 * BS];   // parent records (5 words per lane), then instance staging (4)
     stage_program(a);
     init_bases(a, L);
     uint32_t level = 0;
