@@ -332,10 +332,26 @@ __global__ void __launch_bounds__(kThreads) k_scatter(const uint32_t* __restrict
 }
 
 // ---- Karras build ----
-__device__ __forceinline__ int delta(const unsigned long long* keys, int64_t n, int64_t i, int64_t j) {
-    if (j < 0 || j >= n) return -1;
-    return __clzll((long long)(keys[i] ^ keys[j]));
-}
+// The keys of the workgroup's nodes and kKarrasHalo keys on either side are
+// staged in LDS: most nodes span few leaves, so their searches stay in the
+// window; a search that leaves it reads the key from global memory.
+constexpr int kKarrasHalo = 256;
+constexpr int kKarrasWindow = 256 + 2 * kKarrasHalo;
+
+struct KeyWindow {
+    const unsigned long long* keys;
+    const unsigned long long* sk;   // LDS copy of keys[ws, ws + kKarrasWindow)
+    int64_t ws, n;
+    unsigned long long ki;          // this node's key
+    __device__ __forceinline__ unsigned long long at(int64_t j) const {
+        const int64_t o = j - ws;
+        return (o >= 0 && o < kKarrasWindow) ? sk[o] : keys[j];
+    }
+    __device__ __forceinline__ int delta(int64_t j) const {
+        if (j < 0 || j >= n) return -1;
+        return __clzll((long long)(ki ^ at(j)));
+    }
+};
 
 // other_end[i]: the end of node i's leaf range other than i (the range is
 // [min(i, j), max(i, j)])
@@ -344,23 +360,31 @@ __global__ void __launch_bounds__(256) k_karras(const unsigned long long* __rest
                                                 int32_t* __restrict__ right, int32_t* __restrict__ parent,
                                                 int32_t* __restrict__ leaf_parent, int32_t* __restrict__ other_end,
                                                 uint32_t* __restrict__ ticket) {
+    __shared__ unsigned long long sk[kKarrasWindow];
     const int64_t nn = leaf_count(n, nd);
+    const int64_t ws = (int64_t)blockIdx.x * blockDim.x - kKarrasHalo;
+    for (int t = threadIdx.x; t < kKarrasWindow; t += blockDim.x) {
+        const int64_t k = ws + t;
+        sk[t] = (k >= 0 && k < nn) ? keys[k] : 0ull;
+    }
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nn - 1) return;
     ticket[i] = 0u;
     if (i == 0) parent[0] = -1;
-    const int dir = (delta(keys, nn, i, i + 1) - delta(keys, nn, i, i - 1)) >= 0 ? 1 : -1;
-    const int dmin = delta(keys, nn, i, i - dir);
+    KeyWindow w{keys, sk, ws, nn, sk[i - ws]};
+    const int dir = (w.delta(i + 1) - w.delta(i - 1)) >= 0 ? 1 : -1;
+    const int dmin = w.delta(i - dir);
     int64_t lmax = 2;
-    while (delta(keys, nn, i, i + lmax * dir) > dmin) lmax *= 2;
+    while (w.delta(i + lmax * dir) > dmin) lmax *= 2;
     int64_t l = 0;
     for (int64_t t = lmax / 2; t >= 1; t /= 2)
-        if (delta(keys, nn, i, i + (l + t) * dir) > dmin) l += t;
+        if (w.delta(i + (l + t) * dir) > dmin) l += t;
     const int64_t j = i + l * dir;
-    const int dnode = delta(keys, nn, i, j);
+    const int dnode = w.delta(j);
     int64_t s = 0;
     for (int64_t t = (l + 1) / 2;; t = (t + 1) / 2) {
-        if (delta(keys, nn, i, i + (s + t) * dir) > dnode) s += t;
+        if (w.delta(i + (s + t) * dir) > dnode) s += t;
         if (t == 1) break;
     }
     const int64_t gamma = i + s * dir + (dir < 0 ? -1 : 0);
