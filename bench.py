@@ -491,11 +491,14 @@ def tune_summary(tune):
             "untuned_ms": round(tune["initial_seconds"] * 1e3, 4)}
 
 
-def load_traffic(workload):
+def load_traffic(workload, kernel=None):
+    """HBM bytes per launch from the committed counter summary: the entry
+    of `kernel` when it was measured, else the summary's main kernel."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(path):
         with open(path) as f:
-            return json.load(f).get("traffic_bytes_per_launch")
+            d = json.load(f)
+        return d.get("per_kernel", {}).get(kernel, d.get("traffic_bytes_per_launch"))
     return None
 
 
@@ -535,7 +538,7 @@ def main():
     alg_bytes = INSTANCE_BYTES * T + AXIOM_BYTES * A
     achieved = alg_bytes / ksec / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(f"facade_lots_{args.lots}"),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(f"facade_lots_{args.lots}", kernel_name(tune)),
                 "kernel": kernel_name(tune), "kernel_ms": round(ksec * 1e3, 4),
                 "algorithmic_bytes_per_launch": alg_bytes}
 
