@@ -428,6 +428,15 @@ typedef struct pga_vertex {
 } pga_vertex; /* 36 bytes */
 PGA_API pga_status pga_tessellate(const pga_instance* instances, uint64_t count, int32_t shape_type,
                                   pga_vertex* vertices, uint32_t* indices, uint32_t base_vertex, void* stream);
+/* The same with the count read on the device: min(*device_count, capacity)
+ *    shapes, e.g. a terminal's counter from pga_device_counts, so that a
+ *    derivation and its tessellation run back to back on one stream with no
+ *    host synchronisation and can be recorded in one graph (the reference's
+ *    generation function 0 writes its mesh during the derivation). Buffers
+ *    are sized for `capacity`; `device_count` 8-byte aligned. */
+PGA_API pga_status pga_tessellate_device_count(const pga_instance* instances, const uint64_t* device_count,
+                                               uint64_t capacity, int32_t shape_type, pga_vertex* vertices,
+                                               uint32_t* indices, uint32_t base_vertex, void* stream);
 
 /* -- grammar front end: the PGA grammar language (*.pga) compiled on the
  *    host into a dispatch table over the interpreter's procedure list.

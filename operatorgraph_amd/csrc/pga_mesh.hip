@@ -81,14 +81,19 @@ __device__ __forceinline__ void build_vertex(const float4& r0, const float4& r1,
     o[8] = c < 2u ? 1.0f : 0.0f;
 }
 
-// vec4: vertices is 16-byte aligned (the tile copy then uses 16-byte stores)
+// vec4: vertices is 16-byte aligned (the tile copy then uses 16-byte stores).
+// dcount: when not null, the shape count is min(*dcount, count), read on the
+// device (the grid covers `count` shapes; tiles past the count return).
 __global__ void __launch_bounds__(kBlock) k_tessellate(const float4* __restrict__ inst, uint64_t count, uint32_t quad,
                                                        float* __restrict__ vert, uint32_t* __restrict__ idx,
-                                                       uint32_t base_vertex, uint32_t vec4) {
+                                                       uint32_t base_vertex, uint32_t vec4,
+                                                       const unsigned long long* __restrict__ dcount) {
     __shared__ __align__(16) float tile[kTileFloats];
+    if (dcount) count = min((uint64_t)*dcount, count);
     const uint32_t V = quad ? 4u : 24u;
     const uint32_t tile_shapes = kTileVerts / V;
     const uint64_t s0 = (uint64_t)blockIdx.x * tile_shapes;
+    if (s0 >= count) return;
     const uint32_t ns = (uint32_t)min((uint64_t)tile_shapes, count - s0);
     const uint32_t nv = ns * V;
     for (uint32_t v = threadIdx.x; v < nv; v += kBlock) {
@@ -119,15 +124,15 @@ __global__ void __launch_bounds__(kBlock) k_tessellate(const float4* __restrict_
 
 } // namespace pgamesh
 
-extern "C" {
+namespace {
 
-// Declared in include/pga.h.
-PGA_API pga_status pga_tessellate(const pga_instance* instances, uint64_t count, int32_t shape_type,
-                                  pga_vertex* vertices, uint32_t* indices, uint32_t base_vertex, void* stream) {
+pga_status launch(const pga_instance* instances, uint64_t count, const uint64_t* device_count, int32_t shape_type,
+                  pga_vertex* vertices, uint32_t* indices, uint32_t base_vertex, void* stream) {
     if (count == 0u) return PGA_OK;
     if (!instances || !vertices || !indices) return PGA_ERR_INVALID_ARG;
     if (shape_type != PGA_SHAPE_BOX && shape_type != PGA_SHAPE_QUAD) return PGA_ERR_UNSUPPORTED;
-    if (((uintptr_t)instances & 15u) != 0u || ((uintptr_t)vertices & 3u) != 0u || ((uintptr_t)indices & 3u) != 0u)
+    if (((uintptr_t)instances & 15u) != 0u || ((uintptr_t)vertices & 3u) != 0u || ((uintptr_t)indices & 3u) != 0u ||
+        ((uintptr_t)device_count & 7u) != 0u)
         return PGA_ERR_INVALID_ARG;
     const uint64_t V = shape_type == PGA_SHAPE_QUAD ? 4u : 24u;
     if (count > (0xFFFFFFFFull - base_vertex) / V) return PGA_ERR_INVALID_ARG;   // 32-bit vertex indices
@@ -137,8 +142,27 @@ PGA_API pga_status pga_tessellate(const pga_instance* instances, uint64_t count,
     hipLaunchKernelGGL(pgamesh::k_tessellate, dim3((uint32_t)blocks), dim3(pgamesh::kBlock), 0, (hipStream_t)stream,
                        reinterpret_cast<const float4*>(instances), count, shape_type == PGA_SHAPE_QUAD ? 1u : 0u,
                        reinterpret_cast<float*>(vertices), indices, base_vertex,
-                       ((uintptr_t)vertices & 15u) == 0u ? 1u : 0u);
+                       ((uintptr_t)vertices & 15u) == 0u ? 1u : 0u,
+                       reinterpret_cast<const unsigned long long*>(device_count));
     return hipGetLastError() == hipSuccess ? PGA_OK : PGA_ERR_HIP;
+}
+
+} // namespace
+
+extern "C" {
+
+// Declared in include/pga.h.
+PGA_API pga_status pga_tessellate(const pga_instance* instances, uint64_t count, int32_t shape_type,
+                                  pga_vertex* vertices, uint32_t* indices, uint32_t base_vertex, void* stream) {
+    return launch(instances, count, nullptr, shape_type, vertices, indices, base_vertex, stream);
+}
+
+// Declared in include/pga.h.
+PGA_API pga_status pga_tessellate_device_count(const pga_instance* instances, const uint64_t* device_count,
+                                               uint64_t capacity, int32_t shape_type, pga_vertex* vertices,
+                                               uint32_t* indices, uint32_t base_vertex, void* stream) {
+    if (!device_count && capacity) return PGA_ERR_INVALID_ARG;
+    return launch(instances, capacity, device_count, shape_type, vertices, indices, base_vertex, stream);
 }
 
 } // extern "C"

@@ -85,6 +85,7 @@ def load_library():
         "pga_center_size_mse": (C.c_int, [P, P, P, C.c_uint64, C.c_float, C.c_float, P, P, P]),
         "pga_generate_partition": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(P)]),
         "pga_tessellate": (C.c_int, [P, C.c_uint64, C.c_int32, P, P, C.c_uint32, P]),
+        "pga_tessellate_device_count": (C.c_int, [P, P, C.c_uint64, C.c_int32, P, P, C.c_uint32, P]),
         "pga_grammar_code": (C.c_int, [P, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64)]),
         "pga_grammar_edges": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.POINTER(C.c_uint8))]),
     }

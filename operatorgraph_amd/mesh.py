@@ -49,6 +49,38 @@ def tessellate(instances, shape_type, base_vertex=0, out=None):
     return verts, idx
 
 
+def tessellate_terminal(engine, terminal, shape_type, base_vertex=0, out=None):
+    """Tessellates a terminal type's instance buffer with the count read on
+    the device (pga_tessellate_device_count on the engine's counter), on the
+    engine's stream: no host synchronisation after the derivation, and the
+    call can be recorded with the derivation in one `engine.capture()`.
+    The buffers are sized for the bound capacity; after synchronising, the
+    first engine.counts()[terminal] shapes of them hold the mesh."""
+    import torch
+
+    from .engine import PGAError
+    buf = engine.buffer(terminal)
+    if buf is None:
+        raise ValueError(f"terminal {terminal} has no bound instance buffer")
+    cap = buf.shape[0]
+    v = 4 if shape_type == abi.QUAD else 24
+    if out is None:
+        verts = torch.empty((cap * v, 9), dtype=torch.float32, device=buf.device)
+        idx = torch.empty((cap * (v + v // 2),), dtype=torch.int32, device=buf.device)
+    else:
+        verts, idx = out
+        if (verts.dtype != torch.float32 or verts.shape[0] < cap * v or not verts.is_contiguous()
+                or idx.dtype != torch.int32 or idx.shape[0] < cap * (v + v // 2) or not idx.is_contiguous()):
+            raise ValueError("out buffers are too small for the bound capacity")
+    counts = engine.device_counts() + 8 * terminal
+    st = engine.L.pga_tessellate_device_count(C.c_void_p(buf.data_ptr()), C.c_void_p(counts), cap, int(shape_type),
+                                              C.c_void_p(verts.data_ptr()), C.c_void_p(idx.data_ptr()),
+                                              int(base_vertex), C.c_void_p(engine.stream()))
+    if st != abi.OK:
+        raise PGAError(st, "pga_tessellate_device_count")
+    return verts, idx
+
+
 def write_obj(path, vertices, indices, name="pga"):
     """Writes an OBJ file: v (xyz), vn, vt per vertex, one `f a/a/a b/b/b
     c/c/c` line per triangle (1-based). Accepts numpy arrays or tensors."""

@@ -196,3 +196,80 @@ def test_vertex_buffer_off_a_16_byte_boundary():
         assert torch.equal(flat[0], torch.tensor(7.0, device="cuda"))
         assert bool((flat[1 + n * V * 9:] == 7.0).all())
         assert torch.equal(idx, want_i)
+
+
+def test_graph_of_derivation_and_device_count_tessellation():
+    """One graph = derivation + tessellation of both terminal types with the
+    counts read on the device. The lots change in place between replays,
+    which changes the counts; every replay's mesh equals the oracle's, and
+    the buffers past the count stay untouched. Eager derivations run
+    between the replays."""
+    import torch
+
+    from oracle import oracle as O
+    from oracle.oracle import mesh_indices
+    from operatorgraph_amd import engine
+    from operatorgraph_amd.mesh import tessellate_terminal
+    _note("graph: derivation + device-count tessellation, lots changed between replays")
+    g = compile_grammar(FACADE0, "facade")
+    lots = [facade_lots(g, 24, s) for s in (11, 12)]
+    refs = [O.derive(g.program, ax) for ax in lots]
+    counts = [(r.count(0), r.count(1)) for r in refs]
+    assert counts[0] != counts[1]
+    cap = [max(c[t] for c in counts) + 16 for t in range(2)]
+    dev = torch.from_numpy(lots[0].view(np.uint8).copy()).cuda()
+    eng = engine.Engine(technique=abi.MEGAKERNEL)
+    eng.load(g.program).set_axioms(dev)
+    for t in range(2):
+        eng.bind(t, cap[t])
+    outs = [(torch.empty((cap[t] * 4, 9), device="cuda"), torch.empty((cap[t] * 6,), dtype=torch.int32, device="cuda"))
+            for t in range(2)]
+    with eng.capture() as gr:
+        eng.execute_async()
+        for t in range(2):
+            tessellate_terminal(eng, t, abi.QUAD, out=outs[t])
+    for rep, k in enumerate((0, 1, 0)):
+        dev.copy_(torch.from_numpy(lots[k].view(np.uint8).copy()).cuda())
+        for t in range(2):
+            outs[t][0].fill_(7.0)
+        torch.cuda.synchronize()
+        gr.launch()
+        eng.synchronize()
+        assert tuple(int(x) for x in eng.counts(2)) == counts[k], rep
+        for t in range(2):
+            n = counts[k][t]
+            got, ref = eng.instances(t), refs[k].instances(t)
+            assert_same_multiset(got, ref, f"replay {rep} terminal {t}")
+            v = outs[t][0][: 4 * n].cpu().numpy().reshape(n, 4, 9)
+            assert np.array_equal(v[canon_order(got)], refs[k].mesh(t).reshape(n, 4, 9)[canon_order(ref)]), rep
+            assert np.array_equal(outs[t][1][: 6 * n].cpu().numpy().view(np.uint32), mesh_indices(n, abi.QUAD)), rep
+            assert bool((outs[t][0][4 * n:] == 7.0).all()), rep
+        eng.execute(n=5)                         # eager derivation between replays
+    gr.close()
+    eng.close()
+
+
+def test_device_count_is_clamped_to_the_capacity():
+    """A device count above the capacity tessellates `capacity` shapes and
+    writes nothing past the buffers; a count of zero writes nothing."""
+    import torch
+
+    from operatorgraph_amd.engine import load_library
+    _note("device-count tessellation: clamping")
+    L = load_library()
+    cap = 100
+    inst = torch.randn((cap, 16), device="cuda")
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    from operatorgraph_amd.mesh import tessellate
+    want, want_i = tessellate(inst, abi.BOX)
+    for dc, n in ((cap + 100, cap), (0, 0), (37, 37)):
+        cnt = torch.tensor([dc], dtype=torch.int64, device="cuda")
+        verts = torch.full((cap * 24 + 64, 9), 7.0, device="cuda")
+        idx = torch.full((cap * 36 + 64,), -1, dtype=torch.int32, device="cuda")
+        assert L.pga_tessellate_device_count(C.c_void_p(inst.data_ptr()), C.c_void_p(cnt.data_ptr()), cap, abi.BOX,
+                                             C.c_void_p(verts.data_ptr()), C.c_void_p(idx.data_ptr()), 0, s) == abi.OK
+        torch.cuda.synchronize()
+        assert torch.equal(verts[: 24 * n], want[: 24 * n]) and bool((verts[24 * n:] == 7.0).all()), dc
+        assert torch.equal(idx[: 36 * n], want_i[: 36 * n]) and bool((idx[36 * n:] == -1).all()), dc
+    assert L.pga_tessellate_device_count(C.c_void_p(inst.data_ptr()), None, cap, abi.BOX, C.c_void_p(verts.data_ptr()),
+                                         C.c_void_p(idx.data_ptr()), 0, s) == abi.ERR_INVALID_ARG
