@@ -343,9 +343,31 @@ def mesh_workload(torch, engine, axioms, n_lots, steps):
     eng.synchronize()
     assert [int(c) for c in eng.counts(2)] == [nw, nq]
     assert int(idx[-1]) == 4 * n - 1
+    # the same step with the counts read on the device, recorded once as a
+    # HIP graph: derivation, then one tessellation launch per terminal type
+    from operatorgraph_amd.mesh import tessellate_terminal
+    outs = [(verts[: 4 * nw], idx[: 6 * nw]), (verts[4 * nw:], idx[6 * nw:])]
+    with eng.capture() as gr:
+        eng.execute_async()
+        tessellate_terminal(eng, 0, abi.QUAD, out=outs[0])
+        tessellate_terminal(eng, 1, abi.QUAD, base_vertex=4 * nw, out=outs[1])
+    gr.launch()
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gr.launch()
+    torch.cuda.synchronize()
+    gsec = (time.perf_counter() - t0) / steps
+    eng.synchronize()
+    gr.close()
+    assert [int(c) for c in eng.counts(2)] == [nw, nq]
+    assert int(idx[-1]) == 4 * n - 1
     alg = QUAD_MESH_BYTES * n
     res = {"lots": n_lots, "quads_per_step": n, "triangles_per_step": 2 * n, "ms_per_step": round(sec * 1e3, 4),
            "triangles_per_s": round(2 * n / sec),
+           "graph": {"ms_per_step": round(gsec * 1e3, 4), "triangles_per_s": round(2 * n / gsec),
+                     "note": "derivation + device-count tessellation per terminal type, one HIP graph"},
            "tessellate": {"kernel": "k_tessellate", "kernel_ms": round(tsec * 1e3, 4),
                           "algorithmic_bytes_per_launch": alg, "bound": "hbm",
                           "achieved": round(alg / tsec / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
