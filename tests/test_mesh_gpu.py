@@ -273,3 +273,69 @@ def test_device_count_is_clamped_to_the_capacity():
         assert torch.equal(idx[: 36 * n], want_i[: 36 * n]) and bool((idx[36 * n:] == -1).all()), dc
     assert L.pga_tessellate_device_count(C.c_void_p(inst.data_ptr()), None, cap, abi.BOX, C.c_void_p(verts.data_ptr()),
                                          C.c_void_p(idx.data_ptr()), 0, s) == abi.ERR_INVALID_ARG
+
+
+def _instance_shapes(program, inst):
+    """Shape type of each instance record: its entry (meta & 0xFFFF) runs a
+    Generate procedure of that shape."""
+    procs = program.procedures.to_array()
+    entries = program.table.entries
+    return np.array([procs[entries[e].operator_code].shape_type for e in (inst["meta"] & 0xFFFF)], dtype=np.int32)
+
+
+@pytest.mark.parametrize("seed", list(range(32)))
+def test_fuzz_grammar_triangle_soup(seed):
+    """Seeded random grammars (tests/grammar_fuzz.py) with every terminal on
+    generation function 0: the GPU derivation's instances, tessellated per
+    shape type (a terminal index may collect boxes and quads), give the
+    oracle's mesh shape for shape, value for value."""
+    import re
+
+    import torch
+
+    from oracle import oracle as O
+    from oracle.oracle import mesh_indices
+    from operatorgraph_amd import engine
+    from operatorgraph_amd.mesh import tessellate
+    from tests.grammar_fuzz import lots, random_grammar
+    text, shape = random_grammar(seed)
+    text = re.sub(r"terminal (\w+) (\w+) \(1, ", r"terminal \1 \2 (0, ", text)
+    g = compile_grammar(text, f"fuzz{seed}_soup")
+    ax = lots(g, 4, seed, quad=shape == "Quad")
+    technique = abi.KERNELS if seed % 4 == 3 else abi.MEGAKERNEL
+    _note(f"triangle soup of fuzz grammar seed={seed} shape={shape} technique={technique}")
+    try:
+        r = O.derive(g.program, ax)
+    except O.OracleError:
+        return                                   # the error status is checked by test_fuzz_gpu.py
+    eng = engine.Engine(technique=technique)
+    try:
+        eng.load(g.program).set_axioms(ax)
+        for t in range(3):
+            eng.bind(t, r.count(t) + 64)
+        eng.execute()
+        for t in range(3):
+            got, ref = eng.instances(t), r.instances(t)
+            assert_same_multiset(got, ref, f"seed {seed} terminal {t}")
+            if not len(ref):
+                continue
+            sg, sr = _instance_shapes(g.program, got), _instance_shapes(g.program, ref)
+            off = np.concatenate([[0], np.cumsum(np.where(sr == abi.QUAD, 4, 24))])
+            mesh = r.mesh(t)
+            assert mesh.shape[0] == off[-1]
+            blocks = [None] * len(got)
+            for st in (abi.BOX, abi.QUAD):
+                sel = np.nonzero(sg == st)[0]
+                if not len(sel):
+                    continue
+                V = 4 if st == abi.QUAD else 24
+                tens = torch.from_numpy(np.ascontiguousarray(got[sel]).view(np.float32).reshape(-1, 16).copy()).cuda()
+                v, i = tessellate(tens, st)
+                vv = v.cpu().numpy()
+                for k, gi in enumerate(sel):
+                    blocks[gi] = vv[k * V:(k + 1) * V]
+                assert np.array_equal(i.cpu().numpy().view(np.uint32), mesh_indices(len(sel), st))
+            for a, b in zip(canon_order(got), canon_order(ref)):
+                assert np.array_equal(blocks[a], mesh[off[b]:off[b + 1]]), (seed, t, a, b)
+    finally:
+        eng.close()
