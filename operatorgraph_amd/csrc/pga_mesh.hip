@@ -21,8 +21,8 @@
 // thread would scatter a wave's addresses over 2.3 KB) and writes the
 // tile's indices as consecutive dwords. Every output store is nontemporal:
 // the mesh is written once and read by a later consumer, and streaming it
-// past the caches measured 3.5 -> 6.0 TB/s on 2^20 boxes. Shapes keep the order of the
-// instance buffer; the reference's atomics leave it unspecified.
+// past the caches measured 3.5 -> 6.0 TB/s on 2^20 boxes. Shapes keep the
+// order of the instance buffer; the reference's atomics leave it unspecified.
 #include <hip/hip_runtime.h>
 
 #include "../../include/pga.h"
@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(kBlock) k_tessellate(const float4* __restrict_
     }
     __syncthreads();
     const uint64_t g0 = (s0 * V) * 9u;                 // first float of the tile in vertices
-    const uint32_t nf = nv * 9u;                       // a multiple of 4: V is
+    const uint32_t nf = nv * 9u;                       // a multiple of 4, as V is
     if (vec4) {
         f4* out = reinterpret_cast<f4*>(vert + g0);
         const f4* src = reinterpret_cast<const f4*>(tile);
