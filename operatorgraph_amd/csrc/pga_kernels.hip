@@ -50,6 +50,7 @@ struct LevelLDS {
 enum Mode : uint32_t { M_DONE = 0, M_SETTLE = 1, M_FAN = 2 };
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) f4 gf4;   // a global-memory f4
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint32_t wave_in_block() { return threadIdx.x >> 6; }
@@ -100,7 +101,9 @@ __device__ __forceinline__ uint32_t claim(LevelLDS& L, bool wr, uint32_t k, unsi
 // them out as consecutive 16-byte nontemporal stores: a lane storing its own
 // record would leave every store instruction a 64-byte stride, and partly
 // written lines cost more than the coalesced stream (measured 0.91 -> 0.80 ms
-// on 16,384 facade lots).
+// on 16,384 facade lots). The copy reads its (at most four) words per lane
+// from LDS before any store, and stores through a global-memory pointer: a
+// flat store would also count against the LDS wait of the next read.
 __device__ __forceinline__ void emit_terminal(const Ctx& c, LevelLDS& L, f4* ST, bool wr, bool has, uint32_t slot,
                                               const Rec& r, unsigned long long& idx) {
     f4 q0 = {}, q1 = {}, q2 = {}, q3 = {};
@@ -140,9 +143,17 @@ __device__ __forceinline__ void emit_terminal(const Ctx& c, LevelLDS& L, f4* ST,
             const uint32_t t = c.prog.slot_term[sl];
             const unsigned long long cap = c.inst_cap[t];
             const uint32_t n4 = (uint32_t)__popcll(m) * 4u;
-            f4* dst = reinterpret_cast<f4*>(c.inst[t] + first);
-            for (uint32_t q = lane_id(); q < n4; q += 64u) {
-                if (first + (q >> 2) < cap) __builtin_nontemporal_store(ST[q], dst + q);
+            gf4* dst = (gf4*)(c.inst[t] + first);
+            f4 v[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u) {
+                const uint32_t q = lane_id() + 64u * u;
+                v[u] = q < n4 ? ST[q] : f4{};
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u) {
+                const uint32_t q = lane_id() + 64u * u;
+                if (q < n4 && first + (q >> 2) < cap) __builtin_nontemporal_store(v[u], dst + q);
             }
             if (mine && idx >= cap && cap != 0ull) raise(c, PGA_ERR_OVERFLOW_INSTANCES);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
