@@ -128,11 +128,15 @@ __device__ __forceinline__ void mul_adj_a(const float b[16], const float dc[12],
 
 // Instance = model4 * scale(size) (GenerationFunctions.cuh:155-189).
 __device__ __forceinline__ void emit_adj(const Rec& r, const float* dl, Adj& d) {
-    float di[12];   // rows 0..2 of dL/dinstance, three 16-byte loads
-    const float4* d4 = reinterpret_cast<const float4*>(dl);
+    float di[12];   // rows 0..2 of dL/dinstance, three 16-byte global loads
+    // (the pointer comes from a table, so without the address space the
+    // loads would be flat loads, which also hold up the next LDS wait)
+    typedef float vf4 __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(1))) const vf4 gcf4;
+    const gcf4* d4 = (const gcf4*)dl;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        const float4 v = d4[i];
+        const vf4 v = d4[i];
         di[i * 4] = v.x; di[i * 4 + 1] = v.y; di[i * 4 + 2] = v.z; di[i * 4 + 3] = v.w;
     }
 #pragma unroll

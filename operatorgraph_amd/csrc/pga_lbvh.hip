@@ -338,21 +338,6 @@ __global__ void __launch_bounds__(kThreads) k_scatter(const uint32_t* __restrict
 constexpr int kKarrasHalo = 256;
 constexpr int kKarrasWindow = 256 + 2 * kKarrasHalo;
 
-struct KeyWindow {
-    const unsigned long long* keys;
-    const unsigned long long* sk;   // LDS copy of keys[ws, ws + kKarrasWindow)
-    int64_t ws, n;
-    unsigned long long ki;          // this node's key
-    __device__ __forceinline__ unsigned long long at(int64_t j) const {
-        const int64_t o = j - ws;
-        return (o >= 0 && o < kKarrasWindow) ? sk[o] : keys[j];
-    }
-    __device__ __forceinline__ int delta(int64_t j) const {
-        if (j < 0 || j >= n) return -1;
-        return __clzll((long long)(ki ^ at(j)));
-    }
-};
-
 // other_end[i]: the end of node i's leaf range other than i (the range is
 // [min(i, j), max(i, j)])
 __global__ void __launch_bounds__(256) k_karras(const unsigned long long* __restrict__ keys, uint32_t n,
@@ -372,19 +357,34 @@ __global__ void __launch_bounds__(256) k_karras(const unsigned long long* __rest
     if (i >= nn - 1) return;
     ticket[i] = 0u;
     if (i == 0) parent[0] = -1;
-    KeyWindow w{keys, sk, ws, nn, sk[i - ws]};
-    const int dir = (w.delta(i + 1) - w.delta(i - 1)) >= 0 ? 1 : -1;
-    const int dmin = w.delta(i - dir);
+    // LDS and global pointers of their own address spaces: the two loads
+    // cannot be merged into one load through a selected generic pointer,
+    // which would be a flat load
+    typedef __attribute__((address_space(3))) const unsigned long long lds_u64;
+    typedef __attribute__((address_space(1))) const unsigned long long glb_u64;
+    const lds_u64* wk = (const lds_u64*)sk;
+    const glb_u64* gk = (const glb_u64*)keys;
+    const unsigned long long ki = wk[i - ws];
+    auto delta = [&](int64_t jj) -> int {
+        if (jj < 0 || jj >= nn) return -1;
+        const int64_t o = jj - ws;
+        unsigned long long kj;
+        if (o >= 0 && o < kKarrasWindow) kj = wk[o];
+        else kj = gk[jj];
+        return __clzll((long long)(ki ^ kj));
+    };
+    const int dir = (delta(i + 1) - delta(i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(i - dir);
     int64_t lmax = 2;
-    while (w.delta(i + lmax * dir) > dmin) lmax *= 2;
+    while (delta(i + lmax * dir) > dmin) lmax *= 2;
     int64_t l = 0;
     for (int64_t t = lmax / 2; t >= 1; t /= 2)
-        if (w.delta(i + (l + t) * dir) > dmin) l += t;
+        if (delta(i + (l + t) * dir) > dmin) l += t;
     const int64_t j = i + l * dir;
-    const int dnode = w.delta(j);
+    const int dnode = delta(j);
     int64_t s = 0;
     for (int64_t t = (l + 1) / 2;; t = (t + 1) / 2) {
-        if (w.delta(i + (s + t) * dir) > dnode) s += t;
+        if (delta(i + (s + t) * dir) > dnode) s += t;
         if (t == 1) break;
     }
     const int64_t gamma = i + s * dir + (dir < 0 ? -1 : 0);
