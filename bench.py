@@ -528,7 +528,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the derivation shards by axiom with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(the derivation shards by axiom with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -464,7 +464,7 @@ PGA_API pga_status pga_grammar_axioms(const pga_grammar* grammar, const pga_gram
                                       uint32_t* count);
 PGA_API void pga_grammar_destroy(pga_grammar* grammar);
 
-/* -- partition code (SURVEY's operator-graph codegen): the grammar's
+/* -- partition code (the operator-graph codegen): the grammar's
  *    operator graph, as Context builds it from the first axiom
  *    (src/compiler/Context.cpp:11-253; this build numbers the edges in
  *    depth-first pre-order), cut by `partition` (one '0' keep / '1' cut per
