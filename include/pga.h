@@ -466,7 +466,7 @@ PGA_API void pga_grammar_destroy(pga_grammar* grammar);
 
 /* -- partition code (the operator-graph codegen): the grammar's
  *    operator graph, as Context builds it from the first axiom
- *    (src/compiler/Context.cpp:11-253; this build numbers the edges in
+ *    (src/compiler/Context.cpp:11-253; the library numbers the edges in
  *    depth-first pre-order), cut by `partition` (one '0' keep / '1' cut per
  *    edge; NULL cuts every edge, as the interpreter does,
  *    src/interpreter/InterpreterApp.cpp:214-223), written as C++ in the

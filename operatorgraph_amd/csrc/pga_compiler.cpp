@@ -800,7 +800,7 @@ private:
 
 // ---- operator graph and partition code ----
 // The graph of the operators reachable from the first axiom
-// (Context::Context, src/compiler/Context.cpp:11-253, in this build's own
+// (Context::Context, src/compiler/Context.cpp:11-253, in the library's own
 // numbering). It has one vertex per rule body and per inline operator
 // occurrence, one Generate vertex per terminal symbol (or per inline
 // Generate()), and one per Discard(). Its edges are the successor links,
