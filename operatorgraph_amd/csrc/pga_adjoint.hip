@@ -444,6 +444,7 @@ __device__ __forceinline__ Ctx adj_ctx(const KArgs& a) {
     c.geom = 1u;
     c.seed = a.use_seed;
     c.tape = 1u;
+    c.prog_lds = 0u;
     return c;
 }
 

@@ -115,6 +115,7 @@ struct Ctx {
     uint32_t geom;                      // 0: skip model-matrix arithmetic (counting only)
     uint32_t seed;                      // 0: no parameter reads a seed, the seed chain is skipped
     uint32_t tape;                      // 1: recording a tape (no in-place leaf expansion)
+    uint32_t prog_lds;                  // 1: the program tables are the LDS copy (derivation kernels), 0: global
 };
 
 __device__ __forceinline__ void raise(const Ctx& c, int status) {
@@ -257,8 +258,11 @@ __device__ __forceinline__ float exp_operate(uint32_t op, float p1, float p2) {
 // EncExp::eval (Parameters.cuh:563-664). Expressions are validated on load
 // (length <= 15, stack depth <= 4, indices in range), so the four stack
 // slots live in registers. Out of line: the shape attributes come in as
-// scalars so the record never has to be addressable.
-__device__ __noinline__ float exp_eval(const uint32_t* __restrict__ b, float m0, float m1, float m2, float m3,
+// scalars so the record never has to be addressable. One instance per
+// address space of the code (WordT: LDS words in the derivation kernels,
+// global words in the adjoint), so that its reads are not flat loads.
+template <typename WordT>
+__device__ __noinline__ float exp_eval(const WordT* __restrict__ b, float m0, float m1, float m2, float m3,
                                        float m4, float m5, float m6, float m7, float m8, float m9, float m10,
                                        float m11, float s0x, float s1x, float s2x, float seed, float custom) {
     float st0 = 0.0f, st1 = 0.0f, st2 = 0.0f, st3 = 0.0f;
@@ -313,8 +317,15 @@ __device__ __forceinline__ float eval_param(const Ctx& c, const DevParam& p, con
         float uf = philox_uniform(axiom, path, depth, (uint32_t)p.c, c.key0, c.key1);
         return (p.b - p.a) * uf + p.a;
     }
-    return exp_eval(c.prog.exp + fbits(p.a), r.m[0], r.m[1], r.m[2], r.m[3], r.m[4], r.m[5], r.m[6], r.m[7],
-                    r.m[8], r.m[9], r.m[10], r.m[11], r.s[0], r.s[1], r.s[2], r.seed, r.custom);
+    typedef __attribute__((address_space(3))) const uint32_t lds_u32;
+    if (c.prog_lds)
+        return exp_eval<lds_u32>((const lds_u32*)(c.prog.exp + fbits(p.a)), r.m[0], r.m[1], r.m[2], r.m[3], r.m[4],
+                                 r.m[5], r.m[6], r.m[7], r.m[8], r.m[9], r.m[10], r.m[11], r.s[0], r.s[1], r.s[2],
+                                 r.seed, r.custom);
+    typedef __attribute__((address_space(1))) const uint32_t glb_u32;   // the adjoint reads the global tables
+    return exp_eval<glb_u32>((const glb_u32*)(c.prog.exp + fbits(p.a)), r.m[0], r.m[1], r.m[2], r.m[3], r.m[4],
+                             r.m[5], r.m[6], r.m[7], r.m[8], r.m[9], r.m[10], r.m[11], r.s[0], r.s[1], r.s[2],
+                             r.seed, r.custom);
 }
 
 __device__ __forceinline__ bool shape_dependent(uint32_t type) {

@@ -253,6 +253,7 @@ __device__ __forceinline__ Ctx make_ctx(const KArgs& a, const LevelLDS& L, uint3
     c.par_cap = a.cap_par;
     c.desc_cap = a.cap_desc;
     c.tape = TAPE ? 1u : 0u;
+    c.prog_lds = 1u;
     if (TAPE) {        // this level's outputs follow the earlier levels' in the tape
         c.par_out = a.tape_par + L.tp_cur;
         c.desc_out = a.tape_desc + L.td_cur;
