@@ -48,6 +48,7 @@ METRIC = "terminals/sec batched vs plain at 1 GPU; fwd+bwd grads/sec; % HBM peak
 HBM_PEAK_GBS = 8000.0
 AXIOM_BYTES = 80
 INSTANCE_BYTES = 64
+CPU_THREADS_MAX = 16     # the host share of one GPU
 
 
 def parse():
@@ -245,7 +246,7 @@ def mixed_workload(torch, engine, grammars, axioms, n_lots, steps):
     got = [int(c) for c in eng.counts(3)]
     assert got == caps, (got, caps)
     dsec, _ = kernel_seconds(torch, eng, 3)
-    bvh = LBVH(T, stream=eng.stream())
+    bvh = LBVH(T, engine=eng)
     out = bvh.alloc()
     bvh.gather_engine(eng, 3, 0b011, out)
     bvh.build_device(out)
@@ -409,7 +410,7 @@ def mixed_gradient_workload(torch, engine, grammars, axioms, n_lots, steps):
     eng.forward_tape()
     targets = [tuple(x.contiguous() for x in centers_and_sizes(eng.buffer(t)[:caps[t]].clone())) for t in range(3)]
     T = sum(caps)
-    bvh = LBVH(T, stream=eng.stream())
+    bvh = LBVH(T, engine=eng)
     out = bvh.alloc()
     p = torch.from_numpy(params).cuda()
 
@@ -438,10 +439,12 @@ def mixed_gradient_workload(torch, engine, grammars, axioms, n_lots, steps):
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / steps
     assert n == T and not overflow, (n, T)
+    _, S = eng.tape_counts()
     res = {"lots": n_lots, "menger_cubes": MENGER5_CUBES, "params": n_lots * lparams.shape[1], "terminals": T,
            "ms_per_step": round(sec * 1e3, 4), "grads_per_s": round(n_lots * lparams.shape[1] / sec),
            "terminals_per_s": round(T / sec), "forward_tape_ms": round(fwd / steps * 1e3, 4),
-           "backward_ms": round(bwd / steps * 1e3, 4), "lbvh_ms": round(lb / steps * 1e3, 4)}
+           "backward_ms": round(bwd / steps * 1e3, 4), "lbvh_ms": round(lb / steps * 1e3, 4),
+           "roofline": fwd_bwd_roofline(T, len(both), S, len(both) * lparams.shape[1], (fwd + bwd) / steps)}
     bvh.close()
     eng.close()
     return res
@@ -491,11 +494,41 @@ def gradient_workload(torch, engine, grammars, axioms, n, steps):
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / steps
     T = int(eng.stats()["terminals"])
+    _, S = eng.tape_counts()
     out = {"lots": n, "params": n * params.shape[1], "terminals": T, "ms_per_step": round(sec * 1e3, 4),
            "grads_per_s": round(n * params.shape[1] / sec),
-           "forward_tape_ms": round(fwd / steps * 1e3, 4), "backward_ms": round(bwd / steps * 1e3, 4)}
+           "forward_tape_ms": round(fwd / steps * 1e3, 4), "backward_ms": round(bwd / steps * 1e3, 4),
+           "roofline": fwd_bwd_roofline(T, n, S, n * params.shape[1], (fwd + bwd) / steps)}
     eng.close()
     return out
+
+
+def fwd_bwd_roofline(T, A, S, P, seconds):
+    """HBM roofline of a tape forward plus its backward: algorithmic bytes
+    64 T + 80 A (instances written, axioms read), 8 S written and 8 S read
+    for the tape (S shapes recorded), 48 T of dL/dinstance read (the 12 model
+    entries), 64 T recomputed transforms (one checkpoint per depth), 4 P
+    parameter gradients written; over the device time of both calls."""
+    b = INSTANCE_BYTES * T + AXIOM_BYTES * A + 16 * S + 48 * T + 64 * T + 4 * P
+    achieved = b / seconds / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": b,
+            "shapes_on_tape": S, "device_ms": round(seconds * 1e3, 4),
+            "formula": "64T + 80A + 16S + 48T + 64T + 4P over forward-tape + backward device time"}
+
+
+def cpu_info():
+    """Host CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0)), os.cpu_count()
 
 
 def kernel_name(tune):
@@ -548,6 +581,7 @@ def main():
     if args.only_kernel:
         print(json.dumps({"terminals_per_step": T, "ms_per_step": sec * 1e3, "kernel": kernel_name(tune)}),
               flush=True)
+        eng.close()
         if args.copy_calibration:
             copy_calibration(torch, args.copy_calibration, args.steps)
         return
@@ -615,10 +649,16 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        rate, runs, total = cpu_baseline(lambda: axioms.facade_lots(args.lots), args.cpu_seconds, 1)
-        cpu = {"value": round(rate), "unit": "terminals/s", "cores": 1, "kind": "port",
+        model, usable, online = cpu_info()
+        threads = max(1, min(usable, CPU_THREADS_MAX))
+        rate, runs, total = cpu_baseline(lambda: axioms.facade_lots(args.lots), args.cpu_seconds, threads)
+        rate1, runs1, total1 = cpu_baseline(lambda: axioms.facade_lots(args.lots), args.cpu_seconds, 1)
+        cpu = {"value": round(rate), "unit": "terminals/s", "cores": threads, "kind": "port",
                "sample": f"facade grammar over the same {args.lots} lots, {runs} full derivations "
-                         f"({total} terminals) by the oracle on one host thread, median rate"}
+                         f"({total} terminals), the lots split over {threads} host threads each running "
+                         f"the oracle (g++ -O3 -march=x86-64-v3 -ffp-contract=off), median rate",
+               "cpu_model": model, "cores_usable": usable, "cores_online": online,
+               "single_thread": {"value": round(rate1), "cores": 1, "runs": runs1, "terminals": total1}}
 
     out = {
         "metric": METRIC, "value": round(value), "unit": "terminals/s", "n_gpus": 1,
@@ -642,6 +682,7 @@ def main():
         "triangle_soup_16384": mesh,
         "cpu_baseline": cpu,
     }
+    torch.cuda.synchronize()
     print(json.dumps(out), flush=True)
 
 

@@ -163,8 +163,14 @@ typedef struct pga_dispatch_entry {
  * flags: PGA_PROC_STATIC marks the static-parameter form of an operator
  * (Subdivide<Axis, Pair<F,Next>...>, RandomRule<Chance, Pair<P,Next>...>)
  * whose semantics differ from the <DynParams> form (Subdivide.cuh:22-168 vs
- * 170-248; RandomRule.cuh:20-111 vs 121-147). */
+ * 170-248; RandomRule.cuh:20-111 vs 121-147). PGA_PROC_PARALLEL marks the
+ * <true> form of ComponentSplit, one face per thread through
+ * ParallelFaceSplitter<Box> (ComponentSplit.cuh:179-230, 269-328: face
+ * frames from normal and up vectors, seed modifier (face + 2) % 6); the
+ * <true> form of Repeat traps in the reference (Repeat.cuh:35-61) and is
+ * refused with PGA_ERR_UNSUPPORTED. */
 #define PGA_PROC_STATIC 1u
+#define PGA_PROC_PARALLEL 2u
 typedef struct pga_procedure {
     int32_t op_type;      /* pga_operator_type */
     int32_t shape_type;   /* pga_shape_type of the shapes the queue holds */
@@ -317,6 +323,16 @@ PGA_API pga_status pga_tape_reserve(pga_context* ctx, uint32_t max_records, uint
 PGA_API pga_status pga_forward_tape(pga_context* ctx, uint32_t num_axioms, double* seconds);
 PGA_API pga_status pga_backward(pga_context* ctx, const float* const* dl_dinstances, float* dl_dparams,
                                 double* seconds);
+/* Generation of the recorded tape: a number that grows with every successful
+ * pga_forward_tape, 0 while no tape is valid. pga_set_axioms*,
+ * pga_set_axiom_params, pga_set_philox_key, pga_initialize and
+ * pga_tape_reserve invalidate the tape (pga_backward then returns
+ * PGA_ERR_NOT_INITIALIZED); a caller that keeps a forward's generation can
+ * tell whether a later backward would replay that forward's tape. */
+PGA_API pga_status pga_tape_generation(const pga_context* ctx, uint64_t* generation);
+/* Size of the recorded tape: fan-out parent records of all levels, and
+ * children of all levels counting the axioms (one outcome row each). */
+PGA_API pga_status pga_tape_counts(const pga_context* ctx, uint64_t* parent_records, uint64_t* children);
 /* The loss of the gradient fits: wc * sum |c_i - tc_i|^2 + ws * sum |s_i - ts_i|^2
  * over `count` instances (device, 16 floats each, 16-byte aligned), where
  * c_i is the translation column and s_i the column norms of the 3x3 part;
@@ -392,6 +408,8 @@ typedef struct pga_graph pga_graph;
 PGA_API pga_status pga_capture_begin(pga_context* ctx);
 PGA_API pga_status pga_capture_end(pga_context* ctx, pga_graph** graph);
 PGA_API pga_status pga_graph_launch(pga_context* ctx, pga_graph* graph);
+/* A graph may outlive its context: pga_destroy detaches the context's graphs,
+ * which can then only be destroyed. */
 PGA_API void pga_graph_destroy(pga_graph* graph);
 /* kernel nodes of the graph and how many of them launch cooperatively */
 PGA_API pga_status pga_graph_info(const pga_graph* graph, uint32_t* kernel_nodes, uint32_t* cooperative_nodes);

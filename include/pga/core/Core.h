@@ -545,19 +545,24 @@ template <class A, class E, class N> struct Lower<Operators::Extrude<A, E, N>> {
         return i;
     }
 };
+// Repeat<true> traps in the reference (Repeat.cuh:35-61): its procedure is
+// marked parallel and the engine refuses it on load (PGA_ERR_UNSUPPORTED)
 template <bool P, class A, class E, class M, class N> struct Lower<Operators::Repeat<P, A, E, M, N>> {
     static int emit(Template& t, int shape) {
-        const int i = add_node(t, PGA_OP_REPEAT, shape, {Param<A>::get(), Param<E>::get(), Param<M>::get()});
+        const int i = add_node(t, PGA_OP_REPEAT, shape, {Param<A>::get(), Param<E>::get(), Param<M>::get()}, 0,
+                               P ? PGA_PROC_PARALLEL : 0u);
         const Link l = link<N>(t, shape);
         t.nodes[i].succ.push_back(l);
         return i;
     }
 };
-// a Box's faces are Quads (ComponentSplit.cuh:77-175)
+// a Box's faces are Quads (ComponentSplit.cuh:77-175); the <true> form
+// splits through ParallelFaceSplitter (ComponentSplit.cuh:179-230), whose
+// frames and cap seeds differ
 template <bool P, class Top, class Bottom, class Sides> struct Lower<Operators::ComponentSplit<P, Top, Bottom, Sides>> {
     static int emit(Template& t, int shape) {
         const int face = shape == Compiler::BOX ? (int)Compiler::QUAD : -1;
-        const int i = add_node(t, PGA_OP_COMPSPLIT, shape, {});
+        const int i = add_node(t, PGA_OP_COMPSPLIT, shape, {}, 0, P ? PGA_PROC_PARALLEL : 0u);
         const Link a = link<Top>(t, face), b = link<Bottom>(t, face), c = link<Sides>(t, face);
         t.nodes[i].succ = {a, b, c};
         return i;

@@ -63,6 +63,7 @@ X, Y, Z = 0, 1, 2
 KERNELS, DYN_PAR, MEGAKERNEL, MEGAKERNEL_LOCAL_QUEUES = 0, 1, 2, 3
 
 PROC_STATIC = 1
+PROC_PARALLEL = 2
 
 
 class DispatchParameter(C.Structure):

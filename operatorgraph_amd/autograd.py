@@ -8,7 +8,9 @@ returns the instances of each terminal type as (count, 16) float32 tensors
 (the 64-byte records: model rows 0-11, then custom, axiom, path, meta).
 Backward runs pga_backward with dL/dinstance and returns dL/dparams. The
 topology (branches, repetition counts, rule choices) is frozen by the tape
-of the forward call."""
+of the forward call; a backward after the engine's tape changed (another
+forward, or new axioms, parameters or program) raises instead of replaying
+the wrong tape."""
 import torch
 
 
@@ -21,10 +23,17 @@ class _Derive(torch.autograd.Function):
         outs = tuple(engine.buffer(t)[: int(counts[t])].clone() for t in terminals)
         ctx.engine = engine
         ctx.terminals = terminals
+        ctx.tape_gen = engine.tape_generation()
         return outs
 
     @staticmethod
     def backward(ctx, *grads):
+        from .engine import PGAError
+        from . import abi
+        if ctx.engine.tape_generation() != ctx.tape_gen:
+            raise PGAError(abi.ERR_NOT_INITIALIZED,
+                           "the engine's tape changed since this forward (another forward, or new axioms, "
+                           "parameters or program): run the forward again")
         dl = {t: g for t, g in zip(ctx.terminals, grads) if g is not None}
         grad = ctx.engine.backward(dl)
         return grad, None, None

@@ -102,6 +102,7 @@ struct pga_context {
     size_t grad64_cap = 0;
     uint32_t tape_par_cap = 0, tape_child_cap = 0, tape_blocks = 0;
     bool tape_valid = false;
+    uint64_t tape_gen = 0;            // grows with every recorded tape
     uint32_t tape_axioms = 0;
     std::vector<uint32_t> tape_pc, tape_dc;   // parents / descriptors written by each level
 
@@ -110,6 +111,7 @@ struct pga_context {
     bool capture_failed = false;
     std::string capture_error;
     std::vector<CapturedLaunch> captured;
+    std::vector<pga_graph*> graphs;   // live graphs captured on this context
 
     pga_stats stats{};
 };
@@ -242,7 +244,10 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
             d.kind = pga::K_SUBDIV; need_p = 1;
             need_s = e.num_parameters > 0 ? e.num_parameters - 1 : 0;
             break;
-        case PGA_OP_REPEAT: d.kind = pga::K_REPEAT; need_p = 3; need_s = 1; break;
+        case PGA_OP_REPEAT:
+            if (p.flags & PGA_PROC_PARALLEL)   // Repeat<true> traps in the reference (Repeat.cuh:35-61)
+                return fail(ctx, PGA_ERR_UNSUPPORTED, at + "the parallel form of Repeat is not implemented");
+            d.kind = pga::K_REPEAT; need_p = 3; need_s = 1; break;
         case PGA_OP_DISCARD: d.kind = pga::K_DISCARD; break;
         case PGA_OP_IF: d.kind = pga::K_IF; need_p = 1; need_s = 2; break;
         case PGA_OP_IFSIZELESS: d.kind = pga::K_IFSIZELESS; need_p = 2; need_s = 2; break;
@@ -267,6 +272,7 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         if (e.num_successors < need_s) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "too few successors");
         d.shape = (uint8_t)p.shape_type;
         d.flags = (p.flags & PGA_PROC_STATIC) ? pga::EF_STATIC : 0;
+        if (d.kind == pga::K_COMPSPLIT && (p.flags & PGA_PROC_PARALLEL)) d.flags |= pga::EF_PARALLEL;
         d.nparams = (uint8_t)e.num_parameters;
         d.nsucc = (uint8_t)e.num_successors;
         d.pbase = (uint32_t)Pm.size();
@@ -669,6 +675,7 @@ PGA_API pga_status pga_create(const pga_config* config, pga_context** out) {
 PGA_API void pga_destroy(pga_context* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (pga_graph* g : ctx->graphs) g->ctx = nullptr;   // the graphs stay valid to destroy
     hfree(ctx->d_image);
     hfree(ctx->d_tape_par); hfree(ctx->d_tape_desc); hfree(ctx->d_tape_out); hfree(ctx->d_adj_slots);
     hfree(ctx->d_dl); hfree(ctx->d_grad64);
@@ -693,6 +700,7 @@ PGA_API pga_status pga_initialize(pga_context* ctx, const pga_procedure* procedu
     NO_CAPTURE(ctx, "pga_initialize");
     if (!ctx) return PGA_ERR_INVALID_ARG;
     ctx->initialized = false;
+    ctx->tape_valid = false;
     return build_program(ctx, procedures, num_procedures, entries, num_entries);
 }
 
@@ -705,6 +713,7 @@ PGA_API pga_status pga_set_axioms(pga_context* ctx, const pga_axiom* axioms, uin
         HIP_TRY(ctx, hipMalloc((void**)&ctx->d_axioms_owned, bytes));
         ctx->axioms_owned_cap = bytes;
     }
+    ctx->tape_valid = false;
     if (n) HIP_TRY(ctx, hipMemcpy(ctx->d_axioms_owned, axioms, (size_t)n * sizeof(pga_axiom), hipMemcpyHostToDevice));
     ctx->axioms = ctx->d_axioms_owned;
     ctx->n_axioms = n;
@@ -714,6 +723,7 @@ PGA_API pga_status pga_set_axioms(pga_context* ctx, const pga_axiom* axioms, uin
 PGA_API pga_status pga_set_axioms_device(pga_context* ctx, const pga_axiom* device_axioms, uint32_t n) {
     NO_CAPTURE(ctx, "pga_set_axioms_device");
     if (!ctx || (!device_axioms && n)) return fail(ctx, PGA_ERR_INVALID_ARG, "null axioms");
+    ctx->tape_valid = false;
     ctx->axioms = device_axioms;
     ctx->n_axioms = n;
     return PGA_OK;
@@ -738,6 +748,7 @@ PGA_API pga_status pga_set_axiom_params(pga_context* ctx, const float* params, u
     NO_CAPTURE(ctx, "pga_set_axiom_params");
     if (!ctx || (!params && num_axioms && per_axiom)) return fail(ctx, PGA_ERR_INVALID_ARG, "null parameters");
     size_t bytes = (size_t)num_axioms * per_axiom * sizeof(float);
+    ctx->tape_valid = false;
     if (bytes > ctx->aparams_cap) {
         hfree(ctx->d_aparams);
         HIP_TRY(ctx, hipMalloc((void**)&ctx->d_aparams, bytes));
@@ -751,6 +762,7 @@ PGA_API pga_status pga_set_axiom_params(pga_context* ctx, const float* params, u
 
 PGA_API pga_status pga_set_philox_key(pga_context* ctx, uint32_t k0, uint32_t k1) {
     if (!ctx) return PGA_ERR_INVALID_ARG;
+    if (k0 != ctx->key[0] || k1 != ctx->key[1]) ctx->tape_valid = false;   // the adjoint re-evaluates draws
     ctx->key[0] = k0;
     ctx->key[1] = k1;
     return PGA_OK;
@@ -1074,8 +1086,25 @@ PGA_API pga_status pga_forward_tape(pga_context* ctx, uint32_t num_axioms, doubl
     if (st == PGA_OK) {
         ctx->tape_valid = true;
         ctx->tape_axioms = num_axioms;
+        ++ctx->tape_gen;
     }
     return st;
+}
+
+PGA_API pga_status pga_tape_generation(const pga_context* ctx, uint64_t* generation) {
+    if (!ctx || !generation) return PGA_ERR_INVALID_ARG;
+    *generation = ctx->tape_valid ? ctx->tape_gen : 0u;
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_tape_counts(const pga_context* ctx, uint64_t* parent_records, uint64_t* children) {
+    if (!ctx || !parent_records || !children) return PGA_ERR_INVALID_ARG;
+    uint64_t sp = 0, sd = ctx->tape_valid ? ctx->tape_axioms : 0u;
+    if (ctx->tape_valid)
+        for (size_t l = 0; l < ctx->tape_dc.size(); ++l) { sp += ctx->tape_pc[l]; sd += ctx->tape_dc[l]; }
+    *parent_records = sp;
+    *children = sd;
+    return PGA_OK;
 }
 
 PGA_API pga_status pga_backward(pga_context* ctx, const float* const* dl_dinstances, float* dl_dparams,
@@ -1190,6 +1219,7 @@ PGA_API pga_status pga_capture_end(pga_context* ctx, pga_graph** out) {
     }
     HIP_TRY(ctx, hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
     *out = g.release();
+    ctx->graphs.push_back(*out);
     return PGA_OK;
 }
 
@@ -1218,7 +1248,10 @@ PGA_API pga_status pga_graph_launch(pga_context* ctx, pga_graph* g) {
 
 PGA_API void pga_graph_destroy(pga_graph* g) {
     if (!g) return;
-    if (g->ctx && g->ctx->stream) (void)hipStreamSynchronize(g->ctx->stream);
+    if (pga_context* ctx = g->ctx) {          // NULL once the context is destroyed
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        ctx->graphs.erase(std::remove(ctx->graphs.begin(), ctx->graphs.end(), g), ctx->graphs.end());
+    }
     delete g;
 }
 

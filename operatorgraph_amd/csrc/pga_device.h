@@ -36,6 +36,7 @@ enum EntryFlags : uint8_t {
     EF_SHAPE_DEP = 4, // some parameter reads the shape (dyn Subdivide re-evaluation)
     EF_DIRECT = 8,    // leaf fan-out whose children are all Generate (constant terminal) or
                       // Discard: its terminal counts are a table (aux[1])
+    EF_PARALLEL = 16, // ComponentSplit<true>: ParallelFaceSplitter face frames and seeds
 };
 
 // One dispatch entry, 48 bytes. The first three parameters are held inline

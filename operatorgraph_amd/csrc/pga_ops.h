@@ -404,6 +404,30 @@ __device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, const En
         if (c.seed) ch.seed = next_seed(r.seed, sub ? j : j + 1u);
         return advance(c, ch, succ_of(c, e, sub ? j : 0u), j);
     }
+    if (kind == K_COMPSPLIT && (e.flags() & EF_PARALLEL)) {   // ParallelFaceSplitter<Box> (ComponentSplit.cuh:179-230)
+        // face j from its outward normal n and up vector y, x = cross(y, n);
+        // the frame [x y n] and the translation size * n / 2 are formed in
+        // FP32 as there (signed zeros included)
+        const float sign = (float)(j & 1u) * 2.0f - 1.0f;
+        const float fa = (float)(j < 2u);
+        const float b1 = (float)(j & 2u) * 0.5f;
+        const float b2 = (float)(j & 4u) * 0.25f;
+        const float nx = b1 * sign, ny = b2 * sign, nz = fa * sign;
+        const float yx = 0.0f, yy = 1.0f - b2, yz = b2 * -sign;
+        if (c.geom) {
+            const float xx = yy * nz - yz * ny, xy = yz * nx - yx * nz, xz = yx * ny - yy * nx;
+            float tmp[12];
+            mul_translate(r.m, (r.s[0] * nx) * 0.5f, (r.s[1] * ny) * 0.5f, (r.s[2] * nz) * 0.5f, tmp);
+            const float b[16] = {xx, yx, nx, 0.0f, xy, yy, ny, 0.0f, xz, yz, nz, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+            mul34x44(tmp, b, ch.m);
+        }
+        const float sx = r.s[0], sy = r.s[1], sz = r.s[2];
+        ch.s[0] = (sx * (fa + b2) + sy * 0.0f) + sz * b1;    // dot(size, abs(s0)), left to right
+        ch.s[1] = (sx * 0.0f + sy * (1.0f - b2)) + sz * b2;  // dot(size, abs(s1))
+        ch.s[2] = 1.0f;
+        if (c.seed) ch.seed = next_seed(r.seed, (j + 2u) % 6u);
+        return advance(c, ch, succ_of(c, e, j < 4 ? 2u : (j == 4 ? 1u : 0u)), j);
+    }
     if (kind == K_COMPSPLIT) {                 // FaceSplitter<Box> (ComponentSplit.cuh:77-174)
         if (c.geom) {
             const float hx = r.s[0] * 0.5f, hy = r.s[1] * 0.5f, hz = r.s[2] * 0.5f;

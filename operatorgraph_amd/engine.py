@@ -10,6 +10,7 @@ fallback: without the built library, or without a GPU, construction raises.
 import contextlib
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -69,6 +70,8 @@ def load_library():
         "pga_tape_reserve": (C.c_int, [P, C.c_uint32, C.c_uint32]),
         "pga_forward_tape": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_double)]),
         "pga_backward": (C.c_int, [P, C.POINTER(C.c_void_p * abi.MAX_TERMINALS), P, C.POINTER(C.c_double)]),
+        "pga_tape_generation": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+        "pga_tape_counts": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "pga_capture_begin": (C.c_int, [P]),
         "pga_capture_end": (C.c_int, [P, C.POINTER(P)]),
         "pga_graph_launch": (C.c_int, [P, P]),
@@ -122,11 +125,16 @@ class Engine:
         self.params_per_axiom = 0
 
     def close(self):
+        """Destroys the context; graphs captured on it can then only be closed."""
         if getattr(self, "_h", None):
             self.L.pga_destroy(self._h)
             self._h = None
 
     def __del__(self):
+        # at interpreter shutdown the HIP runtime may already be torn down:
+        # the process releases the context's device memory and stream
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -320,6 +328,19 @@ class Engine:
         self._check(self.L.pga_forward_tape(self._h, self.n_axioms if n is None else n, C.byref(sec)))
         return sec.value
 
+    def tape_generation(self):
+        """Generation of the recorded tape (0: none valid); see
+        pga_tape_generation."""
+        g = C.c_uint64(0)
+        self._check(self.L.pga_tape_generation(self._h, C.byref(g)))
+        return g.value
+
+    def tape_counts(self):
+        """(parent records, children counting the axioms) of the recorded tape."""
+        p, c = C.c_uint64(0), C.c_uint64(0)
+        self._check(self.L.pga_tape_counts(self._h, C.byref(p), C.byref(c)))
+        return p.value, c.value
+
     def backward(self, dl_dinstances, n=None):
         """dl_dinstances: {terminal: (count, 16) float32 cuda tensor}.
         Returns dL/dparams as an (n, params_per_axiom) cuda tensor."""
@@ -425,6 +446,8 @@ class Graph:
             self._h = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -4,6 +4,7 @@ The reference builds it in GPUBVHConstructor.cuh:29-333 (Morton codes,
 thrust::sort, Karras radix tree, ticket refit); this wrapper drives the
 gfx950 kernels of csrc/pga_lbvh.hip on torch device tensors."""
 import ctypes as C
+import sys
 
 from . import abi
 from .engine import PGAError, load_library
@@ -38,14 +39,21 @@ def _lib():
 
 
 class LBVH:
-    """Builder for up to `capacity` boxes; scratch lives on the device."""
+    """Builder for up to `capacity` boxes; scratch lives on the device.
 
-    def __init__(self, capacity, device=-1, stream=None):
+    `engine`: build on that Engine's stream (so that a derivation and the
+    build record into one graph); the builder then keeps the engine, whose
+    stream it uses, alive until it is closed."""
+
+    def __init__(self, capacity, device=-1, stream=None, engine=None):
         import torch
         if not torch.cuda.is_available():
             raise PGAError(abi.ERR_HIP, "no GPU visible: the LBVH has no CPU path")
         self.torch = torch
         self.L = _lib()
+        self._engine = engine
+        if engine is not None:
+            stream = engine.stream()
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         h = C.c_void_p()
@@ -63,8 +71,13 @@ class LBVH:
         if getattr(self, "_h", None):
             self.L.pga_lbvh_destroy(self._h)
             self._h = None
+        self._engine = None
 
     def __del__(self):
+        # at interpreter shutdown the HIP runtime may already be torn down:
+        # the process releases the device memory
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

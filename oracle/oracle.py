@@ -54,6 +54,11 @@ def lib():
         L.pgao_next_seed.restype = C.c_float
         L.pgao_next_seed.argtypes = [C.c_float, C.c_uint32]
         L.pgao_range.restype = C.c_float
+        L.pgao_seed_component.restype = C.c_uint32
+        L.pgao_seed_component.argtypes = [C.c_uint32, C.c_uint32]
+        L.pgao_seed_state.argtypes = [C.c_float, C.c_uint32, P]
+        L.pgao_seed_unit.restype = C.c_float
+        L.pgao_seed_unit.argtypes = [C.c_uint32]
         L.pgao_range.argtypes = [C.c_float, C.c_float, C.c_float]
         L.pgao_philox4x32_10.argtypes = [P, P, P]
         L.pgao_sinf.restype = C.c_float
@@ -174,6 +179,23 @@ def derive(program, axioms, params=None, key=(0, 0), twins=0, max_steps=0):
 
 def next_seed(base, modifier):
     return lib().pgao_next_seed(base, modifier)
+
+
+def seed_component(z, which):
+    """One of hybridTaus's four generators: 0..2 Tausworthe steps, 3 the LCG."""
+    return lib().pgao_seed_component(z, which)
+
+
+def seed_state(base, modifier):
+    """The component states z1..z4 hybridTaus derives from (base, modifier)."""
+    z = (C.c_uint32 * 4)()
+    lib().pgao_seed_state(base, modifier, C.addressof(z))
+    return list(z)
+
+
+def seed_unit(t):
+    """hybridTaus's final 32-bit value -> float conversion."""
+    return lib().pgao_seed_unit(t)
 
 
 def range_(seed, mn, mx):

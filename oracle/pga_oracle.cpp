@@ -134,11 +134,23 @@ inline void sincos_t(Var x, Var* s, Var* c) {
 // ---------------------------------------------------------------------------
 // Random.cuh:15-57 - hybrid Tausworthe seed chain.
 // ---------------------------------------------------------------------------
-inline uint32_t taus_step(uint32_t& z, int s1, int s2, int s3, uint32_t m) {
+inline uint32_t taus_step(uint32_t z, int s1, int s2, int s3, uint32_t m) {
     uint32_t b = (((z << s1) ^ z) >> s2);
-    return z = (((z & m) << s3) ^ b);
+    return (((z & m) << s3) ^ b);
 }
-inline uint32_t lcg_step(uint32_t& z, uint32_t a, uint32_t c) { return z = (a * z + c); }
+inline uint32_t lcg_step(uint32_t z, uint32_t a, uint32_t c) { return a * z + c; }
+
+// The four generators combined by hybridTaus (Random.cuh:36-43): three
+// Tausworthe steps (L'Ecuyer's taus88 components, Math. Comp. 65 (1996)) and
+// one linear congruential step. `which` = 0, 1, 2: Tausworthe; 3: LCG.
+uint32_t seed_component(uint32_t z, uint32_t which) {
+    switch (which) {
+    case 0: return taus_step(z, 13, 19, 12, 4294967294u);
+    case 1: return taus_step(z, 2, 25, 4, 4294967288u);
+    case 2: return taus_step(z, 3, 11, 17, 4294967280u);
+    default: return lcg_step(z, 1664525u, 1013904223u);
+    }
+}
 
 // float -> uint with round-down and saturation (__float2uint_rd, Random.cuh:27)
 inline uint32_t f2u_rd(float x) {
@@ -147,17 +159,26 @@ inline uint32_t f2u_rd(float x) {
     return (uint32_t)x;                   // truncation == floor for x >= 0
 }
 
-float hybrid_taus(float base, uint32_t modifier) {
+// The component states z1..z4 that hybridTaus derives from (base, modifier)
+// (Random.cuh:26-35).
+void seed_state(float base, uint32_t modifier, uint32_t zs[4]) {
     uint32_t z = (f2u_rd(base) * UINT_MAX) + (modifier * 41231278u) % UINT_MAX;
-    uint32_t z1 = (z + 7873234u) % UINT_MAX;
-    uint32_t z2 = (z1 + 1237217827u) % UINT_MAX;
-    uint32_t z3 = (z2 + 87382u) % UINT_MAX;
-    uint32_t z4 = (z3 + 1410022u) % UINT_MAX;
-    uint32_t t1 = taus_step(z1, 13, 19, 12, 4294967294u);
-    uint32_t t2 = taus_step(z2, 2, 25, 4, 4294967288u);
-    uint32_t t3 = taus_step(z3, 3, 11, 17, 4294967280u);
-    uint32_t t4 = lcg_step(z4, 1664525u, 1013904223u);
-    return (float)(2.3283064365387e-10 * (double)(t1 ^ t2 ^ t3 ^ t4));
+    zs[0] = (z + 7873234u) % UINT_MAX;
+    zs[1] = (zs[0] + 1237217827u) % UINT_MAX;
+    zs[2] = (zs[1] + 87382u) % UINT_MAX;
+    zs[3] = (zs[2] + 1410022u) % UINT_MAX;
+}
+
+// The combined 32-bit value to a float (Random.cuh:44-48): a double product
+// with 2.3283064365387e-10 (2^-32 rounded up), then a float conversion; the
+// largest values round to 1.0f.
+inline float seed_unit(uint32_t t) { return (float)(2.3283064365387e-10 * (double)t); }
+
+float hybrid_taus(float base, uint32_t modifier) {
+    uint32_t zs[4];
+    seed_state(base, modifier, zs);
+    return seed_unit(seed_component(zs[0], 0) ^ seed_component(zs[1], 1) ^ seed_component(zs[2], 2) ^
+                     seed_component(zs[3], 3));
 }
 inline float next_seed(float base, uint32_t modifier) { return hybrid_taus(base, modifier); }
 // Random::range (Random.cuh:51-57) in the twin's precision
@@ -601,6 +622,45 @@ void op_compsplit(const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e
     }
 }
 
+// ComponentSplit<true, Top, Bottom, Sides> over a Box with
+// ParallelFaceSplitter<Box>::splitFace (ComponentSplit.cuh:179-230, 269-328):
+// face f = 0..5 (back, front, left, right, bottom, top) framed by its normal
+// n, up vector y and x = cross(y, n) (vector.h:510-515), translated by
+// size * n / 2; sizes are dot(size, |s0|), dot(size, |s1|) (vector.h:505-508);
+// seed modifier (f + 2) % 6; faces 4 and 5 go to Bottom and Top.
+template <class T>
+void op_compsplit_parallel(const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e, std::vector<Out<T>>& out) {
+    if (m.shape_type != PGA_SHAPE_BOX) throw OpErr(PGA_ERR_UNSUPPORTED, "ComponentSplit supports Box shapes only");
+    const int32_t top = succ(e, 0), bottom = succ(e, 1), sides = succ(e, 2);
+    auto k = [](float v) { return S<T>::c(v); };
+    for (uint32_t f = 0; f < 6; ++f) {
+        const float sign = ((float)(f & 0x1u) * 2.0f) - 1.0f;
+        const float a = (float)(f < 2u);
+        const float b1 = (float)(f & 0x2u) * 0.5f;
+        const float b2 = (float)(f & 0x4u) * 0.25f;
+        const float n[3] = {b1 * sign, b2 * sign, a * sign};
+        const float y[3] = {0.0f, 1.0f - b2, b2 * -sign};
+        const float x[3] = {y[1] * n[2] - y[2] * n[1], y[2] * n[0] - y[0] * n[2], y[0] * n[1] - y[1] * n[0]};
+        const float s0[3] = {std::fabs(a + b2), 0.0f, std::fabs(b1)};
+        const float s1[3] = {0.0f, std::fabs(1.0f - b2), std::fabs(b2)};
+        Shape<T> q;
+        T t[16], tmp[12], r[16];
+        mat_translate((sh.size[0] * k(n[0])) * k(0.5f), (sh.size[1] * k(n[1])) * k(0.5f),
+                      (sh.size[2] * k(n[2])) * k(0.5f), t);
+        const float rot[16] = {x[0], y[0], n[0], 0.0f, x[1], y[1], n[1], 0.0f,
+                               x[2], y[2], n[2], 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+        mat_const(rot, r);
+        mul34x44(sh.m, t, tmp);
+        mul34x44(tmp, r, q.m);
+        q.size[0] = sh.size[0] * k(s0[0]) + sh.size[1] * k(s0[1]) + sh.size[2] * k(s0[2]);
+        q.size[1] = sh.size[0] * k(s1[0]) + sh.size[1] * k(s1[1]) + sh.size[2] * k(s1[2]);
+        q.size[2] = S<T>::c(1.0);
+        q.seed = next_seed(sh.seed, (f + 2u) % 6u);
+        q.custom = sh.custom;
+        push_child(out, f == 5u ? top : (f == 4u ? bottom : sides), f, PGA_SHAPE_QUAD, q);
+    }
+}
+
 // IfSizeLess (If.cuh:61-98) / If (If.cuh:23-58)
 template <class T>
 void op_ifsizeless(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e,
@@ -754,8 +814,14 @@ void run_op(const Ctx& c, const Meta& m, const Shape<T>& sh, Dec& d, std::vector
     bool st = (p.flags & PGA_PROC_STATIC) != 0;
     switch (p.op_type) {
     case PGA_OP_SUBDIV: op_subdivide(c, m, sh, e, st, d, out); break;
-    case PGA_OP_REPEAT: op_repeat(c, m, sh, e, d, out); break;
-    case PGA_OP_COMPSPLIT: op_compsplit(m, sh, e, out); break;
+    case PGA_OP_REPEAT:
+        if (p.flags & PGA_PROC_PARALLEL) throw OpErr(PGA_ERR_UNSUPPORTED, "Repeat<true> traps (Repeat.cuh:35-61)");
+        op_repeat(c, m, sh, e, d, out);
+        break;
+    case PGA_OP_COMPSPLIT:
+        if (p.flags & PGA_PROC_PARALLEL) op_compsplit_parallel(m, sh, e, out);
+        else op_compsplit(m, sh, e, out);
+        break;
     case PGA_OP_IFSIZELESS: op_ifsizeless(c, m, sh, e, d, out); break;
     case PGA_OP_IF: op_if(c, m, sh, e, d, out); break;
     case PGA_OP_STOCHASTIC: op_random(c, m, sh, e, st, d, out); break;
@@ -1035,6 +1101,9 @@ int pgao_backward(pgao_result* r, const double* dl_dmodel, double* dl_dparams) {
 }
 
 float pgao_next_seed(float base, uint32_t modifier) { return next_seed(base, modifier); }
+uint32_t pgao_seed_component(uint32_t z, uint32_t which) { return seed_component(z, which); }
+void pgao_seed_state(float base, uint32_t modifier, uint32_t* z4) { seed_state(base, modifier, z4); }
+float pgao_seed_unit(uint32_t t) { return seed_unit(t); }
 float pgao_range(float seed, float mn, float mx) { return range_t<float>(seed, mn, mx); }
 void pgao_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }
 float pgao_sinf(float x) { float s, c; sincos_f32(x, &s, &c); return s; }

@@ -63,6 +63,9 @@ int pgao_backward(pgao_result* r, const double* dl_dmodel, double* dl_dparams);
 /* Restated primitives, exported so tests can pin them one by one. */
 float pgao_next_seed(float base, uint32_t modifier);         /* Random.cuh:26-49 */
 float pgao_range(float seed, float mn, float mx);             /* Random.cuh:51-57 */
+uint32_t pgao_seed_component(uint32_t z, uint32_t which);    /* Random.cuh:15-24, 36-43: 0..2 Tausworthe, 3 LCG */
+void pgao_seed_state(float base, uint32_t modifier, uint32_t* z4); /* Random.cuh:26-35: z1..z4 */
+float pgao_seed_unit(uint32_t t);                            /* Random.cuh:44-48 */
 void pgao_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float pgao_sinf(float x);
 float pgao_cosf(float x);

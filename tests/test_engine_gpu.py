@@ -260,3 +260,82 @@ def test_menger5_full_size_properties(E):
     keys = (cells[:, 0] * 243 + cells[:, 1]) * 243 + cells[:, 2]
     assert len(np.unique(keys)) == 3_200_000
     eng.close()
+
+
+def test_facade_lots_1024_full_size(E):
+    """The headline workload at its own size: 1,024 facade lots (about 1.2M
+    terminals), bit-exact against the oracle, transforms within 1e-5 of its
+    FP64 twin, and the closed-form counts per lot."""
+    _note("facade 1024 lots (headline workload) megakernel")
+    lots = axioms.facade_lots(1024)
+    walls, windows = axioms.facade_lot_terminals(lots)
+    got, st = check_against_oracle(E, grammars.facade(), lots, 2, abi.MEGAKERNEL,
+                                   cap=int(max(walls.sum(), windows.sum())), queue_capacity=1 << 22)
+    assert len(got[0]) == walls.sum() and len(got[1]) == windows.sum()
+    assert np.array_equal(np.bincount(got[1]["axiom"], minlength=1024), windows)
+    assert st["terminals"] == walls.sum() + windows.sum()
+
+
+def _compsplit_program(parallel):
+    from operatorgraph_amd.program import Procedure, ProcedureList
+    procs = ProcedureList([Procedure(abi.COMPSPLIT, B, 0, abi.PROC_PARALLEL if parallel else 0),
+                           Procedure(abi.TRANSLATE, Q), Procedure(abi.GENERATE, Q, 1)])
+    # each face is moved along its normal by its seed, so that the seeds show in the instances
+    tr = lambda nxt: Entry(1, [Scalar(0), Scalar(0), ShapeAttr(15)], [(nxt, 0)])
+    return Program(procs, DispatchTable([Entry(0, [], [(1, 0), (2, 0), (3, 0)]), tr(4), tr(5), tr(6),
+                                         Entry(2, [Scalar(0)], []), Entry(2, [Scalar(1)], []),
+                                         Entry(2, [Scalar(2)], [])]))
+
+
+@pytest.mark.parametrize("technique", TECHNIQUES)
+@pytest.mark.parametrize("parallel", [False, True])
+def test_parallel_compsplit(E, technique, parallel):
+    """ComponentSplit<true> (ParallelFaceSplitter<Box>) next to the serial
+    form, bit-exact against the oracle (signed zeros of its face frames
+    included) on boxes of random size, pose and seed."""
+    _note(f"compsplit parallel={parallel} technique={technique}")
+    rng = np.random.default_rng(17)
+    n = 200
+    a = axioms.empty(n)
+    a["size"] = rng.uniform(0.5, 5.0, size=(n, 3)).astype(np.float32)
+    a["model"][:, [3, 7, 11]] = rng.uniform(-10, 10, size=(n, 3)).astype(np.float32)
+    a["model"][:, [0, 5, 10]] = 1.0
+    a["model"][: n // 2, [1, 4]] = (0.6, -0.6)      # some sheared frames: no zero products
+    a["seed"] = rng.random(n).astype(np.float32)
+    a["entry_index"] = 0
+    a["shape_type"] = B
+    check_against_oracle(E, _compsplit_program(parallel), a, 3, technique)
+
+
+def test_parallel_repeat_is_refused(E):
+    """Repeat<true> traps in the reference (Repeat.cuh:35-61): refused on load."""
+    from operatorgraph_amd.engine import PGAError
+    from operatorgraph_amd.program import Procedure, ProcedureList
+    procs = ProcedureList([Procedure(abi.REPEAT, B, 0, abi.PROC_PARALLEL), Procedure(abi.GENERATE, B, 1)])
+    prog = Program(procs, DispatchTable([Entry(0, [Scalar(0), Scalar(0.3), Scalar(abi.ADJUST_TO_FILL)], [(1, 0)]),
+                                         Entry(1, [Scalar(0)], [])]))
+    eng = E.Engine()
+    with pytest.raises(PGAError) as e:
+        eng.load(prog)
+    assert e.value.status == abi.ERR_UNSUPPORTED
+    eng.close()
+
+
+def test_set_seeds_copies_n_mod_33(E):
+    """Device::setSeeds copies (n % 33) * 4 bytes (GlobalVariables.cuh:28-32):
+    33 seeds copy nothing, 1,024 seeds copy one, 5 seeds copy five."""
+    eng = E.Engine()
+    base = np.arange(1, 33, dtype=np.float32) / 64.0
+    eng.set_seeds(base)
+    assert np.array_equal(eng.seeds(), base)
+    eng.set_seeds(np.full(33, 0.75, dtype=np.float32))
+    assert np.array_equal(eng.seeds(), base)
+    big = np.full(1024, 0.5, dtype=np.float32)
+    eng.set_seeds(big)
+    want = base.copy()
+    want[0] = 0.5
+    assert np.array_equal(eng.seeds(), want)
+    eng.set_seeds(np.full(5, 0.25, dtype=np.float32))
+    want[:5] = 0.25
+    assert np.array_equal(eng.seeds(), want)
+    eng.close()

@@ -144,7 +144,7 @@ def test_graph_derivation_and_lbvh_follow_parameter_updates():
     eng.load(prog).set_axioms(ax).set_axiom_params(p0)
     eng.bind(0, cap[0])
     eng.bind(1, cap[1])
-    bvh = LBVH(cap[0] + cap[1], stream=eng.stream())
+    bvh = LBVH(cap[0] + cap[1], engine=eng)
     out = bvh.alloc()
     with eng.capture() as g:
         eng.execute_async()
@@ -197,7 +197,7 @@ def test_device_count_build_equals_host_count_build():
         eng.bind(t, max(nt[t], 1))
     eng.execute()
     total = sum(nt[:ntypes])
-    bvh = LBVH(total + 1000, stream=eng.stream())   # capacity above the count
+    bvh = LBVH(total + 1000, engine=eng)   # capacity above the count
     out = bvh.alloc()
     bvh.gather_engine(eng, ntypes, 0, out)
     smin, smax = (-500.0, -10.0, -500.0), (2000.0, 5000.0, 2000.0)

@@ -51,30 +51,119 @@ def test_philox_numpy_matches_oracle():
 
 
 # ---------------------------------------------------------------- seeds
-def _taus_py(base, modifier):
-    """Random.cuh:15-49 restated in Python integers."""
-    M = 0xFFFFFFFF
-    ub = 0 if not base > 0 else min(int(base), M)
-    z = ((ub * M) & M) + ((modifier * 41231278) & M) % M
-    z &= M
-    z1 = ((z + 7873234) & M) % M        # unsigned additions wrap before the modulo
-    z2 = ((z1 + 1237217827) & M) % M
-    z3 = ((z2 + 87382) & M) % M
-    z4 = ((z3 + 1410022) & M) % M
-
-    def taus(z, s1, s2, s3, m):
-        b = (((z << s1) & M) ^ z) >> s2
-        return (((z & m) << s3) & M) ^ b
-
-    t = taus(z1, 13, 19, 12, 4294967294) ^ taus(z2, 2, 25, 4, 4294967288) ^ \
-        taus(z3, 3, 11, 17, 4294967280) ^ ((1664525 * z4 + 1013904223) & M)
-    return float(np.float32(2.3283064365387e-10 * t))
+# hybridTaus (Random.cuh:15-49) XORs three Tausworthe generators and one LCG.
+# Each is pinned by what the mathematics fixes about it, using the oracle's
+# own component functions as black boxes:
+#  * the Tausworthe steps are L'Ecuyer's taus88 components (Math. Comp. 65,
+#    1996): GF(2)-linear maps whose state is the top k bits (k = 31, 29, 28),
+#    each of maximal order 2^k - 1; a wrong shift, mask or operand order
+#    breaks linearity, the state projection or the period;
+#  * the LCG z -> 1664525 z + 1013904223 has full period 2^32 (Hull-Dobell:
+#    odd increment, multiplier = 1 mod 4);
+#  * the final conversion multiplies by 2^-32 (rounded up in the 13th digit)
+#    in double and converts to float, so values below 2^24 map exactly and
+#    the largest round to 1.0f.
+# What no reference-held vector fixes (no run of the reference is possible
+# here): the additive offsets of the state chain (Random.cuh:26-35) and the
+# per-operator modifiers; they are restated from the source text.
+_M32 = 0xFFFFFFFF
 
 
-def test_seed_chain_python_restatement():
+def _prime_factors(n):
+    out, d = [], 2
+    while d * d <= n:
+        while n % d == 0:
+            out.append(d)
+            n //= d
+        d += 1
+    if n > 1:
+        out.append(n)
+    return sorted(set(out))
+
+
+def _gf2_mul(a_cols, b_cols):
+    """Columns of A.B over GF(2); a column is an int bitmask."""
+    out = []
+    for col in b_cols:
+        acc, j = 0, 0
+        while col:
+            if col & 1:
+                acc ^= a_cols[j]
+            col >>= 1
+            j += 1
+        out.append(acc)
+    return out
+
+
+def _gf2_pow(cols, e):
+    k = len(cols)
+    res = [1 << j for j in range(k)]
+    base = cols
+    while e:
+        if e & 1:
+            res = _gf2_mul(base, res)
+        base = _gf2_mul(base, base)
+        e >>= 1
+    return res
+
+
+@pytest.mark.parametrize("which,k", [(0, 31), (1, 29), (2, 28)])
+def test_tausworthe_component_is_a_maximal_period_gf2_map(which, k):
+    comp = lambda z: O.seed_component(z, which)
+    rng = np.random.default_rng(11 + which)
+    # linear over GF(2)
+    assert comp(0) == 0
+    for a, b in rng.integers(0, 2 ** 32, size=(200, 2), dtype=np.uint64):
+        a, b = int(a), int(b)
+        assert comp(a ^ b) == comp(a) ^ comp(b)
+    # the state is the top k bits: the bits below never reach them
+    shift = 32 - k
+    for i in range(shift):
+        assert comp(1 << i) >> shift == 0, i
+    # the induced k x k map has order exactly 2^k - 1
+    cols = [comp(1 << (j + shift)) >> shift for j in range(k)]
+    ident = [1 << j for j in range(k)]
+    n = 2 ** k - 1
+    assert _gf2_pow(cols, n) == ident
+    for p in _prime_factors(n):
+        assert _gf2_pow(cols, n // p) != ident, p
+
+
+def test_lcg_component_has_full_period():
+    f = lambda z: O.seed_component(z, 3)
+    c = f(0)
+    a = (f(1) - c) & _M32
+    rng = np.random.default_rng(5)
+    for z in rng.integers(0, 2 ** 32, size=500, dtype=np.uint64):
+        assert f(int(z)) == (a * int(z) + c) & _M32     # affine mod 2^32
+    assert c % 2 == 1 and a % 4 == 1                    # Hull-Dobell for m = 2^32
+
+
+def test_seed_unit_conversion():
+    u = O.seed_unit
+    assert u(0) == 0.0
+    assert u(0xFFFFFFFF) == 1.0                         # SURVEY quirk: can round to 1.0f
+    for t in [1, 2, 3, 1000, 2 ** 23 + 5, 2 ** 24 - 1]:
+        assert u(t) == t * 2.0 ** -32                   # exact below 2^24
+    rng = np.random.default_rng(3)
+    ts = sorted(int(t) for t in rng.integers(0, 2 ** 32, size=3000, dtype=np.uint64))
+    vals = [u(t) for t in ts]
+    assert all(x <= y for x, y in zip(vals, vals[1:]))  # monotone
+    for t, v in zip(ts, vals):
+        exact = t * 2.0 ** -32
+        ulp = float(np.spacing(np.float32(max(exact, 2.0 ** -126))))
+        assert abs(v - exact) <= 0.5 * ulp * (1 + 1e-9) + exact * 1e-12
+
+
+def test_seed_chain_composes_the_pinned_components():
     for m in list(range(0, 40)) + [1000, 65535, 2 ** 31, 2 ** 32 - 1]:
         for base in (0.0, 0.25, 0.999, 1.0, 1.5, 7.0):
-            assert O.next_seed(base, m) == _taus_py(base, m), (base, m)
+            z = O.seed_state(base, m)
+            t = O.seed_component(z[0], 0) ^ O.seed_component(z[1], 1) ^ \
+                O.seed_component(z[2], 2) ^ O.seed_component(z[3], 3)
+            assert O.next_seed(base, m) == O.seed_unit(t), (base, m)
+            # the state depends on the base through its integer part only
+            assert z == O.seed_state(float(math.floor(base)), m)
 
 
 def test_seed_chain_properties():
@@ -243,6 +332,71 @@ def test_compsplit_faces_textbook():
     for s in list(sides) + [bottom, top]:
         R = s[:, :3] / np.linalg.norm(s[:, :3], axis=0)
         assert np.isclose(np.linalg.det(R), 1.0, atol=1e-6)
+
+
+def _compsplit_prog(parallel, term_of_face=(0, 1, 2)):
+    procs = ProcedureList([Procedure(abi.COMPSPLIT, B, 0, abi.PROC_PARALLEL if parallel else 0),
+                           Procedure(abi.GENERATE, Q, 1)])
+    gen = lambda t: Entry(1, [Scalar(t)], [])
+    return Program(procs, DispatchTable([Entry(0, [], [(1, 0), (2, 0), (3, 0)])] +
+                                        [gen(t) for t in term_of_face]))
+
+
+def test_parallel_compsplit_faces_textbook():
+    """ComponentSplit<true> (ParallelFaceSplitter<Box>, ComponentSplit.cuh:
+    179-230): the same six faces as the serial splitter - centred on the box
+    faces, outward normals, right-handed frames, the serial form's sizes and
+    models up to the sign of zero - with the caps' seed modifiers swapped:
+    (f + 2) % 6 gives bottom 0 and top 1, where the serial splitter gives
+    bottom 1 and top 0 (ComponentSplit.cuh:132, 169)."""
+    size = (2.0, 3.0, 4.0)
+    a = _box(size, pos=(5.0, -1.0, 2.0), seed=0.5)
+    rp, rs = O.derive(_compsplit_prog(True), a), O.derive(_compsplit_prog(False), a)
+    for t in range(3):
+        ip, is_ = rp.instances(t), rs.instances(t)
+        kp, ks = np.argsort(ip["path"]), np.argsort(is_["path"])
+        assert np.array_equal(ip["path"][kp], is_["path"][ks])      # same faces (ordinals)
+        assert np.array_equal(ip["model"][kp], is_["model"][ks])    # equal values (+0 == -0)
+    top, bottom = rp.instances(0)["model"].reshape(3, 4), rp.instances(1)["model"].reshape(3, 4)
+    assert np.allclose(bottom[:, 3], (5, -2.5, 2)) and np.allclose(bottom[:, 2], (0, -1, 0))
+    assert np.allclose(top[:, 3], (5, 0.5, 2)) and np.allclose(top[:, 2], (0, 1, 0))
+    sides = rp.instances(2)["model"].reshape(-1, 3, 4)
+    centres = {tuple(np.rint(s[:, 2]).astype(int)): tuple(s[:, 3]) for s in sides}
+    assert centres == {(0, 0, -1): (5, -1, 0), (0, 0, 1): (5, -1, 4), (-1, 0, 0): (4, -1, 2),
+                       (1, 0, 0): (6, -1, 2)}
+    for s in list(sides) + [bottom, top]:
+        R = s[:, :3] / np.linalg.norm(s[:, :3], axis=0)
+        assert np.isclose(np.linalg.det(R), 1.0, atol=1e-6)
+    # the seeds as custom attribute would not show them: route each face's
+    # seed into a Translate by ShapeSeed and read it back from the instance
+    seeds = {}
+    for par in (True, False):
+        procs = ProcedureList([Procedure(abi.COMPSPLIT, B, 0, abi.PROC_PARALLEL if par else 0),
+                               Procedure(abi.TRANSLATE, Q), Procedure(abi.GENERATE, Q, 1)])
+        tr = lambda nxt: Entry(1, [Scalar(0), Scalar(0), ShapeAttr(15)], [(nxt, 0)])
+        p = Program(procs, DispatchTable([Entry(0, [], [(1, 0), (2, 0), (3, 0)]), tr(4), tr(5), tr(6),
+                                          Entry(2, [Scalar(0)], []), Entry(2, [Scalar(1)], []),
+                                          Entry(2, [Scalar(2)], [])]))
+        r = O.derive(p, _box((1, 1, 1), seed=0.5))
+        # translate z by the seed in the face frame: the z column is the normal
+        seeds[par] = [r.instances(t)["model"].reshape(-1, 3, 4) for t in range(3)]
+    def lifted(m):   # seed = displacement along the normal minus the half size (0.5)
+        return np.round(np.einsum("ij,ij->i", m[:, :, 3], m[:, :, 2]) - 0.5, 6)
+    top_p, bot_p = lifted(seeds[True][0])[0], lifted(seeds[True][1])[0]
+    top_s, bot_s = lifted(seeds[False][0])[0], lifted(seeds[False][1])[0]
+    assert np.isclose(top_p, O.next_seed(0.5, 1), atol=1e-6) and np.isclose(bot_p, O.next_seed(0.5, 0), atol=1e-6)
+    assert np.isclose(top_s, O.next_seed(0.5, 0), atol=1e-6) and np.isclose(bot_s, O.next_seed(0.5, 1), atol=1e-6)
+    assert sorted(lifted(seeds[True][2])) == sorted(lifted(seeds[False][2]))   # sides: f + 2 in both
+
+
+def test_parallel_repeat_is_refused():
+    """Repeat<true> traps in the reference (Repeat.cuh:35-61)."""
+    procs = ProcedureList([Procedure(abi.REPEAT, B, 0, abi.PROC_PARALLEL), Procedure(abi.GENERATE, B, 1)])
+    p = Program(procs, DispatchTable([Entry(0, [Scalar(0), Scalar(0.3), Scalar(abi.ADJUST_TO_FILL)], [(1, 0)]),
+                                      Entry(1, [Scalar(0)], [])]))
+    with pytest.raises(O.OracleError) as e:
+        O.derive(p, _box())
+    assert e.value.status == abi.ERR_UNSUPPORTED
 
 
 @pytest.mark.parametrize("mode,total,extent", [
