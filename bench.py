@@ -87,8 +87,21 @@ def make_engine(torch, engine_mod, grammars, axioms, n_lots, key=None):
     eng.set_axioms(dev_axioms)
     eng.bind(0, int(walls.sum()))
     eng.bind(1, int(windows.sum()))
-    tune = eng.autotune(repeats=3)
+    tune = tune_engine(eng, 3)
     return eng, lots, T, tune
+
+
+SPEC_BLOCK = 768   # workgroup size of the kernel specialised to the program
+
+
+def tune_engine(eng, repeats):
+    """Compiles the program's specialised kernel (pga_specialize), then lets
+    the autotuner time it beside the interpreting kernels and keep the
+    fastest configuration."""
+    compile_s = eng.specialize(SPEC_BLOCK)
+    tune = eng.autotune(repeats=repeats)
+    tune["compile_seconds"] = compile_s
+    return tune
 
 
 def timed_steps(torch, eng, steps, technique):
@@ -192,7 +205,7 @@ def stochastic_workload(torch, engine, grammars, axioms, n, steps):
     eng.execute()
     T = int(eng.counts(1)[0])
     buf = eng.bind(0, T)
-    tune = eng.autotune(repeats=3)
+    tune = tune_engine(eng, 3)
     eng.execute()
     sec = timed_steps(torch, eng, steps, tune["technique"])
     ksec, _ = kernel_seconds(torch, eng, 3)
@@ -241,7 +254,7 @@ def mixed_workload(torch, engine, grammars, axioms, n_lots, steps):
     eng.set_axioms(torch.from_numpy(both.view(np.uint8).copy()).cuda())
     for t in range(3):
         eng.bind(t, caps[t])
-    tune = eng.autotune(repeats=2)
+    tune = tune_engine(eng, 2)
     eng.execute()
     got = [int(c) for c in eng.counts(3)]
     assert got == caps, (got, caps)
@@ -319,7 +332,7 @@ def mesh_workload(torch, engine, axioms, n_lots, steps):
     inst = torch.empty((n, 16), dtype=torch.float32, device="cuda")
     eng.bind(0, nw, buffer=inst[:nw])
     eng.bind(1, nq, buffer=inst[nw:])
-    tune = eng.autotune(repeats=2)
+    tune = tune_engine(eng, 2)
     eng.execute()
     assert [int(c) for c in eng.counts(2)] == [nw, nq]
     verts = torch.empty((4 * n, 9), dtype=torch.float32, device="cuda")
@@ -533,6 +546,8 @@ def cpu_info():
 
 def kernel_name(tune):
     from operatorgraph_amd import abi
+    if tune.get("specialized"):
+        return f"pga_spec_mega<{tune['block_size']}>"
     k = "k_mega" if tune["technique"] == abi.MEGAKERNEL else "k_level"
     return f"pga::{k}<{tune['block_size']}>"
 
@@ -541,9 +556,13 @@ def tune_summary(tune):
     from operatorgraph_amd import abi
     tech = "MEGAKERNEL (cooperative persistent launch)" if tune["technique"] == abi.MEGAKERNEL \
         else "KERNELS (one cooperative launch per level)"
+    if tune.get("specialized"):
+        tech = "MEGAKERNEL specialised to the program (hipRTC)"
     return {"technique": tech, "workgroup_size": tune["block_size"], "workgroups": tune["grid_blocks"],
             "candidates": tune["candidates"], "best_ms": round(tune["seconds"] * 1e3, 4),
-            "untuned_ms": round(tune["initial_seconds"] * 1e3, 4)}
+            "untuned_ms": round(tune["initial_seconds"] * 1e3, 4),
+            "specialized": bool(tune.get("specialized")),
+            "compile_s": round(tune.get("compile_seconds", 0.0), 3)}
 
 
 def load_traffic(workload, kernel=None):

@@ -18,8 +18,10 @@
 #ifndef PGA_H
 #define PGA_H
 
+#ifndef __HIPCC_RTC__   /* runtime-compiled device code has no system headers */
 #include <stddef.h>
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -295,8 +297,33 @@ typedef struct pga_tune_result {
     double initial_seconds;        /* best device time of the configuration before tuning */
     uint64_t max_level_parents;    /* largest number of parent records written by one level */
     uint64_t max_level_children;   /* largest number of children of one level */
+    uint32_t specialized;          /* 1: the chosen kernel is the one specialised to the program */
+    uint32_t reserved;
 } pga_tune_result;
 PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t repeats, pga_tune_result* out);
+
+/* -- kernels specialised to the loaded program. The reference compiles each
+ *    grammar into its own kernels (the C++ idiom's templates, the code the
+ *    analyzer generates per partition, CodeGenerator.cpp:1691-1722, rebuilt
+ *    with nvcc). Here the library writes the loaded program as C++ - one
+ *    type of compile-time constants per dispatch entry - and compiles it at
+ *    run time (hipRTC) with the persistent derivation kernel, workgroups of
+ *    block_size threads (256, 512, 768 or 1024; 0: 512). enable != 0 makes
+ *    the MEGAKERNEL technique run it (eager, asynchronous and captured
+ *    derivations) and pga_autotune time it beside the interpreting kernels;
+ *    enable = 0 goes back to the interpreting kernel. The instances are
+ *    those of the interpreting kernel. pga_initialize drops the
+ *    specialisation. seconds (optional): the compile time, 0 when the code
+ *    object was compiled before in this process. -- */
+PGA_API pga_status pga_specialize(pga_context* ctx, uint32_t block_size, int32_t enable, double* seconds);
+/* The same compilation without a device or context: validates the program
+ * as pga_initialize does, writes its specialised kernel and compiles it.
+ * log (optional) receives the validation error or the compiler's messages,
+ * at most log_size bytes. For tests and tools that check a program before a
+ * device is at hand. */
+PGA_API pga_status pga_specialize_check(const pga_procedure* procedures, uint32_t num_procedures,
+                                        const pga_dispatch_entry* entries, uint32_t num_entries, uint32_t block_size,
+                                        char* log, uint32_t log_size, double* seconds);
 
 /* -- instrumentation (the reference's clock64 edge/procedure timers,
  *    Instrumentation.cuh:47-76, become phase timestamps): when a device

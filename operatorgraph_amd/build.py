@@ -16,9 +16,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpga_amd.so")
-SOURCES = ["pga_kernels.hip", "pga_adjoint.hip", "pga_lbvh.hip", "pga_loss.hip", "pga_mesh.hip", "pga_api.cpp"]
+SOURCES = ["pga_kernels.hip", "pga_adjoint.hip", "pga_lbvh.hip", "pga_loss.hip", "pga_mesh.hip", "pga_api.cpp",
+           "pga_spec.cpp"]
 HOST_SOURCES = ["pga_compiler.cpp"]   # no device code: built with the host compiler
-HEADERS = ["pga_device.h", "pga_ops.h", "pga_kernels.h"]
+HEADERS = ["pga_device.h", "pga_ops.h", "pga_kernels.h", "pga_level.h", "pga_spec.h", "pga_spec_host.h"]
+# device headers compiled again at run time (hipRTC) for the kernels
+# specialised to one program; embedded in the library as text
+RTC_HEADERS = [("pga.h", os.path.join(ROOT, "include", "pga.h"))] + \
+    [(h, os.path.join(CSRC, h)) for h in ("pga_device.h", "pga_ops.h", "pga_kernels.h", "pga_level.h", "pga_spec.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
@@ -40,15 +45,38 @@ def _run(cmd):
     return res
 
 
+def write_rtc_headers(build_dir):
+    """The device headers as C++ raw string literals (pga_rtc_headers.inc)."""
+    delim = "PGA_RTC_TEXT"
+    names, texts = [], []
+    for name, path in RTC_HEADERS:
+        with open(path, encoding="utf-8") as f:
+            text = f.read()
+        if ")" + delim + '"' in text:
+            raise RuntimeError(f"{path} contains the raw-string delimiter")
+        names.append(f'"{name}"')
+        texts.append(f'R"{delim}({text}){delim}"')
+    out = os.path.join(build_dir, "pga_rtc_headers.inc")
+    body = ("// written by operatorgraph_amd/build.py: device headers for hipRTC\n"
+            "static const char* const kRtcHeaderNames[] = {" + ", ".join(names) + "};\n"
+            "static const char* const kRtcHeaderTexts[] = {\n" + ",\n".join(texts) + "};\n")
+    old = open(out, encoding="utf-8").read() if os.path.exists(out) else None
+    if old != body:
+        with open(out, "w", encoding="utf-8") as f:
+            f.write(body)
+    return out
+
+
 def build(verbose=False):
     """Compiles every source for gfx950 into operatorgraph_amd/libpga_amd.so."""
     objs = []
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
+    write_rtc_headers(build_dir)
     for src in SOURCES:
         obj = os.path.join(build_dir, src + ".o")
         cmd = [HIPCC, f"--offload-arch={ARCH}", *COMMON_FLAGS, "-I", os.path.join(ROOT, "include"),
-               "-c", os.path.join(CSRC, src), "-o", obj]
+               "-I", build_dir, "-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
             cmd.insert(1, "-x")
             cmd.insert(2, "hip")
@@ -65,7 +93,7 @@ def build(verbose=False):
         _run(cmd)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lhiprtc"])
     os.replace(tmp, LIB)
     return LIB
 

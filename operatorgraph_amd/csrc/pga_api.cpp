@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "pga_kernels.h"
+#include "pga_spec_host.h"
 
 using pga::DevEntry;
 using pga::DevParam;
@@ -26,6 +28,7 @@ using pga::DevParam;
 struct CapturedLaunch {
     pga::KArgs args;
     uint32_t blocks, block_size, first, count;
+    const void* kernel;   // the specialised kernel it ran, or NULL
 };
 
 struct pga_context {
@@ -112,6 +115,12 @@ struct pga_context {
     std::string capture_error;
     std::vector<CapturedLaunch> captured;
     std::vector<pga_graph*> graphs;   // live graphs captured on this context
+
+    // kernel specialised to the program (pga_specialize)
+    hipModule_t spec_module = nullptr;
+    hipFunction_t spec_fn = nullptr;
+    uint32_t spec_bs = 0, spec_blocks = 0;
+    bool spec_on = false;
 
     pga_stats stats{};
 };
@@ -208,8 +217,10 @@ bool check_exp(const pga_dispatch_parameter& p, std::string& why) {
     return true;
 }
 
+// Validates and lowers a program into the context's tables; `upload` = false
+// stops before any device call (the tables stay on the host).
 pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t nprocs,
-                         const pga_dispatch_entry* entries, uint32_t n) {
+                         const pga_dispatch_entry* entries, uint32_t n, bool upload = true) {
     if (!procs || !entries || nprocs == 0 || n == 0) return fail(ctx, PGA_ERR_INVALID_ARG, "empty program");
     if (n > 0xFFFFu) return fail(ctx, PGA_ERR_BAD_PROGRAM, "more than 65535 dispatch entries");
     std::vector<DevEntry> E(n);
@@ -473,6 +484,16 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
     std::memcpy(image.data() + off_succ, Sc.data(), Sc.size() * sizeof(int32_t));
     std::memcpy(image.data() + off_exp, Ex.data(), Ex.size() * sizeof(uint32_t));
     std::memcpy(image.data() + off_slots, slots, sizeof(slots));
+    if (!upload) {
+        ctx->h_entries = E;
+        ctx->h_params = Pm;
+        ctx->h_succ = Sc;
+        ctx->h_exp = Ex;
+        ctx->count_geom = count_geom ? 1u : 0u;
+        ctx->use_seed = use_seed ? 1u : 0u;
+        ctx->max_param_slot = max_slot;
+        return PGA_OK;
+    }
     hfree(ctx->d_image);
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_image, image_bytes));
     HIP_TRY(ctx, hipMemcpy(ctx->d_image, image.data(), image_bytes, hipMemcpyHostToDevice));
@@ -582,6 +603,31 @@ pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
     return a;
 }
 
+void drop_spec(pga_context* ctx) {
+    if (ctx->spec_module) {
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        (void)hipModuleUnload(ctx->spec_module);
+    }
+    ctx->spec_module = nullptr;
+    ctx->spec_fn = nullptr;
+    ctx->spec_bs = ctx->spec_blocks = 0;
+    ctx->spec_on = false;
+}
+
+// The persistent derivation: the specialised kernel when it is on, else the
+// interpreting kernel of the configured workgroup size.
+hipError_t launch_derivation(pga_context* ctx, const pga::KArgs& a, const void** kernel) {
+    if (ctx->spec_on) {
+        pga::KArgs copy = a;
+        void* params[] = {&copy};
+        *kernel = (const void*)ctx->spec_fn;
+        return hipModuleLaunchCooperativeKernel(ctx->spec_fn, ctx->spec_blocks, 1, 1, ctx->spec_bs, 1, 1, a.lds_bytes,
+                                                ctx->stream, params);
+    }
+    *kernel = nullptr;
+    return pga_launch_mega(a, ctx->mega_blocks, ctx->cfg.block_size, ctx->stream);
+}
+
 pga_status status_from_err(pga_context* ctx, uint32_t err) {
     if (!err) return PGA_OK;
     static const pga_status order[] = {PGA_ERR_BAD_PROGRAM, PGA_ERR_SUBDIV_OVERCONSTRAINED,
@@ -689,6 +735,7 @@ PGA_API void pga_destroy(pga_context* ctx) {
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->evk0) (void)hipEventDestroy(ctx->evk0);
     if (ctx->evk1) (void)hipEventDestroy(ctx->evk1);
+    if (ctx->spec_module) (void)hipModuleUnload(ctx->spec_module);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -701,6 +748,7 @@ PGA_API pga_status pga_initialize(pga_context* ctx, const pga_procedure* procedu
     if (!ctx) return PGA_ERR_INVALID_ARG;
     ctx->initialized = false;
     ctx->tape_valid = false;
+    drop_spec(ctx);
     return build_program(ctx, procedures, num_procedures, entries, num_entries);
 }
 
@@ -838,9 +886,11 @@ static pga_status execute_async(pga_context* ctx, uint32_t first, uint32_t count
     const bool timed = !ctx->capturing;     // the events time eager launches only
     HIP_TRY(ctx, pga_launch_reset(a, append ? 0 : 1, ctx->stream));
     if (timed) HIP_TRY(ctx, hipEventRecord(ctx->evk0, ctx->stream));
-    if (count > 0) HIP_TRY(ctx, pga_launch_mega(a, ctx->mega_blocks, ctx->cfg.block_size, ctx->stream));
+    const void* kernel = nullptr;
+    if (count > 0) HIP_TRY(ctx, launch_derivation(ctx, a, &kernel));
+    else kernel = ctx->spec_on ? (const void*)ctx->spec_fn : nullptr;
     if (timed) HIP_TRY(ctx, hipEventRecord(ctx->evk1, ctx->stream));
-    if (ctx->capturing) ctx->captured.push_back({a, ctx->mega_blocks, ctx->cfg.block_size, first, count});
+    if (ctx->capturing) ctx->captured.push_back({a, ctx->mega_blocks, ctx->cfg.block_size, first, count, kernel});
     ctx->pending_count = count;
     return PGA_OK;
 }
@@ -898,7 +948,8 @@ PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first, uint32_t 
             ++levels;
         }
     } else if (count > 0) {
-        HIP_TRY(ctx, pga_launch_mega(a, ctx->mega_blocks, bs, ctx->stream));
+        const void* kernel = nullptr;
+        HIP_TRY(ctx, launch_derivation(ctx, a, &kernel));
     }
     HIP_TRY(ctx, hipEventRecord(ctx->evk1, ctx->stream));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -943,6 +994,76 @@ static pga_status resident_blocks(pga_context* ctx, uint32_t bs, uint32_t* block
     return PGA_OK;
 }
 
+PGA_API pga_status pga_specialize(pga_context* ctx, uint32_t block_size, int32_t enable, double* seconds) {
+    NO_CAPTURE(ctx, "pga_specialize");
+    if (!ctx) return PGA_ERR_INVALID_ARG;
+    if (seconds) *seconds = 0.0;
+    if (!enable) {
+        ctx->spec_on = false;
+        return PGA_OK;
+    }
+    if (!ctx->initialized) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no program loaded");
+    if (block_size == 0) block_size = kDefaultBlockSize;
+    if (!pga_block_size_supported(block_size)) return fail(ctx, PGA_ERR_INVALID_ARG, "unsupported workgroup size");
+    if (ctx->spec_fn && ctx->spec_bs == block_size) {
+        ctx->spec_on = true;
+        return PGA_OK;
+    }
+    drop_spec(ctx);
+    const auto t0 = std::chrono::steady_clock::now();
+    const std::string src = pga_spec_source(ctx->h_entries, ctx->count_geom, ctx->use_seed, block_size);
+    std::vector<char> code;
+    std::string log;
+    bool cached = false;
+    if (!pga_spec_compile(src, code, log, &cached))
+        return fail(ctx, PGA_ERR_UNSUPPORTED, "run-time compilation of the specialised kernel failed: " + log);
+    HIP_TRY(ctx, hipModuleLoadData(&ctx->spec_module, code.data()));
+    HIP_TRY(ctx, hipModuleGetFunction(&ctx->spec_fn, ctx->spec_module, "pga_spec_mega"));
+    int per_cu = 0;
+    HIP_TRY(ctx, hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ctx->spec_fn, (int)block_size,
+                                                                    ctx->lds_bytes));
+    if (per_cu < 1) {
+        drop_spec(ctx);
+        return fail(ctx, PGA_ERR_UNSUPPORTED, "the specialised kernel does not fit on a compute unit");
+    }
+    ctx->spec_bs = block_size;
+    ctx->spec_blocks = std::min((uint32_t)per_cu, kMaxBlocksPerCu) * (uint32_t)ctx->num_cus;
+    ctx->spec_on = true;
+    if (seconds && !cached) seconds[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_specialize_check(const pga_procedure* procedures, uint32_t num_procedures,
+                                        const pga_dispatch_entry* entries, uint32_t num_entries, uint32_t block_size,
+                                        char* log, uint32_t log_size, double* seconds) {
+    if (seconds) *seconds = 0.0;
+    if (log && log_size) log[0] = '\0';
+    if (block_size == 0) block_size = kDefaultBlockSize;
+    if (!pga_block_size_supported(block_size)) return PGA_ERR_INVALID_ARG;
+    std::unique_ptr<pga_context> ctx(new pga_context());   // host tables only: no device call
+    ctx->cfg.block_size = block_size;
+    std::string msg;
+    pga_status st = build_program(ctx.get(), procedures, num_procedures, entries, num_entries, false);
+    if (st == PGA_OK) {
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<char> code;
+        bool cached = false;
+        if (!pga_spec_compile(pga_spec_source(ctx->h_entries, ctx->count_geom, ctx->use_seed, block_size), code, msg,
+                              &cached))
+            st = PGA_ERR_UNSUPPORTED;
+        else if (seconds && !cached)
+            *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    } else {
+        msg = ctx->error;
+    }
+    if (log && log_size) {
+        const size_t m = std::min<size_t>(msg.size(), log_size - 1u);
+        std::memcpy(log, msg.data(), m);
+        log[m] = '\0';
+    }
+    return st;
+}
+
 PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t repeats, pga_tune_result* out) {
     NO_CAPTURE(ctx, "pga_autotune");
     if (!ctx || !out) return fail(ctx, PGA_ERR_INVALID_ARG, "null argument");
@@ -959,19 +1080,49 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
         }
         return PGA_OK;
     };
+    const bool saved_spec = ctx->spec_on;
+    const uint32_t saved_spec_blocks = ctx->spec_blocks;
     pga_tune_result res{};
     pga_status st = time_config(&res.initial_seconds);
     if (st) return st;
     res.technique = saved.technique;
-    res.block_size = saved.block_size;
-    res.grid_blocks = saved_blocks;
+    res.block_size = saved_spec ? ctx->spec_bs : saved.block_size;
+    res.grid_blocks = saved_spec ? saved_spec_blocks : saved_blocks;
+    res.specialized = saved_spec ? 1u : 0u;
     res.seconds = res.initial_seconds;
     res.max_level_parents = ctx->max_level_parents;
     res.max_level_children = ctx->max_level_children;
     res.candidates = 1;
     const uint32_t techniques[] = {PGA_TECHNIQUE_MEGAKERNEL, PGA_TECHNIQUE_KERNELS};
     const uint32_t sizes[] = {256u, 512u, 768u, 1024u};
+    ctx->spec_on = false;
+    if (ctx->spec_fn) {   // the specialised kernel (compiled by pga_specialize) at a full and a half grid
+        int per_cu = 0;
+        HIP_TRY(ctx, hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ctx->spec_fn, (int)ctx->spec_bs,
+                                                                        ctx->lds_bytes));
+        const uint32_t full = std::min((uint32_t)std::max(per_cu, 1), kMaxBlocksPerCu) * (uint32_t)ctx->num_cus;
+        const uint32_t grids[] = {full, std::max(full / 2u, (uint32_t)ctx->num_cus)};
+        ctx->cfg.technique = PGA_TECHNIQUE_MEGAKERNEL;
+        for (int g = 0; g < 2 && !st; ++g) {
+            if (g == 1 && grids[1] >= full) continue;
+            ctx->spec_on = true;
+            ctx->spec_blocks = grids[g];
+            double s = 0.0;
+            st = time_config(&s);
+            ++res.candidates;
+            if (!st && s < res.seconds) {
+                res.seconds = s;
+                res.technique = PGA_TECHNIQUE_MEGAKERNEL;
+                res.block_size = ctx->spec_bs;
+                res.grid_blocks = grids[g];
+                res.specialized = 1u;
+            }
+        }
+        ctx->spec_on = false;
+        ctx->cfg.technique = saved.technique;
+    }
     for (uint32_t tech : techniques) {
+        if (st) break;
         for (uint32_t bs : sizes) {
             uint32_t full = 0;
             st = resident_blocks(ctx, bs, &full);
@@ -1001,11 +1152,20 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
     if (st) {   // leave the context as it was
         ctx->cfg = saved;
         ctx->mega_blocks = saved_blocks;
+        ctx->spec_on = saved_spec;
+        ctx->spec_blocks = saved_spec_blocks;
         return st;
     }
     ctx->cfg.technique = res.technique;
-    ctx->cfg.block_size = res.block_size;
-    ctx->mega_blocks = res.grid_blocks;
+    if (res.specialized) {         // the interpreting kernel keeps its configuration for when it is back
+        ctx->cfg.block_size = saved.block_size;
+        ctx->mega_blocks = saved_blocks;
+        ctx->spec_on = true;
+        ctx->spec_blocks = res.grid_blocks;
+    } else {
+        ctx->cfg.block_size = res.block_size;
+        ctx->mega_blocks = res.grid_blocks;
+    }
     st = pga_execute_range(ctx, 0, num_axioms, 0, nullptr);   // the buffers hold the chosen run
     *out = res;
     return st;
@@ -1209,7 +1369,8 @@ PGA_API pga_status pga_capture_end(pga_context* ctx, pga_graph** out) {
         HIP_TRY(ctx, hipGraphKernelNodeGetParams(node, &kp));
         hipKernelNodeAttrValue v{};
         HIP_TRY(ctx, hipGraphKernelNodeGetAttribute(node, hipKernelNodeAttributeCooperative, &v));
-        if (!v.cooperative && pga_needs_coresidency(kp.func)) {   // the grid barrier needs every workgroup resident
+        const bool spec = ctx->spec_fn != nullptr && kp.func == (void*)ctx->spec_fn;
+        if (!v.cooperative && (spec || pga_needs_coresidency(kp.func))) {   // the grid barrier needs every workgroup resident
             v.cooperative = 1;
             HIP_TRY(ctx, hipGraphKernelNodeSetAttribute(node, hipKernelNodeAttributeCooperative, &v));
             HIP_TRY(ctx, hipGraphKernelNodeGetAttribute(node, hipKernelNodeAttributeCooperative, &v));
@@ -1233,8 +1394,9 @@ PGA_API pga_status pga_graph_launch(pga_context* ctx, pga_graph* g) {
         pga_status st = check_launch(ctx, l.first, l.count);
         if (st) return st;
         const pga::KArgs a = make_args(ctx, l.first, l.count);
+        const void* kernel = ctx->spec_on ? (const void*)ctx->spec_fn : nullptr;
         if (std::memcmp(&a, &l.args, sizeof(a)) != 0 || l.blocks != ctx->mega_blocks ||
-            l.block_size != ctx->cfg.block_size)
+            l.block_size != ctx->cfg.block_size || l.kernel != kernel)
             return fail(ctx, PGA_ERR_INVALID_ARG,
                         "the context changed since the capture (bindings, program layout or configuration); "
                         "capture again");

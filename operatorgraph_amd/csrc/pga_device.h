@@ -6,10 +6,12 @@
 // bit-identical to a host derivation of the same grammar.
 #pragma once
 
+#ifndef __HIPCC_RTC__   // kernels compiled at run time get the HIP built-ins from hipRTC
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
-#include "../../include/pga.h"
+#include "pga.h"
 
 namespace pga {
 

@@ -2,7 +2,9 @@
 // host translation units.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 
 #include "pga_ops.h"
 
@@ -75,6 +77,7 @@ constexpr uint32_t kSlotFloats = 20;    // adjoint slot of a child (80 bytes)
 
 } // namespace pga
 
+#ifndef __HIPCC_RTC__   // host launchers
 bool pga_block_size_supported(uint32_t block_size);   // 256, 512, 768 or 1024
 hipError_t pga_launch_reset(const pga::KArgs& a, int reset_terms, hipStream_t s);
 hipError_t pga_launch_level(const pga::KArgs& a, uint32_t level, uint32_t blocks, uint32_t block_size,
@@ -88,3 +91,4 @@ hipError_t pga_tape_occupancy(uint32_t lds_bytes, int* blocks_per_cu);
 hipError_t pga_launch_mega_tape(const pga::KArgs& a, uint32_t blocks, hipStream_t s);
 hipError_t pga_launch_adjoint(const pga::KArgs& a, const pga::AdjArgs& g, hipStream_t s);
 hipError_t pga_launch_grad_out(const double* acc, float* out, uint32_t n, hipStream_t s);
+#endif

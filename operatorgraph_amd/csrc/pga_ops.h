@@ -23,6 +23,13 @@ namespace pga {
 // chosen by a run-time index (successor k, inline parameter k, the aux words)
 // are read from the LDS entry by index: a select between register copies
 // would be turned into an indexed load from a stack copy of the entry.
+//
+// The operators below take the entry as a template parameter E: EntryV here
+// (the interpreting kernels, one copy of every operator), or a type whose
+// fields are compile-time constants (kernels specialised to one program,
+// where the same operator code folds to the entry at hand). E provides the
+// fields idx, w0, w1, pbase, sbase, pv0..pv2, sv0, the accessors below, and
+// pvk / svk / auxk / param for fields chosen by index.
 struct EntryV {
     uint32_t idx;     // entry index
     uint32_t w0;      // kind | shape << 8 | flags << 16 | nparams << 24
@@ -38,6 +45,12 @@ struct EntryV {
     __device__ __forceinline__ uint32_t inl() const { return (w1 >> 8) & 0xFFu; }
     __device__ __forceinline__ uint32_t tslot() const { return (w1 >> 16) & 0xFFu; }
     __device__ __forceinline__ uint32_t ndirect() const { return w1 >> 24; }
+    // inline constant parameter k < 3, successor k < 3, aux word k
+    __device__ __forceinline__ float pvk(const Ctx& c, uint32_t k) const { return c.prog.entries[idx].pv[k]; }
+    __device__ __forceinline__ int32_t svk(const Ctx& c, uint32_t k) const { return c.prog.entries[idx].sv[k]; }
+    __device__ __forceinline__ uint32_t auxk(const Ctx& c, uint32_t k) const { return c.prog.entries[idx].aux[k]; }
+    // parameter k (any k < nparams) from the program tables
+    __device__ __forceinline__ DevParam param(const Ctx& c, uint32_t k) const { return c.prog.params[pbase + k]; }
 };
 static_assert(sizeof(DevEntry) == 48, "DevEntry must be three 16-byte words");
 
@@ -52,8 +65,9 @@ __device__ __forceinline__ EntryV load_entry(const Ctx& c, uint32_t i) {
     e.sv0 = (int32_t)b.w;
     return e;
 }
-__device__ __forceinline__ uint32_t entry_aux(const Ctx& c, const EntryV& e, uint32_t k) {
-    return c.prog.entries[e.idx].aux[k];
+template <class E>
+__device__ __forceinline__ uint32_t entry_aux(const Ctx& c, const E& e, uint32_t k) {
+    return e.auxk(c, k);
 }
 
 // Shape type of entry i.
@@ -84,21 +98,24 @@ __device__ __forceinline__ void set_model(Rec& r, const float m[12]) {
 }
 
 // Parameter k of entry e for the shape r.
-__device__ __forceinline__ float P(const Ctx& c, const EntryV& e, uint32_t k, const Rec& r, uint32_t axiom,
+template <class E>
+__device__ __forceinline__ float P(const Ctx& c, const E& e, uint32_t k, const Rec& r, uint32_t axiom,
                                    uint32_t path, uint32_t depth) {
-    if (k < 3u && ((e.inl() >> k) & 1u)) return c.prog.entries[e.idx].pv[k];
-    return eval_param(c, c.prog.params[e.pbase + k], r, axiom, path, depth);
+    if (k < 3u && ((e.inl() >> k) & 1u)) return e.pvk(c, k);
+    return eval_param(c, e.param(c, k), r, axiom, path, depth);
 }
 
 // Successor k of an entry (the first three are held in the entry).
-__device__ __forceinline__ int32_t succ_of(const Ctx& c, const EntryV& e, uint32_t k) {
-    if (k < 3u) return c.prog.entries[e.idx].sv[k];
+template <class E>
+__device__ __forceinline__ int32_t succ_of(const Ctx& c, const E& e, uint32_t k) {
+    if (k < 3u) return e.svk(c, k);
     return c.prog.succ[e.sbase + k];
 }
 
 // The first min(nparams, 3) parameters; constants come from the entry, the
 // others through one evaluation site.
-__device__ __forceinline__ void eval3(const Ctx& c, const EntryV& e, const Rec& r, float& v0, float& v1,
+template <class E>
+__device__ __forceinline__ void eval3(const Ctx& c, const E& e, const Rec& r, float& v0, float& v1,
                                       float& v2) {
     const uint32_t np = e.nparams() < 3u ? e.nparams() : 3u;
     v0 = e.pv0; v1 = e.pv1; v2 = e.pv2;
@@ -108,7 +125,7 @@ __device__ __forceinline__ void eval3(const Ctx& c, const EntryV& e, const Rec& 
 #pragma unroll 1
     for (uint32_t k = 0; k < np; ++k) {
         if (!((todo >> k) & 1u)) continue;
-        float x = eval_param(c, c.prog.params[e.pbase + k], r, ax, pa, de);
+        float x = eval_param(c, e.param(c, k), r, ax, pa, de);
         v0 = k == 0 ? x : v0;
         v1 = k == 1 ? x : v1;
         v2 = k == 2 ? x : v2;
@@ -118,7 +135,8 @@ __device__ __forceinline__ void eval3(const Ctx& c, const EntryV& e, const Rec& 
 // RandomRule<DynParams> (RandomRule.cuh:121-147) and the static form
 // (RandomRule.cuh:20-111: right-fold sum, zero-probability branches skipped,
 // last branch unconditional). Returns the chosen branch or -1.
-__device__ __forceinline__ int32_t random_choice(const Ctx& c, const EntryV& e, float chance, const Rec& rr) {
+template <class E>
+__device__ __forceinline__ int32_t random_choice(const Ctx& c, const E& e, float chance, const Rec& rr) {
     const uint32_t n = e.nparams();
     const bool is_static = (e.flags() & EF_STATIC) != 0;
     const uint32_t axiom = rr.axiom, path = rr.path, depth = rec_depth(rr);
@@ -143,7 +161,8 @@ __device__ __forceinline__ int32_t random_choice(const Ctx& c, const EntryV& e, 
 
 // Applies one single-child operator in place. Returns the successor entry
 // (or -1 when nothing is dispatched) and the child ordinal.
-__device__ __forceinline__ int32_t apply_single(const Ctx& c, const EntryV& e, Rec& r, uint32_t* ordinal) {
+template <class E>
+__device__ __forceinline__ int32_t apply_single(const Ctx& c, const E& e, Rec& r, uint32_t* ordinal) {
     const uint32_t k = e.kind();
     *ordinal = 0;
     if (!c.geom && (k == K_TRANSLATE || k == K_ROTATE)) return e.sv0;   // model only
@@ -232,7 +251,8 @@ struct Fan {
     uint32_t reps;       // Repeat repetitions
 };
 
-__device__ __forceinline__ void fan_init(const Ctx& c, const Rec& r, const EntryV& e, Fan& f) {
+template <class E>
+__device__ __forceinline__ void fan_init(const Ctx& c, const Rec& r, const E& e, Fan& f) {
     const uint32_t kind = e.kind();
     f.n = 0;
     f.reps = 0;
@@ -336,13 +356,14 @@ __device__ __forceinline__ void axis_vec(uint32_t axis, float o, float& tx, floa
 // sequential running offsets (Subdivide.cuh:49-69, 218-236; Repeat.cuh:
 // 146-197). Returns false when the child is not dispatched (a zero factor of
 // the static Subdivide form).
-__device__ __forceinline__ bool fan_geom(const Ctx& c, const Rec& r, const EntryV& e, Fan& f, uint32_t j, float& o,
+template <class E>
+__device__ __forceinline__ bool fan_geom(const Ctx& c, const Rec& r, const E& e, Fan& f, uint32_t j, float& o,
                                          float& ns, float* fac_out = nullptr) {
     const uint32_t kind = e.kind();
     o = 0.0f;
     ns = 0.0f;
     if (kind == K_SUBDIV) {
-        const DevParam p = c.prog.params[e.pbase + 1 + j];
+        const DevParam p = e.param(c, 1u + j);
         const bool is_static = (e.flags() & EF_STATIC) != 0;
         float fac;
         if (p.type == PGA_PT_SCALAR) {
@@ -389,7 +410,8 @@ __device__ __forceinline__ bool fan_geom(const Ctx& c, const Rec& r, const Entry
 // Child j of a fan-out parent from its geometry (o, ns along `axis`).
 // Repeat pieces use seed modifier j + 1 (1 for the first piece, i for the
 // middle pieces i = 2 .. reps-1, reps for the last, Repeat.cuh:156-184).
-__device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, const EntryV& e, uint32_t j,
+template <class E>
+__device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, const E& e, uint32_t j,
                                             uint32_t axis, float o, float ns, Rec& ch) {
     const uint32_t kind = e.kind();
     ch = r;

@@ -1,0 +1,283 @@
+// pga_spec.h - derivation kernels specialised to one program.
+//
+// The library writes, for the program loaded by pga_initialize, one type per
+// dispatch entry whose fields are compile-time constants (operator kind,
+// flags, inline parameters, successors), and a dispatcher from an entry
+// index to those types; hipRTC compiles that source together with this
+// header into a persistent kernel for the program. The operators are the
+// same templates the interpreting kernels run (pga_ops.h); given a constant
+// entry they fold to the operator at hand: no operator-kind dispatch, no
+// table reads for constant parameters or successors, the constant parameter
+// values as immediates.
+//
+// The state machine differs from the interpreter's in two ways:
+//  * a lane runs its shape until it needs the wave - a terminal to emit or a
+//    fan-out to start - or is done: a single-operator chain, a leaf fan-out's
+//    child and that child's chain, or all descriptors of a started fan-out,
+//    take one trip round the wave loop instead of one trip per operator;
+//  * every step reaches its entry through `SpecVisit`: the lanes that share
+//    an entry run together under a wave-uniform switch (one pass per distinct
+//    entry among the active lanes), so a level whose children share an entry
+//    runs that entry's code once per wave with scalar branches.
+// Output slots, LDS staging, the count/scan/write structure and the grid
+// barrier are the ones of pga_level.h, so the instances are those of the
+// interpreting kernels (the same records; the order within a claim may
+// differ).
+#pragma once
+
+#include "pga_level.h"
+
+namespace pga {
+
+// Fields every constant entry type shares, from D's constants idx, w0, w1,
+// pbase, sbase, pv0..pv2, sv0..sv2, aux0, aux1.
+template <class D>
+struct SpecEntry {
+    __device__ __forceinline__ static constexpr uint32_t kind() { return D::w0 & 0xFFu; }
+    __device__ __forceinline__ static constexpr uint32_t shape() { return (D::w0 >> 8) & 0xFFu; }
+    __device__ __forceinline__ static constexpr uint32_t flags() { return (D::w0 >> 16) & 0xFFu; }
+    __device__ __forceinline__ static constexpr uint32_t nparams() { return D::w0 >> 24; }
+    __device__ __forceinline__ static constexpr uint32_t nsucc() { return D::w1 & 0xFFu; }
+    __device__ __forceinline__ static constexpr uint32_t inl() { return (D::w1 >> 8) & 0xFFu; }
+    __device__ __forceinline__ static constexpr uint32_t tslot() { return (D::w1 >> 16) & 0xFFu; }
+    __device__ __forceinline__ static constexpr uint32_t ndirect() { return D::w1 >> 24; }
+    __device__ __forceinline__ static float pvk(const Ctx&, uint32_t k) {
+        return k == 0u ? D::pv0 : (k == 1u ? D::pv1 : D::pv2);
+    }
+    __device__ __forceinline__ static int32_t svk(const Ctx&, uint32_t k) {
+        return k == 0u ? D::sv0 : (k == 1u ? D::sv1 : D::sv2);
+    }
+    __device__ __forceinline__ static uint32_t auxk(const Ctx&, uint32_t k) { return k == 0u ? D::aux0 : D::aux1; }
+    __device__ __forceinline__ static DevParam param(const Ctx& c, uint32_t k) { return c.prog.params[D::pbase + k]; }
+};
+
+// The entry type for an index outside the program (a dispatcher's default):
+// its kind matches no operator.
+constexpr uint32_t K_NONE = 0xFFu;
+struct SpecNoEntry : SpecEntry<SpecNoEntry> {
+    static constexpr uint32_t idx = 0u, w0 = K_NONE, w1 = 0u, pbase = 0u, sbase = 0u, aux0 = 0u, aux1 = 0u;
+    static constexpr float pv0 = 0.0f, pv1 = 0.0f, pv2 = 0.0f;
+    static constexpr int32_t sv0 = -1, sv1 = -1, sv2 = -1;
+};
+
+// Calls f(entry) for each lane with the entry of index `i` as its constant
+// type: one pass per distinct index among the active lanes, each under a
+// wave-uniform switch (S::dispatch).
+template <class S>
+struct SpecVisit {
+    template <class F>
+    __device__ __forceinline__ void operator()(const Ctx& c, uint32_t i, F&& f) const {
+        bool pending = true;
+#pragma unroll 1
+        while (pending) {
+            const uint32_t u = __builtin_amdgcn_readfirstlane(i);
+            if (i == u) {
+                S::dispatch(c, u, f);
+                pending = false;
+            }
+        }
+    }
+};
+
+// One pass (WR: the write pass) over the children of a level.
+template <class S, bool WR>
+__device__ __forceinline__ void spec_pass(const KArgs& a, LevelLDS& L, float4* PW, uint32_t level, uint32_t n,
+                                          uint32_t wave, uint32_t nwaves) {
+    Ctx c = make_ctx<false>(a, L, level, WR);
+    c.geom = WR ? 1u : S::kCountGeom;     // constants: the count pass drops the unread arithmetic
+    c.seed = S::kUseSeed;
+    const SpecVisit<S> visit;
+    const Rec* par_in = (level & 1u) ? a.par0 : a.par1;
+    const Desc* desc_in = (level & 1u) ? a.desc0 : a.desc1;
+    f4* ST = reinterpret_cast<f4*>(PW + 5u * blockDim.x) + wave_in_block() * 256u;
+#pragma unroll 1
+    for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u) {
+        const uint32_t i = base + lane_id();
+        Rec cur;
+        const bool valid = i < n && load_child(a, c, par_in, desc_in, WR, level, i, cur, visit);
+        uint32_t mode = valid ? M_SETTLE : M_DONE;
+        bool leaf = false, self = false;
+        uint32_t steps = 0, j = 0, dslot = 0, pidx = ~0u, fe = 0;
+        uint32_t skipm = 0, nd = 0;
+        Fan f;
+        f.n = 0; f.axis = 0; f.total = 0.0f; f.r2a = 0.0f; f.off = 0.0f;
+        f.e1 = f.e2 = f.e3 = 0.0f; f.prev_ns = 0.0f; f.prev_o = 0.0f; f.reps = 0;
+#pragma unroll 1
+        while (__ballot(mode != M_DONE) != 0ull) {
+            bool emit = false, start = false, direct = false;
+            uint32_t slot = 0;
+            // this lane alone, until it needs the wave or is done
+#pragma unroll 1
+            while (mode != M_DONE && !emit && !start && !direct) {
+                visit(c, mode == M_FAN ? fe : rec_entry(cur), [&](const auto& e) {
+                    const uint32_t k = e.kind();
+                    if (k == K_NONE) {          // entries are validated on load: cannot happen
+                        raise(c, PGA_ERR_BAD_PROGRAM);
+                        mode = M_DONE;
+                        return;
+                    }
+                    if (mode == M_FAN) {
+                        // child j of this lane's fan-out (entry e, record in PW), or
+                        // the record itself continuing on the next level (self)
+                        if (j >= f.n) { mode = M_DONE; return; }
+                        float o = 0.0f, ns = 0.0f;
+                        const Rec par = par_load(PW);
+                        const bool keep = (self || !is_fanout(k)) ? true : fan_geom(c, par, e, f, j, o, ns);
+                        const bool skip = j < 32u ? ((skipm >> j) & 1u) != 0u : skipm == 0xFFFFFFFFu;
+                        if (leaf && is_fanout(k)) {
+                            if (keep && !skip && build_child(c, par, e, j, f.axis, o, ns, cur)) {
+                                mode = M_SETTLE;
+                                steps = 0;
+                            }
+                        } else if (!skip) {     // write pass only: the count pass claimed nd slots
+                            const uint32_t jx = pidx == ~0u ? kDescNone
+                                              : self        ? kDescSelf
+                                              : keep        ? (j | (f.axis << 30))
+                                                            : kDescNone;
+                            if (dslot < c.desc_cap)
+                                *reinterpret_cast<uint4*>(c.desc_out + dslot) =
+                                    make_uint4(pidx, jx, __float_as_uint(o), __float_as_uint(ns));
+                            else
+                                raise(c, PGA_ERR_OVERFLOW_QUEUE);
+                            ++dslot;
+                        }
+                        ++j;
+                        return;
+                    }
+                    // SETTLE: the operator of the lane's current record
+                    const uint32_t back = leaf ? (j < f.n ? M_FAN : M_DONE) : M_DONE;
+                    if (k == K_GENERATE) {
+                        slot = e.tslot();
+                        if (slot == 0xFFu) {    // terminal index computed per shape
+                            const uint32_t t = f2u_rd(P(c, e, 0, cur, cur.axiom, cur.path, rec_depth(cur)));
+                            slot = t < PGA_MAX_TERMINALS ? c.prog.term_slot[t] : 0xFFu;
+                        }
+                        if (slot >= c.prog.nslots) { raise(c, PGA_ERR_BAD_PROGRAM); slot = 0; }
+                        emit = true;
+                        mode = back;
+                    } else if (k == K_DISCARD) {
+                        mode = back;
+                    } else if (is_fanout(k)) {
+                        if (leaf) {             // the load-time leaf analysis excludes fan-outs below a leaf
+                            raise(c, PGA_ERR_BAD_PROGRAM);
+                            mode = back;
+                            return;
+                        }
+                        par_store(PW, cur);
+                        fe = e.idx;
+                        fan_init(c, cur, e, f);
+                        j = 0;
+                        leaf = (e.flags() & EF_LEAF) != 0;
+                        start = !leaf;
+                        const uint32_t sk = e.auxk(c, 0);
+                        if (k == K_REPEAT) {
+                            skipm = (sk & 1u) ? 0xFFFFFFFFu : 0u;
+                            nd = skipm ? 0u : f.n;
+                        } else {
+                            skipm = sk;
+                            nd = f.n - (uint32_t)__popc(skipm & (f.n < 32u ? (1u << f.n) - 1u : 0xFFFFFFFFu));
+                        }
+                        if (leaf && (e.flags() & EF_DIRECT)) {
+                            if (WR) {                   // the children in lock step with the wave's other ones
+                                direct = true;
+                            } else {
+                                count_direct(c, L, e, f);
+                            }
+                            mode = M_DONE;
+                        } else {
+                            mode = (leaf || WR) ? M_FAN : M_DONE;
+                        }
+                    } else if (steps >= c.max_chain) {
+                        if (leaf) {             // the leaf analysis bounds every chain below a leaf
+                            raise(c, PGA_ERR_BAD_PROGRAM);
+                            mode = back;
+                            return;
+                        }
+                        // over-long chain: continues on the next level
+                        par_store(PW, cur);
+                        fe = e.idx;
+                        self = true;
+                        f.n = 1;
+                        nd = 1;
+                        skipm = 0;
+                        j = 0;
+                        start = true;
+                        mode = WR ? M_FAN : M_DONE;
+                    } else {
+                        uint32_t ord;
+                        const int32_t next = apply_single(c, e, cur, &ord);
+                        if (!advance(c, cur, next, ord)) mode = back;
+                        ++steps;
+                    }
+                });
+            }
+            if (__ballot(emit) != 0ull) {
+                unsigned long long idx;
+                emit_terminal(c, L, ST, WR, emit, slot, cur, idx);
+            }
+            if (__ballot(start) != 0ull) start_fanout(c, L, WR, start, par_load(PW), nd, pidx, dslot);
+            if (__ballot(direct) != 0ull) {
+                // a direct lane is done with its own record (the parent is in
+                // PW): its children are built into `cur`
+#pragma unroll 1
+                for (uint32_t jj = 0; __ballot(direct && jj < f.n) != 0ull; ++jj) {
+                    uint32_t sl = 0;
+                    bool has = false;
+                    if (direct && jj < f.n)
+                        visit(c, fe, [&](const auto& e) {
+                            if (is_fanout(e.kind())) {
+                                const Rec par = par_load(PW);
+                                has = direct_child(c, par, e, f, skipm, jj, cur, sl);
+                            }
+                        });
+                    unsigned long long idx;
+                    emit_terminal(c, L, ST, true, has, sl, cur, idx);
+                }
+            }
+        }
+    }
+}
+
+// One derivation level: count pass, scan, write pass (as run_level of the
+// interpreting kernels).
+template <class S>
+__device__ __forceinline__ void spec_level(const KArgs& a, LevelLDS& L, float4* PW, uint32_t level, uint32_t n) {
+    const uint32_t nslots = a.prog.nslots, ncnt = nslots + 2u;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t w = wave_in_block();
+    stamp(a, level, 0);
+    for (uint32_t k = lane_id(); k < ncnt; k += 64u) L.cnt[w][k] = 0u;
+    spec_pass<S, false>(a, L, PW, level, n, wave, nwaves);
+    __syncthreads();
+    scan_counts(a, L, level);
+    spec_pass<S, true>(a, L, PW, level, n, wave, nwaves);
+    __syncthreads();
+    stamp(a, level, 3);
+    if (threadIdx.x == 0)
+        for (uint32_t k = 0; k < nslots; ++k) L.term_base[k] += L.level_total[k];
+    __syncthreads();
+}
+
+// The persistent derivation of every level (the MEGAKERNEL technique).
+template <class S, int BS>
+__device__ __forceinline__ void spec_mega(const KArgs& a) {
+    __shared__ LevelLDS L;
+    __shared__ float4 PW[9 * BS];   // parent records (5 words per lane), then instance staging (4)
+    stage_program(a);
+    init_bases(a, L);
+    uint32_t level = 0;
+    uint32_t n = a.n_axioms;
+    for (; level < a.max_levels && n != 0u; ++level) {
+        spec_level<S>(a, L, PW, level, n);
+        n = min(L.next_n, a.cap_desc);
+        grid_barrier(a.bar, gridDim.x);       // next level reads this level's records
+    }
+    store_bases(a, L);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (n != 0u) atomicOr(a.err, 1u << PGA_ERR_MAX_DEPTH);
+        a.desc_counts[a.max_levels + 1u] = level;
+    }
+}
+
+} // namespace pga

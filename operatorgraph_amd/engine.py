@@ -67,6 +67,9 @@ def load_library():
         "pga_set_timestamps": (C.c_int, [P, P, C.c_uint32]),
         "pga_timestamp_hz": (C.c_double, [P]),
         "pga_autotune": (C.c_int, [P, C.c_uint32, C.c_uint32, C.POINTER(abi.TuneResult)]),
+        "pga_specialize": (C.c_int, [P, C.c_uint32, C.c_int32, C.POINTER(C.c_double)]),
+        "pga_specialize_check": (C.c_int, [P, C.c_uint32, P, C.c_uint32, C.c_uint32, C.c_char_p, C.c_uint32,
+                                           C.POINTER(C.c_double)]),
         "pga_tape_reserve": (C.c_int, [P, C.c_uint32, C.c_uint32]),
         "pga_forward_tape": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_double)]),
         "pga_backward": (C.c_int, [P, C.POINTER(C.c_void_p * abi.MAX_TERMINALS), P, C.POINTER(C.c_double)]),
@@ -306,6 +309,14 @@ class Engine:
         self._check(self.L.pga_capture_end(self._h, C.byref(h)))
         g._h = h.value
 
+    def specialize(self, block_size=0, enable=True):
+        """Compiles the loaded program into its own derivation kernel
+        (pga_specialize) and runs it from now on; returns the compile time
+        in seconds (0 when this process compiled the program before)."""
+        sec = C.c_double(0.0)
+        self._check(self.L.pga_specialize(self._h, int(block_size), 1 if enable else 0, C.byref(sec)))
+        return sec.value
+
     def autotune(self, n=None, repeats=3):
         """Times every launch configuration on the bound axioms and keeps the
         fastest (pga_autotune); returns the chosen configuration."""
@@ -401,6 +412,18 @@ class Engine:
         count = min(count, buf.shape[0])
         host = buf[:count].cpu().numpy()
         return host.view(abi.INSTANCE_DTYPE).reshape(count)
+
+
+def specialize_check(program, block_size=0):
+    """Compiles a program's specialised kernel without a device
+    (pga_specialize_check); returns (status, compiler log, seconds)."""
+    L = load_library()
+    procs, entries = program.arrays()
+    log = C.create_string_buffer(1 << 16)
+    sec = C.c_double(0.0)
+    st = L.pga_specialize_check(C.addressof(procs), len(program.procedures), C.addressof(entries),
+                                len(program.table.entries), int(block_size), log, len(log), C.byref(sec))
+    return st, log.value.decode(errors="replace"), sec.value
 
 
 def derive(program, axioms, capacities, technique=abi.MEGAKERNEL, params=None, key=None, **cfg):

@@ -83,3 +83,22 @@ def test_engine_refuses_without_gpu():
         pytest.skip("a GPU is visible")
     with pytest.raises(engine.PGAError):
         engine.Engine()
+
+
+@pytest.mark.parametrize("name", ["facade", "menger_interpreter", "stochastic", "facade_params"])
+def test_specialised_kernel_compiles_without_a_device(name):
+    """The program written as constant entry types compiles at run time
+    (pga_specialize_check: validation, source, hipRTC for gfx950) on a
+    machine without a GPU."""
+    from operatorgraph_amd import engine, grammars
+    st, log, _ = engine.specialize_check(getattr(grammars, name)(), 512)
+    assert st == 0, log
+
+
+def test_specialised_kernel_check_reports_a_bad_program():
+    from operatorgraph_amd import abi, engine, grammars
+    from operatorgraph_amd.program import DispatchTable, Entry, Program, Scalar
+    bad = Program(grammars.INTERPRETER_PROCEDURES, DispatchTable([
+        Entry(grammars.op(abi.SUBDIV, abi.BOX), [Scalar(0), Scalar(1)], [(7, 0)])]))   # no entry 7
+    st, log, _ = engine.specialize_check(bad, 512)
+    assert st != 0 and log
