@@ -382,6 +382,8 @@ PGA_API pga_status pga_center_size_mse(const float* instances, const float* targ
 typedef struct pga_lbvh pga_lbvh;
 PGA_API pga_status pga_lbvh_create(uint32_t capacity, int32_t device, void* stream, pga_lbvh** out);
 PGA_API void pga_lbvh_destroy(pga_lbvh* bvh);
+/* message of the builder's last failed call; bvh = NULL: of the last failed
+ * pga_lbvh_build_noalloc on this thread */
 PGA_API const char* pga_lbvh_last_error(const pga_lbvh* bvh);
 /* box of each instance: the unit cube (planar = 0) or the unit square in the
  * z = 0 plane (planar = 1) under the instance's model rows */
@@ -394,6 +396,23 @@ PGA_API pga_status pga_lbvh_build(pga_lbvh* bvh, const float* aabbs, uint32_t n,
                                   int32_t* parent, int32_t* leaf_parent, float* node_aabb);
 /* device time of the last build (Morton codes to refit) */
 PGA_API pga_status pga_lbvh_seconds(pga_lbvh* bvh, double* seconds);
+
+/* The build of pga_lbvh_build with no builder and no allocation. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code recording a HIP graph, serving loops, harnesses that
+ * forbid hipMalloc). The sort and refit scratch is a device array inside the
+ * library, one per GPU, looked up with hipGetSymbolAddress on the first call
+ * on that GPU and held until the process ends (about 24.3 MiB): at most
+ * PGA_LBVH_NOALLOC_MAX_LEAVES leaves, larger n is refused with
+ * PGA_ERR_INVALID_ARG (message: pga_lbvh_last_error(NULL)). The build is
+ * enqueued on `stream` (a hipStream_t of the current device; NULL: the
+ * default stream) and the call returns without waiting. Calls that overlap
+ * on one GPU share the array and race: order them on one stream or wait in
+ * between. Outputs as pga_lbvh_build. */
+#define PGA_LBVH_NOALLOC_MAX_LEAVES (1u << 20)
+PGA_API pga_status pga_lbvh_build_noalloc(const float* aabbs, uint32_t n, const float scene_min[3],
+                                          const float scene_max[3], uint64_t* keys, int32_t* left, int32_t* right,
+                                          int32_t* parent, int32_t* leaf_parent, float* node_aabb, void* stream);
 
 /* -- device-count form (no counterpart in the reference, whose build takes a
  *    host count, GPUBVHConstructor.cuh:275-333): the leaf count stays in

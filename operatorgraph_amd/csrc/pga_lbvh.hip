@@ -38,6 +38,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "../../include/pga.h"
@@ -685,9 +686,83 @@ hipError_t enqueue_build(pga_lbvh* b, const float* aabbs, uint32_t n, const uint
     return hipGetLastError();
 }
 
+// ---- the allocation-free build (pga_lbvh_build_noalloc) ----
+// Its sort and refit scratch is this file-scope device array: two code and
+// two index arrays, the tickets and range ends of the internal nodes (4 bytes
+// per leaf each), the digit histograms and the scan partials, for at most
+// kNoAllocLeaves leaves (about 24.3 MiB, held by every GPU the library runs
+// on until the process ends). Host code uses the address that
+// hipGetSymbolAddress gives on each device, never the array's name.
+constexpr uint32_t kNoAllocLeaves = PGA_LBVH_NOALLOC_MAX_LEAVES;
+constexpr size_t kNoAllocHist = (size_t)256 * ((kNoAllocLeaves + kTile - 1u) / kTile);
+constexpr size_t kNoAllocPart = (kNoAllocHist + kScanTile - 1u) / kScanTile + 1u;
+constexpr size_t kNoAllocWords = 6u * (size_t)kNoAllocLeaves + kNoAllocHist + kNoAllocPart;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch[kMaxDevices] = {nullptr};   // the array's address on each device
+thread_local std::string g_noalloc_error;      // message of the last failed call on this thread
+
+pga_status noalloc_fail(pga_status st, const std::string& m) {
+    g_noalloc_error = m;
+    return st;
+}
+
+// The array's address on the current device, looked up once per device.
+pga_status scratch_on_device(uint32_t** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return noalloc_fail(PGA_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return noalloc_fail(PGA_ERR_UNSUPPORTED, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return noalloc_fail(PGA_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = static_cast<uint32_t*>(p);
+    }
+    *out = g_scratch[dev];
+    return PGA_OK;
+}
+
 } // namespace
 
 extern "C" {
+
+PGA_API pga_status pga_lbvh_build_noalloc(const float* aabbs, uint32_t n, const float scene_min[3],
+                                          const float scene_max[3], uint64_t* keys, int32_t* left, int32_t* right,
+                                          int32_t* parent, int32_t* leaf_parent, float* node_aabb, void* stream) {
+    if (!aabbs || !scene_min || !scene_max || !keys || !left || !right || !parent || !leaf_parent || !node_aabb)
+        return noalloc_fail(PGA_ERR_INVALID_ARG, "null argument");
+    if (n < 2u || n > kNoAllocLeaves)
+        return noalloc_fail(PGA_ERR_INVALID_ARG, "pga_lbvh_build_noalloc: need 2 <= n <= " +
+                                                     std::to_string(kNoAllocLeaves) + " leaves, got " +
+                                                     std::to_string(n));
+    const float ex[3] = {scene_max[0] - scene_min[0], scene_max[1] - scene_min[1], scene_max[2] - scene_min[2]};
+    if (!(ex[0] > 0.0f) || !(ex[1] > 0.0f) || !(ex[2] > 0.0f)) return noalloc_fail(PGA_ERR_INVALID_ARG, "empty scene box");
+    uint32_t* base = nullptr;
+    pga_status st = scratch_on_device(&base);
+    if (st) return st;
+    // a builder view of the array, sized for n leaves (no events, no allocation)
+    pga_lbvh view;
+    view.capacity = n;
+    view.stream = (hipStream_t)stream;
+    const size_t N = kNoAllocLeaves;
+    view.code[0] = base;
+    view.code[1] = base + N;
+    view.idx[0] = base + 2u * N;
+    view.idx[1] = base + 3u * N;
+    view.ticket = base + 4u * N;
+    view.other_end = reinterpret_cast<int32_t*>(base + 5u * N);
+    view.hist = base + 6u * N;
+    view.part = base + 6u * N + kNoAllocHist;
+    const hipError_t e = enqueue_build(&view, aabbs, n, nullptr, scene_min, ex, nullptr, keys, left, right, parent,
+                                       leaf_parent, node_aabb);
+    if (e != hipSuccess) return noalloc_fail(PGA_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
+    return PGA_OK;
+}
 
 PGA_API pga_status pga_lbvh_create(uint32_t capacity, int32_t device, void* stream, pga_lbvh** out) {
     if (!out || capacity < 2u || capacity > 0x7FFFFFFFu) return PGA_ERR_INVALID_ARG;
@@ -726,7 +801,7 @@ PGA_API void pga_lbvh_destroy(pga_lbvh* b) {
     delete b;
 }
 
-PGA_API const char* pga_lbvh_last_error(const pga_lbvh* b) { return b ? b->error.c_str() : ""; }
+PGA_API const char* pga_lbvh_last_error(const pga_lbvh* b) { return b ? b->error.c_str() : g_noalloc_error.c_str(); }
 
 PGA_API pga_status pga_lbvh_instance_aabbs(pga_lbvh* b, const pga_instance* instances, uint32_t n, int32_t planar,
                                            float* aabbs) {

@@ -24,7 +24,31 @@ _SIGS = {
     "pga_lbvh_build_device": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float * 3),
                                         C.POINTER(C.c_float * 3)] + [C.c_void_p] * 6),
     "pga_lbvh_leaf_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]),
+    "pga_lbvh_build_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float * 3),
+                                         C.POINTER(C.c_float * 3)] + [C.c_void_p] * 7),
 }
+
+NOALLOC_MAX_LEAVES = 1 << 20   # PGA_LBVH_NOALLOC_MAX_LEAVES
+
+
+def build_noalloc(aabbs, scene_min, scene_max, out, stream=None):
+    """pga_lbvh_build_noalloc: the build into the caller's device arrays
+    `out` (keys, left, right, parent, leaf_parent, node_aabb, as from
+    LBVH.alloc() or LBVH.build()) with the library's own per-device scratch;
+    nothing is allocated. Enqueued on `stream` (torch's current stream when
+    None); the caller synchronises before reading the outputs."""
+    import torch
+    L = _lib()
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    smin = (C.c_float * 3)(*[float(v) for v in scene_min])
+    smax = (C.c_float * 3)(*[float(v) for v in scene_max])
+    ptr = lambda t: C.c_void_p(t.data_ptr())
+    st = L.pga_lbvh_build_noalloc(ptr(aabbs), int(aabbs.shape[0]), C.byref(smin), C.byref(smax), ptr(out["keys"]),
+                                  ptr(out["left"]), ptr(out["right"]), ptr(out["parent"]), ptr(out["leaf_parent"]),
+                                  ptr(out["node_aabb"]), C.c_void_p(stream))
+    if st != abi.OK:
+        raise PGAError(st, L.pga_lbvh_last_error(None).decode())
 
 
 def _lib():
