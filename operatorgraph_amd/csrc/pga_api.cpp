@@ -184,15 +184,22 @@ bool is_fan(uint8_t k) {
 }
 
 // Checks an expression parameter (Parameters.cuh:563-664): length, operand
-// tags, integral operation codes and attribute indices, a stack that never
-// exceeds four entries nor underflows.
-bool check_exp(const pga_dispatch_parameter& p, std::string& why) {
+// tags, operation codes and attribute indices in range, a stack that never
+// exceeds four entries nor underflows. Writes the words to upload to `out`
+// (length word first): operation codes and attribute indices are stored
+// rounded, as the reference reads them (round(x), Parameters.cuh:615 and
+// 632), because generated tables print the packed words with six digits and
+// carry values such as 12.99997 for attribute 13.
+bool check_exp(const pga_dispatch_parameter& p, std::string& why, std::vector<uint32_t>& out) {
     float lenf = p.value[0];
     if (!(lenf >= 1.0f) || lenf != std::floor(lenf) || lenf > (float)(PGA_MAX_NUM_PARAMETER_VALUES - 1)) {
         why = "bad expression length";
         return false;
     }
     uint32_t len = (uint32_t)lenf;
+    uint32_t word;
+    std::memcpy(&word, &p.value[0], 4);
+    out.push_back(word);
     int depth = 0;
     for (uint32_t i = 1; i <= len; ++i) {
         uint32_t raw;
@@ -201,17 +208,25 @@ bool check_exp(const pga_dispatch_parameter& p, std::string& why) {
         uint32_t xb = raw << 2;
         float x;
         std::memcpy(&x, &xb, 4);
-        if (tag == PGA_ORT_OP) {
-            if (x != std::floor(x) || x < 0.0f || x > 11.0f) { why = "bad expression operation"; return false; }
-            if (depth < 2) { why = "expression stack underflow"; return false; }
-            depth -= 1;
-        } else {
-            if (tag == PGA_ORT_SHAPE_ATTR && (x != std::floor(x) || x < 0.0f || x > 16.0f)) {
-                why = "bad expression shape attribute";
+        if (tag == PGA_ORT_OP || tag == PGA_ORT_SHAPE_ATTR) {
+            const float r = std::round(x);
+            const float hi = tag == PGA_ORT_OP ? 11.0f : 16.0f;
+            if (!(r >= 0.0f && r <= hi)) {
+                why = tag == PGA_ORT_OP ? "bad expression operation" : "bad expression shape attribute";
                 return false;
             }
-            if (++depth > PGA_MAX_EXP_STACK) { why = "expression stack overflow"; return false; }
+            uint32_t rb;
+            std::memcpy(&rb, &r, 4);
+            raw = (tag << 30) | (rb >> 2);   // integral values keep their two lowest mantissa bits at zero
         }
+        if (tag == PGA_ORT_OP) {
+            if (depth < 2) { why = "expression stack underflow"; return false; }
+            depth -= 1;
+        } else if (++depth > PGA_MAX_EXP_STACK) {
+            why = "expression stack overflow";
+            return false;
+        }
+        out.push_back(raw);
     }
     if (depth < 1) { why = "empty expression"; return false; }
     return true;
@@ -304,16 +319,10 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
                 break;
             case PGA_PT_EXP: {
                 std::string why;
-                if (!check_exp(q, why)) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + why);
+                uint32_t off = (uint32_t)Ex.size();
+                if (!check_exp(q, why, Ex)) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + why);
                 count_geom = true;   // conservatively: an expression may read the model matrix
                 use_seed = true;     // ... and the seed
-                uint32_t off = (uint32_t)Ex.size();
-                uint32_t len = (uint32_t)q.value[0];
-                for (uint32_t v = 0; v <= len; ++v) {
-                    uint32_t raw;
-                    std::memcpy(&raw, &q.value[v], 4);
-                    Ex.push_back(raw);
-                }
                 std::memcpy(&dp.a, &off, 4);
                 break;
             }

@@ -2,7 +2,11 @@
 //
 // pga_spec_source writes the program as C++: one type per dispatch entry with
 // the entry's fields as compile-time constants and a dispatcher from entry
-// index to type (pga_spec.h runs the derivation over them). pga_spec_compile
+// index to type (pga_spec.h runs the derivation over them). Programs of more
+// than kSpecConstantEntriesMax entries are written as groups of entries that
+// share their header words (operator, shape, counts, flags) instead: the
+// compile time grows with the code, and a grammar of a few hundred entries
+// has a few dozen operator forms. pga_spec_compile
 // compiles that source with hipRTC for gfx950 with the flags of the static
 // build (no FP contraction, correctly rounded division, no packed FP32), so
 // the specialised kernel computes the interpreter's instance records bit for
@@ -48,6 +52,9 @@ std::string hex32(uint32_t v) {
     return b;
 }
 
+// Programs up to this many entries get one constant type per entry.
+constexpr size_t kSpecConstantEntriesMax = 32;
+
 std::string fbits(float f) {
     uint32_t u;
     std::memcpy(&u, &f, 4);
@@ -58,25 +65,44 @@ std::string fbits(float f) {
 
 std::string pga_spec_source(const std::vector<pga::DevEntry>& entries, uint32_t count_geom, uint32_t use_seed,
                             uint32_t block_size) {
+    const size_t n = entries.size();
+    std::vector<uint64_t> key(n);
+    for (size_t i = 0; i < n; ++i) {
+        const pga::DevEntry& d = entries[i];
+        const uint32_t w0 = (uint32_t)d.kind | ((uint32_t)d.shape << 8) | ((uint32_t)d.flags << 16) |
+                            ((uint32_t)d.nparams << 24);
+        const uint32_t w1 = (uint32_t)d.nsucc | ((uint32_t)d.inl << 8) | ((uint32_t)d.tslot << 16) |
+                            ((uint32_t)d.ndirect << 24);
+        key[i] = ((uint64_t)w1 << 32) | w0;
+    }
+    // entries per group key, in order of first appearance
+    std::vector<uint64_t> groups;
+    std::unordered_map<uint64_t, std::vector<size_t>> members;
+    for (size_t i = 0; i < n; ++i) {
+        auto& m = members[key[i]];
+        if (m.empty()) groups.push_back(key[i]);
+        m.push_back(i);
+    }
+    const bool grouped = n > kSpecConstantEntriesMax;
+
     std::string s;
-    s.reserve(4096 + entries.size() * 400);
+    s.reserve(4096 + n * 400);
     s += "// derivation kernel specialised to one program (written by pga_spec_source)\n";
     s += "typedef __hip_internal::int8_t int8_t;\ntypedef __hip_internal::uint8_t uint8_t;\n"
          "typedef __hip_internal::int16_t int16_t;\ntypedef __hip_internal::uint16_t uint16_t;\n"
          "typedef __hip_internal::int32_t int32_t;\ntypedef __hip_internal::uint32_t uint32_t;\n"
          "typedef __hip_internal::int64_t int64_t;\ntypedef __hip_internal::uint64_t uint64_t;\n";
     s += "#include \"pga_spec.h\"\nnamespace pga {\nnamespace gen {\n";
-    for (size_t i = 0; i < entries.size(); ++i) {
+    // one constant type per entry (per entry alone in its group, when grouped)
+    for (size_t i = 0; i < n; ++i) {
+        if (grouped && members[key[i]].size() > 1) continue;
         const pga::DevEntry& d = entries[i];
-        const uint32_t w0 = (uint32_t)d.kind | ((uint32_t)d.shape << 8) | ((uint32_t)d.flags << 16) |
-                            ((uint32_t)d.nparams << 24);
-        const uint32_t w1 = (uint32_t)d.nsucc | ((uint32_t)d.inl << 8) | ((uint32_t)d.tslot << 16) |
-                            ((uint32_t)d.ndirect << 24);
-        const std::string n = "E" + std::to_string(i);
-        s += "struct " + n + " : SpecEntry<" + n + "> {\n";
-        s += "  static constexpr uint32_t idx = " + std::to_string(i) + "u, w0 = " + hex32(w0) + ", w1 = " + hex32(w1) +
-             ", pbase = " + std::to_string(d.pbase) + "u, sbase = " + std::to_string(d.sbase) +
-             "u, aux0 = " + hex32(d.aux[0]) + ", aux1 = " + hex32(d.aux[1]) + ";\n";
+        const std::string nm = "E" + std::to_string(i);
+        s += "struct " + nm + " : SpecEntry<" + nm + "> {\n";
+        s += "  static constexpr uint32_t idx = " + std::to_string(i) + "u, w0 = " + hex32((uint32_t)key[i]) +
+             ", w1 = " + hex32((uint32_t)(key[i] >> 32)) + ", pbase = " + std::to_string(d.pbase) +
+             "u, sbase = " + std::to_string(d.sbase) + "u, aux0 = " + hex32(d.aux[0]) + ", aux1 = " +
+             hex32(d.aux[1]) + ";\n";
         s += "  static constexpr float pv0 = " + fbits(d.pv[0]) + ", pv1 = " + fbits(d.pv[1]) +
              ", pv2 = " + fbits(d.pv[2]) + ";\n";
         s += "  static constexpr int32_t sv0 = " + std::to_string(d.sv[0]) + ", sv1 = " + std::to_string(d.sv[1]) +
@@ -84,11 +110,26 @@ std::string pga_spec_source(const std::vector<pga::DevEntry>& entries, uint32_t 
     }
     s += "struct Prog {\n";
     s += "  static constexpr uint32_t kCountGeom = " + std::to_string(count_geom) +
-         "u, kUseSeed = " + std::to_string(use_seed) + "u;\n";
-    s += "  template <class F> __device__ __forceinline__ static void dispatch(const Ctx& c, uint32_t u, F& f) {\n";
+         "u, kUseSeed = " + std::to_string(use_seed) + "u, kEntries = " + std::to_string(n) + "u;\n";
+    s += std::string("  static constexpr bool kGrouped = ") + (grouped ? "true" : "false") + ";\n";
+    s += "  template <class F> __device__ __forceinline__ static void dispatch(const Ctx& c, uint64_t u, uint32_t i, "
+         "F& f) {\n";
     s += "    switch (u) {\n";
-    for (size_t i = 0; i < entries.size(); ++i)
-        s += "    case " + std::to_string(i) + "u: f(E" + std::to_string(i) + "{}); return;\n";
+    if (!grouped) {
+        for (size_t i = 0; i < n; ++i)
+            s += "    case " + std::to_string(i) + "ull: f(E" + std::to_string(i) + "{}); return;\n";
+    } else {
+        for (uint64_t g : groups) {
+            const auto& m = members[g];
+            char k[32];
+            std::snprintf(k, sizeof k, "0x%016llXull", (unsigned long long)g);
+            if (m.size() == 1)
+                s += std::string("    case ") + k + ": f(E" + std::to_string(m[0]) + "{}); return;\n";
+            else
+                s += std::string("    case ") + k + ": f(SpecGroupEntry<" + hex32((uint32_t)g) + ", " +
+                     hex32((uint32_t)(g >> 32)) + ">::at(c, i)); return;\n";
+        }
+    }
     s += "    default: f(SpecNoEntry{}); return;\n    }\n  }\n};\n";
     s += "}  // namespace gen\n}  // namespace pga\n";
     s += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(block_size) +

@@ -16,9 +16,10 @@
 //    child and that child's chain, or all descriptors of a started fan-out,
 //    take one trip round the wave loop instead of one trip per operator;
 //  * every step reaches its entry through `SpecVisit`: the lanes that share
-//    an entry run together under a wave-uniform switch (one pass per distinct
-//    entry among the active lanes), so a level whose children share an entry
-//    runs that entry's code once per wave with scalar branches.
+//    an entry (a group of entries, for large programs) run together under a
+//    wave-uniform switch (one pass per distinct entry among the active
+//    lanes), so a level whose children share an entry runs that entry's code
+//    once per wave with scalar branches.
 // Output slots, LDS staging, the count/scan/write structure and the grid
 // barrier are the ones of pga_level.h, so the instances are those of the
 // interpreting kernels (the same records; the order within a claim may
@@ -60,19 +61,75 @@ struct SpecNoEntry : SpecEntry<SpecNoEntry> {
     static constexpr int32_t sv0 = -1, sv1 = -1, sv2 = -1;
 };
 
-// Calls f(entry) for each lane with the entry of index `i` as its constant
-// type: one pass per distinct index among the active lanes, each under a
-// wave-uniform switch (S::dispatch).
+// An entry of a group: the entries of a program that share both header words
+// (operator kind, shape, flags, parameter and successor counts, inline mask,
+// terminal slot, direct fan-out width). Those fields are compile-time
+// constants; the index, parameter values, successors and aux words are this
+// lane's, read from the LDS copy of the program. Large programs are written
+// as groups so that the code to compile grows with the number of distinct
+// operator forms rather than with the number of entries.
+template <uint32_t W0, uint32_t W1>
+struct SpecGroupEntry {
+    static constexpr uint32_t w0 = W0, w1 = W1;
+    uint32_t idx, pbase, sbase;
+    float pv0, pv1, pv2;
+    int32_t sv0;
+    __device__ __forceinline__ static constexpr uint32_t kind() { return W0 & 0xFFu; }
+    __device__ __forceinline__ static constexpr uint32_t shape() { return (W0 >> 8) & 0xFFu; }
+    __device__ __forceinline__ static constexpr uint32_t flags() { return (W0 >> 16) & 0xFFu; }
+    __device__ __forceinline__ static constexpr uint32_t nparams() { return W0 >> 24; }
+    __device__ __forceinline__ static constexpr uint32_t nsucc() { return W1 & 0xFFu; }
+    __device__ __forceinline__ static constexpr uint32_t inl() { return (W1 >> 8) & 0xFFu; }
+    __device__ __forceinline__ static constexpr uint32_t tslot() { return (W1 >> 16) & 0xFFu; }
+    __device__ __forceinline__ static constexpr uint32_t ndirect() { return W1 >> 24; }
+    __device__ __forceinline__ float pvk(const Ctx&, uint32_t k) const { return k == 0u ? pv0 : (k == 1u ? pv1 : pv2); }
+    __device__ __forceinline__ int32_t svk(const Ctx& c, uint32_t k) const {
+        return k == 0u ? sv0 : c.prog.entries[idx].sv[k];
+    }
+    __device__ __forceinline__ uint32_t auxk(const Ctx& c, uint32_t k) const { return c.prog.entries[idx].aux[k]; }
+    __device__ __forceinline__ DevParam param(const Ctx& c, uint32_t k) const { return c.prog.params[pbase + k]; }
+    __device__ __forceinline__ static SpecGroupEntry at(const Ctx& c, uint32_t i) {
+        const uint4* q = reinterpret_cast<const uint4*>(c.prog.entries) + i * 3u;
+        const uint4 a = q[0], b = q[1];
+        SpecGroupEntry e;
+        e.idx = i;
+        e.pbase = a.z;
+        e.sbase = a.w;
+        e.pv0 = __uint_as_float(b.x);
+        e.pv1 = __uint_as_float(b.y);
+        e.pv2 = __uint_as_float(b.z);
+        e.sv0 = (int32_t)b.w;
+        return e;
+    }
+};
+
+// The dispatch key of entry i: its index when every entry has a type of its
+// own, its two header words when the program is written as groups (an index
+// outside the program gets K_NONE, which no group has).
+template <class S>
+__device__ __forceinline__ uint64_t spec_key(const Ctx& c, uint32_t i) {
+    if (!S::kGrouped) return i;
+    if (i >= S::kEntries) return K_NONE;
+    const uint2 w = *reinterpret_cast<const uint2*>(c.prog.entries + i);
+    return ((uint64_t)w.y << 32) | w.x;
+}
+
+// Calls f(entry) for each lane with the entry of index `i` as its type: one
+// pass per distinct key among the active lanes, each under a wave-uniform
+// switch (S::dispatch).
 template <class S>
 struct SpecVisit {
     template <class F>
     __device__ __forceinline__ void operator()(const Ctx& c, uint32_t i, F&& f) const {
+        const uint64_t key = spec_key<S>(c, i);
         bool pending = true;
 #pragma unroll 1
         while (pending) {
-            const uint32_t u = __builtin_amdgcn_readfirstlane(i);
-            if (i == u) {
-                S::dispatch(c, u, f);
+            const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)key);
+            const uint32_t hi = S::kGrouped ? __builtin_amdgcn_readfirstlane((uint32_t)(key >> 32)) : 0u;
+            const uint64_t u = ((uint64_t)hi << 32) | lo;
+            if (key == u) {
+                S::dispatch(c, u, i, f);
                 pending = false;
             }
         }

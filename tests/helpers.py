@@ -101,3 +101,71 @@ TREE_COUNTS = {
     "3d_tree_8_3": ((3 ** 9 - 1) // 2, 2 * 3 ** 8),
     "3d_tree_12_2": (2 ** 12 - 1, 3 * 2 ** 11),
 }
+
+
+# ---- the reference's compiled test-runner scenes (tests/golden/scene_tables.json) ----
+SCENE_TABLES = os.path.join(os.path.dirname(__file__), "golden", "scene_tables.json")
+
+# MengerSponge.cuh:55-59 sizes its axiom 3^(attribute % MaxIterations); the
+# scene's MaxIterations is 5.
+MENGER_SCENE_MAX_ITERATIONS = 5
+
+
+def scene_fixture():
+    import json
+    with open(SCENE_TABLES, encoding="utf-8") as f:
+        return json.load(f)
+
+
+class SceneProgram:
+    """A scene's lowered procedure and entry arrays, in the shape
+    oracle.derive and Engine.load take (arrays(), procedures, table.entries)."""
+
+    def __init__(self, name, rec=None):
+        import base64
+        import ctypes as C
+        import zlib
+        rec = rec or scene_fixture()[name]
+        pb = zlib.decompress(base64.b64decode(rec["procedures"]))
+        eb = zlib.decompress(base64.b64decode(rec["entries"]))
+        assert len(pb) == rec["num_procedures"] * C.sizeof(abi.Procedure)
+        assert len(eb) == rec["num_entries"] * C.sizeof(abi.DispatchEntry)
+        self.procs = (abi.Procedure * rec["num_procedures"]).from_buffer_copy(pb)
+        self.entries = (abi.DispatchEntry * rec["num_entries"]).from_buffer_copy(eb)
+        self.procedures = list(range(rec["num_procedures"]))
+        self.table = type("Table", (), {"entries": list(range(rec["num_entries"]))})()
+        self.name = name
+        self.axiom = rec["axiom"]
+
+    def arrays(self):
+        return self.procs, self.entries
+
+
+def scene_axioms(name, n, seed=5, menger_attribute=2):
+    """n axioms of one scene as its AxiomTraits make them: the constant size
+    and entry index the header returns, or its seed-driven size
+    (MC_Skyscrapers.cuh:91-102: side (round(2s) + 2) * 5, height
+    (round(3s) + 2) * 15) and MengerSponge's 3^(attribute % 5). Lots sit on
+    a 28-unit grid, as the skyscraper scene's getPosition places them."""
+    from operatorgraph_amd import axioms as A
+    rec = scene_fixture()[name]
+    rng = np.random.default_rng(seed)
+    s = rng.random(n).astype(np.float32)
+    ax = A.empty(n)
+    if name == "MC_Skyscrapers":
+        side = (np.round(2.0 * s) + 2.0) * 5.0
+        height = (np.round(3.0 * s) + 2.0) * 15.0
+        ax["size"] = np.stack([side, height, side], 1).astype(np.float32)
+    elif name == "MengerSponge":
+        ax["size"] = float(3 ** (menger_attribute % MENGER_SCENE_MAX_ITERATIONS))
+    else:
+        ax["size"] = rec["axiom"]["size"]
+    i = np.arange(n)
+    ax["model"][:, 3] = (i % 16) * 28.0 + 14.0
+    ax["model"][:, 7] = ax["size"][:, 1] * 0.5
+    ax["model"][:, 11] = (i // 16) * 28.0 + 14.0
+    ax["seed"] = s
+    ax["custom"] = s
+    ax["entry_index"] = rec["axiom"]["entry"]
+    ax["shape_type"] = abi.BOX
+    return ax

@@ -1,0 +1,171 @@
+"""The reference's compiled test-runner scenes (src/test_runner/*.cuh: the
+analyzer's generated procedure lists and dispatch tables) lowered through
+include/pga/core/Core.h and stored in tests/golden/scene_tables.json.
+
+The oracle's derivation of each table is pinned by what the scene's
+geometry fixes: closed-form counts and volumes (Menger sponge, the simple
+house, the recursive trees), and, for the scenes that also exist as
+interpreter grammars (src/interpreter/grammars/*.pga), equal per-terminal
+counts from the two independent front ends (the C++ template lowering of
+the header and the DSL compiler of the grammar text) on the same axioms."""
+import os
+
+import numpy as np
+import pytest
+
+from operatorgraph_amd import abi
+from tests.helpers import (MENGER_SCENE_MAX_ITERATIONS, TREE_COUNTS, SceneProgram, fixture_grammar, grammar_lots,
+                           scene_axioms, scene_fixture)
+
+REFERENCE = "/root/reference"
+# scene -> interpreter grammar of the same building
+SAME_AS_GRAMMAR = {"3DTree_4_3": "3d_tree_4_3", "3DTree_8_3": "3d_tree_8_3", "3DTree_12_2": "3d_tree_12_2",
+                   "Commercial": "commercial", "MC_Skyscrapers": "mc_skyscraper", "MC_Spaceship": "mc_spaceship",
+                   "MengerSponge": "menger_sponge_3"}
+# scenes whose tables set polygon shapes (SetAs operators): the derivation
+# supports Box and Quad only
+POLYGON_SCENES = ("Balcony", "SuburbanHouse")
+SUPPORTED = sorted(k for k in scene_fixture() if k not in POLYGON_SCENES)
+
+
+def _counts(r):
+    return [r.count(t) for t in range(abi.MAX_TERMINALS)]
+
+
+def _volumes(inst, n_axioms):
+    m = inst["model"].reshape(-1, 3, 4).astype(np.float64)
+    v = np.abs(np.linalg.det(m[:, :, :3]))
+    return np.bincount(inst["axiom"], weights=v, minlength=n_axioms)
+
+
+def test_fixture_integrity():
+    fx = scene_fixture()
+    assert set(fx) == set(SUPPORTED) | set(POLYGON_SCENES)
+    for name, rec in fx.items():
+        assert rec["header"] == f"src/test_runner/{name}.cuh"
+        assert len(rec["sha256"]) == 64
+        assert "refused" not in rec, name
+        p = SceneProgram(name)
+        procs, entries = p.arrays()
+        for e in entries:
+            assert 0 <= e.operator_code < len(procs), name
+            for s in range(e.num_successors):
+                assert -1 <= e.successors[s].entry_index < len(entries), name
+        assert 0 <= rec["axiom"]["entry"] < len(entries)
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REFERENCE, "src", "test_runner")),
+                    reason="the reference sources are only in the build container")
+def test_fixture_regenerates(tmp_path):
+    """The committed tables are what the script makes from the reference's
+    headers, byte for byte."""
+    import importlib.util
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location("make_scene_tables",
+                                                  os.path.join(here, "golden", "make_scene_tables.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    out = tmp_path / "scene_tables.json"
+    mod.main(REFERENCE, str(out))
+    with open(os.path.join(here, "golden", "scene_tables.json"), "rb") as f:
+        assert out.read_bytes() == f.read()
+
+
+@pytest.mark.parametrize("attribute", [1, 2, 3, 6])
+def test_menger_scene_closed_form(attribute):
+    """Side 3^k (MengerSponge.cuh:55-59, k = attribute % 5): the scene
+    splits a cube until its side is below 1, so a cube of side 1 is split
+    once more and the sponge keeps 20^(k+1) cubes of side 1/3, of volume
+    20^(k+1) / 27."""
+    from oracle import oracle as O
+    k = attribute % MENGER_SCENE_MAX_ITERATIONS
+    ax = scene_axioms("MengerSponge", 3, menger_attribute=attribute)
+    r = O.derive(SceneProgram("MengerSponge"), ax)
+    assert _counts(r)[0] == 3 * 20 ** (k + 1) and sum(_counts(r)) == 3 * 20 ** (k + 1)
+    np.testing.assert_allclose(_volumes(r.instances(0), 3), 20.0 ** (k + 1) / 27.0, rtol=1e-5)
+
+
+def test_simple_house_closed_form():
+    """SimpleHouse at its axiom size (2, 5, 2): 5 floors (Repeat Y 1.0);
+    each floor's top is one slab and its 4 sides are 4 tiles each (Repeat X
+    0.5); a tile keeps its two outer columns and the top and bottom of the
+    middle one (1:2:1 splits, the centre discarded): 5 * (1 + 4 * 4 * 4) =
+    325 boxes. Their volume at depth 0.1: per floor the 2 x 2 slab plus 4
+    sides of 4 tiles of 0.5 * 1 - 0.25 * 0.5, 10 in all, times 0.1 and 5
+    floors: 5.0."""
+    from oracle import oracle as O
+    ax = scene_axioms("SimpleHouse", 4)
+    r = O.derive(SceneProgram("SimpleHouse"), ax)
+    assert _counts(r)[0] == 4 * 5 * (1 + 4 * 4 * 4) and sum(_counts(r)) == _counts(r)[0]
+    np.testing.assert_allclose(_volumes(r.instances(0), 4), 5.0, rtol=1e-6)
+
+
+@pytest.mark.parametrize("name", ["3DTree_4_3", "3DTree_8_3", "3DTree_12_2"])
+def test_tree_scenes_closed_form(name):
+    from oracle import oracle as O
+    trunks, leaves = TREE_COUNTS[SAME_AS_GRAMMAR[name]]
+    r = O.derive(SceneProgram(name), scene_axioms(name, 2))
+    assert _counts(r)[:2] == [2 * trunks, 2 * leaves] and sum(_counts(r)) == 2 * (trunks + leaves)
+
+
+@pytest.mark.parametrize("name", sorted(SAME_AS_GRAMMAR))
+def test_scene_counts_equal_the_interpreter_grammar(name):
+    """The header's table and the grammar text of the same building, lowered
+    by independent front ends, derive the same number of instances per
+    terminal from the same seeds, each with its own entry index. Grammars
+    whose axiom is a unit box scale themselves where the scene's
+    AxiomTraits size the axiom instead (the trees' 16, the sponge's 9, the
+    spaceship's 5 x 4.5 x 8); the building lots are the same boxes for
+    both."""
+    from oracle import oracle as O
+    g = fixture_grammar(SAME_AS_GRAMMAR[name])
+    dsl_axioms = grammar_lots(g, SAME_AS_GRAMMAR[name], 6)
+    scene = SceneProgram(name)
+    ax = dsl_axioms.copy()
+    ax["entry_index"] = scene.axiom["entry"]
+    if np.all(dsl_axioms["size"] == 1.0):
+        ax["size"] = scene_axioms(name, len(ax))["size"]
+    a, b = O.derive(scene, ax), O.derive(g.program, dsl_axioms)
+    assert _counts(a) == _counts(b) and sum(_counts(a)) > 0
+
+
+@pytest.mark.parametrize("name", SUPPORTED)
+def test_scene_derives_at_its_own_axioms(name):
+    """Every supported scene derives at the sizes its AxiomTraits give, the
+    FP64 twins of the oracle agreeing with its FP32 transforms."""
+    from oracle import oracle as O
+    from tests.helpers import canon_order, rel_err
+    ax = scene_axioms(name, 8)
+    r = O.derive(SceneProgram(name), ax, twins=O.TWIN_F64)
+    assert sum(_counts(r)) > 0
+    for t in range(abi.MAX_TERMINALS):
+        inst = r.instances(t)
+        if len(inst):
+            assert set(np.unique(inst["axiom"])) <= set(range(8))
+            assert rel_err(inst["model"], r.instances_f64(t)[:, :12]) < 1e-5
+
+
+@pytest.mark.parametrize("name", POLYGON_SCENES)
+def test_polygon_scenes_are_refused(name):
+    from oracle import oracle as O
+    procs, _ = SceneProgram(name).arrays()
+    assert any(abi.SET_AS_DYNAMIC_CONVEX_POLYGON <= p.op_type <= abi.SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM
+               for p in procs)
+    with pytest.raises(O.OracleError) as e:
+        O.derive(SceneProgram(name), scene_axioms(name, 2))
+    assert e.value.status == abi.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("name", ["Commercial"] + list(POLYGON_SCENES))
+def test_scene_tables_load_and_specialise_without_a_device(name):
+    """The library's validation accepts the commercial scene, whose
+    expression operands carry 12.99997 for attribute 13 (read rounded, as
+    the reference reads them), and compiles its specialised kernel; the
+    polygon scenes are refused as unsupported. (The GPU tests load and
+    specialise every scene.)"""
+    from operatorgraph_amd import engine
+    st, log, _ = engine.specialize_check(SceneProgram(name), 512)
+    if name in POLYGON_SCENES:
+        assert st == abi.ERR_UNSUPPORTED, log
+    else:
+        assert st == 0, log
