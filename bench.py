@@ -562,6 +562,8 @@ def tune_summary(tune):
             "candidates": tune["candidates"], "best_ms": round(tune["seconds"] * 1e3, 4),
             "untuned_ms": round(tune["initial_seconds"] * 1e3, 4),
             "specialized": bool(tune.get("specialized")),
+            "level_schedule": (f"workgroup-local levels, {tune['local_grab']} axioms per hand-out"
+                               if tune.get("specialized") and tune.get("local_grab") else "grid-wide levels"),
             "compile_s": round(tune.get("compile_seconds", 0.0), 3)}
 
 

@@ -298,7 +298,8 @@ typedef struct pga_tune_result {
     uint64_t max_level_parents;    /* largest number of parent records written by one level */
     uint64_t max_level_children;   /* largest number of children of one level */
     uint32_t specialized;          /* 1: the chosen kernel is the one specialised to the program */
-    uint32_t reserved;
+    uint32_t local_grab;           /* specialised kernel: axioms per workgroup hand-out of its
+                                      workgroup-local levels (pga_set_schedule), 0: grid-wide levels */
 } pga_tune_result;
 PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t repeats, pga_tune_result* out);
 
@@ -308,7 +309,7 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
  *    with nvcc). Here the library writes the loaded program as C++ - one
  *    type of compile-time constants per dispatch entry - and compiles it at
  *    run time (hipRTC) with the persistent derivation kernel, workgroups of
- *    block_size threads (256, 512, 768 or 1024; 0: 512). enable != 0 makes
+ *    block_size threads (a multiple of 64 up to 1024; 0: 512). enable != 0 makes
  *    the MEGAKERNEL technique run it (eager, asynchronous and captured
  *    derivations) and pga_autotune time it beside the interpreting kernels;
  *    enable = 0 goes back to the interpreting kernel. The instances are
@@ -316,6 +317,20 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
  *    specialisation. seconds (optional): the compile time, 0 when the code
  *    object was compiled before in this process. -- */
 PGA_API pga_status pga_specialize(pga_context* ctx, uint32_t block_size, int32_t enable, double* seconds);
+/* Level scheduling of the specialised kernel. local != 0 (the default): a
+ * workgroup takes `grab` axioms at a time (0: an equal share per
+ * workgroup, at most 64) and derives them level by level on its own - the
+ * same passes, but a workgroup scan in place of the grid scan and no grid
+ * barrier - until a level's fan-outs outgrow its arrays (two parents and
+ * eight children per thread, at least 512 / 2048, at most the queue
+ * capacity); those, and every later level of them, run over the whole grid.
+ * local = 0: every level runs over the whole grid. The instances are the
+ * same multiset either way; with local levels their order within a
+ * terminal buffer differs from run to run. No counterpart in the reference,
+ * whose Whippletree queues hand shapes out one procedure queue at a time
+ * (queuingPerProc.cuh:524-580); this is its knob for small and uneven
+ * batches, and pga_autotune tries both. */
+PGA_API pga_status pga_set_schedule(pga_context* ctx, int32_t local, uint32_t grab);
 /* The same compilation without a device or context: validates the program
  * as pga_initialize does, writes its specialised kernel and compiles it.
  * log (optional) receives the validation error or the compiler's messages,

@@ -119,6 +119,13 @@ struct pga_context {
     // kernel specialised to the program (pga_specialize)
     hipModule_t spec_module = nullptr;
     hipFunction_t spec_fn = nullptr;
+    // workgroup-local frontiers of the specialised kernel: per workgroup of
+    // the full grid two arrays of loc_kp parents and two of loc_kd descriptors
+    void* d_loc = nullptr;
+    size_t loc_bytes = 0;
+    uint32_t loc_kp = 0, loc_kd = 0, loc_blocks = 0;
+    bool sched_local = true;       // pga_set_schedule
+    uint32_t sched_grab = 0;
     uint32_t spec_bs = 0, spec_blocks = 0;
     bool spec_on = false;
 
@@ -146,6 +153,11 @@ const uint32_t kDefaultMaxLevels = 4096;
 const uint32_t kMaxChain = 256;
 const uint32_t kMaxProgramLds = 60u * 1024u;   // program image + terminal tables
 const uint32_t kMaxBlocksPerCu = 8;
+// workgroup-local frontiers of the specialised kernel, per thread of the
+// workgroup: parent records and descriptors a level may write before it
+// spills to the grid's arrays; most axioms handed to a workgroup at once
+const uint32_t kLocalParentsPerThread = 2, kLocalDescsPerThread = 8, kLocalGrabMax = 64;
+const uint32_t kLocalParentsMin = 512, kLocalDescsMin = 2048;
 const uint32_t kDefaultBlockSize = 512;        // one workgroup of 8 waves per compute unit
 
 pga_status fail(pga_context* ctx, pga_status st, const std::string& msg) {
@@ -564,6 +576,15 @@ pga_status ensure_buffers(pga_context* ctx) {
     return PGA_OK;
 }
 
+// Axioms per hand-out of the workgroup-local levels: the scheduled number,
+// or an equal share of the axioms per workgroup (one round: every round
+// pays the derivation's depth in level latencies).
+uint32_t local_grab(const pga_context* ctx, uint32_t count) {
+    if (ctx->sched_grab) return std::min(ctx->sched_grab, kLocalGrabMax);
+    const uint32_t blocks = std::max(ctx->spec_blocks, 1u);
+    return std::max<uint32_t>(1u, std::min<uint32_t>(kLocalGrabMax, (count + blocks - 1u) / blocks));
+}
+
 pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
     pga::KArgs a{};
     a.prog.entries = ctx->d_entries;
@@ -609,7 +630,40 @@ pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
     a.max_chain = kMaxChain;
     a.count_geom = ctx->count_geom;
     a.use_seed = ctx->use_seed;
+    if (ctx->spec_on && ctx->sched_local && ctx->d_loc && ctx->spec_blocks <= ctx->loc_blocks) {
+        a.loc_par = reinterpret_cast<pga::Rec*>(ctx->d_loc);
+        a.loc_desc = reinterpret_cast<pga::Desc*>(reinterpret_cast<char*>(ctx->d_loc) +
+                                                  (size_t)ctx->loc_blocks * 2u * ctx->loc_kp * sizeof(pga::Rec));
+        a.loc_kp = ctx->loc_kp;
+        a.loc_kd = ctx->loc_kd;
+        a.loc_cap_p = std::min(ctx->loc_kp, ctx->cap);   // no level holds more than the queues would
+        a.loc_cap_d = std::min(ctx->loc_kd, ctx->cap);
+        a.loc_grab = local_grab(ctx, count);
+    }
     return a;
+}
+
+// Workgroup sizes of the specialised kernel: whole waves, 64 to 1024.
+bool spec_block_size_supported(uint32_t bs) { return bs >= 64u && bs <= 1024u && bs % 64u == 0u; }
+
+// The specialised kernel's workgroup-local arrays for `blocks` workgroups of
+// `bs` threads (kept when large enough).
+pga_status ensure_local(pga_context* ctx, uint32_t blocks, uint32_t bs) {
+    const uint32_t kp = std::max(kLocalParentsPerThread * bs, kLocalParentsMin);
+    const uint32_t kd = std::max(kLocalDescsPerThread * bs, kLocalDescsMin);
+    const size_t bytes = (size_t)blocks * 2u * ((size_t)kp * sizeof(pga::Rec) + (size_t)kd * sizeof(pga::Desc));
+    if (ctx->d_loc && ctx->loc_kp == kp && ctx->loc_kd == kd && ctx->loc_blocks >= blocks) return PGA_OK;
+    if (ctx->stream) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    hfree(ctx->d_loc);
+    ctx->d_loc = nullptr;
+    ctx->loc_bytes = 0;
+    ctx->loc_blocks = 0;
+    HIP_TRY(ctx, hipMalloc(&ctx->d_loc, bytes));
+    ctx->loc_bytes = bytes;
+    ctx->loc_kp = kp;
+    ctx->loc_kd = kd;
+    ctx->loc_blocks = blocks;
+    return PGA_OK;
 }
 
 void drop_spec(pga_context* ctx) {
@@ -738,7 +792,7 @@ PGA_API void pga_destroy(pga_context* ctx) {
     hfree(ctx->d_inst_ptr); hfree(ctx->d_inst_cap);
     for (int b = 0; b < 2; ++b) { hfree(ctx->d_par[b]); hfree(ctx->d_desc[b]); }
     hfree(ctx->d_term_counts); hfree(ctx->d_par_counts); hfree(ctx->d_desc_counts);
-    hfree(ctx->d_err); hfree(ctx->d_bar); hfree(ctx->d_block_totals);
+    hfree(ctx->d_err); hfree(ctx->d_bar); hfree(ctx->d_block_totals); hfree(ctx->d_loc);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -871,8 +925,10 @@ static pga_status finish(pga_context* ctx, uint32_t count, uint32_t host_levels)
     HIP_TRY(ctx, hipMemcpy(tc, ctx->d_term_counts, sizeof(tc), hipMemcpyDeviceToHost));
     uint32_t levels = host_levels;
     if (ctx->cfg.technique == PGA_TECHNIQUE_MEGAKERNEL && count > 0) levels = lc[ctx->cfg.max_levels + 1];
-    ctx->stats.levels = levels;
-    ctx->stats.records = 0;
+    // the specialised kernel's workgroup-local levels: the deepest and the
+    // parent records they wrote
+    ctx->stats.levels = std::max(levels, pc[ctx->cfg.max_levels + 1]);
+    ctx->stats.records = pc[ctx->cfg.max_levels];
     ctx->max_level_parents = 0;
     ctx->max_level_children = 0;
     for (uint32_t l = 0; l < levels && l < ctx->cfg.max_levels; ++l) {
@@ -1013,7 +1069,8 @@ PGA_API pga_status pga_specialize(pga_context* ctx, uint32_t block_size, int32_t
     }
     if (!ctx->initialized) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no program loaded");
     if (block_size == 0) block_size = kDefaultBlockSize;
-    if (!pga_block_size_supported(block_size)) return fail(ctx, PGA_ERR_INVALID_ARG, "unsupported workgroup size");
+    if (!spec_block_size_supported(block_size))
+        return fail(ctx, PGA_ERR_INVALID_ARG, "unsupported workgroup size");
     if (ctx->spec_fn && ctx->spec_bs == block_size) {
         ctx->spec_on = true;
         return PGA_OK;
@@ -1037,8 +1094,20 @@ PGA_API pga_status pga_specialize(pga_context* ctx, uint32_t block_size, int32_t
     }
     ctx->spec_bs = block_size;
     ctx->spec_blocks = std::min((uint32_t)per_cu, kMaxBlocksPerCu) * (uint32_t)ctx->num_cus;
+    if (pga_status st = ensure_local(ctx, ctx->spec_blocks, block_size)) {
+        drop_spec(ctx);
+        return st;
+    }
     ctx->spec_on = true;
     if (seconds && !cached) seconds[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_set_schedule(pga_context* ctx, int32_t local, uint32_t grab) {
+    NO_CAPTURE(ctx, "pga_set_schedule");
+    if (!ctx) return PGA_ERR_INVALID_ARG;
+    ctx->sched_local = local != 0;
+    ctx->sched_grab = grab;
     return PGA_OK;
 }
 
@@ -1048,7 +1117,7 @@ PGA_API pga_status pga_specialize_check(const pga_procedure* procedures, uint32_
     if (seconds) *seconds = 0.0;
     if (log && log_size) log[0] = '\0';
     if (block_size == 0) block_size = kDefaultBlockSize;
-    if (!pga_block_size_supported(block_size)) return PGA_ERR_INVALID_ARG;
+    if (!spec_block_size_supported(block_size)) return PGA_ERR_INVALID_ARG;
     std::unique_ptr<pga_context> ctx(new pga_context());   // host tables only: no device call
     ctx->cfg.block_size = block_size;
     std::string msg;
@@ -1091,6 +1160,8 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
     };
     const bool saved_spec = ctx->spec_on;
     const uint32_t saved_spec_blocks = ctx->spec_blocks;
+    const bool saved_sched_local = ctx->sched_local;
+    const uint32_t saved_sched_grab = ctx->sched_grab;
     pga_tune_result res{};
     pga_status st = time_config(&res.initial_seconds);
     if (st) return st;
@@ -1098,6 +1169,7 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
     res.block_size = saved_spec ? ctx->spec_bs : saved.block_size;
     res.grid_blocks = saved_spec ? saved_spec_blocks : saved_blocks;
     res.specialized = saved_spec ? 1u : 0u;
+    res.local_grab = saved_spec && saved_sched_local ? local_grab(ctx, num_axioms) : 0u;
     res.seconds = res.initial_seconds;
     res.max_level_parents = ctx->max_level_parents;
     res.max_level_children = ctx->max_level_children;
@@ -1116,17 +1188,31 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
             if (g == 1 && grids[1] >= full) continue;
             ctx->spec_on = true;
             ctx->spec_blocks = grids[g];
-            double s = 0.0;
-            st = time_config(&s);
-            ++res.candidates;
-            if (!st && s < res.seconds) {
-                res.seconds = s;
-                res.technique = PGA_TECHNIQUE_MEGAKERNEL;
-                res.block_size = ctx->spec_bs;
-                res.grid_blocks = grids[g];
-                res.specialized = 1u;
+            // level schedules: grid-wide, and workgroup-local with the
+            // automatic hand-out, half of it (two rounds) and four times it
+            ctx->sched_local = true;
+            ctx->sched_grab = 0;
+            const uint32_t autog = local_grab(ctx, num_axioms);
+            const uint32_t grabs[] = {0u, autog, std::max(autog / 2u, 1u), std::min(4u * autog, kLocalGrabMax)};
+            for (int k = 0; k < 4 && !st; ++k) {
+                if (k >= 2 && (grabs[k] == autog || grabs[k] == grabs[k - 1])) continue;
+                ctx->sched_local = k != 0;
+                ctx->sched_grab = grabs[k];
+                double s = 0.0;
+                st = time_config(&s);
+                ++res.candidates;
+                if (!st && s < res.seconds) {
+                    res.seconds = s;
+                    res.technique = PGA_TECHNIQUE_MEGAKERNEL;
+                    res.block_size = ctx->spec_bs;
+                    res.grid_blocks = grids[g];
+                    res.specialized = 1u;
+                    res.local_grab = grabs[k];
+                }
             }
         }
+        ctx->sched_local = saved_sched_local;
+        ctx->sched_grab = saved_sched_grab;
         ctx->spec_on = false;
         ctx->cfg.technique = saved.technique;
     }
@@ -1171,6 +1257,8 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
         ctx->mega_blocks = saved_blocks;
         ctx->spec_on = true;
         ctx->spec_blocks = res.grid_blocks;
+        ctx->sched_local = res.local_grab != 0u;
+        ctx->sched_grab = res.local_grab;
     } else {
         ctx->cfg.block_size = res.block_size;
         ctx->mega_blocks = res.grid_blocks;

@@ -10,7 +10,12 @@
 
 namespace pga {
 
-constexpr uint32_t kBarWords = 160;     // grid barrier: generation + 9 counters, 64 bytes apart
+constexpr uint32_t kBarWords = 192;     // grid barrier: generation + 9 counters, 64 bytes apart (words 0-159)
+// words after the grid barrier's, on a line of their own and zeroed with
+// them before every launch (k_reset, at most 256 words): the next axiom to
+// hand out and the parents / descriptors spilled from workgroup-local
+// frontiers to the grid's arrays
+constexpr uint32_t kBarGrab = 160, kBarSpillPar = 161, kBarSpillDesc = 162;
 
 struct KArgs {
     Prog prog;                          // tables in the global program image
@@ -48,6 +53,14 @@ struct KArgs {
     // derivation tape (reverse mode): every level's parent records and
     // descriptors kept (cap_par / cap_desc = their capacities), every
     // fan-out materialised, and one outcome per child
+    // workgroup-local frontiers of the specialised kernel: per workgroup two
+    // arrays of loc_kp parent records and two of loc_kd descriptors (NULL:
+    // every level runs over the grid), of which a level may fill loc_cap_p /
+    // loc_cap_d (at most the queue capacity); axioms handed out loc_grab at
+    // a time
+    Rec* loc_par;
+    Desc* loc_desc;
+    uint32_t loc_kp, loc_kd, loc_cap_p, loc_cap_d, loc_grab;
     uint32_t tape;
     Rec* tape_par;
     Desc* tape_desc;

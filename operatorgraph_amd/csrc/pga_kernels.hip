@@ -189,7 +189,7 @@ __device__ __forceinline__ void run_level(const KArgs& a, LevelLDS& L, float4* P
         const Ctx c = make_ctx<TAPE>(a, L, level, wr);
         run_pass<TAPE>(a, c, L, PW, wr, level, n, wave, nwaves);
         __syncthreads();
-        if (!wr) scan_counts(a, L, level);
+        if (!wr) scan_counts(a, L, level, level);
     }
     stamp(a, level, 3);
     // terminals accumulate across levels: this level wrote after the running

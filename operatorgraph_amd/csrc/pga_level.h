@@ -20,6 +20,8 @@ struct LevelLDS {
     uint32_t level_total[kCounters];   // totals of the level over the grid
     unsigned long long term_base[PGA_MAX_TERMINALS];  // running instance count per slot
     uint32_t next_n;                   // children of the next level (all blocks agree)
+    uint32_t grab;                     // workgroup-local derivation: first axiom handed to this block
+    uint32_t spill;                    // ... 1: this level's fan-outs go to the grid's arrays
     uint32_t tp_prev, tp_cur;          // tape: parent-record base of the previous / this level
     uint32_t td_prev, td_cur;          // tape: descriptor base of the previous / this level
 };
@@ -302,19 +304,21 @@ struct LoadEntry {
     __device__ __forceinline__ void operator()(const Ctx& c, uint32_t i, F&& f) const { f(load_entry(c, i)); }
 };
 
-// Child i of the level: an axiom (level 0) or the child that a descriptor
-// of the previous level names. Returns false when there is none.
-template <class Visit = LoadEntry>
-__device__ __forceinline__ bool load_child(const KArgs& a, const Ctx& c, const Rec* par_in, const Desc* desc_in,
-                                           bool wr, uint32_t level, uint32_t i, Rec& r, Visit visit = Visit()) {
+// Child i of a level: an axiom (level 0: axiom first + i) or the child that
+// a descriptor of the previous level names (parents below `in_cap`).
+// Returns false when there is none.
+template <class Visit>
+__device__ __forceinline__ bool load_child_at(const KArgs& a, const Ctx& c, uint32_t first, const Rec* par_in,
+                                              uint32_t in_cap, const Desc* desc_in, bool wr, uint32_t level,
+                                              uint32_t i, Rec& r, Visit visit) {
     if (level == 0u) {
-        const pga_axiom& ax = a.axioms[a.first_axiom + i];
+        const pga_axiom& ax = a.axioms[first + i];
 #pragma unroll
         for (int k = 0; k < 12; ++k) r.m[k] = ax.model[k];
         r.s[0] = ax.size[0]; r.s[1] = ax.size[1]; r.s[2] = ax.size[2];
         r.seed = ax.seed;
         r.custom = ax.custom;
-        r.axiom = a.first_axiom + i;
+        r.axiom = first + i;
         r.path = 0u;
         r.ed = 0u;
         const int32_t ent = ax.entry_index;
@@ -328,7 +332,7 @@ __device__ __forceinline__ bool load_child(const KArgs& a, const Ctx& c, const R
     }
     const uint4 dv = *reinterpret_cast<const uint4*>(desc_in + i);
     const uint32_t parent = dv.x, jx = dv.y;
-    if (parent >= a.cap_par) return false;       // parent was dropped on overflow
+    if (parent >= in_cap) return false;          // parent was dropped on overflow
     const uint32_t j = jx & 0x3FFFFFFFu;
     if (j == kDescNone) return false;
     Rec p;
@@ -339,6 +343,13 @@ __device__ __forceinline__ bool load_child(const KArgs& a, const Ctx& c, const R
         ok = build_child(c, p, e, j, jx >> 30, __uint_as_float(dv.z), __uint_as_float(dv.w), r);
     });
     return ok;
+}
+
+// The same over the launch's axioms and the grid's parent arrays.
+template <class Visit = LoadEntry>
+__device__ __forceinline__ bool load_child(const KArgs& a, const Ctx& c, const Rec* par_in, const Desc* desc_in,
+                                           bool wr, uint32_t level, uint32_t i, Rec& r, Visit visit = Visit()) {
+    return load_child_at(a, c, a.first_axiom, par_in, a.cap_par, desc_in, wr, level, i, r, visit);
 }
 
 // Grid-wide barrier of a cooperative launch. Words (64 bytes apart):
@@ -383,10 +394,11 @@ __device__ __forceinline__ void stamp(const KArgs& a, uint32_t level, uint32_t p
 }
 
 // Between the passes: publish this block's totals, grid barrier, exclusive
-// prefix over the preceding blocks into the write cursors, level totals.
-__device__ __forceinline__ void scan_counts(const KArgs& a, LevelLDS& L, uint32_t level) {
+// prefix over the preceding blocks into the write cursors, level totals
+// (timestamps in row `row`).
+__device__ __forceinline__ void scan_counts(const KArgs& a, LevelLDS& L, uint32_t level, uint32_t row) {
     const uint32_t nslots = a.prog.nslots, ncnt = nslots + 2u;
-    stamp(a, level, 1);
+    stamp(a, row, 1);
     const uint32_t nw = blockDim.x >> 6;
     uint32_t* totals = a.block_totals + (size_t)blockIdx.x * ncnt;
     for (uint32_t k = threadIdx.x; k < ncnt; k += blockDim.x) {
@@ -425,7 +437,7 @@ __device__ __forceinline__ void scan_counts(const KArgs& a, LevelLDS& L, uint32_
         if (blockIdx.x == 0 && k == nslots + 1u) a.desc_counts[level] = all;
     }
     __syncthreads();
-    stamp(a, level, 2);
+    stamp(a, row, 2);
 }
 
 __device__ __forceinline__ void init_bases(const KArgs& a, LevelLDS& L) {

@@ -136,22 +136,51 @@ struct SpecVisit {
     }
 };
 
-// One pass (WR: the write pass) over the children of a level.
+// Where a pass reads its children and writes its fan-outs.
+struct PassIO {
+    uint32_t first;                 // level 0: the first axiom
+    const Rec* par_in;              // parents of the children (levels > 0), `in_cap` of them
+    const Desc* desc_in;
+    uint32_t in_cap;
+    Rec* par_out;                   // fan-outs started by the children
+    Desc* desc_out;
+    uint32_t par_cap, desc_cap;
+};
+
+// The arrays of a grid-wide level: the even / odd level buffers.
+__device__ __forceinline__ PassIO grid_io(const KArgs& a, uint32_t level) {
+    PassIO io;
+    io.first = a.first_axiom;
+    io.par_in = (level & 1u) ? a.par0 : a.par1;
+    io.desc_in = (level & 1u) ? a.desc0 : a.desc1;
+    io.in_cap = a.cap_par;
+    io.par_out = (level & 1u) ? a.par1 : a.par0;
+    io.desc_out = (level & 1u) ? a.desc1 : a.desc0;
+    io.par_cap = a.cap_par;
+    io.desc_cap = a.cap_desc;
+    return io;
+}
+
+// One pass (WR: the write pass) over the n children of a level, 64 a wave,
+// the waves numbered `wave` of `nwaves`.
 template <class S, bool WR>
 __device__ __forceinline__ void spec_pass(const KArgs& a, LevelLDS& L, float4* PW, uint32_t level, uint32_t n,
-                                          uint32_t wave, uint32_t nwaves) {
+                                          uint32_t wave, uint32_t nwaves, const PassIO& io) {
     Ctx c = make_ctx<false>(a, L, level, WR);
     c.geom = WR ? 1u : S::kCountGeom;     // constants: the count pass drops the unread arithmetic
     c.seed = S::kUseSeed;
+    c.par_out = io.par_out;
+    c.desc_out = io.desc_out;
+    c.par_cap = io.par_cap;
+    c.desc_cap = io.desc_cap;
     const SpecVisit<S> visit;
-    const Rec* par_in = (level & 1u) ? a.par0 : a.par1;
-    const Desc* desc_in = (level & 1u) ? a.desc0 : a.desc1;
     f4* ST = reinterpret_cast<f4*>(PW + 5u * blockDim.x) + wave_in_block() * 256u;
 #pragma unroll 1
     for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u) {
         const uint32_t i = base + lane_id();
         Rec cur;
-        const bool valid = i < n && load_child(a, c, par_in, desc_in, WR, level, i, cur, visit);
+        const bool valid = i < n && load_child_at(a, c, io.first, io.par_in, io.in_cap, io.desc_in, WR, level, i,
+                                                  cur, visit);
         uint32_t mode = valid ? M_SETTLE : M_DONE;
         bool leaf = false, self = false;
         uint32_t steps = 0, j = 0, dslot = 0, pidx = ~0u, fe = 0;
@@ -174,29 +203,15 @@ __device__ __forceinline__ void spec_pass(const KArgs& a, LevelLDS& L, float4* P
                         return;
                     }
                     if (mode == M_FAN) {
-                        // child j of this lane's fan-out (entry e, record in PW), or
-                        // the record itself continuing on the next level (self)
-                        if (j >= f.n) { mode = M_DONE; return; }
+                        // child j of this lane's leaf fan-out (entry e, record in PW)
+                        if (j >= f.n || !is_fanout(k)) { mode = M_DONE; return; }
                         float o = 0.0f, ns = 0.0f;
                         const Rec par = par_load(PW);
-                        const bool keep = (self || !is_fanout(k)) ? true : fan_geom(c, par, e, f, j, o, ns);
+                        const bool keep = fan_geom(c, par, e, f, j, o, ns);
                         const bool skip = j < 32u ? ((skipm >> j) & 1u) != 0u : skipm == 0xFFFFFFFFu;
-                        if (leaf && is_fanout(k)) {
-                            if (keep && !skip && build_child(c, par, e, j, f.axis, o, ns, cur)) {
-                                mode = M_SETTLE;
-                                steps = 0;
-                            }
-                        } else if (!skip) {     // write pass only: the count pass claimed nd slots
-                            const uint32_t jx = pidx == ~0u ? kDescNone
-                                              : self        ? kDescSelf
-                                              : keep        ? (j | (f.axis << 30))
-                                                            : kDescNone;
-                            if (dslot < c.desc_cap)
-                                *reinterpret_cast<uint4*>(c.desc_out + dslot) =
-                                    make_uint4(pidx, jx, __float_as_uint(o), __float_as_uint(ns));
-                            else
-                                raise(c, PGA_ERR_OVERFLOW_QUEUE);
-                            ++dslot;
+                        if (keep && !skip && build_child(c, par, e, j, f.axis, o, ns, cur)) {
+                            mode = M_SETTLE;
+                            steps = 0;
                         }
                         ++j;
                         return;
@@ -242,7 +257,10 @@ __device__ __forceinline__ void spec_pass(const KArgs& a, LevelLDS& L, float4* P
                             }
                             mode = M_DONE;
                         } else {
-                            mode = (leaf || WR) ? M_FAN : M_DONE;
+                            // a leaf fan-out takes its children round the state
+                            // machine; the write pass writes the descriptors of
+                            // any other once the fan-out is started
+                            mode = leaf ? M_FAN : M_DONE;
                         }
                     } else if (steps >= c.max_chain) {
                         if (leaf) {             // the leaf analysis bounds every chain below a leaf
@@ -259,7 +277,7 @@ __device__ __forceinline__ void spec_pass(const KArgs& a, LevelLDS& L, float4* P
                         skipm = 0;
                         j = 0;
                         start = true;
-                        mode = WR ? M_FAN : M_DONE;
+                        mode = M_DONE;
                     } else {
                         uint32_t ord;
                         const int32_t next = apply_single(c, e, cur, &ord);
@@ -272,7 +290,35 @@ __device__ __forceinline__ void spec_pass(const KArgs& a, LevelLDS& L, float4* P
                 unsigned long long idx;
                 emit_terminal(c, L, ST, WR, emit, slot, cur, idx);
             }
-            if (__ballot(start) != 0ull) start_fanout(c, L, WR, start, par_load(PW), nd, pidx, dslot);
+            if (__ballot(start) != 0ull) {
+                start_fanout(c, L, WR, start, par_load(PW), nd, pidx, dslot);
+                if (WR && start) {
+                    // the fan-out's descriptors in one loop (the running offsets
+                    // of fan_geom in child order), not one trip round the state
+                    // machine per child
+                    visit(c, fe, [&](const auto& e) {
+                        const Rec par = par_load(PW);
+                        const bool geom = !self && is_fanout(e.kind());
+#pragma unroll 1
+                        for (uint32_t jj = 0; jj < f.n; ++jj) {
+                            float o = 0.0f, ns = 0.0f;
+                            const bool keep = geom ? fan_geom(c, par, e, f, jj, o, ns) : true;
+                            const bool skip = jj < 32u ? ((skipm >> jj) & 1u) != 0u : skipm == 0xFFFFFFFFu;
+                            if (skip) continue;
+                            const uint32_t jx = pidx == ~0u ? kDescNone
+                                              : self        ? kDescSelf
+                                              : keep        ? (jj | (f.axis << 30))
+                                                            : kDescNone;
+                            if (dslot < c.desc_cap)
+                                *reinterpret_cast<uint4*>(c.desc_out + dslot) =
+                                    make_uint4(pidx, jx, __float_as_uint(o), __float_as_uint(ns));
+                            else
+                                raise(c, PGA_ERR_OVERFLOW_QUEUE);
+                            ++dslot;
+                        }
+                    });
+                }
+            }
             if (__ballot(direct) != 0ull) {
                 // a direct lane is done with its own record (the parent is in
                 // PW): its children are built into `cur`
@@ -295,40 +341,169 @@ __device__ __forceinline__ void spec_pass(const KArgs& a, LevelLDS& L, float4* P
     }
 }
 
-// One derivation level: count pass, scan, write pass (as run_level of the
-// interpreting kernels).
-template <class S>
-__device__ __forceinline__ void spec_level(const KArgs& a, LevelLDS& L, float4* PW, uint32_t level, uint32_t n) {
+// The block scan of a workgroup-local level (see spec_mega): wave offsets
+// and totals; the level's instances of each slot claimed after those already
+// claimed (one atomic per slot); the fan-outs at 0 of the block's next
+// arrays or, when they do not fit (more than kp parents or kd descriptors),
+// spilled to the grid's level-1 input (one atomic per array). Returns the
+// spill.
+__device__ __forceinline__ bool local_scan(const KArgs& a, LevelLDS& L, uint32_t level, uint32_t kp, uint32_t kd) {
     const uint32_t nslots = a.prog.nslots, ncnt = nslots + 2u;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t w = wave_in_block();
-    stamp(a, level, 0);
-    for (uint32_t k = lane_id(); k < ncnt; k += 64u) L.cnt[w][k] = 0u;
-    spec_pass<S, false>(a, L, PW, level, n, wave, nwaves);
+    const uint32_t nw = blockDim.x >> 6;
+    for (uint32_t k = threadIdx.x; k < ncnt; k += blockDim.x) {
+        uint32_t sum = 0;
+        for (uint32_t ww = 0; ww < nw; ++ww) {
+            L.cur[ww][k] = sum;
+            sum += L.cnt[ww][k];
+        }
+        L.level_total[k] = sum;
+    }
     __syncthreads();
-    scan_counts(a, L, level);
-    spec_pass<S, true>(a, L, PW, level, n, wave, nwaves);
+    for (uint32_t k = threadIdx.x; k < ncnt; k += blockDim.x) {
+        const uint32_t tot = L.level_total[k];
+        if (k < nslots) {
+            if (tot != 0u)
+                L.term_base[k] = __hip_atomic_fetch_add(&a.term_counts[pga_smem[a.off_slots + PGA_MAX_TERMINALS + k]],
+                                                        (unsigned long long)tot, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+        } else if (k == nslots) {
+            const uint32_t td = L.level_total[nslots + 1u];
+            const bool spill = tot > kp || td > kd;
+            L.spill = spill ? 1u : 0u;
+            L.before[nslots] = spill && tot != 0u ? __hip_atomic_fetch_add(&a.bar[kBarSpillPar], tot,
+                                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                                  : 0u;
+            L.before[nslots + 1u] = spill && td != 0u ? __hip_atomic_fetch_add(&a.bar[kBarSpillDesc], td,
+                                                                               __ATOMIC_RELAXED,
+                                                                               __HIP_MEMORY_SCOPE_AGENT)
+                                                      : 0u;
+            if (tot != 0u) atomicAdd(&a.par_counts[a.max_levels], tot);
+            atomicMax(&a.par_counts[a.max_levels + 1u], level + 1u);
+        }
+    }
     __syncthreads();
-    stamp(a, level, 3);
-    if (threadIdx.x == 0)
-        for (uint32_t k = 0; k < nslots; ++k) L.term_base[k] += L.level_total[k];
+    for (uint32_t k = nslots + threadIdx.x; k < ncnt; k += blockDim.x)
+        for (uint32_t ww = 0; ww < nw; ++ww) L.cur[ww][k] += L.before[k];
+    const bool spill = L.spill != 0u;
     __syncthreads();
+    return spill;
 }
 
 // The persistent derivation of every level (the MEGAKERNEL technique).
+//
+// With local arrays (a.loc_par), the levels run first inside each
+// workgroup: a block takes loc_grab axioms at a time (an atomic cursor) and
+// derives them level by level on its own - the same count and write passes
+// over the block's waves, a block scan (local_scan) for the grid scan, the
+// fan-outs into the block's double-buffered arrays and each level's
+// terminals claimed with one atomic per slot (the instances are the same
+// records; their order across blocks is unspecified). A level whose fan-outs
+// do not fit the block's arrays writes them to the grid's level-1 input
+// instead; when no axiom is left the grid-wide levels (count pass, grid scan,
+// write pass, grid barrier) take that spill up. A batch of small or uneven
+// derivations thus pays no grid barrier per level. Without local arrays the
+// grid-wide levels start from the axioms.
 template <class S, int BS>
 __device__ __forceinline__ void spec_mega(const KArgs& a) {
     __shared__ LevelLDS L;
     __shared__ float4 PW[9 * BS];   // parent records (5 words per lane), then instance staging (4)
     stage_program(a);
-    init_bases(a, L);
-    uint32_t level = 0;
-    uint32_t n = a.n_axioms;
-    for (; level < a.max_levels && n != 0u; ++level) {
-        spec_level<S>(a, L, PW, level, n);
-        n = min(L.next_n, a.cap_desc);
-        grid_barrier(a.bar, gridDim.x);       // next level reads this level's records
+    const uint32_t nslots = a.prog.nslots, ncnt = nslots + 2u;
+    const uint32_t w = wave_in_block();
+    const uint32_t kp = a.loc_kp, kd = a.loc_kd;
+    Rec* const lp = a.loc_par + (size_t)blockIdx.x * 2u * kp;
+    Desc* const ld = a.loc_desc + (size_t)blockIdx.x * 2u * kd;
+    bool local = a.loc_par != nullptr;
+    uint32_t level = 0, buf = 0, first = a.first_axiom;
+    uint32_t n = local ? 0u : a.n_axioms;
+    uint32_t tick = 0;              // timestamp row: local levels, then grid levels
+    if (!local) init_bases(a, L);
+#pragma unroll 1
+    for (;;) {
+        if (local) {
+            if (n == 0u) {          // the block's next axioms, or the grid-wide levels
+                if (threadIdx.x == 0)
+                    L.grab = __hip_atomic_fetch_add(&a.bar[kBarGrab], a.loc_grab, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+                __syncthreads();
+                const uint32_t g = L.grab;
+                __syncthreads();
+                if (g < a.n_axioms) {
+                    first = a.first_axiom + g;
+                    n = min(a.loc_grab, a.n_axioms - g);
+                    level = 0;
+                    buf = 0;
+                } else {
+                    grid_barrier(a.bar, gridDim.x);     // every spill and instance claim is done
+                    const uint32_t spilled =
+                        __hip_atomic_load(&a.bar[kBarSpillDesc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (blockIdx.x == 0 && threadIdx.x == 0) a.desc_counts[0] = spilled;
+                    local = false;
+                    level = 1;
+                    n = min(spilled, a.cap_desc);
+                    init_bases(a, L);
+                    continue;
+                }
+            }
+            if (level >= a.max_levels) {    // as the grid-wide levels: the frontier is dropped
+                if (threadIdx.x == 0) atomicOr(a.err, 1u << PGA_ERR_MAX_DEPTH);
+                n = 0;
+                continue;
+            }
+        } else if (level >= a.max_levels || n == 0u) {
+            break;
+        }
+        PassIO io;
+        uint32_t wave, nwaves;
+        if (local) {
+            io.first = first;
+            io.par_in = lp + buf * kp;
+            io.desc_in = ld + buf * kd;
+            io.in_cap = kp;
+            io.par_out = lp + (buf ^ 1u) * kp;
+            io.desc_out = ld + (buf ^ 1u) * kd;
+            io.par_cap = kp;
+            io.desc_cap = kd;
+            wave = w;
+            nwaves = blockDim.x >> 6;
+        } else {
+            io = grid_io(a, level);
+            wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+            nwaves = (gridDim.x * blockDim.x) >> 6;
+        }
+        stamp(a, tick, 0);
+        for (uint32_t k = lane_id(); k < ncnt; k += 64u) L.cnt[w][k] = 0u;
+        spec_pass<S, false>(a, L, PW, level, n, wave, nwaves, io);
+        __syncthreads();
+        bool spill = false;
+        if (local) {
+            stamp(a, tick, 1);
+            spill = local_scan(a, L, level, a.loc_cap_p, a.loc_cap_d);
+            stamp(a, tick, 2);
+            if (spill) {
+                io.par_out = a.par0;
+                io.desc_out = a.desc0;
+                io.par_cap = a.cap_par;
+                io.desc_cap = a.cap_desc;
+            }
+        } else {
+            scan_counts(a, L, level, tick);
+        }
+        spec_pass<S, true>(a, L, PW, level, n, wave, nwaves, io);
+        __syncthreads();
+        stamp(a, tick, 3);
+        ++tick;
+        if (local) {
+            n = spill ? 0u : L.level_total[nslots + 1u];
+            buf ^= 1u;
+        } else {
+            if (threadIdx.x == 0)
+                for (uint32_t k = 0; k < nslots; ++k) L.term_base[k] += L.level_total[k];
+            __syncthreads();
+            n = min(L.next_n, a.cap_desc);
+            grid_barrier(a.bar, gridDim.x);       // next level reads this level's records
+        }
+        ++level;
     }
     store_bases(a, L);
     if (blockIdx.x == 0 && threadIdx.x == 0) {

@@ -132,6 +132,37 @@ def test_spec_fuzz_grammar(seed):
         eng.close()
 
 
+@pytest.mark.parametrize("local,grab", [(False, 0), (True, 0), (True, 1), (True, 7), (True, 64)])
+def test_spec_level_schedules(local, grab):
+    """Every level schedule (pga_set_schedule: grid-wide levels, or
+    workgroup-local levels with any hand-out size) derives the oracle's
+    multisets: lots of different depths, Philox rule choices, and a sponge
+    whose frontier outgrows the workgroup arrays (the spill to the grid-wide
+    levels)."""
+    from oracle import oracle as O
+    from operatorgraph_amd import engine
+    _note(f"specialised: schedule local={local} grab={grab}")
+    cases = [(grammars.facade(), axioms.facade_lots(300), None, 2),
+             (grammars.stochastic(), axioms.stochastic_axioms(2000)[0], axioms.STOCHASTIC_KEY, 1),
+             (grammars.menger_test_runner(),
+              axioms.menger_axiom(27.0, grammars.MENGER_TEST_RUNNER_AXIOM_ENTRY, centred=False), None, 1)]
+    for program, ax, key, n_terms in cases:
+        r = O.derive(program, ax, key=key or (0, 0))
+        eng = engine.Engine(queue_capacity=1 << 22)
+        eng.load(program).set_axioms(ax)
+        if key is not None:
+            eng.set_philox_key(key)
+        for t in range(n_terms):
+            eng.bind(t, r.count(t) + 64)
+        eng.specialize(256)
+        eng.set_schedule(local, grab)
+        for rep in range(2):
+            eng.execute()
+            for t in range(n_terms):
+                assert_same_multiset(eng.instances(t), r.instances(t), f"{program.name} terminal {t} run {rep}")
+        eng.close()
+
+
 def test_spec_errors_counts_append_and_switch_back():
     from operatorgraph_amd import engine
     from operatorgraph_amd.engine import PGAError
@@ -203,8 +234,8 @@ def test_spec_graph_replay_and_autotune():
         eng.execute()            # eager work between replays
     g.close()
     res = eng.autotune(repeats=2)
-    _note(f"  autotune chose specialized={res['specialized']} block={res['block_size']}")
-    assert res["candidates"] >= 3
+    _note(f"  autotune chose specialized={res['specialized']} block={res['block_size']} grab={res['local_grab']}")
+    assert res["candidates"] >= 5
     eng.execute()
     for t in range(2):
         assert_same_multiset(eng.instances(t), ref[t], f"tuned terminal {t}")
