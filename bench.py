@@ -563,6 +563,8 @@ def tune_summary(tune):
             "untuned_ms": round(tune["initial_seconds"] * 1e3, 4),
             "specialized": bool(tune.get("specialized")),
             "level_schedule": (f"workgroup-local levels, {tune['local_grab']} axioms per hand-out"
+                               + (f", levels over {tune['local_spill']} children to the grid"
+                                  if tune.get("local_spill") else "")
                                if tune.get("specialized") and tune.get("local_grab") else "grid-wide levels"),
             "compile_s": round(tune.get("compile_seconds", 0.0), 3)}
 

@@ -217,7 +217,8 @@ typedef struct pga_config {
 } pga_config;
 
 typedef struct pga_stats {
-    uint64_t levels;          /* derivation levels executed */
+    uint64_t levels;          /* derivation levels executed (with workgroup-local levels: the
+                                 passes of the busiest workgroup plus the grid-wide levels) */
     uint64_t records;         /* shape records written to frontiers */
     uint64_t terminals;       /* instances written, all terminal types */
     double seconds;           /* device time of the last execute */
@@ -300,6 +301,8 @@ typedef struct pga_tune_result {
     uint32_t specialized;          /* 1: the chosen kernel is the one specialised to the program */
     uint32_t local_grab;           /* specialised kernel: axioms per workgroup hand-out of its
                                       workgroup-local levels (pga_set_schedule), 0: grid-wide levels */
+    uint32_t local_spill;          /* ... and their spill limit (0: the arrays' capacity) */
+    uint32_t reserved;
 } pga_tune_result;
 PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t repeats, pga_tune_result* out);
 
@@ -319,18 +322,20 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
 PGA_API pga_status pga_specialize(pga_context* ctx, uint32_t block_size, int32_t enable, double* seconds);
 /* Level scheduling of the specialised kernel. local != 0 (the default): a
  * workgroup takes `grab` axioms at a time (0: an equal share per
- * workgroup, at most 64) and derives them level by level on its own - the
- * same passes, but a workgroup scan in place of the grid scan and no grid
- * barrier - until a level's fan-outs outgrow its arrays (two parents and
- * eight children per thread, at least 512 / 2048, at most the queue
- * capacity); those, and every later level of them, run over the whole grid.
+ * workgroup, at most 64) whenever its level is narrower than the workgroup,
+ * and derives what it holds level by level on its own - the same passes,
+ * but a workgroup scan in place of the grid scan and no grid barrier -
+ * until a level's fan-outs outgrow its arrays (two parents and eight
+ * children per thread, at least 512 / 2048, at most the queue capacity) or
+ * number more than `spill` children (0: no such limit); those, and every
+ * later level of them, run over the whole grid.
  * local = 0: every level runs over the whole grid. The instances are the
  * same multiset either way; with local levels their order within a
  * terminal buffer differs from run to run. No counterpart in the reference,
  * whose Whippletree queues hand shapes out one procedure queue at a time
  * (queuingPerProc.cuh:524-580); this is its knob for small and uneven
  * batches, and pga_autotune tries both. */
-PGA_API pga_status pga_set_schedule(pga_context* ctx, int32_t local, uint32_t grab);
+PGA_API pga_status pga_set_schedule(pga_context* ctx, int32_t local, uint32_t grab, uint32_t spill);
 /* The same compilation without a device or context: validates the program
  * as pga_initialize does, writes its specialised kernel and compiles it.
  * log (optional) receives the validation error or the compiler's messages,

@@ -127,7 +127,8 @@ class TuneResult(C.Structure):
     _fields_ = [("technique", C.c_uint32), ("block_size", C.c_uint32), ("grid_blocks", C.c_uint32),
                 ("candidates", C.c_uint32), ("seconds", C.c_double), ("initial_seconds", C.c_double),
                 ("max_level_parents", C.c_uint64), ("max_level_children", C.c_uint64),
-                ("specialized", C.c_uint32), ("local_grab", C.c_uint32)]
+                ("specialized", C.c_uint32), ("local_grab", C.c_uint32),
+                ("local_spill", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 # numpy views of the 80-byte axiom and 64-byte instance records

@@ -125,7 +125,7 @@ struct pga_context {
     size_t loc_bytes = 0;
     uint32_t loc_kp = 0, loc_kd = 0, loc_blocks = 0;
     bool sched_local = true;       // pga_set_schedule
-    uint32_t sched_grab = 0;
+    uint32_t sched_grab = 0, sched_spill = 0;
     uint32_t spec_bs = 0, spec_blocks = 0;
     bool spec_on = false;
 
@@ -638,6 +638,7 @@ pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
         a.loc_kd = ctx->loc_kd;
         a.loc_cap_p = std::min(ctx->loc_kp, ctx->cap);   // no level holds more than the queues would
         a.loc_cap_d = std::min(ctx->loc_kd, ctx->cap);
+        if (ctx->sched_spill) a.loc_cap_d = std::min(a.loc_cap_d, ctx->sched_spill);
         a.loc_grab = local_grab(ctx, count);
     }
     return a;
@@ -1103,11 +1104,12 @@ PGA_API pga_status pga_specialize(pga_context* ctx, uint32_t block_size, int32_t
     return PGA_OK;
 }
 
-PGA_API pga_status pga_set_schedule(pga_context* ctx, int32_t local, uint32_t grab) {
+PGA_API pga_status pga_set_schedule(pga_context* ctx, int32_t local, uint32_t grab, uint32_t spill) {
     NO_CAPTURE(ctx, "pga_set_schedule");
     if (!ctx) return PGA_ERR_INVALID_ARG;
     ctx->sched_local = local != 0;
     ctx->sched_grab = grab;
+    ctx->sched_spill = spill;
     return PGA_OK;
 }
 
@@ -1162,6 +1164,7 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
     const uint32_t saved_spec_blocks = ctx->spec_blocks;
     const bool saved_sched_local = ctx->sched_local;
     const uint32_t saved_sched_grab = ctx->sched_grab;
+    const uint32_t saved_sched_spill = ctx->sched_spill;
     pga_tune_result res{};
     pga_status st = time_config(&res.initial_seconds);
     if (st) return st;
@@ -1170,6 +1173,7 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
     res.grid_blocks = saved_spec ? saved_spec_blocks : saved_blocks;
     res.specialized = saved_spec ? 1u : 0u;
     res.local_grab = saved_spec && saved_sched_local ? local_grab(ctx, num_axioms) : 0u;
+    res.local_spill = saved_spec && saved_sched_local ? saved_sched_spill : 0u;
     res.seconds = res.initial_seconds;
     res.max_level_parents = ctx->max_level_parents;
     res.max_level_children = ctx->max_level_children;
@@ -1188,16 +1192,22 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
             if (g == 1 && grids[1] >= full) continue;
             ctx->spec_on = true;
             ctx->spec_blocks = grids[g];
-            // level schedules: grid-wide, and workgroup-local with the
-            // automatic hand-out, half of it (two rounds) and four times it
+            // level schedules: grid-wide; workgroup-local with the automatic
+            // hand-out, half of it and four times it; and the automatic
+            // hand-out with levels wider than two rounds of the workgroup
+            // left to the grid
             ctx->sched_local = true;
             ctx->sched_grab = 0;
+            ctx->sched_spill = 0;
             const uint32_t autog = local_grab(ctx, num_axioms);
-            const uint32_t grabs[] = {0u, autog, std::max(autog / 2u, 1u), std::min(4u * autog, kLocalGrabMax)};
-            for (int k = 0; k < 4 && !st; ++k) {
-                if (k >= 2 && (grabs[k] == autog || grabs[k] == grabs[k - 1])) continue;
+            struct Sched { uint32_t grab, spill; };
+            const Sched scheds[] = {{0u, 0u}, {autog, 0u}, {std::max(autog / 2u, 1u), 0u},
+                                    {std::min(4u * autog, kLocalGrabMax), 0u}, {autog, 2u * ctx->spec_bs}};
+            for (int k = 0; k < 5 && !st; ++k) {
+                if ((k == 2 || k == 3) && (scheds[k].grab == autog || scheds[k].grab == scheds[k - 1].grab)) continue;
                 ctx->sched_local = k != 0;
-                ctx->sched_grab = grabs[k];
+                ctx->sched_grab = scheds[k].grab;
+                ctx->sched_spill = scheds[k].spill;
                 double s = 0.0;
                 st = time_config(&s);
                 ++res.candidates;
@@ -1207,12 +1217,14 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
                     res.block_size = ctx->spec_bs;
                     res.grid_blocks = grids[g];
                     res.specialized = 1u;
-                    res.local_grab = grabs[k];
+                    res.local_grab = scheds[k].grab;
+                    res.local_spill = scheds[k].spill;
                 }
             }
         }
         ctx->sched_local = saved_sched_local;
         ctx->sched_grab = saved_sched_grab;
+        ctx->sched_spill = saved_sched_spill;
         ctx->spec_on = false;
         ctx->cfg.technique = saved.technique;
     }
@@ -1259,6 +1271,7 @@ PGA_API pga_status pga_autotune(pga_context* ctx, uint32_t num_axioms, uint32_t 
         ctx->spec_blocks = res.grid_blocks;
         ctx->sched_local = res.local_grab != 0u;
         ctx->sched_grab = res.local_grab;
+        ctx->sched_spill = res.local_spill;
     } else {
         ctx->cfg.block_size = res.block_size;
         ctx->mega_blocks = res.grid_blocks;

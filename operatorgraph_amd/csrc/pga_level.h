@@ -304,36 +304,40 @@ struct LoadEntry {
     __device__ __forceinline__ void operator()(const Ctx& c, uint32_t i, F&& f) const { f(load_entry(c, i)); }
 };
 
-// Child i of a level: an axiom (level 0: axiom first + i) or the child that
-// a descriptor of the previous level names (parents below `in_cap`).
-// Returns false when there is none.
+// Child i of a level: an axiom (level 0: axiom first + i) or what a
+// descriptor names - the child of a parent below `in_cap`, or an axiom
+// (kDescAxiom). Returns false when there is none.
+// The record of axiom k. Returns false for an axiom without an entry.
+__device__ __forceinline__ bool load_axiom(const KArgs& a, const Ctx& c, uint32_t k, bool wr, Rec& r) {
+    const pga_axiom& ax = a.axioms[k];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) r.m[q] = ax.model[q];
+    r.s[0] = ax.size[0]; r.s[1] = ax.size[1]; r.s[2] = ax.size[2];
+    r.seed = ax.seed;
+    r.custom = ax.custom;
+    r.axiom = k;
+    r.path = 0u;
+    r.ed = 0u;
+    const int32_t ent = ax.entry_index;
+    if (ent < 0) return false;              // PGAFacade.cu:708
+    if ((uint32_t)ent >= c.prog.nentries || (int32_t)entry_shape(c, (uint32_t)ent) != ax.shape_type) {
+        if (wr) raise(c, PGA_ERR_BAD_PROGRAM);
+        return false;
+    }
+    r.ed = (uint32_t)ent;
+    return true;
+}
+
 template <class Visit>
 __device__ __forceinline__ bool load_child_at(const KArgs& a, const Ctx& c, uint32_t first, const Rec* par_in,
                                               uint32_t in_cap, const Desc* desc_in, bool wr, uint32_t level,
                                               uint32_t i, Rec& r, Visit visit) {
-    if (level == 0u) {
-        const pga_axiom& ax = a.axioms[first + i];
-#pragma unroll
-        for (int k = 0; k < 12; ++k) r.m[k] = ax.model[k];
-        r.s[0] = ax.size[0]; r.s[1] = ax.size[1]; r.s[2] = ax.size[2];
-        r.seed = ax.seed;
-        r.custom = ax.custom;
-        r.axiom = first + i;
-        r.path = 0u;
-        r.ed = 0u;
-        const int32_t ent = ax.entry_index;
-        if (ent < 0) return false;              // PGAFacade.cu:708
-        if ((uint32_t)ent >= c.prog.nentries || (int32_t)entry_shape(c, (uint32_t)ent) != ax.shape_type) {
-            if (wr) raise(c, PGA_ERR_BAD_PROGRAM);
-            return false;
-        }
-        r.ed = (uint32_t)ent;
-        return true;
-    }
+    if (level == 0u) return load_axiom(a, c, first + i, wr, r);
     const uint4 dv = *reinterpret_cast<const uint4*>(desc_in + i);
     const uint32_t parent = dv.x, jx = dv.y;
-    if (parent >= in_cap) return false;          // parent was dropped on overflow
     const uint32_t j = jx & 0x3FFFFFFFu;
+    if (j == kDescAxiom) return load_axiom(a, c, parent, wr, r);
+    if (parent >= in_cap) return false;          // parent was dropped on overflow
     if (j == kDescNone) return false;
     Rec p;
     load_rec(par_in + parent, p);

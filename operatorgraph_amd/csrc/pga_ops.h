@@ -344,6 +344,7 @@ struct __align__(16) Desc {
 static_assert(sizeof(Desc) == 16, "Desc must be 16 bytes");
 const uint32_t kDescSelf = 0x3FFFFFFFu;   // child = the parent record itself (chain continues)
 const uint32_t kDescNone = 0x3FFFFFFEu;   // no child (static Subdivide zero factor)
+const uint32_t kDescAxiom = 0x3FFFFFFDu;  // the axiom whose index is in `parent` (workgroup-local levels)
 
 __device__ __forceinline__ void axis_vec(uint32_t axis, float o, float& tx, float& ty, float& tz) {
     tx = axis == 0 ? o : 0.0f;

@@ -392,17 +392,20 @@ __device__ __forceinline__ bool local_scan(const KArgs& a, LevelLDS& L, uint32_t
 // The persistent derivation of every level (the MEGAKERNEL technique).
 //
 // With local arrays (a.loc_par), the levels run first inside each
-// workgroup: a block takes loc_grab axioms at a time (an atomic cursor) and
-// derives them level by level on its own - the same count and write passes
-// over the block's waves, a block scan (local_scan) for the grid scan, the
-// fan-outs into the block's double-buffered arrays and each level's
-// terminals claimed with one atomic per slot (the instances are the same
-// records; their order across blocks is unspecified). A level whose fan-outs
-// do not fit the block's arrays writes them to the grid's level-1 input
-// instead; when no axiom is left the grid-wide levels (count pass, grid scan,
-// write pass, grid barrier) take that spill up. A batch of small or uneven
-// derivations thus pays no grid barrier per level. Without local arrays the
-// grid-wide levels start from the axioms.
+// workgroup: a block takes loc_grab axioms at a time (an atomic cursor),
+// whenever its level is narrower than the block, and derives what it holds
+// level by level on its own - the same count and write passes over the
+// block's waves, a block scan (local_scan) for the grid scan, the fan-outs
+// into the block's double-buffered arrays and each level's terminals claimed
+// with one atomic per slot (the instances are the same records; their order
+// across blocks is unspecified). New axioms enter a level as kDescAxiom
+// descriptors beside the children of the ones taken earlier, so derivations
+// of different depths share passes. A level whose fan-outs do not fit the
+// block's arrays writes them to the grid's level-1 input instead; when a
+// block holds nothing and no axiom is left, the grid-wide levels (count
+// pass, grid scan, write pass, grid barrier) take that spill up. A batch of
+// small or uneven derivations thus pays no grid barrier per level. Without
+// local arrays the grid-wide levels start from the axioms.
 template <class S, int BS>
 __device__ __forceinline__ void spec_mega(const KArgs& a) {
     __shared__ LevelLDS L;
@@ -414,38 +417,47 @@ __device__ __forceinline__ void spec_mega(const KArgs& a) {
     Rec* const lp = a.loc_par + (size_t)blockIdx.x * 2u * kp;
     Desc* const ld = a.loc_desc + (size_t)blockIdx.x * 2u * kd;
     bool local = a.loc_par != nullptr;
-    uint32_t level = 0, buf = 0, first = a.first_axiom;
+    bool more = true;               // local: axioms may be left to hand out
+    uint32_t level = 0, buf = 0, since = 0;
     uint32_t n = local ? 0u : a.n_axioms;
     uint32_t tick = 0;              // timestamp row: local levels, then grid levels
     if (!local) init_bases(a, L);
 #pragma unroll 1
     for (;;) {
         if (local) {
-            if (n == 0u) {          // the block's next axioms, or the grid-wide levels
+            // a narrow level takes loc_grab more axioms (as kDescAxiom
+            // descriptors after the level's own), so the block's passes stay
+            // full while the derivations it holds go deeper
+            if (more && n < blockDim.x) {
                 if (threadIdx.x == 0)
                     L.grab = __hip_atomic_fetch_add(&a.bar[kBarGrab], a.loc_grab, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
                 __syncthreads();
                 const uint32_t g = L.grab;
-                __syncthreads();
                 if (g < a.n_axioms) {
-                    first = a.first_axiom + g;
-                    n = min(a.loc_grab, a.n_axioms - g);
-                    level = 0;
-                    buf = 0;
+                    const uint32_t cnt = min(a.loc_grab, a.n_axioms - g);
+                    Desc* const din = ld + buf * kd;      // n + cnt < blockDim + 64 <= kd
+                    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x)
+                        *reinterpret_cast<uint4*>(din + n + i) = make_uint4(a.first_axiom + g + i, kDescAxiom, 0u, 0u);
+                    n += cnt;
+                    since = 0;
                 } else {
-                    grid_barrier(a.bar, gridDim.x);     // every spill and instance claim is done
-                    const uint32_t spilled =
-                        __hip_atomic_load(&a.bar[kBarSpillDesc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (blockIdx.x == 0 && threadIdx.x == 0) a.desc_counts[0] = spilled;
-                    local = false;
-                    level = 1;
-                    n = min(spilled, a.cap_desc);
-                    init_bases(a, L);
-                    continue;
+                    more = false;
                 }
+                __syncthreads();
             }
-            if (level >= a.max_levels) {    // as the grid-wide levels: the frontier is dropped
+            if (n == 0u) {          // nothing held and nothing left: the grid-wide levels
+                grid_barrier(a.bar, gridDim.x);     // every spill and instance claim is done
+                const uint32_t spilled =
+                    __hip_atomic_load(&a.bar[kBarSpillDesc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (blockIdx.x == 0 && threadIdx.x == 0) a.desc_counts[0] = spilled;
+                local = false;
+                level = 1;
+                n = min(spilled, a.cap_desc);
+                init_bases(a, L);
+                continue;
+            }
+            if (since >= a.max_levels) {    // as the grid-wide levels: the frontier is dropped
                 if (threadIdx.x == 0) atomicOr(a.err, 1u << PGA_ERR_MAX_DEPTH);
                 n = 0;
                 continue;
@@ -456,7 +468,7 @@ __device__ __forceinline__ void spec_mega(const KArgs& a) {
         PassIO io;
         uint32_t wave, nwaves;
         if (local) {
-            io.first = first;
+            io.first = 0;
             io.par_in = lp + buf * kp;
             io.desc_in = ld + buf * kd;
             io.in_cap = kp;
@@ -472,8 +484,9 @@ __device__ __forceinline__ void spec_mega(const KArgs& a) {
             nwaves = (gridDim.x * blockDim.x) >> 6;
         }
         stamp(a, tick, 0);
+        const uint32_t plevel = local ? 1u : level;     // local levels read descriptors only
         for (uint32_t k = lane_id(); k < ncnt; k += 64u) L.cnt[w][k] = 0u;
-        spec_pass<S, false>(a, L, PW, level, n, wave, nwaves, io);
+        spec_pass<S, false>(a, L, PW, plevel, n, wave, nwaves, io);
         __syncthreads();
         bool spill = false;
         if (local) {
@@ -489,13 +502,14 @@ __device__ __forceinline__ void spec_mega(const KArgs& a) {
         } else {
             scan_counts(a, L, level, tick);
         }
-        spec_pass<S, true>(a, L, PW, level, n, wave, nwaves, io);
+        spec_pass<S, true>(a, L, PW, plevel, n, wave, nwaves, io);
         __syncthreads();
         stamp(a, tick, 3);
         ++tick;
         if (local) {
             n = spill ? 0u : L.level_total[nslots + 1u];
             buf ^= 1u;
+            ++since;
         } else {
             if (threadIdx.x == 0)
                 for (uint32_t k = 0; k < nslots; ++k) L.term_base[k] += L.level_total[k];

@@ -68,7 +68,7 @@ def load_library():
         "pga_timestamp_hz": (C.c_double, [P]),
         "pga_autotune": (C.c_int, [P, C.c_uint32, C.c_uint32, C.POINTER(abi.TuneResult)]),
         "pga_specialize": (C.c_int, [P, C.c_uint32, C.c_int32, C.POINTER(C.c_double)]),
-        "pga_set_schedule": (C.c_int, [P, C.c_int32, C.c_uint32]),
+        "pga_set_schedule": (C.c_int, [P, C.c_int32, C.c_uint32, C.c_uint32]),
         "pga_specialize_check": (C.c_int, [P, C.c_uint32, P, C.c_uint32, C.c_uint32, C.c_char_p, C.c_uint32,
                                            C.POINTER(C.c_double)]),
         "pga_tape_reserve": (C.c_int, [P, C.c_uint32, C.c_uint32]),
@@ -318,11 +318,12 @@ class Engine:
         self._check(self.L.pga_specialize(self._h, int(block_size), 1 if enable else 0, C.byref(sec)))
         return sec.value
 
-    def set_schedule(self, local=True, grab=0):
+    def set_schedule(self, local=True, grab=0, spill=0):
         """Level scheduling of the specialised kernel (pga_set_schedule):
-        workgroup-local levels with `grab` axioms per hand-out (0: automatic),
-        or every level over the grid (local=False)."""
-        self._check(self.L.pga_set_schedule(self._h, 1 if local else 0, int(grab)))
+        workgroup-local levels with `grab` axioms per hand-out (0: automatic)
+        and levels of more than `spill` children (0: no limit) left to the
+        grid, or every level over the grid (local=False)."""
+        self._check(self.L.pga_set_schedule(self._h, 1 if local else 0, int(grab), int(spill)))
         return self
 
     def autotune(self, n=None, repeats=3):
