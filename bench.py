@@ -68,6 +68,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--only-kernel", action="store_true",
                    help="run the timed batched steps only (for profilers)")
+    p.add_argument("--workload", choices=("facade", "stochastic"), default="facade",
+                   help="with --only-kernel: the facade lots (--lots) or the stochastic batch "
+                        "(--stochastic-axioms)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="device-to-device copies of MIB MiB (scale HBM byte counters); alone, or after "
                         "the steps of --only-kernel")
@@ -229,6 +232,26 @@ def stochastic_workload(torch, engine, grammars, axioms, n, steps):
     lb.close()
     eng.close()
     return out
+
+
+def stochastic_steps(torch, engine, grammars, axioms, n, steps, warmup):
+    """The stochastic batch's tuned derivation alone, `steps` timed launches
+    (for profilers)."""
+    ax, _ = axioms.stochastic_axioms(n)
+    eng = engine.Engine(queue_capacity=1 << 23)
+    eng.load(grammars.stochastic()).set_axioms(ax)
+    eng.set_philox_key(axioms.STOCHASTIC_KEY)
+    eng.bind(0, 0)
+    eng.execute()
+    T = int(eng.counts(1)[0])
+    eng.bind(0, T)
+    tune = tune_engine(eng, 3)
+    for _ in range(warmup):
+        eng.execute()
+    sec = timed_steps(torch, eng, steps, tune["technique"])
+    print(json.dumps({"workload": f"stochastic_{n}", "terminals_per_step": T, "ms_per_step": sec * 1e3,
+                      "kernel": kernel_name(tune), "autotune": tune_summary(tune)}), flush=True)
+    eng.close()
 
 
 MENGER5_CUBES = 20 ** 5
@@ -594,6 +617,9 @@ def main():
 
     from operatorgraph_amd import abi, axioms, engine, grammars
 
+    if args.only_kernel and args.workload == "stochastic":
+        stochastic_steps(torch, engine, grammars, axioms, args.stochastic_axioms, args.steps, args.warmup)
+        return
     eng, lots, T, tune = make_engine(torch, engine, grammars, axioms, args.lots)
     A = len(lots)
     for _ in range(args.warmup):
