@@ -134,10 +134,10 @@ class Engine:
             self.L.pga_destroy(self._h)
             self._h = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):   # bound now: module globals are gone at shutdown
         # at interpreter shutdown the HIP runtime may already be torn down:
         # the process releases the context's device memory and stream
-        if sys.is_finalizing():
+        if _finalizing():
             return
         try:
             self.close()
@@ -477,8 +477,8 @@ class Graph:
             self.engine.L.pga_graph_destroy(self._h)
             self._h = None
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):   # bound now: module globals are gone at shutdown
+        if _finalizing():
             return
         try:
             self.close()

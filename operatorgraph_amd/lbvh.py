@@ -97,10 +97,10 @@ class LBVH:
             self._h = None
         self._engine = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):   # bound now: module globals are gone at shutdown
         # at interpreter shutdown the HIP runtime may already be torn down:
         # the process releases the device memory
-        if sys.is_finalizing():
+        if _finalizing():
             return
         try:
             self.close()
