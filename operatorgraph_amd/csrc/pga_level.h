@@ -184,9 +184,10 @@ __device__ __forceinline__ void count_direct(const Ctx& c, LevelLDS& L, const E&
 
 // Starts the non-leaf fan-outs of the active lanes: claims and writes each
 // parent record (pidx = its index, ~0u when the queue is full) and claims
-// `n` consecutive descriptor slots from `dslot` on.
+// `n` consecutive descriptor slots from `dslot` on (unless !claim_desc; the
+// count pass counts them either way).
 __device__ __forceinline__ void start_fanout(const Ctx& c, LevelLDS& L, bool wr, bool active, const Rec& par,
-                                             uint32_t n, uint32_t& pidx, uint32_t& dslot) {
+                                             uint32_t n, uint32_t& pidx, uint32_t& dslot, bool claim_desc = true) {
     const uint32_t kP = c.prog.nslots, kD = c.prog.nslots + 1u;
     const unsigned long long pm = __ballot(active);
     const int pl = __ffsll((long long)pm) - 1;
@@ -209,6 +210,7 @@ __device__ __forceinline__ void start_fanout(const Ctx& c, LevelLDS& L, bool wr,
         if (lane_id() == 0) L.cnt[w][kD] += total;
         return;
     }
+    if (!claim_desc) return;        // the caller claims its descriptors child by child
     const uint32_t dbase = L.cur[w][kD];
     if (lane_id() == 0) L.cur[w][kD] = dbase + total;
     if (active) dslot = dbase + incl - nn;

@@ -291,30 +291,39 @@ __device__ __forceinline__ void spec_pass(const KArgs& a, LevelLDS& L, float4* P
                 emit_terminal(c, L, ST, WR, emit, slot, cur, idx);
             }
             if (__ballot(start) != 0ull) {
-                start_fanout(c, L, WR, start, par_load(PW), nd, pidx, dslot);
+                start_fanout(c, L, WR, start, par_load(PW), nd, pidx, dslot, false);
                 if (WR && start) {
-                    // the fan-out's descriptors in one loop (the running offsets
-                    // of fan_geom in child order), not one trip round the state
-                    // machine per child
+                    // the fan-out's descriptors in one loop over its children
+                    // (the running offsets of fan_geom in child order), child
+                    // jj of every lane of the wave side by side: each store
+                    // writes consecutive descriptors
                     visit(c, fe, [&](const auto& e) {
                         const Rec par = par_load(PW);
                         const bool geom = !self && is_fanout(e.kind());
+                        const uint32_t kD = c.prog.nslots + 1u;
 #pragma unroll 1
-                        for (uint32_t jj = 0; jj < f.n; ++jj) {
+                        for (uint32_t jj = 0; __ballot(jj < f.n) != 0ull; ++jj) {
+                            const bool act = jj < f.n;
                             float o = 0.0f, ns = 0.0f;
-                            const bool keep = geom ? fan_geom(c, par, e, f, jj, o, ns) : true;
+                            bool keep = true;
+                            if (act && geom) keep = fan_geom(c, par, e, f, jj, o, ns);
                             const bool skip = jj < 32u ? ((skipm >> jj) & 1u) != 0u : skipm == 0xFFFFFFFFu;
-                            if (skip) continue;
-                            const uint32_t jx = pidx == ~0u ? kDescNone
-                                              : self        ? kDescSelf
-                                              : keep        ? (jj | (f.axis << 30))
-                                                            : kDescNone;
-                            if (dslot < c.desc_cap)
-                                *reinterpret_cast<uint4*>(c.desc_out + dslot) =
-                                    make_uint4(pidx, jx, __float_as_uint(o), __float_as_uint(ns));
-                            else
-                                raise(c, PGA_ERR_OVERFLOW_QUEUE);
-                            ++dslot;
+                            const bool has = act && !skip;
+                            const unsigned long long m = __ballot(has);
+                            if (m == 0ull) continue;
+                            const uint32_t base = claim(L, true, kD, m, __ffsll((long long)m) - 1);
+                            if (has) {
+                                const uint32_t ds = base + mbcnt(m);
+                                const uint32_t jx = pidx == ~0u ? kDescNone
+                                                  : self        ? kDescSelf
+                                                  : keep        ? (jj | (f.axis << 30))
+                                                                : kDescNone;
+                                if (ds < c.desc_cap)
+                                    *reinterpret_cast<uint4*>(c.desc_out + ds) =
+                                        make_uint4(pidx, jx, __float_as_uint(o), __float_as_uint(ns));
+                                else
+                                    raise(c, PGA_ERR_OVERFLOW_QUEUE);
+                            }
                         }
                     });
                 }
