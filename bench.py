@@ -68,6 +68,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--only-kernel", action="store_true",
                    help="run the timed batched steps only (for profilers)")
+    p.add_argument("--spec-block", type=int, default=768,
+                   help="workgroup size of the kernel specialised to the program")
     p.add_argument("--workload", choices=("facade", "stochastic"), default="facade",
                    help="with --only-kernel: the facade lots (--lots) or the stochastic batch "
                         "(--stochastic-axioms)")
@@ -94,7 +96,10 @@ def make_engine(torch, engine_mod, grammars, axioms, n_lots, key=None):
     return eng, lots, T, tune
 
 
-SPEC_BLOCK = 768   # workgroup size of the kernel specialised to the program
+SPEC_BLOCK = 768   # workgroup size of the kernel specialised to the program (--spec-block)
+# the kernel sources a traffic measurement under profiles/ was taken with
+KERNEL_SOURCES = ["pga.h", "pga_device.h", "pga_ops.h", "pga_kernels.h", "pga_level.h", "pga_spec.h",
+                  "pga_spec.cpp", "pga_kernels.hip"]
 
 
 def tune_engine(eng, repeats):
@@ -593,18 +598,36 @@ def tune_summary(tune):
 
 
 def load_traffic(workload, kernel=None):
-    """HBM bytes per launch from the committed counter summary: the entry
-    of `kernel` when it was measured, else the summary's main kernel."""
+    """HBM bytes per launch from the committed counter summary (the entry of
+    `kernel`, else the summary's main kernel) and whether the summary was
+    measured with the kernel sources of this tree (their digest matches)."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
-    if os.path.exists(path):
-        with open(path) as f:
-            d = json.load(f)
-        return d.get("per_kernel", {}).get(kernel, d.get("traffic_bytes_per_launch"))
-    return None
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        d = json.load(f)
+    value = d.get("per_kernel", {}).get(kernel, d.get("traffic_bytes_per_launch"))
+    return value, d.get("kernel_sources_sha256") == kernel_sources_digest()
+
+
+def kernel_sources_digest():
+    """SHA-256 over the derivation kernel's sources, in KERNEL_SOURCES order."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        for d in (os.path.join(ROOT, "operatorgraph_amd", "csrc"), os.path.join(ROOT, "include")):
+            path = os.path.join(d, name)
+            if os.path.exists(path):
+                with open(path, "rb") as f:
+                    h.update(f.read())
+                break
+    return h.hexdigest()
 
 
 def main():
+    global SPEC_BLOCK
     args = parse()
+    SPEC_BLOCK = args.spec_block
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the derivation shards by axiom with no exchange between GPUs)", file=sys.stderr)
@@ -641,9 +664,11 @@ def main():
         graph = {"ms_per_step": round(gsec * 1e3, 4), "terminals_per_s": round(T / gsec),
                  "device_ms": round(gdev * 1e3, 4)}
     alg_bytes = INSTANCE_BYTES * T + AXIOM_BYTES * A
+    traffic, traffic_current = load_traffic(f"facade_lots_{args.lots}", kernel_name(tune))
     achieved = alg_bytes / ksec / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(f"facade_lots_{args.lots}", kernel_name(tune)),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_measured_on_these_sources": traffic_current,
                 "kernel": kernel_name(tune), "kernel_ms": round(ksec * 1e3, 4),
                 "algorithmic_bytes_per_launch": alg_bytes}
 
