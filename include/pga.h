@@ -41,6 +41,10 @@ extern "C" {
 #define PGA_MAX_EXP_STACK 4
 /* instanced meshes per scene (include/pga/rendering/RenderingConstants.h) */
 #define PGA_MAX_TERMINALS 64
+/* phases of a multi-phase program (MultiPhaseEvaluator<..., NumPhasesT, ...>,
+ * GPUEvaluator.cuh:395-456): successor phase indices 0..15; the instance
+ * meta then holds entry | depth << 16 | phase << 28, with depth < 4096 */
+#define PGA_MAX_PHASES 16
 
 /* ---- status codes (the reference traps or throws; see each code) ---- */
 typedef enum pga_status {
@@ -270,6 +274,21 @@ PGA_API pga_status pga_execute(pga_context* ctx, uint32_t num_axioms, double* se
 PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first_axiom, uint32_t num_axioms,
                                      int32_t append, double* seconds);
 PGA_API pga_status pga_get_stats(const pga_context* ctx, pga_stats* stats);
+
+/* -- context sensitivity (MultiPhaseEvaluator::execute,
+ *    GPUEvaluator.cuh:416-440; Collider.cuh:26-42, IfCollides.cuh:26-58).
+ *    A program whose successors name phases 1..15 runs one derivation per
+ *    phase inside each execute: phase p emits the terminals of the shapes
+ *    dispatched into phase p, and Collider stores shapes that IfCollides in
+ *    later phases tests against (an oriented-box separating-axis test,
+ *    Collision.cuh). Such programs, and any program with Collider or
+ *    IfCollides, run on the interpreting kernels only: pga_specialize,
+ *    pga_forward_tape and graph capture return PGA_ERR_UNSUPPORTED.
+ *    pga_set_collider_capacity: colliders one execute may store (0: 2^20,
+ *    80 bytes each; more raise PGA_ERR_OVERFLOW_QUEUE). pga_num_colliders:
+ *    the colliders the last execute stored and the program's phases. -- */
+PGA_API pga_status pga_set_collider_capacity(pga_context* ctx, uint32_t capacity);
+PGA_API pga_status pga_num_colliders(const pga_context* ctx, uint32_t* colliders, uint32_t* phases);
 
 /* -- asynchronous form: enqueues the derivation on the context stream and
  *    returns without waiting (MEGAKERNEL technique only). pga_synchronize

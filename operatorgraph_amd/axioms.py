@@ -185,3 +185,56 @@ def param_lots(n, key=GRADIENT_KEY, perturb=0.0, defaults=None):
             noise[:, 2 * k + 1] = r * np.sin(2.0 * np.pi * u[:, 1])
         p = p + perturb * noise
     return a, p.astype(np.float32)
+
+
+CITY_KEY = (0x5EED0005, 0)
+
+
+def city_lots(n, key=CITY_KEY, columns=None, pitch=16.0, entry_index=0):
+    """Lots of `grammars.context_city()`: width and depth uniform in [6, 12),
+    height 3, seeds uniform in [0, 1) (Philox4x32-10, counter (lot, 3, 0, 0)),
+    on a grid of `pitch` (lots never overlap)."""
+    ids = np.arange(n, dtype=np.uint32)
+    ctr = np.zeros((n, 4), dtype=np.uint32)
+    ctr[:, 0] = ids
+    ctr[:, 1] = 3
+    u = uniform24(philox4x32_10(ctr, key)).astype(np.float64)
+    if columns is None:
+        columns = max(1, int(np.ceil(np.sqrt(n))))
+    a = empty(n)
+    a["size"][:, 0] = (6.0 + 6.0 * u[:, 0]).astype(np.float32)
+    a["size"][:, 1] = 3.0
+    a["size"][:, 2] = (6.0 + 6.0 * u[:, 1]).astype(np.float32)
+    a["model"][:, 3] = (ids % columns).astype(np.float32) * np.float32(pitch)
+    a["model"][:, 11] = (ids // columns).astype(np.float32) * np.float32(pitch)
+    a["seed"] = u[:, 2].astype(np.float32)
+    a["custom"] = a["seed"]
+    a["entry_index"] = entry_index
+    a["shape_type"] = abi.BOX
+    return a
+
+
+def collision_pair_axioms(kinds, rng, spread=1.6, quad_depth=0.0):
+    """Axioms and per-axiom parameters of `grammars.collision_pairs()`: pair i
+    is axioms 2i (A) and 2i + 1 (B) with shape types kinds[i] = (type_a,
+    type_b). Sizes uniform in [0.4, 2), rotations uniform in [0, 360)
+    degrees, A's translation uniform in [-0.2, 0.2)^3 and B's in
+    [-spread, spread)^3; Quads get z extent `quad_depth`. Tag i + 1.
+    Returns (axioms, params[2n, COLLISION_PAIR_SLOTS])."""
+    from .grammars import COLLISION_PAIR_ENTRIES, COLLISION_PAIR_SLOTS
+    n = len(kinds)
+    a = empty(2 * n)
+    p = np.zeros((2 * n, COLLISION_PAIR_SLOTS), dtype=np.float32)
+    for i, (ta, tb) in enumerate(kinds):
+        for r, (role, t) in enumerate((("A", ta), ("B", tb))):
+            k = 2 * i + r
+            a[k]["entry_index"] = COLLISION_PAIR_ENTRIES[(role, t)]
+            a[k]["shape_type"] = t
+            size = rng.uniform(0.4, 2.0, 3)
+            if t == abi.QUAD:
+                size[2] = quad_depth
+            p[k, 0:3] = size
+            p[k, 3:6] = rng.uniform(0.0, 360.0, 3)
+            p[k, 6:9] = rng.uniform(-0.2, 0.2, 3) if role == "A" else rng.uniform(-spread, spread, 3)
+            p[k, 9] = i + 1
+    return a, p

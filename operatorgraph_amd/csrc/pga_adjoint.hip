@@ -371,7 +371,7 @@ __device__ __forceinline__ void fan_adj(const Ctx& c, const AdjArgs& g, GradAcc&
     if (kind == K_SUBDIV) {
         // r2a = (total - sum of absolute factors) / (sum of |relative factors|)
         float abs_sum = 0.0f, rel_sum = 0.0f;
-        const uint32_t ax = r.axiom, pa = r.path, de = rec_depth(r);
+        const uint32_t ax = r.axiom, pa = r.path, de = rec_depth(c, r);
         for (uint32_t i = 1; i < e.nparams(); ++i) {
             const float fac = P(c, e, i, r, ax, pa, de);
             abs_sum += (float)(fac > 0.0f) * fac;
@@ -445,6 +445,13 @@ __device__ __forceinline__ Ctx adj_ctx(const KArgs& a) {
     c.seed = a.use_seed;
     c.tape = 1u;
     c.prog_lds = 0u;
+    c.dmask = 0xFFFFu;       // one-phase programs only
+    c.phase = 0u;
+    c.wr = 0u;
+    c.coll = nullptr;
+    c.coll_count = nullptr;
+    c.coll_visible = nullptr;
+    c.coll_cap = 0u;
     return c;
 }
 

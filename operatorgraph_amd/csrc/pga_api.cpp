@@ -53,6 +53,8 @@ struct pga_context {
     uint32_t lds_bytes = 0;           // dynamic LDS of the derivation kernels
     uint32_t count_geom = 0;          // 1: some decision of the program reads the model matrix
     uint32_t use_seed = 0;            // 1: some parameter reads a shape seed
+    uint32_t nphases = 1;             // 1 + the largest phase a successor names
+    bool collision = false;           // the program holds Collider or IfCollides entries
     DevEntry* d_entries = nullptr;    // views into d_image
     DevParam* d_params = nullptr;
     int32_t* d_succ = nullptr;
@@ -94,6 +96,14 @@ struct pga_context {
     uint32_t* h_pinned = nullptr;   // count readback
     uint32_t mega_blocks = 0;
     uint64_t max_level_parents = 0, max_level_children = 0;   // of the last execute
+
+    // collider store of Collider / IfCollides (80 bytes per collider) and its
+    // counters: [0] stored, [1] visible to the running phase
+    float4* d_coll = nullptr;
+    uint32_t coll_cap = 0;
+    uint32_t coll_cap_req = 0;        // pga_set_collider_capacity (0: kDefaultColliders)
+    uint32_t* d_coll_count = nullptr;
+    uint32_t colliders = 0;           // stored by the last execute
 
     // derivation tape (reverse mode)
     pga::Rec* d_tape_par = nullptr;
@@ -149,6 +159,7 @@ struct pga_graph {
 namespace {
 
 const uint32_t kDefaultCapacity = 1u << 22;
+const uint32_t kDefaultColliders = 1u << 20;   // 80 MB of collider store
 const uint32_t kDefaultMaxLevels = 4096;
 const uint32_t kMaxChain = 256;
 const uint32_t kMaxProgramLds = 60u * 1024u;   // program image + terminal tables
@@ -189,7 +200,12 @@ template <class T> void hfree(T*& p) {
 
 bool is_single(uint8_t k) {
     return k == pga::K_IFSIZELESS || k == pga::K_IF || k == pga::K_RANDOM || k == pga::K_TRANSLATE ||
-           k == pga::K_ROTATE || k == pga::K_SCALE || k == pga::K_SWAPSIZE || k == pga::K_EXTRUDE;
+           k == pga::K_ROTATE || k == pga::K_SCALE || k == pga::K_SWAPSIZE || k == pga::K_EXTRUDE ||
+           k == pga::K_COLLIDER || k == pga::K_IFCOLLIDES;
+}
+// single-child operators that choose among their successors
+bool is_branch(uint8_t k) {
+    return k == pga::K_IFSIZELESS || k == pga::K_IF || k == pga::K_RANDOM || k == pga::K_IFCOLLIDES;
 }
 bool is_fan(uint8_t k) {
     return k == pga::K_SUBDIV || k == pga::K_REPEAT || k == pga::K_COMPSPLIT || k == pga::K_REPLICATE;
@@ -257,6 +273,9 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
     int32_t max_slot = -1;
     bool count_geom = false;
     bool use_seed = false;
+    bool collision = false;
+    uint32_t nphases = 1;
+    std::vector<uint8_t> Sp;   // phase of each successor in Sc
     for (uint32_t i = 0; i < n; ++i) {
         const pga_dispatch_entry& e = entries[i];
         std::string at = "entry " + std::to_string(i) + ": ";
@@ -294,6 +313,9 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
             need_s = e.num_parameters > 0 ? e.num_parameters - 1 : 0;
             break;
         case PGA_OP_REPLICATE: d.kind = pga::K_REPLICATE; break;
+        case PGA_OP_COLLIDER: d.kind = pga::K_COLLIDER; need_p = 1; need_s = 1; collision = true; break;
+        case PGA_OP_IFCOLLIDES:   // decides on the model matrix in the count pass too
+            d.kind = pga::K_IFCOLLIDES; need_p = 1; need_s = 2; collision = true; count_geom = true; break;
         case PGA_OP_GENERATE:
             // generation function 0 (triangle soup) is derived into instance
             // records as well; pga_tessellate turns them into its geometry
@@ -356,7 +378,12 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         for (uint32_t k = 0; k < e.num_successors; ++k) {
             int32_t s = e.successors[k].entry_index;
             if (s < -1 || s >= (int32_t)n) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "successor out of range");
+            // a negative (unassigned) phase index is phase 0
+            const int32_t ph = e.successors[k].phase_index;
+            if (ph >= (int32_t)PGA_MAX_PHASES) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "phase index out of range");
             Sc.push_back(s);
+            Sp.push_back(ph > 0 ? (uint8_t)ph : (uint8_t)0);
+            if (ph > 0) nphases = std::max(nphases, (uint32_t)ph + 1u);
         }
         E[i] = d;
     }
@@ -385,7 +412,7 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         if (!is_single(d.kind)) { memo[s] = 0; return false; }
         memo[s] = 2;
         bool ok = true;
-        uint32_t ns = (d.kind == pga::K_IFSIZELESS || d.kind == pga::K_IF || d.kind == pga::K_RANDOM) ? d.nsucc : 1u;
+        uint32_t ns = is_branch(d.kind) ? d.nsucc : 1u;
         for (uint32_t k = 0; k < ns && ok; ++k) ok = ends(Sc[d.sbase + k], len + 1);
         memo[s] = ok ? 1 : 0;
         return ok;
@@ -489,6 +516,20 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         d.ndirect = (uint8_t)nd;
         d.flags |= pga::EF_DIRECT;
     }
+    // multi-phase programs: each successor carries its phase in bits 16-19,
+    // and no fan-out counts its terminals from a table (a child's phase
+    // decides whether a launch emits it)
+    if (nphases > 1) {
+        for (size_t k = 0; k < Sc.size(); ++k)
+            if (Sc[k] >= 0) Sc[k] |= (int32_t)Sp[k] << 16;
+        for (uint32_t i = 0; i < n; ++i) {
+            DevEntry& d = E[i];
+            d.flags &= (uint8_t)~pga::EF_DIRECT;
+            d.ndirect = 0;
+            d.aux[1] = 0u;
+            for (uint32_t k = 0; k < 3; ++k) d.sv[k] = k < d.nsucc ? Sc[d.sbase + k] : -1;
+        }
+    }
     if (Sc.empty()) Sc.push_back(-1);
     if (Pm.empty()) Pm.push_back(DevParam{});
     if (Ex.empty()) Ex.push_back(0u);
@@ -513,7 +554,20 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         ctx->count_geom = count_geom ? 1u : 0u;
         ctx->use_seed = use_seed ? 1u : 0u;
         ctx->max_param_slot = max_slot;
+        ctx->nphases = nphases;
+        ctx->collision = collision;
         return PGA_OK;
+    }
+    if (collision) {
+        const uint32_t want = ctx->coll_cap_req ? ctx->coll_cap_req : kDefaultColliders;
+        if (!ctx->d_coll || ctx->coll_cap != want) {
+            if (ctx->stream) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            hfree(ctx->d_coll);
+            ctx->coll_cap = 0;
+            HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll, (size_t)want * 5u * sizeof(float4)));
+            ctx->coll_cap = want;
+        }
+        if (!ctx->d_coll_count) HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_count, 2u * sizeof(uint32_t)));
     }
     hfree(ctx->d_image);
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_image, image_bytes));
@@ -548,6 +602,8 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
     ctx->h_succ = Sc;
     ctx->h_exp = Ex;
     ctx->max_param_slot = max_slot;
+    ctx->nphases = nphases;
+    ctx->collision = collision;
     ctx->initialized = true;
     return PGA_OK;
 }
@@ -630,6 +686,14 @@ pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
     a.max_chain = kMaxChain;
     a.count_geom = ctx->count_geom;
     a.use_seed = ctx->use_seed;
+    a.dmask = ctx->nphases > 1 ? 0x0FFFu : 0xFFFFu;
+    a.phase = 0;
+    if (ctx->collision) {
+        a.coll = ctx->d_coll;
+        a.coll_count = ctx->d_coll_count;
+        a.coll_visible = ctx->d_coll_count + 1;
+        a.coll_cap = ctx->coll_cap;
+    }
     if (ctx->spec_on && ctx->sched_local && ctx->d_loc && ctx->spec_blocks <= ctx->loc_blocks) {
         a.loc_par = reinterpret_cast<pga::Rec*>(ctx->d_loc);
         a.loc_desc = reinterpret_cast<pga::Desc*>(reinterpret_cast<char*>(ctx->d_loc) +
@@ -794,6 +858,7 @@ PGA_API void pga_destroy(pga_context* ctx) {
     for (int b = 0; b < 2; ++b) { hfree(ctx->d_par[b]); hfree(ctx->d_desc[b]); }
     hfree(ctx->d_term_counts); hfree(ctx->d_par_counts); hfree(ctx->d_desc_counts);
     hfree(ctx->d_err); hfree(ctx->d_bar); hfree(ctx->d_block_totals); hfree(ctx->d_loc);
+    hfree(ctx->d_coll); hfree(ctx->d_coll_count);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -926,6 +991,11 @@ static pga_status finish(pga_context* ctx, uint32_t count, uint32_t host_levels)
     HIP_TRY(ctx, hipMemcpy(tc, ctx->d_term_counts, sizeof(tc), hipMemcpyDeviceToHost));
     uint32_t levels = host_levels;
     if (ctx->cfg.technique == PGA_TECHNIQUE_MEGAKERNEL && count > 0) levels = lc[ctx->cfg.max_levels + 1];
+    ctx->colliders = 0;
+    if (ctx->collision && ctx->d_coll_count) {
+        HIP_TRY(ctx, hipMemcpy(&ctx->colliders, ctx->d_coll_count, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        ctx->colliders = std::min(ctx->colliders, ctx->coll_cap);
+    }
     // the specialised kernel's workgroup-local levels: the deepest and the
     // parent records they wrote
     ctx->stats.levels = std::max(levels, pc[ctx->cfg.max_levels + 1]);
@@ -943,18 +1013,46 @@ static pga_status finish(pga_context* ctx, uint32_t count, uint32_t host_levels)
     return status_from_err(ctx, err);
 }
 
+// Programs with context sensitivity run on the interpreting kernels only,
+// eagerly: returns an error for the paths that cannot run them.
+static pga_status one_phase_only(pga_context* ctx, const char* what) {
+    if (ctx->nphases > 1 || ctx->collision)
+        return fail(ctx, PGA_ERR_UNSUPPORTED, std::string(what) + " does not run programs with Collider, "
+                                              "IfCollides or more than one phase");
+    return PGA_OK;
+}
+
+// The derivation of a multi-phase program is one persistent launch per phase
+// (MultiPhaseEvaluator::execute, GPUEvaluator.cuh:416-440): every launch
+// derives from the axioms again, emits the terminals and stores the
+// colliders of its own phase only, and carries the shapes of the phases up
+// to its own (their IfCollides decisions read the colliders of earlier
+// phases, so the re-derivation repeats them). Between launches the reset
+// keeps the error word and the terminal counts, and publishes the colliders
+// stored so far.
+static pga_status launch_phases(pga_context* ctx, pga::KArgs& a, const void** kernel) {
+    for (uint32_t p = 0; p < ctx->nphases; ++p) {
+        a.phase = p;
+        if (p > 0) HIP_TRY(ctx, pga_launch_reset(a, 2, ctx->stream));
+        HIP_TRY(ctx, launch_derivation(ctx, a, kernel));
+    }
+    a.phase = 0;
+    return PGA_OK;
+}
+
 static pga_status execute_async(pga_context* ctx, uint32_t first, uint32_t count, int32_t append) {
     pga_status st = check_launch(ctx, first, count);
     if (st) return st;
     if (ctx->cfg.technique != PGA_TECHNIQUE_MEGAKERNEL)
         return fail(ctx, PGA_ERR_UNSUPPORTED, "asynchronous execution needs the MEGAKERNEL technique");
+    if (ctx->capturing && (st = one_phase_only(ctx, "graph capture"))) return st;
     pga::KArgs a = make_args(ctx, first, count);
     const bool timed = !ctx->capturing;     // the events time eager launches only
     HIP_TRY(ctx, pga_launch_reset(a, append ? 0 : 1, ctx->stream));
     if (timed) HIP_TRY(ctx, hipEventRecord(ctx->evk0, ctx->stream));
     const void* kernel = nullptr;
-    if (count > 0) HIP_TRY(ctx, launch_derivation(ctx, a, &kernel));
-    else kernel = ctx->spec_on ? (const void*)ctx->spec_fn : nullptr;
+    if (count > 0 && (st = launch_phases(ctx, a, &kernel))) return st;
+    if (count == 0) kernel = ctx->spec_on ? (const void*)ctx->spec_fn : nullptr;
     if (timed) HIP_TRY(ctx, hipEventRecord(ctx->evk1, ctx->stream));
     if (ctx->capturing) ctx->captured.push_back({a, ctx->mega_blocks, ctx->cfg.block_size, first, count, kernel});
     ctx->pending_count = count;
@@ -1000,22 +1098,27 @@ PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first, uint32_t 
     const uint32_t bs = ctx->cfg.block_size;
     HIP_TRY(ctx, hipEventRecord(ctx->evk0, ctx->stream));
     if (ctx->cfg.technique == PGA_TECHNIQUE_KERNELS) {
-        uint32_t n = count;
-        while (n > 0) {
-            if (levels >= ctx->cfg.max_levels) {
+        for (uint32_t p = 0; p < ctx->nphases; ++p) {   // one level loop per phase
+            a.phase = p;
+            if (p > 0) HIP_TRY(ctx, pga_launch_reset(a, 2, ctx->stream));
+            uint32_t n = count, level = 0;
+            while (n > 0) {
+                if (level >= ctx->cfg.max_levels) {
+                    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                    return fail(ctx, PGA_ERR_MAX_DEPTH, "maximum number of derivation levels reached");
+                }
+                HIP_TRY(ctx, pga_launch_level(a, level, ctx->mega_blocks, bs, ctx->stream));
+                HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->d_desc_counts + level, sizeof(uint32_t),
+                                            hipMemcpyDeviceToHost, ctx->stream));
                 HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-                return fail(ctx, PGA_ERR_MAX_DEPTH, "maximum number of derivation levels reached");
+                n = std::min(ctx->h_pinned[0], ctx->cap);
+                ++level;
             }
-            HIP_TRY(ctx, pga_launch_level(a, levels, ctx->mega_blocks, bs, ctx->stream));
-            HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->d_desc_counts + levels, sizeof(uint32_t),
-                                        hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-            n = std::min(ctx->h_pinned[0], ctx->cap);
-            ++levels;
+            levels += level;
         }
     } else if (count > 0) {
         const void* kernel = nullptr;
-        HIP_TRY(ctx, launch_derivation(ctx, a, &kernel));
+        if ((st = launch_phases(ctx, a, &kernel))) return st;
     }
     HIP_TRY(ctx, hipEventRecord(ctx->evk1, ctx->stream));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -1051,6 +1154,29 @@ PGA_API pga_status pga_get_stats(const pga_context* ctx, pga_stats* stats) {
     return PGA_OK;
 }
 
+PGA_API pga_status pga_set_collider_capacity(pga_context* ctx, uint32_t capacity) {
+    if (!ctx) return PGA_ERR_INVALID_ARG;
+    NO_CAPTURE(ctx, "pga_set_collider_capacity");
+    if (capacity > (1u << 26)) return fail(ctx, PGA_ERR_INVALID_ARG, "more than 2^26 colliders");
+    ctx->coll_cap_req = capacity;
+    if (!ctx->collision) return PGA_OK;      // allocated when a program stores colliders
+    const uint32_t want = capacity ? capacity : kDefaultColliders;
+    if (ctx->d_coll && ctx->coll_cap == want) return PGA_OK;
+    if (ctx->stream) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    hfree(ctx->d_coll);
+    ctx->coll_cap = 0;
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll, (size_t)want * 5u * sizeof(float4)));
+    ctx->coll_cap = want;
+    return PGA_OK;
+}
+
+PGA_API pga_status pga_num_colliders(const pga_context* ctx, uint32_t* colliders, uint32_t* phases) {
+    if (!ctx) return PGA_ERR_INVALID_ARG;
+    if (colliders) *colliders = ctx->colliders;
+    if (phases) *phases = ctx->initialized ? ctx->nphases : 0u;
+    return PGA_OK;
+}
+
 // Workgroups of one launch at workgroup size bs: all that fit on the device
 // at once (cooperative launch), at most kMaxBlocksPerCu per compute unit.
 static pga_status resident_blocks(pga_context* ctx, uint32_t bs, uint32_t* blocks) {
@@ -1069,6 +1195,7 @@ PGA_API pga_status pga_specialize(pga_context* ctx, uint32_t block_size, int32_t
         return PGA_OK;
     }
     if (!ctx->initialized) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no program loaded");
+    if (pga_status st = one_phase_only(ctx, "the specialised kernel")) return st;
     if (block_size == 0) block_size = kDefaultBlockSize;
     if (!spec_block_size_supported(block_size))
         return fail(ctx, PGA_ERR_INVALID_ARG, "unsupported workgroup size");
@@ -1124,7 +1251,10 @@ PGA_API pga_status pga_specialize_check(const pga_procedure* procedures, uint32_
     ctx->cfg.block_size = block_size;
     std::string msg;
     pga_status st = build_program(ctx.get(), procedures, num_procedures, entries, num_entries, false);
-    if (st == PGA_OK) {
+    if (st == PGA_OK) st = one_phase_only(ctx.get(), "the specialised kernel");
+    if (st != PGA_OK) {
+        msg = ctx->error;
+    } else {
         const auto t0 = std::chrono::steady_clock::now();
         std::vector<char> code;
         bool cached = false;
@@ -1133,8 +1263,6 @@ PGA_API pga_status pga_specialize_check(const pga_procedure* procedures, uint32_
             st = PGA_ERR_UNSUPPORTED;
         else if (seconds && !cached)
             *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    } else {
-        msg = ctx->error;
     }
     if (log && log_size) {
         const size_t m = std::min<size_t>(msg.size(), log_size - 1u);
@@ -1304,6 +1432,7 @@ PGA_API pga_status pga_forward_tape(pga_context* ctx, uint32_t num_axioms, doubl
     NO_CAPTURE(ctx, "pga_forward_tape");
     pga_status st = check_launch(ctx, 0, num_axioms);
     if (st) return st;
+    if ((st = one_phase_only(ctx, "the derivation tape"))) return st;
     if (!ctx->d_tape_par) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no tape: call pga_tape_reserve first");
     if (num_axioms > ctx->tape_child_cap) return fail(ctx, PGA_ERR_OVERFLOW_QUEUE, "tape smaller than the axioms");
     if (ctx->tape_blocks == 0) {

@@ -31,6 +31,8 @@ enum Kind : uint8_t {
     K_REPEAT,
     K_COMPSPLIT,
     K_REPLICATE,
+    K_COLLIDER,       // stores the shape for IfCollides of later phases, one successor
+    K_IFCOLLIDES,     // successor 0 if the shape collides with a stored collider, else 1
 };
 enum EntryFlags : uint8_t {
     EF_STATIC = 1,    // static-parameter operator form
@@ -87,14 +89,14 @@ struct __align__(16) Rec {
     float s[3];         // size
     float seed;
     float custom;
-    uint32_t ed;        // entry | depth << 16
+    uint32_t ed;        // entry | depth << 16; in multi-phase programs
+                        // entry | depth << 16 | phase << 28 (depth < 4096)
     uint32_t axiom;
     uint32_t path;
 };
 static_assert(sizeof(Rec) == 80, "Rec must be 80 bytes");
 
 __device__ __forceinline__ uint32_t rec_entry(const Rec& r) { return r.ed & 0xFFFFu; }
-__device__ __forceinline__ uint32_t rec_depth(const Rec& r) { return r.ed >> 16; }
 
 struct Desc;
 
@@ -119,7 +121,23 @@ struct Ctx {
     uint32_t seed;                      // 0: no parameter reads a seed, the seed chain is skipped
     uint32_t tape;                      // 1: recording a tape (no in-place leaf expansion)
     uint32_t prog_lds;                  // 1: the program tables are the LDS copy (derivation kernels), 0: global
+    // multi-phase programs: depth bits of Rec::ed (0xFFFF: one phase, 0x0FFF:
+    // the phase in bits 28-31), the phase this launch derives, and the
+    // collider store (5 x 16 bytes per collider: model rows, size and tag,
+    // shape type and phase), of which the first *coll_visible were stored by
+    // earlier phases
+    uint32_t dmask;
+    uint32_t phase;
+    uint32_t wr;                        // 1: write pass (colliders are stored once, here)
+    float4* coll;
+    uint32_t* coll_count;
+    const uint32_t* coll_visible;
+    uint32_t coll_cap;
 };
+
+__device__ __forceinline__ uint32_t rec_depth(const Ctx& c, const Rec& r) { return (r.ed >> 16) & c.dmask; }
+__device__ __forceinline__ bool multi_phase(const Ctx& c) { return c.dmask != 0xFFFFu; }
+__device__ __forceinline__ uint32_t rec_phase(const Ctx& c, const Rec& r) { return multi_phase(c) ? r.ed >> 28 : 0u; }
 
 __device__ __forceinline__ void raise(const Ctx& c, int status) {
     atomicOr(c.err, 1u << status);

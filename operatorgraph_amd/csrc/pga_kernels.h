@@ -50,6 +50,15 @@ struct KArgs {
     uint32_t max_chain;
     uint32_t count_geom;                // 1: some decision reads the model matrix
     uint32_t use_seed;                  // 1: some parameter reads a shape seed
+    // multi-phase programs (one launch per phase): Rec::ed depth bits
+    // (0xFFFF: one phase), the phase derived, and the collider store with its
+    // append counter and the count stored by earlier phases
+    uint32_t dmask;
+    uint32_t phase;
+    float4* coll;
+    uint32_t* coll_count;
+    const uint32_t* coll_visible;
+    uint32_t coll_cap;
     // derivation tape (reverse mode): every level's parent records and
     // descriptors kept (cap_par / cap_desc = their capacities), every
     // fan-out materialised, and one outcome per child
@@ -92,7 +101,9 @@ constexpr uint32_t kSlotFloats = 20;    // adjoint slot of a child (80 bytes)
 
 #ifndef __HIPCC_RTC__   // host launchers
 bool pga_block_size_supported(uint32_t block_size);   // 256, 512, 768 or 1024
-hipError_t pga_launch_reset(const pga::KArgs& a, int reset_terms, hipStream_t s);
+// flags: bit 0 zeroes the terminal counts (a fresh run), bit 1 keeps the
+// error word and snapshots the collider count (a later phase of the run)
+hipError_t pga_launch_reset(const pga::KArgs& a, int flags, hipStream_t s);
 hipError_t pga_launch_level(const pga::KArgs& a, uint32_t level, uint32_t blocks, uint32_t block_size,
                             hipStream_t s);
 hipError_t pga_mega_occupancy(uint32_t block_size, uint32_t lds_bytes, int* blocks_per_cu);

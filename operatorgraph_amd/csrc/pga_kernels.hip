@@ -92,10 +92,12 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
                 const EntryV e = load_entry(c, rec_entry(cur));
                 const uint32_t k = e.kind();
                 const uint32_t back = leaf ? M_FAN : M_DONE;
-                if (k == K_GENERATE) {
+                if (k == K_GENERATE && rec_phase(c, cur) != c.phase) {
+                    mode = back;                    // emitted by the launch of its own phase
+                } else if (k == K_GENERATE) {
                     slot = e.tslot();
                     if (slot == 0xFFu) {            // terminal index computed per shape
-                        const uint32_t t = f2u_rd(P(c, e, 0, cur, cur.axiom, cur.path, rec_depth(cur)));
+                        const uint32_t t = f2u_rd(P(c, e, 0, cur, cur.axiom, cur.path, rec_depth(c, cur)));
                         slot = t < PGA_MAX_TERMINALS ? c.prog.term_slot[t] : 0xFFu;
                     }
                     if (slot >= c.prog.nslots) { raise(c, PGA_ERR_BAD_PROGRAM); slot = 0; }
@@ -245,23 +247,30 @@ __global__ void __launch_bounds__(BS) k_mega(KArgs a) {
 
 // Zeroes the per-run counters (a kernel, so that the reset can sit in a
 // captured graph).
+// flags bit 0: zero the terminal counts; bit 1: a later phase of a
+// multi-phase run (the error word stays, the colliders stored so far become
+// visible). A fresh run zeroes the collider counters.
 __global__ void k_reset(uint32_t* par_counts, uint32_t* desc_counts, uint32_t n_levels, uint32_t* err, uint32_t* bar,
-                        unsigned long long* term_counts, uint32_t reset_terms) {
+                        unsigned long long* term_counts, uint32_t flags, uint32_t* coll_count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_levels) { par_counts[i] = 0u; desc_counts[i] = 0u; }
-    if (i == 0) *err = 0u;
+    if (i == 0 && !(flags & 2u)) *err = 0u;
+    if (i == 0 && coll_count) {           // [0] appended, [1] visible to this launch
+        if (flags & 2u) coll_count[1] = coll_count[0];
+        else coll_count[0] = coll_count[1] = 0u;
+    }
     if (i < kBarWords) bar[i] = 0u;
-    if (reset_terms && i < PGA_MAX_TERMINALS) term_counts[i] = 0ull;
+    if ((flags & 1u) && i < PGA_MAX_TERMINALS) term_counts[i] = 0ull;
 }
 
 } // namespace pga
 
 // ---- launchers (host side) ----
-hipError_t pga_launch_reset(const pga::KArgs& a, int reset_terms, hipStream_t s) {
+hipError_t pga_launch_reset(const pga::KArgs& a, int flags, hipStream_t s) {
     const uint32_t n = a.max_levels + 2u;
     const uint32_t blocks = (n + 255u) / 256u;
     hipLaunchKernelGGL(pga::k_reset, dim3(blocks), dim3(256), 0, s, a.par_counts, a.desc_counts, n, a.err, a.bar,
-                       a.term_counts, (uint32_t)reset_terms);
+                       a.term_counts, (uint32_t)flags, a.coll_count);
     return hipGetLastError();
 }
 

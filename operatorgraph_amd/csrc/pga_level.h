@@ -270,6 +270,13 @@ __device__ __forceinline__ Ctx make_ctx(const KArgs& a, const LevelLDS& L, uint3
     c.max_chain = a.max_chain;
     c.geom = wr ? 1u : a.count_geom;
     c.seed = a.use_seed;
+    c.dmask = a.dmask;
+    c.phase = a.phase;
+    c.wr = wr ? 1u : 0u;
+    c.coll = a.coll;
+    c.coll_count = a.coll_count;
+    c.coll_visible = a.coll_visible;
+    c.coll_cap = a.coll_cap;
     return c;
 }
 

@@ -1,8 +1,8 @@
 // pga_ops.h - operator execution on one shape record per lane.
 //
 // Single-child operators (IfSizeLess, If, RandomRule, Translate, Rotate,
-// Scale, SwapSize, Extrude) transform the record in registers and continue
-// with the successor entry. Fan-out operators (Subdivide, Repeat,
+// Scale, SwapSize, Extrude, Collider, IfCollides) transform the record in
+// registers and continue with the successor entry. Fan-out operators (Subdivide, Repeat,
 // ComponentSplit, Replicate) are split into `fan_init` (per-parent prelude:
 // axis, relative-to-absolute scale, repetition counts), `fan_geom` (the
 // geometry of child j, called for j = 0, 1, ... in order, carrying the
@@ -75,9 +75,16 @@ __device__ __forceinline__ uint32_t entry_shape(const Ctx& c, uint32_t i) { retu
 
 __device__ __forceinline__ bool advance(const Ctx& c, Rec& r, int32_t next, uint32_t ordinal) {
     if (next < 0) return false;   // unused dynamic edge: nothing dispatched
-    uint32_t depth = rec_depth(r) + 1u;
-    if (depth > 0xFFFFu) { raise(c, PGA_ERR_MAX_DEPTH); return false; }
-    r.ed = (uint32_t)next | (depth << 16);
+    // successors of a multi-phase program carry their phase in bits 16-19
+    // (DispatchTableEntry.h:8-36, the successor's phaseIndex); a child bound
+    // for an earlier phase than its parent's is never derived (that phase
+    // has run), one bound for a later phase than this launch's waits for its
+    // own launch (Call.cuh:69-70 dispatches into that phase's queues)
+    const uint32_t cph = (uint32_t)next >> 16;
+    if (multi_phase(c) && (cph < (r.ed >> 28) || cph > c.phase)) return false;
+    uint32_t depth = rec_depth(c, r) + 1u;
+    if (depth > c.dmask) { raise(c, PGA_ERR_MAX_DEPTH); return false; }
+    r.ed = ((uint32_t)next & 0xFFFFu) | (depth << 16) | (cph << 28);
     r.path = path_mix(r.path, ordinal);
     return true;
 }
@@ -121,7 +128,7 @@ __device__ __forceinline__ void eval3(const Ctx& c, const E& e, const Rec& r, fl
     v0 = e.pv0; v1 = e.pv1; v2 = e.pv2;
     const uint32_t todo = ((1u << np) - 1u) & ~e.inl();
     if (todo == 0u) return;
-    const uint32_t ax = r.axiom, pa = r.path, de = rec_depth(r);
+    const uint32_t ax = r.axiom, pa = r.path, de = rec_depth(c, r);
 #pragma unroll 1
     for (uint32_t k = 0; k < np; ++k) {
         if (!((todo >> k) & 1u)) continue;
@@ -139,7 +146,7 @@ template <class E>
 __device__ __forceinline__ int32_t random_choice(const Ctx& c, const E& e, float chance, const Rec& rr) {
     const uint32_t n = e.nparams();
     const bool is_static = (e.flags() & EF_STATIC) != 0;
-    const uint32_t axiom = rr.axiom, path = rr.path, depth = rec_depth(rr);
+    const uint32_t axiom = rr.axiom, path = rr.path, depth = rec_depth(c, rr);
     float sum = 0.0f;
     if (!is_static) {
         for (uint32_t i = 1; i < n; ++i) sum += P(c, e, i, rr, axiom, path, depth);
@@ -157,6 +164,98 @@ __device__ __forceinline__ int32_t random_choice(const Ctx& c, const E& e, float
         if ((!is_static || np != 0.0f) && chance <= acc) return (int32_t)i - 1;
     }
     return -1;
+}
+
+// ---- context sensitivity (Collider.cuh, IfCollides.cuh, Collision.cuh) ----
+
+// Separating-axis test of two oriented boxes A (a stored collider) and B (the
+// shape asking) (Gottschalk 1996; Ericson, Real-Time Collision Detection
+// 4.4.1), with the axes in the order of Collision.cuh:35-262: A's three face
+// axes, B's three, the nine edge cross products. Models are row-major 3x4,
+// the box axes are their columns. af / bf: half extents on the shape's own
+// face axes; ap / bp: in projections onto the other shape's axes, where a
+// Quad's z half extent drops out (Box x Quad :263-490, Quad x Quad :506-733).
+__device__ __forceinline__ bool sat_overlap(const float A[12], const float af[3], const float ap[3],
+                                            const float B[12], const float bf[3], const float bp[3]) {
+    const float v0 = B[3] - A[3], v1 = B[7] - A[7], v2 = B[11] - A[11];
+    float T[3], R[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        T[i] = v0 * A[i] + v1 * A[4 + i] + v2 * A[8 + i];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) R[i][k] = A[i] * B[k] + A[4 + i] * B[4 + k] + A[8 + i] * B[8 + k];
+    }
+    bool sep = false;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        sep |= fabsf(T[i]) > af[i] + (bp[0] * fabsf(R[i][0]) + bp[1] * fabsf(R[i][1]) + bp[2] * fabsf(R[i][2]));
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        sep |= fabsf(T[0] * R[0][k] + T[1] * R[1][k] + T[2] * R[2][k]) >
+               (ap[0] * fabsf(R[0][k]) + ap[1] * fabsf(R[1][k]) + ap[2] * fabsf(R[2][k])) + bf[k];
+    // L = A_i x B_k: the two A axes other than i and the two B axes other than k
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int i1 = i == 0 ? 1 : 0, i2 = i == 2 ? 1 : 2;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int k1 = k == 0 ? 1 : 0, k2 = k == 2 ? 1 : 2;
+            const float ra = ap[i1] * fabsf(R[i2][k]) + ap[i2] * fabsf(R[i1][k]);
+            const float rb = bp[k1] * fabsf(R[i][k2]) + bp[k2] * fabsf(R[i][k1]);
+            sep |= fabsf(T[i2] * R[i1][k] - T[i1] * R[i2][k]) > ra + rb;
+        }
+    }
+    return !sep;
+}
+
+// Collision::check(stored, asking) of two Box / Quad shapes, half extents
+// size * 0.5f (Shape.cuh:70-73); Quad x Box is Box x Quad with the shapes
+// swapped (Collision.cuh:734-737).
+__device__ __forceinline__ bool shapes_overlap(const float am[12], const float as[3], uint32_t at,
+                                               const float bm[12], const float bs[3], uint32_t bt) {
+    const float ah[3] = {as[0] * 0.5f, as[1] * 0.5f, as[2] * 0.5f};
+    const float bh[3] = {bs[0] * 0.5f, bs[1] * 0.5f, bs[2] * 0.5f};
+    const float aq[3] = {ah[0], ah[1], 0.0f}, bq[3] = {bh[0], bh[1], 0.0f};
+    const bool aquad = at == PGA_SHAPE_QUAD, bquad = bt == PGA_SHAPE_QUAD;
+    if (aquad && !bquad) return sat_overlap(bm, bh, bh, am, ah, aq);
+    return sat_overlap(am, ah, aquad ? aq : ah, bm, bh, bquad ? bq : bh);
+}
+
+// IfCollides' query (IntermediateSymbolsBufferAdapter.cuh:89-136 over the
+// candidates of BVHTraversal.cuh:21-148): does a collider that an earlier
+// phase than the shape's stored with this tag overlap the shape? The store
+// holds `n` such colliders; the answer is an existence, independent of the
+// order they are visited in.
+__device__ __forceinline__ bool collider_hit(const float4* __restrict__ coll, uint32_t n, uint32_t tag,
+                                             uint32_t phase, uint32_t shape, float m0, float m1, float m2, float m3,
+                                             float m4, float m5, float m6, float m7, float m8, float m9, float m10,
+                                             float m11, float s0, float s1, float s2) {
+    const float bm[12] = {m0, m1, m2, m3, m4, m5, m6, m7, m8, m9, m10, m11};
+    const float bs[3] = {s0, s1, s2};
+#pragma unroll 1
+    for (uint32_t i = 0; i < n; ++i) {
+        const float4* k = coll + 5u * i;
+        const float4 w3 = k[3], w4 = k[4];
+        if (__float_as_uint(w3.w) != tag || __float_as_uint(w4.y) >= phase) continue;
+        const float4 w0 = k[0], w1 = k[1], w2 = k[2];
+        const float am[12] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w};
+        const float as[3] = {w3.x, w3.y, w3.z};
+        if (shapes_overlap(am, as, __float_as_uint(w4.x), bm, bs, shape)) return true;
+    }
+    return false;
+}
+
+// Collider (Collider.cuh:26-42): a positive tag stores the shape with the
+// phase it is derived in; once per shape (the write pass of its own phase).
+__device__ __forceinline__ void store_collider(const Ctx& c, const Rec& r, uint32_t shape, uint32_t tag) {
+    const uint32_t i = atomicAdd(c.coll_count, 1u);
+    if (i >= c.coll_cap) { raise(c, PGA_ERR_OVERFLOW_QUEUE); return; }
+    float4* k = c.coll + 5u * i;
+    k[0] = make_float4(r.m[0], r.m[1], r.m[2], r.m[3]);
+    k[1] = make_float4(r.m[4], r.m[5], r.m[6], r.m[7]);
+    k[2] = make_float4(r.m[8], r.m[9], r.m[10], r.m[11]);
+    k[3] = make_float4(r.s[0], r.s[1], r.s[2], __uint_as_float(tag));
+    k[4] = make_float4(__uint_as_float(shape), __uint_as_float(c.phase), 0.0f, 0.0f);
 }
 
 // Applies one single-child operator in place. Returns the successor entry
@@ -220,6 +319,20 @@ __device__ __forceinline__ int32_t apply_single(const Ctx& c, const E& e, Rec& r
         r.s[0] = s0; r.s[1] = s1; r.s[2] = s2;
         return e.sv0;
     }
+    if (k == K_COLLIDER) {                     // Collider.cuh:26-42
+        const uint32_t tag = f2u_rd(v0);
+        if (c.wr && tag > 0u && rec_phase(c, r) == c.phase) store_collider(c, r, e.shape(), tag);
+        return e.sv0;
+    }
+    if (k == K_IFCOLLIDES) {                   // IfCollides.cuh:26-58
+        const uint32_t n = min(*c.coll_visible, c.coll_cap);
+        const uint32_t t = (n != 0u && collider_hit(c.coll, n, f2u_rd(v0), rec_phase(c, r), e.shape(), r.m[0],
+                                                    r.m[1], r.m[2], r.m[3], r.m[4], r.m[5], r.m[6], r.m[7], r.m[8],
+                                                    r.m[9], r.m[10], r.m[11], r.s[0], r.s[1], r.s[2]))
+                             ? 0u : 1u;
+        *ordinal = t;
+        return succ_of(c, e, t);
+    }
     // K_EXTRUDE, Extrude.cuh:25-71 (Quad -> Box); parameter 0 (axis) unused
     if (c.geom) {
         const float b[16] = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1.0f, 0.0f,
@@ -263,7 +376,7 @@ __device__ __forceinline__ void fan_init(const Ctx& c, const Rec& r, const E& e,
     f.e1 = f.e2 = f.e3 = 0.0f;
     f.prev_ns = 0.0f; f.prev_o = 0.0f;
     if (kind == K_SUBDIV) {                    // Subdivide.cuh:142-153 / 181-216
-        const uint32_t ax = r.axiom, pa = r.path, de = rec_depth(r);
+        const uint32_t ax = r.axiom, pa = r.path, de = rec_depth(c, r);
         float abs_sum = 0.0f, rel_sum = 0.0f;
         uint32_t axis = 0;
 #pragma unroll 1
@@ -378,9 +491,9 @@ __device__ __forceinline__ bool fan_geom(const Ctx& c, const Rec& r, const E& e,
             if (c.geom) mul_translate(r.m, tx, ty, tz, sym.m);
             if (f.axis == 0) sym.s[0] = f.prev_ns; else if (f.axis == 1) sym.s[1] = f.prev_ns; else sym.s[2] = f.prev_ns;
             if (c.seed) sym.seed = next_seed(r.seed, j - 1u);
-            fac = eval_param(c, p, sym, r.axiom, r.path, rec_depth(r));
+            fac = eval_param(c, p, sym, r.axiom, r.path, rec_depth(c, r));
         } else {
-            fac = eval_param(c, p, r, r.axiom, r.path, rec_depth(r));
+            fac = eval_param(c, p, r, r.axiom, r.path, rec_depth(c, r));
         }
         if (fac_out) *fac_out = fac;
         if (is_static && fac == 0.0f) return false;

@@ -167,6 +167,8 @@ template <class S, bool WR>
 __device__ __forceinline__ void spec_pass(const KArgs& a, LevelLDS& L, float4* PW, uint32_t level, uint32_t n,
                                           uint32_t wave, uint32_t nwaves, const PassIO& io) {
     Ctx c = make_ctx<false>(a, L, level, WR);
+    c.dmask = 0xFFFFu;   // one-phase programs only: the phase tests fold away
+    c.phase = 0u;
     c.geom = WR ? 1u : S::kCountGeom;     // constants: the count pass drops the unread arithmetic
     c.seed = S::kUseSeed;
     c.par_out = io.par_out;
@@ -221,7 +223,7 @@ __device__ __forceinline__ void spec_pass(const KArgs& a, LevelLDS& L, float4* P
                     if (k == K_GENERATE) {
                         slot = e.tslot();
                         if (slot == 0xFFu) {    // terminal index computed per shape
-                            const uint32_t t = f2u_rd(P(c, e, 0, cur, cur.axiom, cur.path, rec_depth(cur)));
+                            const uint32_t t = f2u_rd(P(c, e, 0, cur, cur.axiom, cur.path, rec_depth(c, cur)));
                             slot = t < PGA_MAX_TERMINALS ? c.prog.term_slot[t] : 0xFFu;
                         }
                         if (slot >= c.prog.nslots) { raise(c, PGA_ERR_BAD_PROGRAM); slot = 0; }

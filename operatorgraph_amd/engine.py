@@ -61,6 +61,8 @@ def load_library():
         "pga_execute": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_double)]),
         "pga_execute_range": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_double)]),
         "pga_get_stats": (C.c_int, [P, C.POINTER(abi.Stats)]),
+        "pga_set_collider_capacity": (C.c_int, [P, C.c_uint32]),
+        "pga_num_colliders": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "pga_execute_async": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int32]),
         "pga_synchronize": (C.c_int, [P]),
         "pga_kernel_seconds": (C.c_int, [P, C.POINTER(C.c_double)]),
@@ -405,6 +407,18 @@ class Engine:
             gap = (t[lv + 1, 0] - t[lv, 3]) / hz * 1e6 if lv + 1 < len(t) and t[lv + 1, 0] else 0.0
             out.append({"count_us": d[0], "scan_us": d[1], "write_us": d[2], "barrier_us": gap})
         return out
+
+    def set_collider_capacity(self, capacity):
+        """Colliders one execute may store (pga_set_collider_capacity, 0:
+        the default of 2^20)."""
+        self._check(self.L.pga_set_collider_capacity(self._h, int(capacity)))
+        return self
+
+    def colliders(self):
+        """(colliders the last execute stored, phases of the program)."""
+        n, p = C.c_uint32(), C.c_uint32()
+        self._check(self.L.pga_num_colliders(self._h, C.byref(n), C.byref(p)))
+        return n.value, p.value
 
     def stats(self):
         s = abi.Stats()

@@ -13,7 +13,7 @@
 * `facade_params()`: the facade with 8 per-lot differentiable parameters.
 """
 from . import abi
-from .program import (AxiomParam, DispatchTable, Entry, Parameter, Philox, Procedure,
+from .program import (AxiomParam, DispatchTable, Entry, Parameter, Philox, Procedure, Rand,
                       ProcedureList, Program, Scalar)
 
 B, Q = abi.BOX, abi.QUAD
@@ -225,3 +225,110 @@ def facade_params():
         Entry(op(abi.GENERATE, Q, 1), [Scalar(1)], []),
     ])
     return Program(INTERPRETER_PROCEDURES, table, "facade_params")
+
+
+# ---- context-sensitive grammars (Collider / IfCollides, two phases) ----
+
+CONTEXT_PROCEDURES = ProcedureList([
+    Procedure(abi.REPLICATE, B), Procedure(abi.SCALE, B), Procedure(abi.TRANSLATE, B),
+    Procedure(abi.ROTATE, B), Procedure(abi.COLLIDER, B), Procedure(abi.GENERATE, B, 1),
+    Procedure(abi.REPEAT, B), Procedure(abi.IFCOLLIDES, B), Procedure(abi.COMPSPLIT, B),
+    Procedure(abi.DISCARD, Q), Procedure(abi.COLLIDER, Q), Procedure(abi.GENERATE, Q, 1),
+    Procedure(abi.REPLICATE, Q), Procedure(abi.SCALE, Q), Procedure(abi.TRANSLATE, Q),
+    Procedure(abi.ROTATE, Q), Procedure(abi.IFCOLLIDES, Q),
+])
+
+
+def _cop(op_type, shape, gen_func=0):
+    idx = CONTEXT_PROCEDURES.index_of(Procedure(op_type, shape, gen_func))
+    if idx < 0:
+        raise KeyError((op_type, shape, gen_func))
+    return idx
+
+
+def context_city(walls=False):
+    """A two-phase city block (the reference's context-sensitive operators,
+    IfCollides.cuh / Collider.cuh, under MultiPhaseEvaluator):
+        Lot      = Replicate { Building (phase 0) | Yard (phase 1) };
+        Building = Scale(-0.45, -1, -0.45) Translate(r, 0, r)
+                   Rotate(0, 90 r', 0) Collider(1) Generate(0);
+        Yard     = Repeat(X, 1, ADJUST_TO_FILL) Repeat(Z, 1, ADJUST_TO_FILL)
+                   Scale(-0.5, -0.3, -0.5) IfCollides(1) { Generate(2) | Generate(1) };
+    r, r' are Rand(-1.5, 1.5) and Rand(0, 1) of the shape's seed. Phase 0
+    derives the buildings and stores them as colliders; phase 1 places one
+    prop per yard cell, blocked (terminal 2) where it overlaps a building,
+    free (terminal 1) elsewhere. walls=True also splits each building into
+    faces stored under tag 2 (Quad colliders) and tests props against them
+    first: IfCollides(2) { Generate(2) | IfCollides(1) {...} }."""
+    (LOT, BUILDING, MOVE, TURN, STORE, GEN_B, YARD_X, YARD_Z, PROP, TEST, GEN_BLOCK, GEN_FREE,
+     SPLIT, QDISC, WALL, GEN_WALL, TEST_WALL) = range(17)
+    P0, P1 = 0, 1
+    after_store = SPLIT if walls else GEN_B
+    first_test = TEST_WALL if walls else TEST
+    table = DispatchTable([
+        Entry(_cop(abi.REPLICATE, B), [], [(BUILDING, P0), (YARD_X, P1)]),
+        Entry(_cop(abi.SCALE, B), [Scalar(-0.45), Scalar(-1.0), Scalar(-0.45)], [(MOVE, P0)]),
+        Entry(_cop(abi.TRANSLATE, B), [Rand(-1.5, 1.5), Scalar(0.0), Rand(-1.5, 1.5)], [(TURN, P0)]),
+        Entry(_cop(abi.ROTATE, B), [Scalar(0.0), Rand(0.0, 90.0), Scalar(0.0)], [(STORE, P0)]),
+        Entry(_cop(abi.COLLIDER, B), [Scalar(1)], [(after_store, P0)]),
+        Entry(_cop(abi.GENERATE, B, 1), [Scalar(0)], []),
+        Entry(_cop(abi.REPEAT, B), [Scalar(abi.X), Scalar(1.0), Scalar(abi.ADJUST_TO_FILL)], [(YARD_Z, P1)]),
+        Entry(_cop(abi.REPEAT, B), [Scalar(abi.Z), Scalar(1.0), Scalar(abi.ADJUST_TO_FILL)], [(PROP, P1)]),
+        Entry(_cop(abi.SCALE, B), [Scalar(-0.5), Scalar(-0.3), Scalar(-0.5)], [(first_test, P1)]),
+        Entry(_cop(abi.IFCOLLIDES, B), [Scalar(1)], [(GEN_BLOCK, P1), (GEN_FREE, P1)]),
+        Entry(_cop(abi.GENERATE, B, 1), [Scalar(2)], []),
+        Entry(_cop(abi.GENERATE, B, 1), [Scalar(1)], []),
+        Entry(_cop(abi.COMPSPLIT, B), [], [(QDISC, P0), (QDISC, P0), (WALL, P0)]),
+        Entry(_cop(abi.DISCARD, Q), [], []),
+        Entry(_cop(abi.COLLIDER, Q), [Scalar(2)], [(GEN_WALL, P0)]),
+        Entry(_cop(abi.GENERATE, Q, 1), [Scalar(3)], []),
+        Entry(_cop(abi.IFCOLLIDES, B), [Scalar(2)], [(GEN_BLOCK, P1), (TEST, P1)]),
+    ])
+    return Program(CONTEXT_PROCEDURES, table, "context_city_walls" if walls else "context_city")
+
+
+# per-axiom slots of collision_pairs(): size, rotation (degrees), translation, tag
+COLLISION_PAIR_SLOTS = 10
+
+
+def collision_pairs():
+    """Shape pairs, the collision test itself as a program. Each pair is two
+    axioms with per-axiom parameters (PT_AXIOM_PARAM) p0..p2 size, p3..p5
+    rotation in degrees, p6..p8 translation, p9 tag:
+        A = Scale(p0..p2) Rotate(p3..p5) Translate(p6..p8) Collider(p9) Generate(0);
+        B = Replicate { Moved (phase 1) };
+        Moved = Scale(p0..p2) Rotate(p3..p5) Translate(p6..p8)
+                IfCollides(p9) { Generate(1) | Generate(2) };
+    A stores its shape in phase 0, B's shape asks in phase 1; a tag per pair
+    keeps each pair to itself. Entry points (COLLISION_PAIR_ENTRIES): A and B
+    on a Box or on a Quad; a Quad's third size component is its z extent (0:
+    planar)."""
+    from .program import AxiomParam as AP
+    entries = []
+    for s in (B, Q):
+        base = len(entries)
+        SC, RO, TR, CO, GA = (base + k for k in range(5))
+        entries += [
+            Entry(_cop(abi.SCALE, s), [AP(0), AP(1), AP(2)], [(RO, 0)]),
+            Entry(_cop(abi.ROTATE, s), [AP(3), AP(4), AP(5)], [(TR, 0)]),
+            Entry(_cop(abi.TRANSLATE, s), [AP(6), AP(7), AP(8)], [(CO, 0)]),
+            Entry(_cop(abi.COLLIDER, s), [AP(9)], [(GA, 0)]),
+            Entry(_cop(abi.GENERATE, s, 1), [Scalar(0)], []),
+        ]
+    for s in (B, Q):
+        base = len(entries)
+        RE, SC, RO, TR, IC, G1, G2 = (base + k for k in range(7))
+        entries += [
+            Entry(_cop(abi.REPLICATE, s), [], [(SC, 1)]),
+            Entry(_cop(abi.SCALE, s), [AP(0), AP(1), AP(2)], [(RO, 1)]),
+            Entry(_cop(abi.ROTATE, s), [AP(3), AP(4), AP(5)], [(TR, 1)]),
+            Entry(_cop(abi.TRANSLATE, s), [AP(6), AP(7), AP(8)], [(IC, 1)]),
+            Entry(_cop(abi.IFCOLLIDES, s), [AP(9)], [(G1, 1), (G2, 1)]),
+            Entry(_cop(abi.GENERATE, s, 1), [Scalar(1)], []),
+            Entry(_cop(abi.GENERATE, s, 1), [Scalar(2)], []),
+        ]
+    return Program(CONTEXT_PROCEDURES, DispatchTable(entries), "collision_pairs")
+
+
+# collision_pairs() entry points: (role, shape type) -> entry index
+COLLISION_PAIR_ENTRIES = {("A", B): 0, ("A", Q): 5, ("B", B): 10, ("B", Q): 17}

@@ -73,6 +73,10 @@ def lib():
         L.pgao_pack_float2.argtypes = [C.c_float, C.c_float]
         L.pgao_lbvh.restype = C.c_int
         L.pgao_lbvh.argtypes = [P, C.c_uint32, P, P, P, P, P, P, P, P]
+        L.pgao_num_colliders.restype = C.c_uint32
+        L.pgao_num_colliders.argtypes = [P]
+        L.pgao_shapes_collide.restype = C.c_int
+        L.pgao_shapes_collide.argtypes = [P, P, C.c_int32, P, P, C.c_int32]
         L.pgao_morton30.restype = C.c_uint32
         L.pgao_morton30.argtypes = [C.c_float, C.c_float, C.c_float]
         _lib = L
@@ -106,6 +110,11 @@ class Result:
     @property
     def steps(self):
         return lib().pgao_steps(self._h)
+
+    @property
+    def colliders(self):
+        """Shapes the Collider operator stored, over all phases."""
+        return lib().pgao_num_colliders(self._h)
 
     @property
     def executions(self):
@@ -175,6 +184,17 @@ def derive(program, axioms, params=None, key=(0, 0), twins=0, max_steps=0):
         msg = L.pgao_error(h.value).decode()
         raise OracleError(st, msg)
     return res
+
+
+def shapes_collide(a_model, a_size, a_type, b_model, b_size, b_type):
+    """Collision::check(stored a, asking b) of two Box/Quad shapes: model as
+    12 floats (3x4 row-major), size as 3 floats, abi.BOX or abi.QUAD."""
+    am = np.ascontiguousarray(a_model, dtype=np.float32)
+    asz = np.ascontiguousarray(a_size, dtype=np.float32)
+    bm = np.ascontiguousarray(b_model, dtype=np.float32)
+    bsz = np.ascontiguousarray(b_size, dtype=np.float32)
+    return bool(lib().pgao_shapes_collide(am.ctypes.data, asz.ctypes.data, int(a_type), bm.ctypes.data,
+                                          bsz.ctypes.data, int(b_type)))
 
 
 def next_seed(base, modifier):

@@ -246,6 +246,7 @@ struct Meta {
     int32_t entry;
     uint32_t axiom, path, depth;
     int32_t shape_type;
+    uint32_t phase;    // multi-phase programs: the phase the shape is derived in
 };
 
 template <class T> void mul34x44(const T a[12], const T b[16], T c[12]) {
@@ -307,6 +308,8 @@ struct Ctx {
     uint32_t params_per_axiom;
     uint32_t naxioms;
     uint32_t key[2];
+    std::vector<struct Collider>* colliders;   // stored by Collider, read by IfCollides
+    uint32_t phase;                            // the phase being derived
 };
 
 template <class T> T axiom_param(const Ctx& c, uint32_t axiom, uint32_t slot);
@@ -415,6 +418,7 @@ template <class T> T dyn_eval(const Ctx& c, const Meta& m, const Shape<T>& s,
 // entry) and terminal instances.
 template <class T> struct Out {
     int kind;          // 0 child, 1 terminal
+    uint32_t phase;    // child: the phase of the successor it goes to
     int32_t entry;     // successor entry (child) / generate entry (terminal)
     uint32_t ordinal;  // child ordinal within the operator
     uint32_t terminal; // terminal index
@@ -433,16 +437,22 @@ inline uint32_t axis_of(float v) {          // __float2uint_rd / static_cast<uns
     return a;
 }
 
-template <class T>
-void push_child(std::vector<Out<T>>& out, int32_t entry, uint32_t ordinal, int32_t st, const Shape<T>& s) {
-    Out<T> o;
-    o.kind = 0; o.entry = entry; o.ordinal = ordinal; o.terminal = 0; o.shape_type = st; o.s = s;
-    out.push_back(o);
-}
-
 inline int32_t succ(const pga_dispatch_entry& e, uint32_t k) {
     if (k >= e.num_successors) throw OpErr(PGA_ERR_BAD_PROGRAM, "successor index out of range");
     return e.successors[k].entry_index;
+}
+
+// The child of successor k of entry e: its entry and the phase the successor
+// names (DispatchTableEntry.h:8-36; Call.cuh:69-70 dispatches to that phase).
+// A negative phase index (an unassigned one) is phase 0.
+template <class T>
+void push_child(std::vector<Out<T>>& out, const pga_dispatch_entry& e, uint32_t k, uint32_t ordinal, int32_t st,
+                const Shape<T>& s) {
+    Out<T> o;
+    o.kind = 0; o.entry = succ(e, k); o.ordinal = ordinal; o.terminal = 0; o.shape_type = st; o.s = s;
+    const int32_t ph = e.successors[k].phase_index;
+    o.phase = ph > 0 ? (uint32_t)ph : 0u;
+    out.push_back(o);
 }
 inline float fparam(const pga_dispatch_entry& e, uint32_t k) {
     if (k >= e.num_parameters) throw OpErr(PGA_ERR_BAD_PROGRAM, "parameter index out of range");
@@ -497,8 +507,7 @@ void op_subdivide(const Ctx& c, const Meta& m, const Shape<T>& parent, const pga
         ch.size[axis] = new_size;
         apply_translate(ch, base_m, off3[0], off3[1], off3[2]);
         ch.seed = next_seed(parent.seed, j);
-        int32_t target = succ(e, j);
-        push_child(out, target, j, m.shape_type, ch);
+        push_child(out, e, j, j, m.shape_type, ch);
         if (!is_static) sym = ch;
         offset = offset + new_size;
     }
@@ -540,7 +549,7 @@ void op_repeat(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispat
     if (reps > (1u << 24)) throw OpErr(PGA_ERR_OVERFLOW_QUEUE, "Repeat: too many repetitions");
     bool start = mode == PGA_ANCHOR_TO_START;
     T start_ext = S<T>::c(start ? 1.0 : 0.0) * e1 + S<T>::c(start ? 0.0 : 1.0) * e3;
-    int32_t target = succ(e, 0);
+    (void)succ(e, 0);
     Shape<T> ns = sh;
     T base_m[12];
     for (int i = 0; i < 12; ++i) base_m[i] = sh.m[i];
@@ -550,7 +559,7 @@ void op_repeat(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispat
     ns.size[axis] = start_ext;
     apply_translate(ns, base_m, off[0], off[1], off[2]);
     ns.seed = next_seed(sh.seed, 1u);
-    push_child(out, target, 0, m.shape_type, ns);
+    push_child(out, e, 0, 0, m.shape_type, ns);
     off[axis] = off[axis] + half_start;
     T half_mid = e2 * half;
     ns.size[axis] = e2;
@@ -558,7 +567,7 @@ void op_repeat(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispat
         off[axis] = off[axis] + half_mid;
         apply_translate(ns, base_m, off[0], off[1], off[2]);
         ns.seed = next_seed(sh.seed, i);
-        push_child(out, target, i - 1, m.shape_type, ns);
+        push_child(out, e, 0, i - 1, m.shape_type, ns);
         off[axis] = off[axis] + half_mid;
     }
     if (reps > 1) {
@@ -567,7 +576,7 @@ void op_repeat(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispat
         ns.size[axis] = end_ext;
         apply_translate(ns, base_m, off[0], off[1], off[2]);
         ns.seed = next_seed(sh.seed, reps);
-        push_child(out, target, reps - 1, m.shape_type, ns);
+        push_child(out, e, 0, reps - 1, m.shape_type, ns);
     }
 }
 
@@ -586,7 +595,7 @@ void op_compsplit(const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e
     if (m.shape_type != PGA_SHAPE_BOX) throw OpErr(PGA_ERR_UNSUPPORTED, "ComponentSplit supports Box shapes only");
     const T zero = S<T>::c(0.0), one = S<T>::c(1.0), half = S<T>::c(0.5);
     T he[3] = {sh.size[0] * half, sh.size[1] * half, sh.size[2] * half};
-    int32_t top = succ(e, 0), bottom = succ(e, 1), sides = succ(e, 2);
+    (void)succ(e, 0); (void)succ(e, 1); (void)succ(e, 2);
     for (uint32_t f = 0; f < 4; ++f) {
         Shape<T> q;
         T t[16], tmp[12], r[16];
@@ -605,7 +614,7 @@ void op_compsplit(const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e
         q.size[2] = one;
         q.seed = next_seed(sh.seed, f + 2u);
         q.custom = sh.custom;
-        push_child(out, sides, f, PGA_SHAPE_QUAD, q);
+        push_child(out, e, 2, f, PGA_SHAPE_QUAD, q);
     }
     for (int cap = 0; cap < 2; ++cap) {     // bottom, then top
         bool bottom_face = cap == 0;
@@ -618,7 +627,7 @@ void op_compsplit(const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e
         q.size[0] = sh.size[0]; q.size[1] = sh.size[2]; q.size[2] = one;
         q.seed = next_seed(sh.seed, bottom_face ? 1u : 0u);
         q.custom = sh.custom;
-        push_child(out, bottom_face ? bottom : top, 4u + (uint32_t)cap, PGA_SHAPE_QUAD, q);
+        push_child(out, e, bottom_face ? 1u : 0u, 4u + (uint32_t)cap, PGA_SHAPE_QUAD, q);
     }
 }
 
@@ -631,7 +640,7 @@ void op_compsplit(const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e
 template <class T>
 void op_compsplit_parallel(const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e, std::vector<Out<T>>& out) {
     if (m.shape_type != PGA_SHAPE_BOX) throw OpErr(PGA_ERR_UNSUPPORTED, "ComponentSplit supports Box shapes only");
-    const int32_t top = succ(e, 0), bottom = succ(e, 1), sides = succ(e, 2);
+    (void)succ(e, 0); (void)succ(e, 1); (void)succ(e, 2);
     auto k = [](float v) { return S<T>::c(v); };
     for (uint32_t f = 0; f < 6; ++f) {
         const float sign = ((float)(f & 0x1u) * 2.0f) - 1.0f;
@@ -657,7 +666,7 @@ void op_compsplit_parallel(const Meta& m, const Shape<T>& sh, const pga_dispatch
         q.size[2] = S<T>::c(1.0);
         q.seed = next_seed(sh.seed, (f + 2u) % 6u);
         q.custom = sh.custom;
-        push_child(out, f == 5u ? top : (f == 4u ? bottom : sides), f, PGA_SHAPE_QUAD, q);
+        push_child(out, e, f == 5u ? 0u : (f == 4u ? 1u : 2u), f, PGA_SHAPE_QUAD, q);
     }
 }
 
@@ -668,14 +677,14 @@ void op_ifsizeless(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_di
     uint32_t axis = (uint32_t)d.pick(axis_of((float)S<T>::val(P(c, m, sh, e, 0))));
     T thr = P(c, m, sh, e, 1);
     int taken = (int)d.pick(sh.size[axis] < thr ? 0 : 1);
-    push_child(out, succ(e, (uint32_t)taken), (uint32_t)taken, m.shape_type, sh);
+    push_child(out, e, (uint32_t)taken, (uint32_t)taken, m.shape_type, sh);
 }
 template <class T>
 void op_if(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e,
            Dec& d, std::vector<Out<T>>& out) {
     float r = (float)S<T>::val(P(c, m, sh, e, 0));
     int taken = (int)d.pick((r != 0.0f) ? 0 : 1);   // `if (result)`: NaN counts as true
-    push_child(out, succ(e, (uint32_t)taken), (uint32_t)taken, m.shape_type, sh);
+    push_child(out, e, (uint32_t)taken, (uint32_t)taken, m.shape_type, sh);
 }
 
 // RandomRule<DynParams> (RandomRule.cuh:121-147) and the static form
@@ -717,7 +726,7 @@ void op_random(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispat
         }
     }
     choice = d.pick(choice);
-    if (choice >= 0) push_child(out, succ(e, (uint32_t)choice), (uint32_t)choice, m.shape_type, sh);
+    if (choice >= 0) push_child(out, e, (uint32_t)choice, (uint32_t)choice, m.shape_type, sh);
 }
 
 template <class T>
@@ -726,7 +735,7 @@ void op_translate(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dis
     T x = P(c, m, sh, e, 0), y = P(c, m, sh, e, 1), z = P(c, m, sh, e, 2);
     Shape<T> ns = sh;
     apply_translate(ns, sh.m, x, y, z);
-    push_child(out, succ(e, 0), 0, m.shape_type, ns);
+    push_child(out, e, 0, 0, m.shape_type, ns);
 }
 
 template <class T>
@@ -746,7 +755,7 @@ void op_rotate(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispat
                zero, zero, zero, one};
     Shape<T> ns = sh;
     mul34x44(sh.m, r, ns.m);
-    push_child(out, succ(e, 0), 0, m.shape_type, ns);
+    push_child(out, e, 0, 0, m.shape_type, ns);
 }
 
 template <class T>
@@ -757,7 +766,7 @@ void op_scale(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispatc
     Shape<T> ns = sh;
     for (int i = 0; i < 3; ++i)
         ns.size[i] = (((v[i] < zero ? one : zero) * sh.size[i]) * v[i]) * m1 + ((v[i] > zero ? one : zero) * v[i]);
-    push_child(out, succ(e, 0), 0, m.shape_type, ns);
+    push_child(out, e, 0, 0, m.shape_type, ns);
 }
 
 template <class T>
@@ -767,7 +776,7 @@ void op_swapsize(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_disp
     for (uint32_t i = 0; i < 3; ++i) ax[i] = (uint32_t)d.pick(axis_of((float)S<T>::val(P(c, m, sh, e, i))));
     Shape<T> ns = sh;
     ns.size[0] = sh.size[ax[0]]; ns.size[1] = sh.size[ax[1]]; ns.size[2] = sh.size[ax[2]];
-    push_child(out, succ(e, 0), 0, m.shape_type, ns);
+    push_child(out, e, 0, 0, m.shape_type, ns);
 }
 
 template <class T>
@@ -782,13 +791,13 @@ void op_extrude(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispa
     ns.size[0] = sh.size[0]; ns.size[1] = extent; ns.size[2] = sh.size[1];
     ns.seed = next_seed(sh.seed, 0u);
     ns.custom = sh.custom;
-    push_child(out, succ(e, 0), 0, PGA_SHAPE_BOX, ns);
+    push_child(out, e, 0, 0, PGA_SHAPE_BOX, ns);
 }
 
 template <class T>
 void op_replicate(const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e,
                   std::vector<Out<T>>& out) {           // Replicate.cuh:102-146
-    for (uint32_t i = 0; i < e.num_successors; ++i) push_child(out, succ(e, i), i, m.shape_type, sh);
+    for (uint32_t i = 0; i < e.num_successors; ++i) push_child(out, e, i, i, m.shape_type, sh);
 }
 
 // Generate<false, 1, TerminalIdx> -> GenerationFunction<..., 1>::perShape
@@ -805,6 +814,147 @@ void op_generate(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_disp
     o.s = sh;
     mul34x44(sh.m, sc, o.s.m);
     out.push_back(o);
+}
+
+// ---------------------------------------------------------------------------
+// Context sensitivity (IfCollides.cuh, Collider.cuh, Collision.cuh,
+// IntermediateSymbolsBufferAdapter.cuh)
+// ---------------------------------------------------------------------------
+struct Collider {
+    float m[12];
+    float size[3];
+    int32_t shape_type;
+    uint32_t tag;
+    uint32_t phase;            // the phase that stored it
+};
+
+inline float dot3(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Collision::check of two shapes A (the stored collider) and B (the shape
+// asking), the separating-axis test of two oriented boxes (Gottschalk 1996;
+// Ericson, Real-Time Collision Detection 4.4.1) as Box x Box
+// (Collision.cuh:35-262) evaluates it: the 3 face axes of A, the 3 of B, the
+// 9 edge cross products. A Quad is a box whose z half extent drops out of
+// every projection onto the other shape's axes and keeps its own face axis
+// (Box x Quad :263-490, Quad x Quad :506-733); Quad x Box is Box x Quad with
+// the shapes swapped (:734-737). af / bf: half extents for the shape's own
+// face axes; ap / bp: for its projections.
+bool sat_check(const float A[12], const float af[3], const float ap[3], const float B[12], const float bf[3],
+               const float bp[3]) {
+    const float ac[3][3] = {{A[0], A[4], A[8]}, {A[1], A[5], A[9]}, {A[2], A[6], A[10]}};   // column1..3
+    const float bc[3][3] = {{B[0], B[4], B[8]}, {B[1], B[5], B[9]}, {B[2], B[6], B[10]}};
+    const float v[3] = {B[3] - A[3], B[7] - A[7], B[11] - A[11]};                          // position (column4)
+    const float T[3] = {dot3(v, ac[0]), dot3(v, ac[1]), dot3(v, ac[2])};
+    float R[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) R[i][k] = dot3(ac[i], bc[k]);
+    float ra, rb, t;
+    for (int i = 0; i < 3; ++i) {
+        ra = af[i];
+        rb = bp[0] * std::fabs(R[i][0]) + bp[1] * std::fabs(R[i][1]) + bp[2] * std::fabs(R[i][2]);
+        t = std::fabs(T[i]);
+        if (t > ra + rb) return false;
+    }
+    for (int k = 0; k < 3; ++k) {
+        ra = ap[0] * std::fabs(R[0][k]) + ap[1] * std::fabs(R[1][k]) + ap[2] * std::fabs(R[2][k]);
+        rb = bf[k];
+        t = std::fabs(T[0] * R[0][k] + T[1] * R[1][k] + T[2] * R[2][k]);
+        if (t > ra + rb) return false;
+    }
+    // L = A0 x B0 .. A2 x B2
+    ra = ap[1] * std::fabs(R[2][0]) + ap[2] * std::fabs(R[1][0]);
+    rb = bp[1] * std::fabs(R[0][2]) + bp[2] * std::fabs(R[0][1]);
+    t = std::fabs(T[2] * R[1][0] - T[1] * R[2][0]);
+    if (t > ra + rb) return false;
+    ra = ap[1] * std::fabs(R[2][1]) + ap[2] * std::fabs(R[1][1]);
+    rb = bp[0] * std::fabs(R[0][2]) + bp[2] * std::fabs(R[0][0]);
+    t = std::fabs(T[2] * R[1][1] - T[1] * R[2][1]);
+    if (t > ra + rb) return false;
+    ra = ap[1] * std::fabs(R[2][2]) + ap[2] * std::fabs(R[1][2]);
+    rb = bp[0] * std::fabs(R[0][1]) + bp[1] * std::fabs(R[0][0]);
+    t = std::fabs(T[2] * R[1][2] - T[1] * R[2][2]);
+    if (t > ra + rb) return false;
+    ra = ap[0] * std::fabs(R[2][0]) + ap[2] * std::fabs(R[0][0]);
+    rb = bp[1] * std::fabs(R[1][2]) + bp[2] * std::fabs(R[1][1]);
+    t = std::fabs(T[0] * R[2][0] - T[2] * R[0][0]);
+    if (t > ra + rb) return false;
+    ra = ap[0] * std::fabs(R[2][1]) + ap[2] * std::fabs(R[0][1]);
+    rb = bp[0] * std::fabs(R[1][2]) + bp[2] * std::fabs(R[1][0]);
+    t = std::fabs(T[0] * R[2][1] - T[2] * R[0][1]);
+    if (t > ra + rb) return false;
+    ra = ap[0] * std::fabs(R[2][2]) + ap[2] * std::fabs(R[0][2]);
+    rb = bp[0] * std::fabs(R[1][1]) + bp[1] * std::fabs(R[1][0]);
+    t = std::fabs(T[0] * R[2][2] - T[2] * R[0][2]);
+    if (t > ra + rb) return false;
+    ra = ap[0] * std::fabs(R[1][0]) + ap[1] * std::fabs(R[0][0]);
+    rb = bp[1] * std::fabs(R[2][2]) + bp[2] * std::fabs(R[2][1]);
+    t = std::fabs(T[1] * R[0][0] - T[0] * R[1][0]);
+    if (t > ra + rb) return false;
+    ra = ap[0] * std::fabs(R[1][1]) + ap[1] * std::fabs(R[0][1]);
+    rb = bp[0] * std::fabs(R[2][2]) + bp[2] * std::fabs(R[2][0]);
+    t = std::fabs(T[1] * R[0][1] - T[0] * R[1][1]);
+    if (t > ra + rb) return false;
+    ra = ap[0] * std::fabs(R[1][2]) + ap[1] * std::fabs(R[0][2]);
+    rb = bp[0] * std::fabs(R[2][1]) + bp[1] * std::fabs(R[2][0]);
+    t = std::fabs(T[1] * R[0][2] - T[0] * R[1][2]);
+    if (t > ra + rb) return false;
+    return true;
+}
+
+// Collision::check(stored, asking) for Box and Quad shapes, half extents
+// size * 0.5f (Shape.cuh:70-73).
+bool shapes_collide(const float am[12], const float asz[3], int32_t at, const float bm[12], const float bsz[3],
+                    int32_t bt) {
+    const float ah[3] = {asz[0] * 0.5f, asz[1] * 0.5f, asz[2] * 0.5f};
+    const float bh[3] = {bsz[0] * 0.5f, bsz[1] * 0.5f, bsz[2] * 0.5f};
+    const float aq[3] = {ah[0], ah[1], 0.0f}, bq[3] = {bh[0], bh[1], 0.0f};
+    const bool aquad = at == PGA_SHAPE_QUAD, bquad = bt == PGA_SHAPE_QUAD;
+    if (aquad && !bquad) return sat_check(bm, bh, bh, am, ah, aq);      // Quad x Box = Box x Quad swapped
+    return sat_check(am, ah, aquad ? aq : ah, bm, bh, bquad ? bq : bh);
+}
+
+// IntermediateSymbolsBufferAdapter::checkCollision (:89-136) over every
+// collider a phase before this one stored: the tag matches and the stored
+// shape collides with the asking one. The reference prunes the candidates
+// with a BVH built between phases (BVHTraversal.cuh:21-148); the answer is
+// the same existence.
+template <class T> bool collides(const Ctx& c, uint32_t tag, const Meta& m, const Shape<T>& sh) {
+    float qm[12], qs[3];
+    for (int i = 0; i < 12; ++i) qm[i] = (float)S<T>::val(sh.m[i]);
+    for (int i = 0; i < 3; ++i) qs[i] = (float)S<T>::val(sh.size[i]);
+    for (const Collider& k : *c.colliders)
+        if (k.phase < c.phase && k.tag == tag && shapes_collide(k.m, k.size, k.shape_type, qm, qs, m.shape_type))
+            return true;
+    return false;
+}
+
+// Collider (Collider.cuh:26-42): a positive tag stores the shape; the shape
+// goes on to successor 0.
+template <class T>
+void op_collider(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e, Dec& d,
+                 std::vector<Out<T>>& out) {
+    const uint32_t tag = (uint32_t)d.pick(f2u_rd((float)S<T>::val(P(c, m, sh, e, 0))));
+    if (tag > 0u && d.rec) {
+        Collider k;
+        for (int i = 0; i < 12; ++i) k.m[i] = (float)S<T>::val(sh.m[i]);
+        for (int i = 0; i < 3; ++i) k.size[i] = (float)S<T>::val(sh.size[i]);
+        k.shape_type = m.shape_type;
+        k.tag = tag;
+        k.phase = c.phase;
+        c.colliders->push_back(k);
+    }
+    push_child(out, e, 0, 0, m.shape_type, sh);
+}
+
+// IfCollides (IfCollides.cuh:26-58): successor 0 when the shape collides
+// with a stored collider of the tag, else successor 1. Decided in FP32 and
+// replayed by the twins.
+template <class T>
+void op_ifcollides(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e, Dec& d,
+                   std::vector<Out<T>>& out) {
+    const uint32_t tag = f2u_rd((float)S<T>::val(P(c, m, sh, e, 0)));
+    const int taken = (int)d.pick(d.rec ? (collides(c, tag, m, sh) ? 0 : 1) : 0);
+    push_child(out, e, (uint32_t)taken, (uint32_t)taken, m.shape_type, sh);
 }
 
 template <class T>
@@ -832,6 +982,8 @@ void run_op(const Ctx& c, const Meta& m, const Shape<T>& sh, Dec& d, std::vector
     case PGA_OP_EXTRUDE: op_extrude(c, m, sh, e, out); break;
     case PGA_OP_REPLICATE: op_replicate(m, sh, e, out); break;
     case PGA_OP_DISCARD: break;                               // Discard.cuh:15-31
+    case PGA_OP_COLLIDER: op_collider(c, m, sh, e, d, out); break;
+    case PGA_OP_IFCOLLIDES: op_ifcollides(c, m, sh, e, d, out); break;
     case PGA_OP_GENERATE:
         if (p.gen_func_idx != 0 && p.gen_func_idx != 1)
             throw OpErr(PGA_ERR_UNSUPPORTED, "generation functions 0 and 1 only");
@@ -910,6 +1062,7 @@ struct pgao_result {
     std::vector<std::vector<int32_t>> inst_ids;   // 12 tape ids per instance
     Tape tape;
     std::vector<int32_t> param_ids;
+    uint32_t ncolliders = 0;   // shapes the Collider operator stored, all phases
 };
 
 extern "C" {
@@ -960,7 +1113,19 @@ int pgao_derive(const pga_procedure* procs, uint32_t num_procs,
             if (op < 0 || (uint32_t)op >= num_procs)
                 throw OpErr(PGA_ERR_BAD_PROGRAM, "operator code out of range");
         };
-        std::vector<std::deque<Item>> queues(num_procs);
+        // phases: one more than the largest phase a successor names
+        // (MultiPhaseEvaluator, GPUEvaluator.cuh:405-456: every phase runs to
+        // completion, then the colliders it stored take part in the next)
+        uint32_t nphases = 1;
+        for (uint32_t i = 0; i < num_entries; ++i)
+            for (uint32_t k = 0; k < entries[i].num_successors && k < PGA_MAX_NUM_SUCCESSORS; ++k) {
+                const int32_t ph = entries[i].successors[k].phase_index;
+                if (ph >= (int32_t)PGAO_MAX_PHASES) throw OpErr(PGA_ERR_BAD_PROGRAM, "bad phase index");
+                nphases = std::max(nphases, ph > 0 ? (uint32_t)ph + 1u : 1u);
+            }
+        std::vector<Collider> colliders;
+        c.colliders = &colliders;
+        std::vector<std::vector<std::deque<Item>>> phase_queues(nphases, std::vector<std::deque<Item>>(num_procs));
         // axioms: AxiomGenerator::generateAxiom -> dispatchAxiom (CPUEvaluator.h:240-256)
         for (uint32_t a = 0; a < num_axioms; ++a) {
             const pga_axiom& ax = axioms[a];
@@ -969,16 +1134,20 @@ int pgao_derive(const pga_procedure* procs, uint32_t num_procs,
             Item it;
             it.meta.entry = ax.entry_index; it.meta.axiom = a; it.meta.path = 0; it.meta.depth = 0;
             it.meta.shape_type = ax.shape_type;
+            it.meta.phase = 0;
             for (int i = 0; i < 12; ++i) { it.f.m[i] = ax.model[i]; it.dd.m[i] = ax.model[i]; it.v.m[i] = Var(ax.model[i]); }
             for (int i = 0; i < 3; ++i) { it.f.size[i] = ax.size[i]; it.dd.size[i] = ax.size[i]; it.v.size[i] = Var(ax.size[i]); }
             it.f.seed = it.dd.seed = it.v.seed = ax.seed;
             it.f.custom = it.dd.custom = it.v.custom = ax.custom;
-            queues[entries[ax.entry_index].operator_code].push_back(it);
+            phase_queues[0][entries[ax.entry_index].operator_code].push_back(it);
         }
         // while (queuesManager.dequeueAll() && ++derivationStep < MaxDerivationSteps)
         // (CPUEvaluator.h:429; RoundRobinQueuesManager::dequeueAll :185-192;
         // Queue::dequeue drains FIFO including self-appended items :45-61)
         uint64_t step = 0;
+        for (uint32_t phase = 0; phase < nphases; ++phase) {
+        c.phase = phase;
+        std::vector<std::deque<Item>>& queues = phase_queues[phase];
         bool proceed = true;
         while (proceed) {
             proceed = false;
@@ -1004,7 +1173,7 @@ int pgao_derive(const pga_procedure* procs, uint32_t num_procs,
                             inst.custom = o.s.custom;
                             inst.axiom = it.meta.axiom;
                             inst.path = it.meta.path;
-                            inst.meta = (it.meta.depth << 16) | ((uint32_t)it.meta.entry & 0xFFFFu);
+                            inst.meta = (it.meta.phase << 28) | (it.meta.depth << 16) | ((uint32_t)it.meta.entry & 0xFFFFu);
                             res->inst[o.terminal].push_back(inst);
                             if (procs[entries[it.meta.entry].operator_code].gen_func_idx == 0)
                                 shape_mesh(it.f, it.meta.shape_type, res->mesh[o.terminal]);
@@ -1024,10 +1193,17 @@ int pgao_derive(const pga_procedure* procs, uint32_t num_procs,
                         ch.meta.path = path_mix(it.meta.path, o.ordinal);
                         ch.meta.depth = it.meta.depth + 1;
                         ch.meta.shape_type = o.shape_type;
+                        ch.meta.phase = o.phase;
+                        // a later phase's shape waits for it; an earlier phase's is
+                        // never derived (that phase has run); with phases, the depth
+                        // shares its 16 bits of the instance meta with the phase
+                        if (o.phase < phase) continue;
+                        if (nphases > 1 && ch.meta.depth >= 4096u)
+                            throw OpErr(PGA_ERR_MAX_DEPTH, "depth 4096 reached in a multi-phase program");
                         ch.f = o.s;
                         if (twins & PGAO_TWIN_F64) ch.dd = od[k].s;
                         if (twins & PGAO_TWIN_ADJ) ch.v = ov[k].s;
-                        queues[entries[o.entry].operator_code].push_back(ch);
+                        phase_queues[o.phase][entries[o.entry].operator_code].push_back(ch);
                     }
                 }
             }
@@ -1038,7 +1214,9 @@ int pgao_derive(const pga_procedure* procs, uint32_t num_procs,
                 break;
             }
         }
+        }
         res->steps = step;
+        res->ncolliders = (uint32_t)colliders.size();
     } catch (const OpErr& e) {
         res->error = e.what();
         g_tape = nullptr;
@@ -1054,6 +1232,10 @@ int pgao_derive(const pga_procedure* procs, uint32_t num_procs,
 
 void pgao_free(pgao_result* r) { delete r; }
 const char* pgao_error(const pgao_result* r) { return r ? r->error.c_str() : "null result"; }
+uint32_t pgao_num_colliders(const pgao_result* r) { return r ? r->ncolliders : 0u; }
+int pgao_shapes_collide(const float* am, const float* asz, int32_t at, const float* bm, const float* bsz, int32_t bt) {
+    return shapes_collide(am, asz, at, bm, bsz, bt) ? 1 : 0;
+}
 uint32_t pgao_num_terminals(const pgao_result* r) {
     uint32_t n = 0;
     for (uint32_t i = 0; i < PGA_MAX_TERMINALS; ++i) if (!r->inst[i].empty()) n = i + 1;

@@ -24,6 +24,7 @@
 extern "C" {
 #endif
 
+#define PGAO_MAX_PHASES 16u   /* successor phase indices 0 .. 15 */
 #define PGAO_TWIN_F64 1u
 #define PGAO_TWIN_ADJ 2u
 
@@ -41,6 +42,11 @@ int pgao_derive(const pga_procedure* procs, uint32_t num_procs,
 void pgao_free(pgao_result* r);
 const char* pgao_error(const pgao_result* r);
 uint32_t pgao_num_terminals(const pgao_result* r);
+/* Shapes the Collider operator stored over all phases. */
+uint32_t pgao_num_colliders(const pgao_result* r);
+/* Collision::check(stored, asking) of two Box/Quad shapes (model 3x4
+ * row-major, size, PGA_SHAPE_*): 1 when they collide. */
+int pgao_shapes_collide(const float* am, const float* asz, int32_t at, const float* bm, const float* bsz, int32_t bt);
 uint64_t pgao_count(const pgao_result* r, uint32_t terminal);
 uint64_t pgao_steps(const pgao_result* r);
 uint64_t pgao_executions(const pgao_result* r);
