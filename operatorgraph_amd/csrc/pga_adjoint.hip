@@ -445,13 +445,15 @@ __device__ __forceinline__ Ctx adj_ctx(const KArgs& a) {
     c.seed = a.use_seed;
     c.tape = 1u;
     c.prog_lds = 0u;
-    c.dmask = 0xFFFFu;       // one-phase programs only
+    c.ctxs = 0u;             // one-phase programs without Collider / IfCollides only
+    c.dmask = 0xFFFFu;
     c.phase = 0u;
     c.wr = 0u;
     c.coll = nullptr;
     c.coll_count = nullptr;
     c.coll_visible = nullptr;
     c.coll_cap = 0u;
+    c.bvh_flat = nullptr;
     return c;
 }
 

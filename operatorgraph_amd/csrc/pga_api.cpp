@@ -104,6 +104,15 @@ struct pga_context {
     uint32_t coll_cap_req = 0;        // pga_set_collider_capacity (0: kDefaultColliders)
     uint32_t* d_coll_count = nullptr;
     uint32_t colliders = 0;           // stored by the last execute
+    // LBVH over the colliders, built between phases (multi-phase programs):
+    // leaf boxes, keys, children, parents and node boxes for coll_cap leaves
+    pga_lbvh* coll_bvh = nullptr;
+    uint32_t coll_bvh_cap = 0;
+    float* d_coll_boxes = nullptr;
+    uint64_t* d_coll_keys = nullptr;
+    int32_t* d_coll_tree = nullptr;   // left | right | parent | leaf parent, coll_bvh_cap each
+    float* d_coll_nodes = nullptr;
+    float4* d_coll_flat = nullptr;    // the tree in depth-first order, 2 coll_bvh_cap - 1 nodes
 
     // derivation tape (reverse mode)
     pga::Rec* d_tape_par = nullptr;
@@ -259,6 +268,8 @@ bool check_exp(const pga_dispatch_parameter& p, std::string& why, std::vector<ui
     if (depth < 1) { why = "empty expression"; return false; }
     return true;
 }
+
+pga_status ensure_colliders(pga_context* ctx);
 
 // Validates and lowers a program into the context's tables; `upload` = false
 // stops before any device call (the tables stay on the host).
@@ -558,17 +569,9 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         ctx->collision = collision;
         return PGA_OK;
     }
-    if (collision) {
-        const uint32_t want = ctx->coll_cap_req ? ctx->coll_cap_req : kDefaultColliders;
-        if (!ctx->d_coll || ctx->coll_cap != want) {
-            if (ctx->stream) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-            hfree(ctx->d_coll);
-            ctx->coll_cap = 0;
-            HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll, (size_t)want * 5u * sizeof(float4)));
-            ctx->coll_cap = want;
-        }
-        if (!ctx->d_coll_count) HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_count, 2u * sizeof(uint32_t)));
-    }
+    ctx->collision = collision;
+    ctx->nphases = nphases;
+    if (pga_status st = ensure_colliders(ctx)) return st;
     hfree(ctx->d_image);
     HIP_TRY(ctx, hipMalloc((void**)&ctx->d_image, image_bytes));
     HIP_TRY(ctx, hipMemcpy(ctx->d_image, image.data(), image_bytes, hipMemcpyHostToDevice));
@@ -593,7 +596,7 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
     ctx->count_geom = count_geom ? 1u : 0u;
     ctx->use_seed = use_seed ? 1u : 0u;
     int per_cu = 0;
-    HIP_TRY(ctx, pga_mega_occupancy(ctx->cfg.block_size, ctx->lds_bytes, &per_cu));
+    HIP_TRY(ctx, pga_mega_occupancy(ctx->cfg.block_size, ctx->lds_bytes, collision || nphases > 1, &per_cu));
     if (per_cu < 1) return fail(ctx, PGA_ERR_UNSUPPORTED, "derivation kernel does not fit on a compute unit");
     const uint32_t max_blocks = std::min((uint32_t)per_cu, kMaxBlocksPerCu) * (uint32_t)ctx->num_cus;
     ctx->mega_blocks = ctx->cfg.grid_blocks ? std::min(ctx->cfg.grid_blocks, max_blocks) : max_blocks;
@@ -639,6 +642,52 @@ uint32_t local_grab(const pga_context* ctx, uint32_t count) {
     if (ctx->sched_grab) return std::min(ctx->sched_grab, kLocalGrabMax);
     const uint32_t blocks = std::max(ctx->spec_blocks, 1u);
     return std::max<uint32_t>(1u, std::min<uint32_t>(kLocalGrabMax, (count + blocks - 1u) / blocks));
+}
+
+void drop_collider_bvh(pga_context* ctx) {
+    if (ctx->coll_bvh) pga_lbvh_destroy(ctx->coll_bvh);
+    ctx->coll_bvh = nullptr;
+    ctx->coll_bvh_cap = 0;
+    hfree(ctx->d_coll_boxes);
+    hfree(ctx->d_coll_keys);
+    hfree(ctx->d_coll_tree);
+    hfree(ctx->d_coll_nodes);
+    hfree(ctx->d_coll_flat);
+}
+
+// The collider store of a program with Collider / IfCollides (coll_cap_req
+// colliders, default kDefaultColliders) and, for more than one phase, the
+// LBVH arrays over it; kept while large enough.
+pga_status ensure_colliders(pga_context* ctx) {
+    if (!ctx->collision) return PGA_OK;
+    const uint32_t want = ctx->coll_cap_req ? ctx->coll_cap_req : kDefaultColliders;
+    if (!ctx->d_coll || ctx->coll_cap != want) {
+        if (ctx->stream) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        hfree(ctx->d_coll);
+        ctx->coll_cap = 0;
+        HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll, (size_t)want * 5u * sizeof(float4)));
+        ctx->coll_cap = want;
+    }
+    if (!ctx->d_coll_count) HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_count, 2u * sizeof(uint32_t)));
+    if (ctx->nphases < 2 || want < 2u) return PGA_OK;
+    if (ctx->coll_bvh && ctx->coll_bvh_cap == want) return PGA_OK;
+    if (ctx->stream) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    drop_collider_bvh(ctx);
+    pga_lbvh* b = nullptr;
+    const pga_status st = pga_lbvh_create(want, ctx->device, ctx->stream, &b);
+    if (st) {
+        const std::string why = b ? pga_lbvh_last_error(b) : "bad capacity";
+        if (b) pga_lbvh_destroy(b);
+        return fail(ctx, st, "collider BVH: " + why);
+    }
+    ctx->coll_bvh = b;
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_boxes, (size_t)want * 6u * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_keys, (size_t)want * sizeof(uint64_t)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_tree, (size_t)want * 4u * sizeof(int32_t)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_nodes, (size_t)want * 6u * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_flat, (size_t)want * 2u * 2u * sizeof(float4)));
+    ctx->coll_bvh_cap = want;
+    return PGA_OK;
 }
 
 pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
@@ -859,6 +908,7 @@ PGA_API void pga_destroy(pga_context* ctx) {
     hfree(ctx->d_term_counts); hfree(ctx->d_par_counts); hfree(ctx->d_desc_counts);
     hfree(ctx->d_err); hfree(ctx->d_bar); hfree(ctx->d_block_totals); hfree(ctx->d_loc);
     hfree(ctx->d_coll); hfree(ctx->d_coll_count);
+    drop_collider_bvh(ctx);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -1030,10 +1080,29 @@ static pga_status one_phase_only(pga_context* ctx, const char* what) {
 // phases, so the re-derivation repeats them). Between launches the reset
 // keeps the error word and the terminal counts, and publishes the colliders
 // stored so far.
+// Before phase p > 0: the LBVH over the colliders stored so far
+// (GPUEvaluator.cuh:427-430 builds it after every phase but the last),
+// then the reset that keeps the error word and publishes the count.
+static pga_status begin_phase(pga_context* ctx, pga::KArgs& a, uint32_t p) {
+    a.phase = p;
+    if (p == 0) return PGA_OK;
+    if (ctx->coll_bvh) {
+        HIP_TRY(ctx, pga_launch_collider_boxes(ctx->d_coll, ctx->coll_cap, ctx->d_coll_count, ctx->d_coll_boxes,
+                                               ctx->stream));
+        const uint32_t c = ctx->coll_bvh_cap;
+        int32_t* t = ctx->d_coll_tree;
+        HIP_TRY(ctx, pga_lbvh_enqueue_counted(ctx->coll_bvh, ctx->d_coll_boxes, ctx->d_coll_count, ctx->d_coll_keys,
+                                              t, t + c, t + 2u * c, t + 3u * c, ctx->d_coll_nodes,
+                                              ctx->d_coll_flat));
+        a.bvh_flat = ctx->d_coll_flat;
+    }
+    HIP_TRY(ctx, pga_launch_reset(a, 2, ctx->stream));
+    return PGA_OK;
+}
+
 static pga_status launch_phases(pga_context* ctx, pga::KArgs& a, const void** kernel) {
     for (uint32_t p = 0; p < ctx->nphases; ++p) {
-        a.phase = p;
-        if (p > 0) HIP_TRY(ctx, pga_launch_reset(a, 2, ctx->stream));
+        if (pga_status st = begin_phase(ctx, a, p)) return st;
         HIP_TRY(ctx, launch_derivation(ctx, a, kernel));
     }
     a.phase = 0;
@@ -1099,8 +1168,7 @@ PGA_API pga_status pga_execute_range(pga_context* ctx, uint32_t first, uint32_t 
     HIP_TRY(ctx, hipEventRecord(ctx->evk0, ctx->stream));
     if (ctx->cfg.technique == PGA_TECHNIQUE_KERNELS) {
         for (uint32_t p = 0; p < ctx->nphases; ++p) {   // one level loop per phase
-            a.phase = p;
-            if (p > 0) HIP_TRY(ctx, pga_launch_reset(a, 2, ctx->stream));
+            if ((st = begin_phase(ctx, a, p))) return st;
             uint32_t n = count, level = 0;
             while (n > 0) {
                 if (level >= ctx->cfg.max_levels) {
@@ -1159,15 +1227,7 @@ PGA_API pga_status pga_set_collider_capacity(pga_context* ctx, uint32_t capacity
     NO_CAPTURE(ctx, "pga_set_collider_capacity");
     if (capacity > (1u << 26)) return fail(ctx, PGA_ERR_INVALID_ARG, "more than 2^26 colliders");
     ctx->coll_cap_req = capacity;
-    if (!ctx->collision) return PGA_OK;      // allocated when a program stores colliders
-    const uint32_t want = capacity ? capacity : kDefaultColliders;
-    if (ctx->d_coll && ctx->coll_cap == want) return PGA_OK;
-    if (ctx->stream) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    hfree(ctx->d_coll);
-    ctx->coll_cap = 0;
-    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll, (size_t)want * 5u * sizeof(float4)));
-    ctx->coll_cap = want;
-    return PGA_OK;
+    return ensure_colliders(ctx);            // allocated when a program stores colliders
 }
 
 PGA_API pga_status pga_num_colliders(const pga_context* ctx, uint32_t* colliders, uint32_t* phases) {
@@ -1181,7 +1241,7 @@ PGA_API pga_status pga_num_colliders(const pga_context* ctx, uint32_t* colliders
 // at once (cooperative launch), at most kMaxBlocksPerCu per compute unit.
 static pga_status resident_blocks(pga_context* ctx, uint32_t bs, uint32_t* blocks) {
     int per_cu = 0;
-    HIP_TRY(ctx, pga_mega_occupancy(bs, ctx->lds_bytes, &per_cu));
+    HIP_TRY(ctx, pga_mega_occupancy(bs, ctx->lds_bytes, ctx->collision || ctx->nphases > 1, &per_cu));
     *blocks = per_cu < 1 ? 0u : std::min((uint32_t)per_cu, kMaxBlocksPerCu) * (uint32_t)ctx->num_cus;
     return PGA_OK;
 }

@@ -126,6 +126,7 @@ struct Ctx {
     // collider store (5 x 16 bytes per collider: model rows, size and tag,
     // shape type and phase), of which the first *coll_visible were stored by
     // earlier phases
+    uint32_t ctxs;                      // 1: the kernel runs Collider / IfCollides and phases
     uint32_t dmask;
     uint32_t phase;
     uint32_t wr;                        // 1: write pass (colliders are stored once, here)
@@ -133,10 +134,31 @@ struct Ctx {
     uint32_t* coll_count;
     const uint32_t* coll_visible;
     uint32_t coll_cap;
+    // linear BVH over the visible colliders in depth-first order (NULL: none
+    // built): per node two 16-byte words, box min xyz, max x | max y, max z,
+    // the node after its subtree, the collider of a leaf (~0: internal)
+    const float4* bvh_flat;
 };
 
+// Axis-aligned box around a Box or Quad shape (a Quad with its z extent,
+// which the collision test keeps on the Quad's own face axis) under its
+// row-major 3x4 model, widened by 1e-5 of the magnitudes of its centre and
+// half extents, so that FP32 rounding never separates the boxes of shapes
+// the collision test finds touching.
+__device__ __forceinline__ void shape_box(const float m[12], const float s[3], float lo[3], float hi[3]) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const float h = 0.5f * (fabsf(m[4 * r]) * fabsf(s[0]) + fabsf(m[4 * r + 1]) * fabsf(s[1]) +
+                                fabsf(m[4 * r + 2]) * fabsf(s[2]));
+        const float c = m[4 * r + 3];
+        const float w = h + 1e-5f * (fabsf(c) + h) + 1e-20f;
+        lo[r] = c - w;
+        hi[r] = c + w;
+    }
+}
+
 __device__ __forceinline__ uint32_t rec_depth(const Ctx& c, const Rec& r) { return (r.ed >> 16) & c.dmask; }
-__device__ __forceinline__ bool multi_phase(const Ctx& c) { return c.dmask != 0xFFFFu; }
+__device__ __forceinline__ bool multi_phase(const Ctx& c) { return c.ctxs && c.dmask != 0xFFFFu; }
 __device__ __forceinline__ uint32_t rec_phase(const Ctx& c, const Rec& r) { return multi_phase(c) ? r.ed >> 28 : 0u; }
 
 __device__ __forceinline__ void raise(const Ctx& c, int status) {

@@ -59,6 +59,7 @@ struct KArgs {
     uint32_t* coll_count;
     const uint32_t* coll_visible;
     uint32_t coll_cap;
+    const float4* bvh_flat;             // BVH over the visible colliders, depth-first (NULL: none)
     // derivation tape (reverse mode): every level's parent records and
     // descriptors kept (cap_par / cap_desc = their capacities), every
     // fan-out materialised, and one outcome per child
@@ -106,13 +107,25 @@ bool pga_block_size_supported(uint32_t block_size);   // 256, 512, 768 or 1024
 hipError_t pga_launch_reset(const pga::KArgs& a, int flags, hipStream_t s);
 hipError_t pga_launch_level(const pga::KArgs& a, uint32_t level, uint32_t blocks, uint32_t block_size,
                             hipStream_t s);
-hipError_t pga_mega_occupancy(uint32_t block_size, uint32_t lds_bytes, int* blocks_per_cu);
+// blocks per compute unit of both interpreting kernels of a workgroup size;
+// context_sensitive: the variants that run Collider / IfCollides and phases
+hipError_t pga_mega_occupancy(uint32_t block_size, uint32_t lds_bytes, bool context_sensitive, int* blocks_per_cu);
 hipError_t pga_launch_mega(const pga::KArgs& a, uint32_t blocks, uint32_t block_size, hipStream_t s);
 // true for the kernels whose workgroups wait on one another (grid barrier):
 // a graph must launch them cooperatively
 bool pga_needs_coresidency(const void* fn);
 hipError_t pga_tape_occupancy(uint32_t lds_bytes, int* blocks_per_cu);
 hipError_t pga_launch_mega_tape(const pga::KArgs& a, uint32_t blocks, hipStream_t s);
+// boxes of the colliders in the store (count in *count, at most cap), for the
+// BVH built between phases
+hipError_t pga_launch_collider_boxes(const float4* coll, uint32_t cap, const uint32_t* count, float* aabbs,
+                                     hipStream_t s);
+// the LBVH build over a device count word: leaves min(*nd, capacity), scene
+// box from the device, then the tree in depth-first order with skip indices
+// (2 * capacity - 1 nodes of two float4 in `flat`; pga_lbvh.hip)
+hipError_t pga_lbvh_enqueue_counted(pga_lbvh* b, const float* aabbs, const uint32_t* nd, uint64_t* keys,
+                                    int32_t* left, int32_t* right, int32_t* parent, int32_t* leaf_parent,
+                                    float* node_aabb, float4* flat);
 hipError_t pga_launch_adjoint(const pga::KArgs& a, const pga::AdjArgs& g, hipStream_t s);
 hipError_t pga_launch_grad_out(const double* acc, float* out, uint32_t n, hipStream_t s);
 #endif

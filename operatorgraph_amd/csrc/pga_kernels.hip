@@ -29,13 +29,15 @@
 //              count (the reference's KERNELS technique, techniqueKernels.cuh);
 //   k_mega   - one cooperative persistent launch for all levels (the
 //              MEGAKERNEL technique, techniqueMegakernel.cuh).
+#include <algorithm>
+
 #include "pga_level.h"
 
 namespace pga {
 
 // One pass over the children of a level; waves take 64 consecutive
 // children at a time, in a fixed order, so both passes see the same lanes.
-template <bool TAPE>
+template <bool TAPE, bool CS>
 __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS& L, float4* PW, bool wr,
                                          uint32_t level, uint32_t n, uint32_t wave, uint32_t nwaves) {
     // inputs: written by level - 1 (ping-pong buffers, or that level's part of the tape)
@@ -177,7 +179,7 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
 // One derivation level: count pass, scan, write pass. Leaves L.next_n =
 // children of the next level; block 0 records the level's parent and
 // descriptor counts.
-template <bool TAPE>
+template <bool TAPE, bool CS>
 __device__ __forceinline__ void run_level(const KArgs& a, LevelLDS& L, float4* PW, uint32_t level, uint32_t n) {
     const uint32_t nslots = a.prog.nslots, ncnt = nslots + 2u;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -188,8 +190,8 @@ __device__ __forceinline__ void run_level(const KArgs& a, LevelLDS& L, float4* P
 #pragma unroll 1
     for (uint32_t pass = 0; pass < 2u; ++pass) {
         const bool wr = pass != 0u;
-        const Ctx c = make_ctx<TAPE>(a, L, level, wr);
-        run_pass<TAPE>(a, c, L, PW, wr, level, n, wave, nwaves);
+        const Ctx c = make_ctx<TAPE, CS>(a, L, level, wr);
+        run_pass<TAPE, CS>(a, c, L, PW, wr, level, n, wave, nwaves);
         __syncthreads();
         if (!wr) scan_counts(a, L, level, level);
     }
@@ -204,21 +206,21 @@ __device__ __forceinline__ void run_level(const KArgs& a, LevelLDS& L, float4* P
 // KERNELS technique: one level per (cooperative) launch. BS = threads per
 // workgroup; the variants differ in the registers the compiler may use
 // (BS / 256 waves per SIMD at one workgroup per compute unit).
-template <int BS>
+template <int BS, bool CS>
 __global__ void __launch_bounds__(BS) k_level(KArgs a, uint32_t level) {
     __shared__ LevelLDS L;
     __shared__ float4 PW[9 * BS];   // parent records (5 words per lane), then instance staging (4)
     stage_program(a);
     init_bases(a, L);
     const uint32_t n = first_level_size(a, level);
-    if (n != 0u) run_level<false>(a, L, PW, level, n);
+    if (n != 0u) run_level<false, CS>(a, L, PW, level, n);
     // every block has read the old bases before the barrier inside run_level
     store_bases(a, L);
 }
 
 // MEGAKERNEL technique: all levels in one persistent launch. TAPE: record
 // the derivation tape (reverse mode), one build only.
-template <int BS, bool TAPE>
+template <int BS, bool TAPE, bool CS>
 __global__ void __launch_bounds__(BS) k_mega(KArgs a) {
     __shared__ LevelLDS L;
     __shared__ float4 PW[9 * BS];   // parent records (5 words per lane), then instance staging (4)
@@ -227,7 +229,7 @@ __global__ void __launch_bounds__(BS) k_mega(KArgs a) {
     uint32_t level = 0;
     uint32_t n = a.n_axioms;
     for (; level < a.max_levels && n != 0u; ++level) {
-        run_level<TAPE>(a, L, PW, level, n);
+        run_level<TAPE, CS>(a, L, PW, level, n);
         n = min(L.next_n, TAPE ? (a.cap_desc > L.td_cur ? a.cap_desc - L.td_cur : 0u) : a.cap_desc);
         if (TAPE && threadIdx.x == 0) {       // the next level's inputs are this level's outputs
             const uint32_t ns = a.prog.nslots;
@@ -263,9 +265,34 @@ __global__ void k_reset(uint32_t* par_counts, uint32_t* desc_counts, uint32_t n_
     if ((flags & 1u) && i < PGA_MAX_TERMINALS) term_counts[i] = 0ull;
 }
 
+// Boxes (shape_box) of the colliders in the store, the leaves of the BVH
+// built between phases.
+__global__ void __launch_bounds__(256) k_collider_boxes(const float4* __restrict__ coll, uint32_t cap,
+                                                        const uint32_t* __restrict__ count, float* __restrict__ aabbs) {
+    const uint32_t n = min(*count, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4* k = coll + 5u * i;
+        const float4 w0 = k[0], w1 = k[1], w2 = k[2], w3 = k[3];
+        const float m[12] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w};
+        const float sz[3] = {w3.x, w3.y, w3.z};
+        float lo[3], hi[3];
+        shape_box(m, sz, lo, hi);
+        float* o = aabbs + (size_t)i * 6u;
+        o[0] = lo[0]; o[1] = lo[1]; o[2] = lo[2];
+        o[3] = hi[0]; o[4] = hi[1]; o[5] = hi[2];
+    }
+}
+
 } // namespace pga
 
 // ---- launchers (host side) ----
+hipError_t pga_launch_collider_boxes(const float4* coll, uint32_t cap, const uint32_t* count, float* aabbs,
+                                     hipStream_t s) {
+    const uint32_t blocks = std::max(1u, std::min(1024u, (cap + 255u) / 256u));
+    hipLaunchKernelGGL(pga::k_collider_boxes, dim3(blocks), dim3(256), 0, s, coll, cap, count, aabbs);
+    return hipGetLastError();
+}
+
 hipError_t pga_launch_reset(const pga::KArgs& a, int flags, hipStream_t s) {
     const uint32_t n = a.max_levels + 2u;
     const uint32_t blocks = (n + 255u) / 256u;
@@ -277,26 +304,42 @@ hipError_t pga_launch_reset(const pga::KArgs& a, int flags, hipStream_t s) {
 namespace {
 
 template <class F>
-bool pick_variant(uint32_t block_size, F&& f) {
+bool pick_variant(uint32_t block_size, bool cs, F&& f) {
+    if (cs) {
+        switch (block_size) {
+        case 256: f(pga::k_mega<256, false, true>, pga::k_level<256, true>); return true;
+        case 512: f(pga::k_mega<512, false, true>, pga::k_level<512, true>); return true;
+        case 768: f(pga::k_mega<768, false, true>, pga::k_level<768, true>); return true;
+        case 1024: f(pga::k_mega<1024, false, true>, pga::k_level<1024, true>); return true;
+        default: return false;
+        }
+    }
     switch (block_size) {
-    case 256: f(pga::k_mega<256, false>, pga::k_level<256>); return true;
-    case 512: f(pga::k_mega<512, false>, pga::k_level<512>); return true;
-    case 768: f(pga::k_mega<768, false>, pga::k_level<768>); return true;
-    case 1024: f(pga::k_mega<1024, false>, pga::k_level<1024>); return true;
+    case 256: f(pga::k_mega<256, false, false>, pga::k_level<256, false>); return true;
+    case 512: f(pga::k_mega<512, false, false>, pga::k_level<512, false>); return true;
+    case 768: f(pga::k_mega<768, false, false>, pga::k_level<768, false>); return true;
+    case 1024: f(pga::k_mega<1024, false, false>, pga::k_level<1024, false>); return true;
     default: return false;
     }
 }
 
+// the variant for a launch: the context-sensitive one when the program
+// stores colliders or has phases
+bool context_sensitive(const pga::KArgs& a) { return a.coll != nullptr || a.dmask != 0xFFFFu; }
+
 } // namespace
 
 bool pga_block_size_supported(uint32_t block_size) {
-    return pick_variant(block_size, [](auto, auto) {});
+    return pick_variant(block_size, false, [](auto, auto) {});
 }
 
 bool pga_needs_coresidency(const void* fn) {
-    bool hit = fn == (const void*)pga::k_mega<pga::kTapeBlock, true>;
+    bool hit = fn == (const void*)pga::k_mega<pga::kTapeBlock, true, false>;
     for (uint32_t bs : {256u, 512u, 768u, 1024u})
-        pick_variant(bs, [&](auto mega, auto lvl) { hit = hit || fn == (const void*)mega || fn == (const void*)lvl; });
+        for (bool cs : {false, true})
+            pick_variant(bs, cs, [&](auto mega, auto lvl) {
+                hit = hit || fn == (const void*)mega || fn == (const void*)lvl;
+            });
     return hit;
 }
 
@@ -305,14 +348,15 @@ hipError_t pga_launch_level(const pga::KArgs& a, uint32_t level, uint32_t blocks
     pga::KArgs copy = a;
     void* args[] = {&copy, &level};
     const void* fn = nullptr;
-    if (!pick_variant(block_size, [&](auto mega, auto lvl) { (void)mega; fn = (const void*)lvl; }))
+    if (!pick_variant(block_size, context_sensitive(a), [&](auto mega, auto lvl) { (void)mega; fn = (const void*)lvl; }))
         return hipErrorInvalidValue;
     return hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(block_size), args, a.lds_bytes, s);
 }
 
-hipError_t pga_mega_occupancy(uint32_t block_size, uint32_t lds_bytes, int* blocks_per_cu) {
+hipError_t pga_mega_occupancy(uint32_t block_size, uint32_t lds_bytes, bool context_sensitive,
+                              int* blocks_per_cu) {
     hipError_t e = hipErrorInvalidValue;
-    pick_variant(block_size, [&](auto mega, auto lvl) {
+    pick_variant(block_size, context_sensitive, [&](auto mega, auto lvl) {
         int a = 0, b = 0;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, mega, (int)block_size, lds_bytes);
         if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, lvl, (int)block_size, lds_bytes);
@@ -325,19 +369,19 @@ hipError_t pga_launch_mega(const pga::KArgs& a, uint32_t blocks, uint32_t block_
     pga::KArgs copy = a;
     void* args[] = {&copy};
     const void* fn = nullptr;
-    if (!pick_variant(block_size, [&](auto mega, auto lvl) { (void)lvl; fn = (const void*)mega; }))
+    if (!pick_variant(block_size, context_sensitive(a), [&](auto mega, auto lvl) { (void)lvl; fn = (const void*)mega; }))
         return hipErrorInvalidValue;
     return hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(block_size), args, a.lds_bytes, s);
 }
 
 hipError_t pga_tape_occupancy(uint32_t lds_bytes, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, pga::k_mega<pga::kTapeBlock, true>,
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, pga::k_mega<pga::kTapeBlock, true, false>,
                                                          (int)pga::kTapeBlock, lds_bytes);
 }
 
 hipError_t pga_launch_mega_tape(const pga::KArgs& a, uint32_t blocks, hipStream_t s) {
     pga::KArgs copy = a;
     void* args[] = {&copy};
-    return hipLaunchCooperativeKernel((const void*)pga::k_mega<pga::kTapeBlock, true>, dim3(blocks),
+    return hipLaunchCooperativeKernel((const void*)pga::k_mega<pga::kTapeBlock, true, false>, dim3(blocks),
                                       dim3(pga::kTapeBlock), args, a.lds_bytes, s);
 }

@@ -617,6 +617,55 @@ __global__ void __launch_bounds__(256) k_refit_global(const float* __restrict__ 
     }
 }
 
+// The tree in depth-first order with skip indices, for walks without a
+// stack: node k is two float4, box min xyz, max x | max y, max z, the index
+// after k's subtree, the box index of a leaf (0xFFFFFFFF: internal). The
+// node whose sorted leaf range starts at a sits at 2a + (the number of its
+// ancestors whose left subtree holds it): the subtrees to its left hold a
+// leaves and a - (ancestors it is the right child of) internal nodes, and
+// each ancestor comes before it. Its subtree spans 2 (leaves) - 1 nodes.
+// Thread t < n: leaf t; n <= t < 2n - 1: internal node t - n.
+__global__ void __launch_bounds__(256) k_flatten(const unsigned long long* __restrict__ keys, uint32_t n,
+                                                 const uint32_t* nd, const int32_t* __restrict__ left,
+                                                 const int32_t* __restrict__ parent,
+                                                 const int32_t* __restrict__ leaf_parent,
+                                                 const int32_t* __restrict__ other_end,
+                                                 const float* __restrict__ node_aabb,
+                                                 const float* __restrict__ aabbs, float4* __restrict__ flat) {
+    const uint32_t nn = leaf_count(n, nd);
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nn == 0u || t >= 2u * nn - 1u) return;
+    int32_t cur, par;
+    uint32_t first, len, item;
+    const float* box;
+    if (t < nn) {
+        cur = (int32_t)t;
+        par = nn > 1u ? leaf_parent[t] : -1;
+        first = t;
+        len = 1u;
+        item = (uint32_t)keys[t];
+        box = aabbs + (size_t)item * 6u;
+    } else {
+        const int32_t j = (int32_t)(t - nn);
+        const int32_t e = other_end[j];
+        cur = -j;
+        par = j == 0 ? -1 : parent[j];
+        first = (uint32_t)min(j, e);
+        len = 2u * ((uint32_t)max(j, e) - first + 1u) - 1u;
+        item = 0xFFFFFFFFu;
+        box = node_aabb + (size_t)j * 6u;
+    }
+    uint32_t lefts = 0;
+    while (par >= 0) {                     // at most the tree's depth
+        if (left[par] == cur) ++lefts;
+        cur = -par;
+        par = parent[par];
+    }
+    const uint32_t k = 2u * first + lefts;
+    flat[2u * k] = make_float4(box[0], box[1], box[2], box[3]);
+    flat[2u * k + 1u] = make_float4(box[4], box[5], __uint_as_float(k + len), __uint_as_float(item));
+}
+
 } // namespace bvhk
 
 // ---- host API ----
@@ -892,6 +941,30 @@ PGA_API pga_status pga_lbvh_build_device(pga_lbvh* b, const float* aabbs, const 
     if (timed) LHIP(b, hipEventRecord(b->e1, b->stream));
     return PGA_OK;
 }
+
+} // extern "C"
+
+// The device-count build over a caller's count word (the colliders a
+// multi-phase derivation stored, built between its phases by pga_api.cpp):
+// min(*nd, capacity) leaves, the scene box from the device.
+hipError_t pga_lbvh_enqueue_counted(pga_lbvh* b, const float* aabbs, const uint32_t* nd, uint64_t* keys,
+                                    int32_t* left, int32_t* right, int32_t* parent, int32_t* leaf_parent,
+                                    float* node_aabb, float4* flat) {
+    const uint32_t n = b->capacity;
+    const uint32_t blocks = std::min<uint32_t>(1024u, (n + 255u) / 256u);
+    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, b->stream, aabbs, n, nd, b->bounds);
+    hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(256), 0, b->stream, b->bounds, blocks);
+    const float zero[3] = {0.0f, 0.0f, 0.0f}, one[3] = {1.0f, 1.0f, 1.0f};
+    const hipError_t e = enqueue_build(b, aabbs, n, nd, zero, one, b->bounds, keys, left, right, parent,
+                                       leaf_parent, node_aabb);
+    if (e != hipSuccess) return e;
+    const uint32_t g = (uint32_t)(((uint64_t)2u * n + 255u) / 256u);
+    hipLaunchKernelGGL(k_flatten, dim3(g), dim3(256), 0, b->stream, reinterpret_cast<const unsigned long long*>(keys),
+                       n, nd, left, parent, leaf_parent, b->other_end, node_aabb, aabbs, flat);
+    return hipGetLastError();
+}
+
+extern "C" {
 
 PGA_API pga_status pga_lbvh_leaf_count(pga_lbvh* b, uint32_t* n, int32_t* overflow) {
     if (!b || !n) return lfail(b, PGA_ERR_INVALID_ARG, "null argument");

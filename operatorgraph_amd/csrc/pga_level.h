@@ -233,7 +233,10 @@ __device__ __forceinline__ void stage_program(const KArgs& a) {
     __syncthreads();
 }
 
-template <bool TAPE>
+// CS: the kernel variant that runs context-sensitive programs (Collider,
+// IfCollides, phases); the others see those fields as constants and drop
+// the code.
+template <bool TAPE, bool CS = false>
 __device__ __forceinline__ Ctx make_ctx(const KArgs& a, const LevelLDS& L, uint32_t level, bool wr) {
     Ctx c;
     c.prog.entries = reinterpret_cast<const DevEntry*>(pga_smem);
@@ -270,13 +273,15 @@ __device__ __forceinline__ Ctx make_ctx(const KArgs& a, const LevelLDS& L, uint3
     c.max_chain = a.max_chain;
     c.geom = wr ? 1u : a.count_geom;
     c.seed = a.use_seed;
-    c.dmask = a.dmask;
-    c.phase = a.phase;
+    c.ctxs = CS ? 1u : 0u;
+    c.dmask = CS ? a.dmask : 0xFFFFu;
+    c.phase = CS ? a.phase : 0u;
     c.wr = wr ? 1u : 0u;
-    c.coll = a.coll;
-    c.coll_count = a.coll_count;
-    c.coll_visible = a.coll_visible;
-    c.coll_cap = a.coll_cap;
+    c.coll = CS ? a.coll : nullptr;
+    c.coll_count = CS ? a.coll_count : nullptr;
+    c.coll_visible = CS ? a.coll_visible : nullptr;
+    c.coll_cap = CS ? a.coll_cap : 0u;
+    c.bvh_flat = CS ? a.bvh_flat : nullptr;
     return c;
 }
 

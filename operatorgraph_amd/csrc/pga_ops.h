@@ -221,26 +221,55 @@ __device__ __forceinline__ bool shapes_overlap(const float am[12], const float a
     return sat_overlap(am, ah, aquad ? aq : ah, bm, bh, bquad ? bq : bh);
 }
 
-// IfCollides' query (IntermediateSymbolsBufferAdapter.cuh:89-136 over the
-// candidates of BVHTraversal.cuh:21-148): does a collider that an earlier
-// phase than the shape's stored with this tag overlap the shape? The store
-// holds `n` such colliders; the answer is an existence, independent of the
-// order they are visited in.
-__device__ __forceinline__ bool collider_hit(const float4* __restrict__ coll, uint32_t n, uint32_t tag,
-                                             uint32_t phase, uint32_t shape, float m0, float m1, float m2, float m3,
-                                             float m4, float m5, float m6, float m7, float m8, float m9, float m10,
-                                             float m11, float s0, float s1, float s2) {
+// Collider i of the store against the asking shape (bm, bs, shape): the tag
+// matches, an earlier phase than the shape's stored it, and the shapes
+// collide (IntermediateSymbolsBufferAdapter.cuh:89-136).
+__device__ __forceinline__ bool collider_test(const float4* __restrict__ coll, uint32_t i, uint32_t tag,
+                                              uint32_t phase, uint32_t shape, const float bm[12], const float bs[3]) {
+    const float4* k = coll + 5u * i;
+    const float4 w3 = k[3], w4 = k[4];
+    if (__float_as_uint(w3.w) != tag || __float_as_uint(w4.y) >= phase) return false;
+    const float4 w0 = k[0], w1 = k[1], w2 = k[2];
+    const float am[12] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w};
+    const float as[3] = {w3.x, w3.y, w3.z};
+    return shapes_overlap(am, as, __float_as_uint(w4.x), bm, bs, shape);
+}
+
+// IfCollides' query: does a collider that an earlier phase than the shape's
+// stored with this tag overlap the shape? The store holds `n` such
+// colliders. With a BVH (built over them between phases, as
+// MultiPhaseEvaluator::execute does, GPUEvaluator.cuh:427-430) the walk of
+// BVHTraversal.cuh:21-148 runs without a stack over the nodes in
+// depth-first order: a node whose box meets the shape's box is entered (the
+// next node), any other skipped (its skip index); at a leaf the collider is
+// tested in full. The boxes are conservative, so the answer is the
+// existence over all n colliders, which a scan gives for few colliders.
+__device__ __forceinline__ bool collider_hit(const Ctx& c, uint32_t n, uint32_t tag, uint32_t phase,
+                                             uint32_t shape, float m0, float m1, float m2, float m3, float m4,
+                                             float m5, float m6, float m7, float m8, float m9, float m10, float m11,
+                                             float s0, float s1, float s2) {
     const float bm[12] = {m0, m1, m2, m3, m4, m5, m6, m7, m8, m9, m10, m11};
     const float bs[3] = {s0, s1, s2};
+    if (c.bvh_flat == nullptr || n <= 32u) {
 #pragma unroll 1
-    for (uint32_t i = 0; i < n; ++i) {
-        const float4* k = coll + 5u * i;
-        const float4 w3 = k[3], w4 = k[4];
-        if (__float_as_uint(w3.w) != tag || __float_as_uint(w4.y) >= phase) continue;
-        const float4 w0 = k[0], w1 = k[1], w2 = k[2];
-        const float am[12] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w};
-        const float as[3] = {w3.x, w3.y, w3.z};
-        if (shapes_overlap(am, as, __float_as_uint(w4.x), bm, bs, shape)) return true;
+        for (uint32_t i = 0; i < n; ++i)
+            if (collider_test(c.coll, i, tag, phase, shape, bm, bs)) return true;
+        return false;
+    }
+    float lo[3], hi[3];
+    shape_box(bm, bs, lo, hi);
+    const uint32_t nodes = 2u * n - 1u;
+    uint32_t i = 0;
+#pragma unroll 1
+    while (i < nodes) {
+        const float4 w0 = c.bvh_flat[2u * i], w1 = c.bvh_flat[2u * i + 1u];
+        if (w0.x > hi[0] || w0.y > hi[1] || w0.z > hi[2] || w0.w < lo[0] || w1.x < lo[1] || w1.y < lo[2]) {
+            i = __float_as_uint(w1.z);          // skip the subtree
+            continue;
+        }
+        const uint32_t item = __float_as_uint(w1.w);
+        if (item != 0xFFFFFFFFu && collider_test(c.coll, item, tag, phase, shape, bm, bs)) return true;
+        ++i;
     }
     return false;
 }
@@ -319,14 +348,14 @@ __device__ __forceinline__ int32_t apply_single(const Ctx& c, const E& e, Rec& r
         r.s[0] = s0; r.s[1] = s1; r.s[2] = s2;
         return e.sv0;
     }
-    if (k == K_COLLIDER) {                     // Collider.cuh:26-42
+    if (c.ctxs && k == K_COLLIDER) {           // Collider.cuh:26-42
         const uint32_t tag = f2u_rd(v0);
         if (c.wr && tag > 0u && rec_phase(c, r) == c.phase) store_collider(c, r, e.shape(), tag);
         return e.sv0;
     }
-    if (k == K_IFCOLLIDES) {                   // IfCollides.cuh:26-58
+    if (c.ctxs && k == K_IFCOLLIDES) {         // IfCollides.cuh:26-58
         const uint32_t n = min(*c.coll_visible, c.coll_cap);
-        const uint32_t t = (n != 0u && collider_hit(c.coll, n, f2u_rd(v0), rec_phase(c, r), e.shape(), r.m[0],
+        const uint32_t t = (n != 0u && collider_hit(c, n, f2u_rd(v0), rec_phase(c, r), e.shape(), r.m[0],
                                                     r.m[1], r.m[2], r.m[3], r.m[4], r.m[5], r.m[6], r.m[7], r.m[8],
                                                     r.m[9], r.m[10], r.m[11], r.s[0], r.s[1], r.s[2]))
                              ? 0u : 1u;

@@ -166,9 +166,7 @@ __device__ __forceinline__ PassIO grid_io(const KArgs& a, uint32_t level) {
 template <class S, bool WR>
 __device__ __forceinline__ void spec_pass(const KArgs& a, LevelLDS& L, float4* PW, uint32_t level, uint32_t n,
                                           uint32_t wave, uint32_t nwaves, const PassIO& io) {
-    Ctx c = make_ctx<false>(a, L, level, WR);
-    c.dmask = 0xFFFFu;   // one-phase programs only: the phase tests fold away
-    c.phase = 0u;
+    Ctx c = make_ctx<false>(a, L, level, WR);   // one-phase programs without Collider / IfCollides
     c.geom = WR ? 1u : S::kCountGeom;     // constants: the count pass drops the unread arithmetic
     c.seed = S::kUseSeed;
     c.par_out = io.par_out;
