@@ -30,6 +30,10 @@ Prints ONE JSON line. Beside the headline (batched terminals/s) it reports:
                   (generation function 0) over 16,384 lots: derivation plus
                   one tessellation launch (4 vertices, 6 indices per quad)
                   per step, with the tessellation kernel's HBM roofline;
+  context_city_16384 - the two-phase city block (Collider / IfCollides)
+                  over 16,384 lots: phase 0 stores each building and its
+                  four walls as colliders, the LBVH over them is built, and
+                  phase 1 tests one prop per yard cell against them;
   cpu_baseline  - the CPU restatement (oracle) on the host, one thread.
 """
 import argparse
@@ -64,15 +68,16 @@ def parse():
     p.add_argument("--mixed-lots", type=int, default=16384)
     p.add_argument("--mixed-grad-lots", type=int, default=16384)
     p.add_argument("--mesh-lots", type=int, default=16384)
+    p.add_argument("--context-lots", type=int, default=16384)
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--only-kernel", action="store_true",
                    help="run the timed batched steps only (for profilers)")
     p.add_argument("--spec-block", type=int, default=768,
                    help="workgroup size of the kernel specialised to the program")
-    p.add_argument("--workload", choices=("facade", "stochastic"), default="facade",
-                   help="with --only-kernel: the facade lots (--lots) or the stochastic batch "
-                        "(--stochastic-axioms)")
+    p.add_argument("--workload", choices=("facade", "stochastic", "context"), default="facade",
+                   help="with --only-kernel: the facade lots (--lots), the stochastic batch "
+                        "(--stochastic-axioms) or the context-sensitive city block (--context-lots)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="device-to-device copies of MIB MiB (scale HBM byte counters); alone, or after "
                         "the steps of --only-kernel")
@@ -237,6 +242,61 @@ def stochastic_workload(torch, engine, grammars, axioms, n, steps):
     lb.close()
     eng.close()
     return out
+
+
+def context_engine(engine, grammars, axioms, n_lots):
+    from operatorgraph_amd import abi
+    eng = engine.Engine(queue_capacity=1 << 22)
+    eng.load(grammars.context_city(walls=True)).set_axioms(axioms.city_lots(n_lots))
+    eng.set_collider_capacity(1 << 17)
+    for t in range(4):
+        eng.bind(t, 0)                       # count-only run sizes the buffers
+    eng.execute()
+    counts = [int(c) for c in eng.counts(4)]
+    for t in range(4):
+        eng.bind(t, max(counts[t], 1))
+    tune = eng.autotune(repeats=3)
+    eng.execute()
+    return eng, counts, tune
+
+
+def context_workload(torch, engine, grammars, axioms, n_lots, steps):
+    """The two-phase city block with walls (grammars.context_city) over
+    n_lots lots, autotuned over the interpreting kernels (programs with
+    Collider / IfCollides are not specialised); the oracle's single-thread
+    rate on 256 of the lots beside it."""
+    from operatorgraph_amd import abi
+    prog = grammars.context_city(walls=True)
+    eng, counts, tune = context_engine(engine, grammars, axioms, n_lots)
+    T = sum(counts)
+    sec = timed_steps(torch, eng, steps, tune["technique"])
+    colliders, phases = eng.colliders()
+    eng.close()
+    from oracle import oracle as O
+    sample = axioms.city_lots(256)
+    t0 = time.perf_counter()
+    r = O.derive(prog, sample)
+    cpu_sec = time.perf_counter() - t0
+    cpu_T = sum(len(r.instances(t)) for t in range(4))
+    return {"lots": n_lots, "phases": phases, "colliders": colliders, "props": counts[1] + counts[2],
+            "props_blocked": counts[2], "terminals_per_step": T, "ms_per_step": round(sec * 1e3, 4),
+            "terminals_per_s": round(T / sec), "queries_per_s": round((counts[1] + counts[2]) / sec),
+            "autotune": {"technique": "MEGAKERNEL" if tune["technique"] == abi.MEGAKERNEL else "KERNELS",
+                         "workgroup_size": tune["block_size"], "workgroups": tune["grid_blocks"],
+                         "candidates": tune["candidates"]},
+            "cpu_oracle": {"lots": 256, "terminals": cpu_T, "terminals_per_s": round(cpu_T / cpu_sec), "cores": 1}}
+
+
+def context_steps(torch, engine, grammars, axioms, n_lots, steps, warmup):
+    """The context-sensitive city block alone, `steps` timed executes (for
+    profilers)."""
+    eng, counts, tune = context_engine(engine, grammars, axioms, n_lots)
+    for _ in range(warmup):
+        eng.execute()
+    sec = timed_steps(torch, eng, steps, tune["technique"])
+    print(json.dumps({"workload": f"context_city_{n_lots}", "terminals_per_step": sum(counts),
+                      "ms_per_step": sec * 1e3, "block_size": tune["block_size"]}), flush=True)
+    eng.close()
 
 
 def stochastic_steps(torch, engine, grammars, axioms, n, steps, warmup):
@@ -640,6 +700,9 @@ def main():
 
     from operatorgraph_amd import abi, axioms, engine, grammars
 
+    if args.only_kernel and args.workload == "context":
+        context_steps(torch, engine, grammars, axioms, args.context_lots, args.steps, args.warmup)
+        return
     if args.only_kernel and args.workload == "stochastic":
         stochastic_steps(torch, engine, grammars, axioms, args.stochastic_axioms, args.steps, args.warmup)
         return
@@ -717,6 +780,10 @@ def main():
     if args.mesh_lots:
         mesh = mesh_workload(torch, engine, axioms, args.mesh_lots, max(3, args.steps // 4))
 
+    context = None
+    if args.context_lots:
+        context = context_workload(torch, engine, grammars, axioms, args.context_lots, max(3, args.steps // 4))
+
     grad = None
     if args.gradient_lots:
         grad = gradient_workload(torch, engine, grammars, axioms, args.gradient_lots, max(5, args.steps // 2))
@@ -754,6 +821,7 @@ def main():
         "mixed_menger5_16384_lbvh": mixed,
         "mixed_fwd_bwd_menger5_16384_lbvh": mixed_grad,
         "triangle_soup_16384": mesh,
+        "context_city_16384": context,
         "cpu_baseline": cpu,
     }
     torch.cuda.synchronize()

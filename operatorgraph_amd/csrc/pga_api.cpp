@@ -420,7 +420,10 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         if (memo[s] == 2) return false;
         const DevEntry& d = E[s];
         if (d.kind == pga::K_GENERATE || d.kind == pga::K_DISCARD) { memo[s] = 1; return true; }
-        if (!is_single(d.kind)) { memo[s] = 0; return false; }
+        // an IfCollides below a fan-out keeps it off the leaf path: its BVH
+        // walk is a chain of dependent loads, which the lanes of the next
+        // level run side by side instead of one child after another
+        if (!is_single(d.kind) || d.kind == pga::K_IFCOLLIDES) { memo[s] = 0; return false; }
         memo[s] = 2;
         bool ok = true;
         uint32_t ns = is_branch(d.kind) ? d.nsucc : 1u;
