@@ -45,6 +45,11 @@ extern "C" {
  * GPUEvaluator.cuh:395-456): successor phase indices 0..15; the instance
  * meta then holds entry | depth << 16 | phase << 28, with depth < 4096 */
 #define PGA_MAX_PHASES 16
+/* a successor phase index that continues in the phase of the shape reaching
+ * it: an operator chained inside one procedure, which the reference runs
+ * without a dispatch (the C++ front end's lowering writes it; no counterpart
+ * in the reference's table format). Other negative indices mean phase 0. */
+#define PGA_PHASE_INHERIT (-2)
 
 /* ---- status codes (the reference traps or throws; see each code) ---- */
 typedef enum pga_status {

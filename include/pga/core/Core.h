@@ -25,6 +25,10 @@
  *   3. the expanded table - one operator per entry, the form the engine
  *      runs - goes to pga_initialize with the procedure list of the
  *      distinct (operator, shape, generation function, static form) kinds.
+ *      A call's successor keeps the phase its dispatch names, an inline
+ *      operator's runs in the phase of the shape reaching it
+ *      (PGA_PHASE_INHERIT), so `MultiPhaseEvaluator<...>` (with IfCollides
+ *      and Collider, GPUEvaluator.cuh:395-456) runs the same tables.
  *
  * Shapes other than Box and Quad, the SetAs* operators and generation
  * function 0 (triangle soup) lower but are refused by pga_initialize
@@ -302,6 +306,9 @@ template <class XT, class YT, class ZT, class NextT> struct Scale {};
 template <class XT, class YT, class ZT, class NextT> struct SwapSize {};
 template <class AxisT, class ExtentT, class NextT> struct Extrude {};
 template <class... ParamsT> struct Replicate {};
+// context sensitivity (IfCollides.cuh:22-76, Collider.cuh:22-69)
+template <class ColliderTagT, class NextIfTrueT, class NextIfFalseT> struct IfCollides {};
+template <class ColliderTagT, class NextT> struct Collider {};
 template <bool ParallelT, unsigned int GenFuncIdxT = 0, class TerminalIdxT = Parameters::Scalar<0>,
           class Attr1T = Parameters::Scalar<0>, class Attr2T = Parameters::Scalar<0>,
           class Attr3T = Parameters::Scalar<0>, class Attr4T = Parameters::Scalar<0>,
@@ -361,6 +368,7 @@ struct Link {
     int kind = LINK_NODE;
     int a = -1;   // NODE: node index; DCALL: successor; PSCALL/FSCALL: procedure
     int b = -1;   // PSCALL: successor; FSCALL: fixed entry (-1: the current one)
+    int phase = PGA_PHASE_INHERIT;   // FSCALL: its PhaseIndexT (the others take it from the entry)
 };
 
 struct Node {
@@ -468,7 +476,7 @@ template <unsigned O, unsigned S, int E> struct CallLink<Operators::PSCall<O, S,
     static Link get() { return Link{LINK_PSCALL, (int)O, (int)S}; }
 };
 template <unsigned O, int P, int E, int N> struct CallLink<Operators::FSCall<O, P, E, N>> {
-    static Link get() { return Link{LINK_FSCALL, (int)O, N}; }
+    static Link get() { return Link{LINK_FSCALL, (int)O, N, P}; }
 };
 template <unsigned S, int E> struct CallLink<Operators::DCall<S, E>> {
     static Link get() { return Link{LINK_DCALL, (int)S, -1}; }
@@ -581,6 +589,22 @@ template <class A, class S, class TT, class F> struct Lower<Operators::IfSizeLes
         const int i = add_node(t, PGA_OP_IFSIZELESS, shape, {Param<A>::get(), Param<S>::get()});
         const Link a = link<TT>(t, shape), b = link<F>(t, shape);
         t.nodes[i].succ = {a, b};
+        return i;
+    }
+};
+template <class Tag, class TT, class F> struct Lower<Operators::IfCollides<Tag, TT, F>> {
+    static int emit(Template& t, int shape) {
+        const int i = add_node(t, PGA_OP_IFCOLLIDES, shape, {Param<Tag>::get()});
+        const Link a = link<TT>(t, shape), b = link<F>(t, shape);
+        t.nodes[i].succ = {a, b};
+        return i;
+    }
+};
+template <class Tag, class N> struct Lower<Operators::Collider<Tag, N>> {
+    static int emit(Template& t, int shape) {
+        const int i = add_node(t, PGA_OP_COLLIDER, shape, {Param<Tag>::get()});
+        const Link l = link<N>(t, shape);
+        t.nodes[i].succ.push_back(l);
         return i;
     }
 };
@@ -751,12 +775,19 @@ inline Program expand(const std::vector<Template>& procs, const pga_dispatch_ent
         };
         std::vector<pga_dispatch_parameter> params;
         std::vector<int> succ;
+        // the phase of each successor: a call dispatches into the phase its
+        // entry's successor names (DCall, PSCall, Call.cuh:69-70, 88-89) or
+        // its PhaseIndexT (FSCall, :184, 202); an operator chained inline
+        // runs in the current one (PGA_PHASE_INHERIT)
+        std::vector<int> phase;
+        auto succ_phase = [&](int s) { return (int)cur->successors[s].phase_index; };
         if (nd.whole_entry) {
             if (!cur) fail("a <DynParams> operator needs a dispatching entry");
             for (uint32_t k = 0; k < cur->num_parameters; ++k) params.push_back(cur->parameters[k]);
             for (uint32_t k = 0; k < cur->num_successors; ++k) {
                 const int te = (int)cur->successors[k].entry_index;
                 succ.push_back(te < 0 ? -1 : call_target(proc_of_entry(te), te));
+                phase.push_back(succ_phase((int)k));
             }
         } else {
             for (const ParamRef& r : nd.params) {
@@ -771,21 +802,27 @@ inline Program expand(const std::vector<Template>& procs, const pga_dispatch_ent
             }
             for (const Link& l : nd.succ) {
                 switch (l.kind) {
-                case LINK_NODE: succ.push_back(slot(p, e, l.a)); break;
+                case LINK_NODE:
+                    succ.push_back(slot(p, e, l.a));
+                    phase.push_back(PGA_PHASE_INHERIT);
+                    break;
                 case LINK_DCALL: {
                     const int te = succ_entry(l.a);
                     succ.push_back(te < 0 ? -1 : call_target(proc_of_entry(te), te));
+                    phase.push_back(succ_phase(l.a));
                     break;
                 }
                 case LINK_PSCALL: {
                     const int te = succ_entry(l.b);
                     if ((size_t)l.a >= procs.size()) fail("PSCall procedure out of range");
                     succ.push_back(call_target(l.a, te));
+                    phase.push_back(succ_phase(l.b));
                     break;
                 }
                 default: {   // FSCall: a fixed entry, or the current one
                     if ((size_t)l.a >= procs.size()) fail("FSCall procedure out of range");
                     succ.push_back(call_target(l.a, l.b >= 0 ? l.b : e));
+                    phase.push_back(l.phase);
                     break;
                 }
                 }
@@ -805,7 +842,7 @@ inline Program expand(const std::vector<Template>& procs, const pga_dispatch_ent
         x.num_successors = (uint32_t)succ.size();
         for (size_t k = 0; k < succ.size(); ++k) {
             x.successors[k].entry_index = succ[k];
-            x.successors[k].phase_index = succ[k] < 0 ? -1 : 0;
+            x.successors[k].phase_index = succ[k] < 0 ? -1 : phase[k];
         }
         if (out.entries.size() <= w) out.entries.resize(w + 1);
         out.entries[w] = x;
@@ -1088,6 +1125,44 @@ private:
     std::vector<uint64_t> caps_;
     std::vector<void*> buffers_;
     Lowering::Program program_;
+};
+
+// MultiPhaseEvaluator (GPUEvaluator.cuh:395-456): the phases come from the
+// dispatch table's successors, and execute() runs them all, each to the end
+// before the next starts, with the colliders stored so far gathered into a
+// BVH between them. PhaseTraitsT, which in the reference decides which
+// procedures get a queue in which phase, is accepted and not consulted:
+// every procedure runs in every phase. initialize() throws when the table
+// names a phase at or beyond NumPhasesT.
+template <class ProcListT, class AxiomGeneratorT, class GenFuncFilterT, unsigned int NumPhasesT,
+          template <class, int> class PhaseTraitsT, bool InstrumentedT, unsigned int NumSubgraphsT,
+          unsigned int NumEdgesT, class ConfigurationT>
+class MultiPhaseEvaluator : public SinglePhaseEvaluator<ProcListT, AxiomGeneratorT, GenFuncFilterT, InstrumentedT,
+                                                        NumSubgraphsT, NumEdgesT, ConfigurationT> {
+    typedef SinglePhaseEvaluator<ProcListT, AxiomGeneratorT, GenFuncFilterT, InstrumentedT, NumSubgraphsT, NumEdgesT,
+                                 ConfigurationT> Base;
+
+public:
+    void initialize(DispatchTableEntry* dispatchTable, size_t numEntries) override {
+        static_assert(NumPhasesT >= 1 && NumPhasesT <= PGA_MAX_PHASES, "NumPhasesT out of range");
+        Base::initialize(dispatchTable, numEntries);
+        uint32_t colliders = 0, phases = 0;
+        pga_num_colliders(this->context(), &colliders, &phases);
+        if (phases > NumPhasesT)
+            throw Error(PGA_ERR_BAD_PROGRAM, "PGA: the dispatch table names phase " + std::to_string(phases - 1) +
+                                             " of a " + std::to_string(NumPhasesT) + "-phase evaluator");
+    }
+    // capacity of the collider store (Collider stores one shape per call)
+    void setMaxNumColliders(uint32_t capacity) {
+        const pga_status st = pga_set_collider_capacity(this->context(), capacity);
+        if (st != PGA_OK) throw Error(st, std::string("PGA: ") + pga_last_error(this->context()));
+    }
+    // colliders the last execute stored
+    uint32_t getNumColliders() {
+        uint32_t colliders = 0, phases = 0;
+        pga_num_colliders(this->context(), &colliders, &phases);
+        return colliders;
+    }
 };
 
 } // namespace PGA

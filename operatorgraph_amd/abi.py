@@ -13,6 +13,8 @@ MAX_NUM_PARAMETER_VALUES = 16
 MAX_NUM_SUCCESSORS = 20
 MAX_NUM_AXIOM_SEEDS = 32
 MAX_TERMINALS = 64
+MAX_PHASES = 16
+PHASE_INHERIT = -2    # successor phase: the phase of the shape reaching it
 MAX_EXP_STACK = 4
 
 # pga_status

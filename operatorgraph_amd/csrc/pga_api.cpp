@@ -389,11 +389,12 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         for (uint32_t k = 0; k < e.num_successors; ++k) {
             int32_t s = e.successors[k].entry_index;
             if (s < -1 || s >= (int32_t)n) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "successor out of range");
-            // a negative (unassigned) phase index is phase 0
+            // PGA_PHASE_INHERIT: the parent's phase (bit 4); any other
+            // negative (unassigned) phase index is phase 0
             const int32_t ph = e.successors[k].phase_index;
             if (ph >= (int32_t)PGA_MAX_PHASES) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "phase index out of range");
             Sc.push_back(s);
-            Sp.push_back(ph > 0 ? (uint8_t)ph : (uint8_t)0);
+            Sp.push_back(ph == PGA_PHASE_INHERIT ? (uint8_t)16 : ph > 0 ? (uint8_t)ph : (uint8_t)0);
             if (ph > 0) nphases = std::max(nphases, (uint32_t)ph + 1u);
         }
         E[i] = d;
@@ -530,9 +531,9 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         d.ndirect = (uint8_t)nd;
         d.flags |= pga::EF_DIRECT;
     }
-    // multi-phase programs: each successor carries its phase in bits 16-19,
-    // and no fan-out counts its terminals from a table (a child's phase
-    // decides whether a launch emits it)
+    // multi-phase programs: each successor carries its phase in bits 16-19
+    // (bit 20: the parent's), and no fan-out counts its terminals from a
+    // table (a child's phase decides whether a launch emits it)
     if (nphases > 1) {
         for (size_t k = 0; k < Sc.size(); ++k)
             if (Sc[k] >= 0) Sc[k] |= (int32_t)Sp[k] << 16;

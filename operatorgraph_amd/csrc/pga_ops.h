@@ -76,12 +76,17 @@ __device__ __forceinline__ uint32_t entry_shape(const Ctx& c, uint32_t i) { retu
 __device__ __forceinline__ bool advance(const Ctx& c, Rec& r, int32_t next, uint32_t ordinal) {
     if (next < 0) return false;   // unused dynamic edge: nothing dispatched
     // successors of a multi-phase program carry their phase in bits 16-19
-    // (DispatchTableEntry.h:8-36, the successor's phaseIndex); a child bound
-    // for an earlier phase than its parent's is never derived (that phase
-    // has run), one bound for a later phase than this launch's waits for its
-    // own launch (Call.cuh:69-70 dispatches into that phase's queues)
-    const uint32_t cph = (uint32_t)next >> 16;
-    if (multi_phase(c) && (cph < (r.ed >> 28) || cph > c.phase)) return false;
+    // (DispatchTableEntry.h:8-36, the successor's phaseIndex), bit 20 for
+    // the parent's own; a child bound for an earlier phase than its parent's
+    // is never derived (that phase has run), one bound for a later phase than
+    // this launch's waits for its own launch (Call.cuh:69-70 dispatches into
+    // that phase's queues)
+    uint32_t cph = (uint32_t)next >> 16;
+    if (multi_phase(c)) {
+        const uint32_t pph = r.ed >> 28;
+        if (cph & 16u) cph = pph;
+        if (cph < pph || cph > c.phase) return false;
+    }
     uint32_t depth = rec_depth(c, r) + 1u;
     if (depth > c.dmask) { raise(c, PGA_ERR_MAX_DEPTH); return false; }
     r.ed = ((uint32_t)next & 0xFFFFu) | (depth << 16) | (cph << 28);

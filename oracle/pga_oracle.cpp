@@ -444,14 +444,16 @@ inline int32_t succ(const pga_dispatch_entry& e, uint32_t k) {
 
 // The child of successor k of entry e: its entry and the phase the successor
 // names (DispatchTableEntry.h:8-36; Call.cuh:69-70 dispatches to that phase).
-// A negative phase index (an unassigned one) is phase 0.
+// PGA_PHASE_INHERIT keeps the parent's phase (resolved by the caller); any
+// other negative phase index (an unassigned one) is phase 0.
+const uint32_t kPhaseInherit = 0xFFFFFFFFu;
 template <class T>
 void push_child(std::vector<Out<T>>& out, const pga_dispatch_entry& e, uint32_t k, uint32_t ordinal, int32_t st,
                 const Shape<T>& s) {
     Out<T> o;
     o.kind = 0; o.entry = succ(e, k); o.ordinal = ordinal; o.terminal = 0; o.shape_type = st; o.s = s;
     const int32_t ph = e.successors[k].phase_index;
-    o.phase = ph > 0 ? (uint32_t)ph : 0u;
+    o.phase = ph == PGA_PHASE_INHERIT ? kPhaseInherit : (ph > 0 ? (uint32_t)ph : 0u);
     out.push_back(o);
 }
 inline float fparam(const pga_dispatch_entry& e, uint32_t k) {
@@ -1193,17 +1195,17 @@ int pgao_derive(const pga_procedure* procs, uint32_t num_procs,
                         ch.meta.path = path_mix(it.meta.path, o.ordinal);
                         ch.meta.depth = it.meta.depth + 1;
                         ch.meta.shape_type = o.shape_type;
-                        ch.meta.phase = o.phase;
+                        ch.meta.phase = o.phase == kPhaseInherit ? it.meta.phase : o.phase;
                         // a later phase's shape waits for it; an earlier phase's is
                         // never derived (that phase has run); with phases, the depth
                         // shares its 16 bits of the instance meta with the phase
-                        if (o.phase < phase) continue;
+                        if (ch.meta.phase < phase) continue;
                         if (nphases > 1 && ch.meta.depth >= 4096u)
                             throw OpErr(PGA_ERR_MAX_DEPTH, "depth 4096 reached in a multi-phase program");
                         ch.f = o.s;
                         if (twins & PGAO_TWIN_F64) ch.dd = od[k].s;
                         if (twins & PGAO_TWIN_ADJ) ch.v = ov[k].s;
-                        phase_queues[o.phase][entries[o.entry].operator_code].push_back(ch);
+                        phase_queues[ch.meta.phase][entries[o.entry].operator_code].push_back(ch);
                     }
                 }
             }

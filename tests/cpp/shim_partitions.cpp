@@ -103,6 +103,37 @@ const unsigned int Terminals = 4;
 
 } // namespace Mix
 
+namespace Yard {   // two phases: the building stored as a collider in phase 0, props that ask in phase 1
+
+// the lot replicates into its building (the entry's successor 0, phase 0)
+// and its yard (a fixed entry in phase 1); the chains inside each procedure
+// stay in the phase their shape was dispatched to
+struct ProcedureList : T::List<
+    /* procedure[0]= */ Proc<Box, Replicate<DCall<0>, FSCall<2, 1, -1, 2>>, 1>,
+    /* procedure[1]= */ Proc<Box, Scale<Scalar<-500, 3>, Scalar<-1000, 3>, Scalar<-500, 3>,
+        Collider<Scalar<1>, Generate<false, 1, Scalar<0>>>>, 1>,
+    /* procedure[2]= */ Proc<Box, Repeat<false, AxisParam<X>, Scalar<1000, 3>, RepeatModeParam<ADJUST_TO_FILL>,
+        Repeat<false, AxisParam<Z>, Scalar<1000, 3>, RepeatModeParam<ADJUST_TO_FILL>,
+            Scale<Scalar<-500, 3>, Scalar<-300, 3>, Scalar<-500, 3>,
+                IfCollides<Scalar<1>, PSCall<3, 0>, PSCall<3, 1>>>>>, 1>,
+    /* procedure[3]= */ Proc<Box, Generate<false, 1, DynParam<0>>, 1>> {};
+
+DispatchTable dispatchTable = {
+    /* entries[0]= */ {0, {}, {{1, 0}}},
+    /* entries[1]= */ {1, {}, {}},
+    /* entries[2]= */ {2, {}, {{3, 1}, {4, 1}}},
+    /* entries[3]= */ {3, {{PT_SCALAR, {2}}}, {}},
+    /* entries[4]= */ {3, {{PT_SCALAR, {1}}}, {}},
+};
+const unsigned int Terminals = 3;
+
+// every procedure has a queue in every phase
+template <class ProcedureT, int PhaseIndexT> struct AllPhases {
+    static const bool Enabled = true;
+};
+
+} // namespace Yard
+
 struct Configuration {
     static const PGA::GPU::Technique Technique = PGA::GPU::MEGAKERNEL;
     static const unsigned int QueueSize = 1u << 20;
@@ -127,10 +158,11 @@ static std::vector<T> read_all(const std::string& path) {
     return v;
 }
 
-template <class ProcListT, class AxiomGenT>
+template <class ProcListT, class AxiomGenT,
+          class Evaluator = SinglePhaseEvaluator<ProcListT, AxiomGenT, Rendering::GenFuncFilter, false, 0, 0,
+                                                 Configuration>>
 static int run_scene(const char* mode, const std::string& scene, const std::string& dir, DispatchTable& table,
                      unsigned int terminals, const std::vector<uint64_t>& caps) {
-    typedef SinglePhaseEvaluator<ProcListT, AxiomGenT, Rendering::GenFuncFilter, false, 0, 0, Configuration> Evaluator;
     const std::string base = dir + "/" + scene;
     if (std::string(mode) == "lower") {
         const auto procs = Lowering::lower<ProcListT>();
@@ -259,6 +291,11 @@ extern "C" __attribute__((visibility("default"))) int shim_main(int argc, char**
         if (scene == "mix")
             return run_scene<Mix::ProcedureList, ListedAxioms>(mode.c_str(), scene, dir, Mix::dispatchTable,
                                                                Mix::Terminals, caps);
+        if (scene == "yard")
+            return run_scene<Yard::ProcedureList, ListedAxioms,
+                             MultiPhaseEvaluator<Yard::ProcedureList, ListedAxioms, Rendering::GenFuncFilter, 2,
+                                                 Yard::AllPhases, false, 0, 0, Configuration>>(
+                mode.c_str(), scene, dir, Yard::dispatchTable, Yard::Terminals, caps);
         std::fprintf(stderr, "unknown scene %s\n", scene.c_str());
         return 2;
     } catch (const std::exception& e) {

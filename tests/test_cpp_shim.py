@@ -319,3 +319,57 @@ def test_interpreter_flow_triangle_soup(tmp_path):
         ref = r.mesh(t).reshape(n, 4, 9)
         assert np.array_equal(v[canon_order(got)], ref[canon_order(r.instances(t))])
         assert np.array_equal(idx, mesh_indices(n, abi.QUAD))
+
+
+def test_yard_partition_lowers_its_phases(tmp_path):
+    """The two-phase Yard partition: the lot's DCall keeps the table's phase
+    0, its FSCall<2, 1, -1, 2> goes to entry 2 in phase 1, the operators
+    chained inside each procedure run in their shape's phase
+    (PGA_PHASE_INHERIT), and the yard's PSCalls keep the table's phase 1.
+    On a 4 x 3 x 4 lot the building (2 x 3 x 2 at the centre) blocks the
+    four inner props of sixteen."""
+    low = lower("yard", tmp_path)
+    phases = {}
+    for i in range(len(low.entries)):
+        e = low.entries[i]
+        op = low.procs[e.operator_code].op_type
+        phases[i] = (op, [(e.successors[k].entry_index, e.successors[k].phase_index) for k in range(e.num_successors)])
+    assert phases[0][0] == abi.REPLICATE and [p for _, p in phases[0][1]] == [0, 1]
+    assert phases[0][1][1][0] == 2
+    assert phases[2][0] == abi.REPEAT and phases[2][1][0][1] == abi.PHASE_INHERIT
+    ifc = [i for i, (op, _) in phases.items() if op == abi.IFCOLLIDES]
+    assert len(ifc) == 1 and [p for _, p in phases[ifc[0]][1]] == [1, 1]
+    col = [i for i, (op, _) in phases.items() if op == abi.COLLIDER]
+    assert len(col) == 1 and [p for _, p in phases[col[0]][1]] == [abi.PHASE_INHERIT]
+    ax = axioms.empty(1)
+    ax["size"] = (4.0, 3.0, 4.0)
+    ax["entry_index"] = 0
+    ax["shape_type"] = abi.BOX
+    r = O.derive(low, ax)
+    assert [r.count(t) for t in range(3)] == [1, 12, 4] and r.colliders == 1
+    assert np.all(r.instances(1)["meta"] >> 28 == 1) and np.all(r.instances(0)["meta"] >> 28 == 0)
+
+
+@pytest.mark.gpu
+def test_cpp_multi_phase_evaluator_yard(tmp_path):
+    _note("C++ MultiPhaseEvaluator: Yard partition, 64 lots")
+    low = lower("yard", tmp_path)
+    rng = np.random.default_rng(5)
+    n = 64
+    ax = axioms.empty(n)
+    ax["size"][:, 0] = rng.uniform(3.0, 9.0, n).astype(np.float32)
+    ax["size"][:, 1] = 3.0
+    ax["size"][:, 2] = rng.uniform(3.0, 9.0, n).astype(np.float32)
+    ax["model"][:, 3] = (np.arange(n) % 8 * 12.0).astype(np.float32)
+    ax["model"][:, 11] = (np.arange(n) // 8 * 12.0).astype(np.float32)
+    ax["entry_index"] = 0
+    ax["shape_type"] = abi.BOX
+    r = O.derive(low, ax)
+    caps = np.array([max(1, r.count(t)) for t in range(3)], dtype=np.uint64)
+    base = os.path.join(str(tmp_path), "yard")
+    open(base + ".axioms", "wb").write(np.ascontiguousarray(ax).tobytes())
+    open(base + ".caps", "wb").write(caps.tobytes())
+    run_shim("run", "yard", tmp_path)
+    for t in range(3):
+        assert_same_multiset(_read_instances(f"{base}.t{t}"), r.instances(t), f"yard terminal {t}")
+    assert r.count(2) > 0 and r.count(1) > 0
