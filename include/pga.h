@@ -580,11 +580,15 @@ PGA_API void pga_grammar_destroy(pga_grammar* grammar);
  *    src/interpreter/InterpreterApp.cpp:214-223), written as C++ in the
  *    form CodeGenerator::fromPartition emits (src/compiler/CodeGenerator.cpp:
  *    1691-1722): Proc<Shape, operator tree, 1> structs P<k>, the procedure
- *    list, a DispatchTable, in `namespace name`. Uncut edges are inlined;
- *    cut edges become FSCall<procedure, 0, -1, entry>. Edges that close a
- *    cycle must be cut. Host code only. The result object holds the code
- *    (pga_grammar_code), the edges' cycle flags (pga_grammar_edges) and an
- *    error message; destroy it with pga_grammar_destroy. -- */
+ *    list, a DispatchTable and NumPhases, in `namespace name`. Uncut edges
+ *    are inlined; cut edges become FSCall<procedure, phase, -1, entry>.
+ *    Phases follow Graph::computePhases (src/compiler/Graph.cpp:149-272):
+ *    an IfCollides(tag) runs one phase after the latest Collider(tag), every
+ *    other operator in its predecessor's phase. Edges that close a cycle or
+ *    enter such an IfCollides must be cut: their flags from
+ *    pga_grammar_edges are 1 (cycle) | 2 (phase). Host code only. The result
+ *    object holds the code (pga_grammar_code), the edge flags and an error
+ *    message; destroy it with pga_grammar_destroy. -- */
 PGA_API pga_status pga_generate_partition(const char* source, const char* partition, const char* name,
                                           pga_grammar** out);
 PGA_API pga_status pga_grammar_code(const pga_grammar* grammar, const char** code, uint64_t* length);

@@ -20,7 +20,8 @@ Errors in the source raise `GrammarError` with the line number.
 graph, cut by a partition, as C++ partition code (Proc<Shape, operator
 tree> structs, the procedure list and a DispatchTable) for the C++ front
 end (include/pga/core/Core.h); `operator_graph_edges(text)` gives the
-graph's edges with their cycle flags (cycle edges must be cut).
+graph's edges with their must-cut flags (edges that close a cycle or enter
+a later phase's IfCollides must be cut).
 """
 import ctypes as C
 
@@ -155,7 +156,8 @@ def _partition(source, partition, name):
 
 
 def operator_graph_edges(source):
-    """Cycle flags of the operator graph's edges, in edge order."""
+    """Must-cut flags of the operator graph's edges, in edge order: the edge
+    closes a cycle, or enters an IfCollides of a later phase."""
     return _partition(source, None, "G")[1]
 
 

@@ -816,6 +816,7 @@ struct GVertex {
 struct GEdge {
     int from = -1, to = -1;
     bool cycle = false;
+    bool phase = false;      // enters an IfCollides that starts a later phase: a partition must cut it
 };
 
 class GraphBuilder {
@@ -824,17 +825,98 @@ public:
 
     std::vector<GVertex> V;
     std::vector<GEdge> E;
+    std::vector<int> phase;   // per vertex: the phase its operator runs in (-1: none assigned)
+    int nphases = 1;
 
     void run() {
         if (g_.axioms.empty()) throw CompileError("no axiom");
         const Axiom& a = g_.axioms[0];
         symbol_vertex(a.symbol, a.shape, 0);
+        compute_phases();
     }
 
 private:
     const Parser& g_;
     std::map<std::string, int> named_;   // rule or terminal symbol -> vertex
     std::vector<char> on_stack_;
+
+    int op_of(int v) const { return V[v].node ? V[v].node->op : PGA_OP_GENERATE; }
+    unsigned tag_of(int v) const {
+        const Successor* s = V[v].node;
+        if (s->params.empty() || s->params[0].kind != OD_NUMBER)
+            throw CompileError(at_line(s->line) + operator_name(s->op) + " takes one number, its collider tag");
+        return (unsigned)s->params[0].a;
+    }
+
+    // Graph::computePhases (src/compiler/Graph.cpp:149-272): a walk from the
+    // axiom's operator, each operator in its predecessor's phase, except an
+    // IfCollides(tag), which waits until every Collider(tag) has been reached
+    // and then runs one phase after the latest of them; the edge into it
+    // must be cut (a dispatch into that phase). The walk order is the
+    // reference's: a deque whose front is taken, the children of an operator
+    // pushed to the front in successor order, an IfCollides put back on a
+    // waiting list (once) while colliders are missing, the waiting list's
+    // front resumed when the deque runs dry. An IfCollides still waiting
+    // for its colliders after that one retry keeps no phase in the
+    // reference (which reports it only in verbose builds); here it is an
+    // error, since its table would test against colliders of its own phase.
+    void compute_phases() {
+        phase.assign(V.size(), -1);
+        std::map<unsigned, std::vector<int>> colliders;
+        for (size_t v = 0; v < V.size(); ++v)
+            if (op_of((int)v) == PGA_OP_COLLIDER) colliders[tag_of((int)v)].push_back((int)v);
+        if (op_of(0) == PGA_OP_IFCOLLIDES) throw CompileError("the first operator can not be an IfCollides");
+        struct Item { int src, dst, edge; };
+        std::deque<Item> stack, waiting;
+        std::set<int> visited;
+        std::map<int, unsigned> pops;
+        std::vector<int> unresolved;
+        auto resume = [&]() {
+            if (!stack.empty() || waiting.empty()) return;
+            const Item w = waiting.front();
+            waiting.pop_front();
+            ++pops[w.dst];
+            stack.push_front(w);
+        };
+        stack.push_back({-1, 0, -1});
+        while (!stack.empty()) {
+            const Item cur = stack.front();
+            stack.pop_front();
+            const int pred = cur.src >= 0 ? phase[cur.src] : 0;
+            if (op_of(cur.dst) == PGA_OP_IFCOLLIDES) {
+                auto it = colliders.find(tag_of(cur.dst));
+                if (it != colliders.end()) {
+                    int latest = -1;
+                    bool missing = false;
+                    for (int c : it->second) {
+                        latest = std::max(latest, phase[c]);
+                        missing |= visited.count(c) == 0;
+                    }
+                    if (missing) {
+                        if (pops[cur.dst] < 1) waiting.push_back(cur);
+                        else unresolved.push_back(cur.dst);
+                        resume();
+                        continue;
+                    }
+                    phase[cur.dst] = latest + 1;
+                    if (cur.edge >= 0) E[cur.edge].phase = true;
+                }
+            } else {
+                phase[cur.dst] = pred;
+            }
+            visited.insert(cur.dst);
+            for (int e : V[cur.dst].out)
+                if (!visited.count(E[e].to)) stack.push_front({cur.dst, E[e].to, e});
+            resume();
+        }
+        for (int v : unresolved)
+            if (phase[v] < 0)
+                throw CompileError(at_line(V[v].node->line) + "IfCollides(" + std::to_string(tag_of(v)) +
+                                   ") is reached before its colliders in the phase walk: list the branch that "
+                                   "stores them first");
+        for (int p : phase) nphases = std::max(nphases, p + 1);
+        if (nphases > (int)PGA_MAX_PHASES) throw CompileError("more than 16 phases");
+    }
 
     std::string resolve(const std::string& name) const {
         std::string s = name;
@@ -920,7 +1002,7 @@ private:
 class CodeGen {
 public:
     CodeGen(const Parser& p, const GraphBuilder& gb, const std::string& cuts)
-        : g_(p), V_(gb.V), E_(gb.E), cut_(gb.E.size(), 0) {
+        : g_(p), V_(gb.V), E_(gb.E), phase_(gb.phase), nphases_(gb.nphases), cut_(gb.E.size(), 0) {
         if (!cuts.empty() && cuts.size() != E_.size())
             throw CompileError("the partition names " + std::to_string(cuts.size()) + " edges; the graph has " +
                                std::to_string(E_.size()));
@@ -930,6 +1012,8 @@ public:
                 throw CompileError("a partition is a string of '0' (keep) and '1' (cut), one per edge");
             if (E_[e].cycle && !c)
                 throw CompileError("edge " + std::to_string(e) + " closes a cycle and must be cut");
+            if (E_[e].phase && !c)
+                throw CompileError("edge " + std::to_string(e) + " enters an IfCollides of a later phase and must be cut");
             cut_[e] = c;
         }
         proc_of_.assign(V_.size(), -1);
@@ -959,7 +1043,8 @@ public:
         for (size_t k = 0; k < roots_.size(); ++k)
             o += "    /* entries[" + std::to_string(k) + "]= */ {" + std::to_string(k) + ", {}, {}},\n";
         o += "};\n\nconst unsigned int AxiomEntry = 0;\nconst unsigned int AxiomShape = " +
-             std::to_string(V_[0].shape) + ";\n\n} // namespace " + name + "\n";
+             std::to_string(V_[0].shape) + ";\nconst unsigned int NumPhases = " + std::to_string(nphases_) +
+             ";\n\n} // namespace " + name + "\n";
         return o;
     }
 
@@ -967,6 +1052,8 @@ private:
     const Parser& g_;
     const std::vector<GVertex>& V_;
     const std::vector<GEdge>& E_;
+    const std::vector<int>& phase_;
+    int nphases_;
     std::vector<char> cut_;
     std::vector<int> proc_of_;
     std::vector<int> roots_;
@@ -1063,9 +1150,9 @@ private:
     std::string succ(int v, size_t i) {
         const int e = V_[v].out[i];
         const int w = E_[e].to;
-        if (cut_[e]) {
+        if (cut_[e]) {   // FSCall<procedure, its phase, -1, entry> (CodeGenerator.cpp:1161)
             const std::string k = std::to_string(proc_of_[w]);
-            return "FSCall<" + k + ", 0, -1, " + k + ">";
+            return "FSCall<" + k + ", " + std::to_string(std::max(phase_[w], 0)) + ", -1, " + k + ">";
         }
         return tree(w);
     }
@@ -1116,6 +1203,12 @@ private:
         case PGA_OP_IFSIZELESS:
             need_succ(2);
             return "IfSizeLess<" + P(0) + ", " + P(1) + ", " + succ(v, 0) + ", " + succ(v, 1) + ">";
+        case PGA_OP_IFCOLLIDES:
+            need_succ(2);
+            return "IfCollides<" + P(0) + ", " + succ(v, 0) + ", " + succ(v, 1) + ">";
+        case PGA_OP_COLLIDER:
+            need_succ(1);
+            return "Collider<" + P(0) + ", " + succ(v, 0) + ">";
         case PGA_OP_STOCHASTIC: {
             if (s->factors.size() != s->next.size() || s->next.empty())
                 throw CompileError(at_line(ln) + "RandomRule requires a factor before each successor");
@@ -1140,7 +1233,8 @@ struct pga_grammar {
     std::vector<pga_dispatch_entry> entries;
     std::vector<pga_grammar_axiom> axioms;
     std::string code;                 // partition code (pga_generate_partition)
-    std::vector<uint8_t> cycle;       // per operator-graph edge: 1 = closes a cycle
+    std::vector<uint8_t> cycle;       // per operator-graph edge: 1 = closes a cycle, 2 = enters a later
+                                      // phase's IfCollides (either: a partition must cut it)
     std::string error;
 };
 
@@ -1221,7 +1315,7 @@ PGA_API pga_status pga_generate_partition(const char* source, const char* partit
         p.parse();
         GraphBuilder gb(p);
         gb.run();
-        for (const GEdge& e : gb.E) g->cycle.push_back(e.cycle ? 1u : 0u);
+        for (const GEdge& e : gb.E) g->cycle.push_back((e.cycle ? 1u : 0u) | (e.phase ? 2u : 0u));
         CodeGen cg(p, gb, partition ? std::string(partition) : std::string());
         g->code = cg.run(name);
     } catch (const std::exception& e) {

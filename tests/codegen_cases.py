@@ -63,6 +63,56 @@ Glazed = If(Exp(Exp(ShapeNormal(Z) > 0.5) || Exp(ShapeSize(X) >= 2.4))) {
 };
 """
 
+# context-sensitive grammars (Collider / IfCollides; their phases come from
+# the operator graph, Graph::computePhases): a yard whose props ask for the
+# building stored in phase 0; a city block whose buildings also store their
+# walls (Quads) under tag 2; three phases, lamps asking for free props (the
+# phase walk takes a Replicate's last branch first, so the lamps, which wait
+# for the yard's colliders, are listed before the yard)
+YARD = """
+axiom Box Lot;
+terminal Building (1, 0);
+terminal Free (1, 1);
+terminal Blocked (1, 2);
+Lot = Replicate() { Scale(-0.5, -1, -0.5) Collider(1) Building | Yard };
+Yard = Repeat(X, 1, ADJUST_TO_FILL) Repeat(Z, 1, ADJUST_TO_FILL) Scale(-0.5, -0.3, -0.5) IfCollides(1) { Blocked | Free };
+"""
+CITY = """
+axiom Box Lot;
+terminal Free (1, 1);
+terminal Blocked (1, 2);
+terminal WallQuad (1, 3);
+Lot = Replicate() { Building | Yard };
+Building = Scale(-0.45, -1, -0.45) Translate(Rand(-1.5, 1.5), 0, Rand(-1.5, 1.5)) Rotate(0, Rand(0, 90), 0)
+           Collider(1) CompSplit() { Discard() | Discard() | Collider(2) WallQuad };
+Yard = Repeat(X, 1, ADJUST_TO_FILL) Repeat(Z, 1, ADJUST_TO_FILL) Scale(-0.5, -0.3, -0.5) Prop;
+Prop = IfCollides(2) { Blocked | IfCollides(1) { Blocked | Free } };
+"""
+LAMPS = """
+axiom Box Lot;
+terminal Building (1, 0);
+terminal Free (1, 1);
+terminal Blocked (1, 2);
+terminal DarkLamp (1, 3);
+terminal Lamp (1, 4);
+Lot = Replicate() { Scale(-0.5, -1, -0.5) Collider(1) Building | Lamps | Yard };
+Yard = Repeat(X, 1, ADJUST_TO_FILL) Repeat(Z, 1, ADJUST_TO_FILL) Scale(-0.5, -0.3, -0.5)
+       IfCollides(1) { Blocked | Collider(2) Free };
+Lamps = Repeat(X, 2, ADJUST_TO_FILL) Repeat(Z, 2, ADJUST_TO_FILL) Scale(-0.6, -0.3, -0.6) IfCollides(2) { DarkLamp | Lamp };
+"""
+
+
+def context_cases():
+    """(case name, grammar text, partition spec) of the context-sensitive
+    grammars (compiled like cases(); compared with the oracle, not with an
+    interpreter table, which cannot hold Collider / IfCollides)."""
+    out = []
+    for name, text in (("yard", YARD), ("city", CITY), ("lamps", LAMPS)):
+        for spec in ("all", "cycles", "random:5"):
+            out.append((f"ctx_{name}_{spec.replace(':', '')}", text, spec))
+    return out
+
+
 # (case name, grammar text, axiom shape, partition spec, lots: "menger" | "facade" | "fuzz:<seed>")
 def cases():
     from tests.grammar_fuzz import random_grammar
@@ -97,6 +147,10 @@ def write_tu(path):
              "using namespace PGA::Parameters;", ""]
     names = []
     for name, text, _, spec, _ in cases():
+        uid = partition_uid(operator_graph_edges(text), spec)
+        parts.append(generate_partition(text, uid, name))
+        names.append(name)
+    for name, text, spec in context_cases():
         uid = partition_uid(operator_graph_edges(text), spec)
         parts.append(generate_partition(text, uid, name))
         names.append(name)
