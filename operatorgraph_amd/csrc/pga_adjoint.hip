@@ -454,6 +454,8 @@ __device__ __forceinline__ Ctx adj_ctx(const KArgs& a) {
     c.coll_visible = nullptr;
     c.coll_cap = 0u;
     c.bvh_flat = nullptr;
+    c.coll_bits = nullptr;
+    c.coll_bits_cap = 0u;
     return c;
 }
 

@@ -113,6 +113,8 @@ struct pga_context {
     int32_t* d_coll_tree = nullptr;   // left | right | parent | leaf parent, coll_bvh_cap each
     float* d_coll_nodes = nullptr;
     float4* d_coll_flat = nullptr;    // the tree in depth-first order, 2 coll_bvh_cap - 1 nodes
+    uint32_t* d_coll_bits = nullptr;  // per child of a level: IfCollides outcomes of the count pass
+    uint32_t coll_bits_n = 0;
 
     // derivation tape (reverse mode)
     pga::Rec* d_tape_par = nullptr;
@@ -673,6 +675,14 @@ pga_status ensure_colliders(pga_context* ctx) {
         ctx->coll_cap = want;
     }
     if (!ctx->d_coll_count) HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_count, 2u * sizeof(uint32_t)));
+    const uint32_t bits = queue_capacity(ctx);
+    if (!ctx->d_coll_bits || ctx->coll_bits_n != bits) {
+        if (ctx->stream) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        hfree(ctx->d_coll_bits);
+        ctx->coll_bits_n = 0;
+        HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_bits, (size_t)bits * sizeof(uint32_t)));
+        ctx->coll_bits_n = bits;
+    }
     if (ctx->nphases < 2 || want < 2u) return PGA_OK;
     if (ctx->coll_bvh && ctx->coll_bvh_cap == want) return PGA_OK;
     if (ctx->stream) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -746,6 +756,8 @@ pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
         a.coll_count = ctx->d_coll_count;
         a.coll_visible = ctx->d_coll_count + 1;
         a.coll_cap = ctx->coll_cap;
+        a.coll_bits = ctx->d_coll_bits;
+        a.coll_bits_cap = ctx->coll_bits_n;
     }
     if (ctx->spec_on && ctx->sched_local && ctx->d_loc && ctx->spec_blocks <= ctx->loc_blocks) {
         a.loc_par = reinterpret_cast<pga::Rec*>(ctx->d_loc);
@@ -911,7 +923,7 @@ PGA_API void pga_destroy(pga_context* ctx) {
     for (int b = 0; b < 2; ++b) { hfree(ctx->d_par[b]); hfree(ctx->d_desc[b]); }
     hfree(ctx->d_term_counts); hfree(ctx->d_par_counts); hfree(ctx->d_desc_counts);
     hfree(ctx->d_err); hfree(ctx->d_bar); hfree(ctx->d_block_totals); hfree(ctx->d_loc);
-    hfree(ctx->d_coll); hfree(ctx->d_coll_count);
+    hfree(ctx->d_coll); hfree(ctx->d_coll_count); hfree(ctx->d_coll_bits);
     drop_collider_bvh(ctx);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

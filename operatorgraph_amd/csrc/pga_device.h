@@ -138,6 +138,19 @@ struct Ctx {
     // built): per node two 16-byte words, box min xyz, max x | max y, max z,
     // the node after its subtree, the collider of a leaf (~0: internal)
     const float4* bvh_flat;
+    // per child of the level: the IfCollides outcomes of the count pass, bit
+    // k for the child's k-th IfCollides (k < 32), replayed by the write pass
+    uint32_t* coll_bits;
+    uint32_t coll_bits_cap;
+};
+
+// A lane's IfCollides outcomes for its current child: recorded by the count
+// pass, replayed by the write pass (both passes meet the same IfCollides in
+// the same order).
+struct CollMemo {
+    uint32_t bits;
+    uint32_t n;        // IfCollides met so far
+    uint32_t mode;     // 0: neither, 1: record, 2: replay
 };
 
 // Axis-aligned box around a Box or Quad shape (a Quad with its z extent,

@@ -60,6 +60,8 @@ struct KArgs {
     const uint32_t* coll_visible;
     uint32_t coll_cap;
     const float4* bvh_flat;             // BVH over the visible colliders, depth-first (NULL: none)
+    uint32_t* coll_bits;                // per child of a level: IfCollides outcomes of the count pass
+    uint32_t coll_bits_cap;
     // derivation tape (reverse mode): every level's parent records and
     // descriptors kept (cap_par / cap_desc = their capacities), every
     // fan-out materialised, and one outcome per child

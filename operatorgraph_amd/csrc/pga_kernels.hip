@@ -49,6 +49,11 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
         const uint32_t i = base + lane_id();
         Rec cur;
         const bool valid = i < n && load_child(a, c, par_in, desc_in, wr, level, i, cur);
+        CollMemo memo;                  // context-sensitive kernels: IfCollides outcomes, count -> write pass
+        memo.bits = 0u;
+        memo.n = 0u;
+        memo.mode = (CS && valid && i < c.coll_bits_cap) ? (wr ? 2u : 1u) : 0u;
+        if (CS && memo.mode == 2u) memo.bits = c.coll_bits[i];
         uint32_t out_kind = T_DROP, out_a = 0, out_b = 0, out_c = 0;   // tape outcome
         uint32_t mode = valid ? M_SETTLE : M_DONE;
         bool leaf = false, self = false;
@@ -151,7 +156,7 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
                     }
                 } else {
                     uint32_t ord;
-                    const int32_t next = apply_single(c, e, cur, &ord);
+                    const int32_t next = apply_single(c, e, cur, &ord, CS ? &memo : nullptr);
                     if (!advance(c, cur, next, ord)) mode = back;
                     ++steps;
                 }
@@ -167,6 +172,7 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
                 if (TAPE && start) { out_kind = self ? T_SELF : T_FAN; out_a = pidx; out_b = dslot; out_c = nd; }
             }
         }
+        if (CS && memo.mode == 1u && memo.n != 0u) c.coll_bits[i] = memo.bits;
         if (TAPE && wr && i < n) {
             if (child_base + i < a.tape_out_cap)
                 a.tape_out[child_base + i] = make_uint4(out_kind, out_a, out_b, out_c);

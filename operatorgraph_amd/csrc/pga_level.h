@@ -282,6 +282,8 @@ __device__ __forceinline__ Ctx make_ctx(const KArgs& a, const LevelLDS& L, uint3
     c.coll_visible = CS ? a.coll_visible : nullptr;
     c.coll_cap = CS ? a.coll_cap : 0u;
     c.bvh_flat = CS ? a.bvh_flat : nullptr;
+    c.coll_bits = CS ? a.coll_bits : nullptr;
+    c.coll_bits_cap = CS ? a.coll_bits_cap : 0u;
     return c;
 }
 

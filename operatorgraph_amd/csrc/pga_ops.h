@@ -295,7 +295,8 @@ __device__ __forceinline__ void store_collider(const Ctx& c, const Rec& r, uint3
 // Applies one single-child operator in place. Returns the successor entry
 // (or -1 when nothing is dispatched) and the child ordinal.
 template <class E>
-__device__ __forceinline__ int32_t apply_single(const Ctx& c, const E& e, Rec& r, uint32_t* ordinal) {
+__device__ __forceinline__ int32_t apply_single(const Ctx& c, const E& e, Rec& r, uint32_t* ordinal,
+                                                CollMemo* memo = nullptr) {
     const uint32_t k = e.kind();
     *ordinal = 0;
     if (!c.geom && (k == K_TRANSLATE || k == K_ROTATE)) return e.sv0;   // model only
@@ -359,11 +360,19 @@ __device__ __forceinline__ int32_t apply_single(const Ctx& c, const E& e, Rec& r
         return e.sv0;
     }
     if (c.ctxs && k == K_IFCOLLIDES) {         // IfCollides.cuh:26-58
-        const uint32_t n = min(*c.coll_visible, c.coll_cap);
-        const uint32_t t = (n != 0u && collider_hit(c, n, f2u_rd(v0), rec_phase(c, r), e.shape(), r.m[0],
-                                                    r.m[1], r.m[2], r.m[3], r.m[4], r.m[5], r.m[6], r.m[7], r.m[8],
-                                                    r.m[9], r.m[10], r.m[11], r.s[0], r.s[1], r.s[2]))
-                             ? 0u : 1u;
+        const uint32_t q = memo ? memo->n : 32u;
+        uint32_t t;
+        if (memo && memo->mode == 2u && q < 32u) {
+            t = (memo->bits >> q) & 1u;         // the count pass's outcome
+        } else {
+            const uint32_t n = min(*c.coll_visible, c.coll_cap);
+            t = (n != 0u && collider_hit(c, n, f2u_rd(v0), rec_phase(c, r), e.shape(), r.m[0], r.m[1], r.m[2],
+                                         r.m[3], r.m[4], r.m[5], r.m[6], r.m[7], r.m[8], r.m[9], r.m[10], r.m[11],
+                                         r.s[0], r.s[1], r.s[2]))
+                    ? 0u : 1u;
+            if (memo && memo->mode == 1u && q < 32u) memo->bits |= t << q;
+        }
+        if (memo) memo->n = q + 1u;
         *ordinal = t;
         return succ_of(c, e, t);
     }
