@@ -535,15 +535,31 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
     }
     // multi-phase programs: each successor carries its phase in bits 16-19
     // (bit 20: the parent's), and no fan-out counts its terminals from a
-    // table (a child's phase decides whether a launch emits it)
+    // table (a child's phase decides whether a launch emits it). aux[1]
+    // holds instead the latest phase a successor reachable from the entry
+    // names: a shape of an earlier phase whose entry reaches no later one
+    // adds nothing to a launch, which skips it.
     if (nphases > 1) {
+        std::vector<uint32_t> reach(n, 0u);
+        for (bool changed = true; changed;) {   // a fixed point: at most 16 rises per entry
+            changed = false;
+            for (uint32_t i = 0; i < n; ++i) {
+                uint32_t r = reach[i];
+                for (uint32_t k = 0; k < E[i].nsucc; ++k) {
+                    const int32_t t = Sc[E[i].sbase + k];
+                    if (t < 0) continue;
+                    r = std::max(r, std::max(Sp[E[i].sbase + k] & 15u, reach[t]));
+                }
+                if (r != reach[i]) { reach[i] = r; changed = true; }
+            }
+        }
         for (size_t k = 0; k < Sc.size(); ++k)
             if (Sc[k] >= 0) Sc[k] |= (int32_t)Sp[k] << 16;
         for (uint32_t i = 0; i < n; ++i) {
             DevEntry& d = E[i];
             d.flags &= (uint8_t)~pga::EF_DIRECT;
             d.ndirect = 0;
-            d.aux[1] = 0u;
+            d.aux[1] = reach[i];
             for (uint32_t k = 0; k < 3; ++k) d.sv[k] = k < d.nsucc ? Sc[d.sbase + k] : -1;
         }
     }
@@ -694,12 +710,18 @@ pga_status ensure_colliders(pga_context* ctx) {
         if (b) pga_lbvh_destroy(b);
         return fail(ctx, st, "collider BVH: " + why);
     }
+    // the builder is kept only with all its arrays: a failed allocation
+    // leaves no BVH, and IfCollides then scans
+    hipError_t e = hipMalloc((void**)&ctx->d_coll_boxes, (size_t)want * 6u * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&ctx->d_coll_keys, (size_t)want * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&ctx->d_coll_tree, (size_t)want * 4u * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&ctx->d_coll_nodes, (size_t)want * 6u * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&ctx->d_coll_flat, (size_t)want * 2u * 2u * sizeof(float4));
     ctx->coll_bvh = b;
-    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_boxes, (size_t)want * 6u * sizeof(float)));
-    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_keys, (size_t)want * sizeof(uint64_t)));
-    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_tree, (size_t)want * 4u * sizeof(int32_t)));
-    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_nodes, (size_t)want * 6u * sizeof(float)));
-    HIP_TRY(ctx, hipMalloc((void**)&ctx->d_coll_flat, (size_t)want * 2u * 2u * sizeof(float4)));
+    if (e != hipSuccess) {
+        drop_collider_bvh(ctx);
+        return hip_fail(ctx, e, "collider BVH arrays");
+    }
     ctx->coll_bvh_cap = want;
     return PGA_OK;
 }
@@ -1102,7 +1124,7 @@ static pga_status one_phase_only(pga_context* ctx, const char* what) {
 static pga_status begin_phase(pga_context* ctx, pga::KArgs& a, uint32_t p) {
     a.phase = p;
     if (p == 0) return PGA_OK;
-    if (ctx->coll_bvh) {
+    if (ctx->collision && ctx->coll_bvh) {
         HIP_TRY(ctx, pga_launch_collider_boxes(ctx->d_coll, ctx->coll_cap, ctx->d_coll_count, ctx->d_coll_boxes,
                                                ctx->stream));
         const uint32_t c = ctx->coll_bvh_cap;

@@ -48,7 +48,11 @@ __device__ __forceinline__ void run_pass(const KArgs& a, const Ctx& c, LevelLDS&
     for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u) {
         const uint32_t i = base + lane_id();
         Rec cur;
-        const bool valid = i < n && load_child(a, c, par_in, desc_in, wr, level, i, cur);
+        bool valid = i < n && load_child(a, c, par_in, desc_in, wr, level, i, cur);
+        // an axiom whose entry reaches no phase of this launch adds nothing
+        if (CS && valid && level == 0u && multi_phase(c) && c.phase != 0u &&
+            c.prog.entries[rec_entry(cur)].aux[1] < c.phase)
+            valid = false;
         CollMemo memo;                  // context-sensitive kernels: IfCollides outcomes, count -> write pass
         memo.bits = 0u;
         memo.n = 0u;

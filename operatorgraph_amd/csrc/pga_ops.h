@@ -86,6 +86,9 @@ __device__ __forceinline__ bool advance(const Ctx& c, Rec& r, int32_t next, uint
         const uint32_t pph = r.ed >> 28;
         if (cph & 16u) cph = pph;
         if (cph < pph || cph > c.phase) return false;
+        // a shape of an earlier phase whose entry reaches no phase of this
+        // launch (the entry's aux[1]) adds nothing here
+        if (cph < c.phase && c.prog.entries[(uint32_t)next & 0xFFFFu].aux[1] < c.phase) return false;
     }
     uint32_t depth = rec_depth(c, r) + 1u;
     if (depth > c.dmask) { raise(c, PGA_ERR_MAX_DEPTH); return false; }
