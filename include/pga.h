@@ -37,6 +37,8 @@ extern "C" {
 #define PGA_MAX_NUM_PARAMETERS 17
 #define PGA_MAX_NUM_PARAMETER_VALUES 16
 #define PGA_MAX_NUM_SUCCESSORS 20
+/* vertices of a dynamic polygon or right prism (Constants::MaxNumSides) */
+#define PGA_MAX_NUM_SIDES 16
 #define PGA_MAX_NUM_AXIOM_SEEDS 32
 #define PGA_MAX_EXP_STACK 4
 /* instanced meshes per scene (include/pga/rendering/RenderingConstants.h) */
@@ -126,7 +128,11 @@ typedef enum pga_operator_type {
 typedef enum pga_shape_type {
     PGA_SHAPE_TRIANGLE = 0,
     PGA_SHAPE_QUAD = 1,
-    PGA_SHAPE_BOX = 9
+    PGA_SHAPE_DYNAMIC_CONVEX_POLYGON = 6,
+    PGA_SHAPE_DYNAMIC_POLYGON = 7,
+    PGA_SHAPE_BOX = 9,
+    PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM = 14,
+    PGA_SHAPE_DYNAMIC_RIGHT_PRISM = 15
 } pga_shape_type;
 
 /* ---- PGA::RepeatMode (include/pga/core/RepeatMode.h) ---- */

@@ -30,8 +30,10 @@
  *      (PGA_PHASE_INHERIT), so `MultiPhaseEvaluator<...>` (with IfCollides
  *      and Collider, GPUEvaluator.cuh:395-456) runs the same tables.
  *
- * Shapes other than Box and Quad, the SetAs* operators and generation
- * function 0 (triangle soup) lower but are refused by pga_initialize
+ * Dynamic polygons and right prisms derive (SetAs*, a prism's
+ * ComponentSplit, a polygon's Extrude), except through generation function
+ * 0 and the collision operators; the static shapes (Triangle .. Octagon,
+ * Prism3 .. Prism8, Sphere) lower but are refused by pga_initialize
  * (PGA_ERR_UNSUPPORTED), as the engine does not derive them.
  */
 #pragma once
@@ -548,7 +550,11 @@ template <class X, class Y, class Z, class N> struct Lower<Operators::SwapSize<X
 template <class A, class E, class N> struct Lower<Operators::Extrude<A, E, N>> {
     static int emit(Template& t, int shape) {
         const int i = add_node(t, PGA_OP_EXTRUDE, shape, {Param<A>::get(), Param<E>::get()});
-        const Link l = link<N>(t, shape == Compiler::QUAD ? (int)Compiler::BOX : -1);
+        // Operator::nextShapeType: a planar shape's extruded type
+        const int to = shape == Compiler::QUAD ? (int)Compiler::BOX
+                     : shape == Compiler::DYNAMIC_CONVEX_POLYGON ? (int)Compiler::DYNAMIC_CONVEX_RIGHT_PRISM
+                     : shape == Compiler::DYNAMIC_POLYGON ? (int)Compiler::DYNAMIC_RIGHT_PRISM : -1;
+        const Link l = link<N>(t, to);
         t.nodes[i].succ.push_back(l);
         return i;
     }
@@ -569,9 +575,15 @@ template <bool P, class A, class E, class M, class N> struct Lower<Operators::Re
 // frames and cap seeds differ
 template <bool P, class Top, class Bottom, class Sides> struct Lower<Operators::ComponentSplit<P, Top, Bottom, Sides>> {
     static int emit(Template& t, int shape) {
-        const int face = shape == Compiler::BOX ? (int)Compiler::QUAD : -1;
+        // a Box's faces are Quads; a dynamic prism's caps are its polygon,
+        // its sides Quads (GetCapFaceType, GetSideFaceType)
+        const int cap = shape == Compiler::BOX ? (int)Compiler::QUAD
+                      : shape == Compiler::DYNAMIC_CONVEX_RIGHT_PRISM ? (int)Compiler::DYNAMIC_CONVEX_POLYGON
+                      : shape == Compiler::DYNAMIC_RIGHT_PRISM ? (int)Compiler::DYNAMIC_POLYGON : -1;
+        const int side = (shape == Compiler::BOX || shape == Compiler::DYNAMIC_CONVEX_RIGHT_PRISM ||
+                          shape == Compiler::DYNAMIC_RIGHT_PRISM) ? (int)Compiler::QUAD : -1;
         const int i = add_node(t, PGA_OP_COMPSPLIT, shape, {}, 0, P ? PGA_PROC_PARALLEL : 0u);
-        const Link a = link<Top>(t, face), b = link<Bottom>(t, face), c = link<Sides>(t, face);
+        const Link a = link<Top>(t, cap), b = link<Bottom>(t, cap), c = link<Sides>(t, side);
         t.nodes[i].succ = {a, b, c};
         return i;
     }
@@ -665,25 +677,36 @@ struct Lower<Operators::Generate<P, G, Term, A1, A2, A3, A4, A5>> {
     }
 };
 
-// SetAs* (SetAs.cuh:24-170): the shape becomes a dynamic polygon or prism,
-// which the engine does not derive; the node keeps the operator so that
-// pga_initialize reports it.
-template <int OpT, class N> struct LowerSetAs {
+// SetAs* (SetAs.cuh:24-170): the shape becomes a dynamic polygon or prism
+// whose vertices are the operator's Vec2 (or scalar) parameters; the
+// <DynParams> form takes them, and its successor, from the dispatching entry.
+template <int OpT> constexpr int set_as_shape() {
+    return OpT == PGA_OP_SET_AS_DYNAMIC_CONVEX_POLYGON ? (int)Compiler::DYNAMIC_CONVEX_POLYGON
+         : OpT == PGA_OP_SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM ? (int)Compiler::DYNAMIC_CONVEX_RIGHT_PRISM
+         : OpT == PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON ? (int)Compiler::DYNAMIC_POLYGON
+                                                        : (int)Compiler::DYNAMIC_RIGHT_PRISM;
+}
+template <int OpT, class N, class... V> struct LowerSetAs {
     static int emit(Template& t, int shape) {
-        const int i = add_node(t, OpT, shape, {});
-        const Link l = link<N>(t, -1);
-        t.nodes[i].succ.push_back(l);
-        return i;
+        if constexpr (std::is_same<N, Parameters::DynParams>::value) {
+            static_assert(sizeof...(V) == 0, "PGA::Parameters::DynParams can only be used as a single parameter");
+            return add_whole_entry(t, OpT, shape);
+        } else {
+            const int i = add_node(t, OpT, shape, {Param<V>::get()...});
+            const Link l = link<N>(t, set_as_shape<OpT>());
+            t.nodes[i].succ.push_back(l);
+            return i;
+        }
     }
 };
 template <class N, class... V> struct Lower<Operators::SetAsDynamicConvexPolygon<N, V...>>
-    : LowerSetAs<PGA_OP_SET_AS_DYNAMIC_CONVEX_POLYGON, N> {};
+    : LowerSetAs<PGA_OP_SET_AS_DYNAMIC_CONVEX_POLYGON, N, V...> {};
 template <class N, class... V> struct Lower<Operators::SetAsDynamicConvexRightPrism<N, V...>>
-    : LowerSetAs<PGA_OP_SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM, N> {};
+    : LowerSetAs<PGA_OP_SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM, N, V...> {};
 template <class N, class... V> struct Lower<Operators::SetAsDynamicConcavePolygon<N, V...>>
-    : LowerSetAs<PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON, N> {};
+    : LowerSetAs<PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON, N, V...> {};
 template <class N, class... V> struct Lower<Operators::SetAsDynamicConcaveRightPrism<N, V...>>
-    : LowerSetAs<PGA_OP_SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM, N> {};
+    : LowerSetAs<PGA_OP_SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM, N, V...> {};
 
 // a procedure of the list (a Proc or a struct deriving from one)
 template <class S, class Op, unsigned N> Template lower_proc(const Proc<S, Op, N>*) {

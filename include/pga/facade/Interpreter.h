@@ -62,8 +62,8 @@ struct ProcList
           Proc<Quad, Repeat<false, DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>, Proc<Quad, Replicate<DynParams>>,
           Proc<Quad, Rotate<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
           Proc<Quad, Scale<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
-          Proc<Quad, SetAsDynamicConvexPolygon<DCall<0>>>, Proc<Quad, SetAsDynamicConvexRightPrism<DCall<0>>>,
-          Proc<Quad, SetAsDynamicConcavePolygon<DCall<0>>>, Proc<Quad, SetAsDynamicConcaveRightPrism<DCall<0>>>,
+          Proc<Quad, SetAsDynamicConvexPolygon<DynParams>>, Proc<Quad, SetAsDynamicConvexRightPrism<DynParams>>,
+          Proc<Quad, SetAsDynamicConcavePolygon<DynParams>>, Proc<Quad, SetAsDynamicConcaveRightPrism<DynParams>>,
           Proc<Quad, Subdivide<DynParams>>, Proc<Quad, SwapSize<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
           Proc<Quad, Translate<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
           Proc<Box, ComponentSplit<false, DCall<0>, DCall<1>, DCall<2>>>, Proc<Box, Discard>,
@@ -73,8 +73,8 @@ struct ProcList
           Proc<Box, Repeat<false, DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>, Proc<Box, Replicate<DynParams>>,
           Proc<Box, Rotate<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
           Proc<Box, Scale<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
-          Proc<Box, SetAsDynamicConvexPolygon<DCall<0>>>, Proc<Box, SetAsDynamicConvexRightPrism<DCall<0>>>,
-          Proc<Box, SetAsDynamicConcavePolygon<DCall<0>>>, Proc<Box, SetAsDynamicConcaveRightPrism<DCall<0>>>,
+          Proc<Box, SetAsDynamicConvexPolygon<DynParams>>, Proc<Box, SetAsDynamicConvexRightPrism<DynParams>>,
+          Proc<Box, SetAsDynamicConcavePolygon<DynParams>>, Proc<Box, SetAsDynamicConcaveRightPrism<DynParams>>,
           Proc<Box, Subdivide<DynParams>>, Proc<Box, SwapSize<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
           Proc<Box, Translate<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>> {};
 

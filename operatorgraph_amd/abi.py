@@ -54,6 +54,9 @@ TRANSLATE, ROTATE, SCALE, EXTRUDE, COMPSPLIT, SUBDIV, REPEAT, DISCARD, IF, \
 
 # PGA::Compiler::ShapeType (include/pga/compiler/ShapeType.h)
 TRIANGLE, QUAD, BOX = 0, 1, 9
+DYNAMIC_CONVEX_POLYGON, DYNAMIC_POLYGON = 6, 7
+DYNAMIC_CONVEX_RIGHT_PRISM, DYNAMIC_RIGHT_PRISM = 14, 15
+MAX_NUM_SIDES = 16
 
 # PGA::RepeatMode
 ANCHOR_TO_START, ANCHOR_TO_END, ADJUST_TO_FILL = 0, 1, 2

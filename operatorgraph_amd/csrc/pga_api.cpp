@@ -55,6 +55,7 @@ struct pga_context {
     uint32_t use_seed = 0;            // 1: some parameter reads a shape seed
     uint32_t nphases = 1;             // 1 + the largest phase a successor names
     bool collision = false;           // the program holds Collider or IfCollides entries
+    bool polygons = false;            // the program holds dynamic polygon or prism entries
     DevEntry* d_entries = nullptr;    // views into d_image
     DevParam* d_params = nullptr;
     int32_t* d_succ = nullptr;
@@ -212,14 +213,23 @@ template <class T> void hfree(T*& p) {
 bool is_single(uint8_t k) {
     return k == pga::K_IFSIZELESS || k == pga::K_IF || k == pga::K_RANDOM || k == pga::K_TRANSLATE ||
            k == pga::K_ROTATE || k == pga::K_SCALE || k == pga::K_SWAPSIZE || k == pga::K_EXTRUDE ||
-           k == pga::K_COLLIDER || k == pga::K_IFCOLLIDES;
+           k == pga::K_COLLIDER || k == pga::K_IFCOLLIDES || k == pga::K_SETAS;
+}
+// shapes with a vertex list: dynamic polygons and right prisms
+bool has_vertices(int32_t st) {
+    return st == PGA_SHAPE_DYNAMIC_CONVEX_POLYGON || st == PGA_SHAPE_DYNAMIC_POLYGON ||
+           st == PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM || st == PGA_SHAPE_DYNAMIC_RIGHT_PRISM;
+}
+bool is_prism(int32_t st) {
+    return st == PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM || st == PGA_SHAPE_DYNAMIC_RIGHT_PRISM;
 }
 // single-child operators that choose among their successors
 bool is_branch(uint8_t k) {
     return k == pga::K_IFSIZELESS || k == pga::K_IF || k == pga::K_RANDOM || k == pga::K_IFCOLLIDES;
 }
 bool is_fan(uint8_t k) {
-    return k == pga::K_SUBDIV || k == pga::K_REPEAT || k == pga::K_COMPSPLIT || k == pga::K_REPLICATE;
+    return k == pga::K_SUBDIV || k == pga::K_REPEAT || k == pga::K_COMPSPLIT || k == pga::K_REPLICATE ||
+           k == pga::K_PRISMSPLIT;
 }
 
 // Checks an expression parameter (Parameters.cuh:563-664): length, operand
@@ -278,7 +288,8 @@ pga_status ensure_colliders(pga_context* ctx);
 pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t nprocs,
                          const pga_dispatch_entry* entries, uint32_t n, bool upload = true) {
     if (!procs || !entries || nprocs == 0 || n == 0) return fail(ctx, PGA_ERR_INVALID_ARG, "empty program");
-    if (n > 0xFFFFu) return fail(ctx, PGA_ERR_BAD_PROGRAM, "more than 65535 dispatch entries");
+    // the entry index shares a word of the shape record with a flag (bit 15)
+    if (n > 0x7FFFu) return fail(ctx, PGA_ERR_BAD_PROGRAM, "more than 32767 dispatch entries");
     std::vector<DevEntry> E(n);
     std::vector<DevParam> Pm;
     std::vector<int32_t> Sc;
@@ -287,6 +298,7 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
     bool count_geom = false;
     bool use_seed = false;
     bool collision = false;
+    bool polygons = false;
     uint32_t nphases = 1;
     std::vector<uint8_t> Sp;   // phase of each successor in Sc
     for (uint32_t i = 0; i < n; ++i) {
@@ -305,11 +317,28 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         case PGA_OP_SCALE: d.kind = pga::K_SCALE; need_p = 3; need_s = 1; break;
         case PGA_OP_SWAPSIZE: d.kind = pga::K_SWAPSIZE; need_p = 3; need_s = 1; break;
         case PGA_OP_EXTRUDE:
-            if (p.shape_type != PGA_SHAPE_QUAD) return fail(ctx, PGA_ERR_UNSUPPORTED, at + "Extrude needs a Quad procedure");
+            if (p.shape_type != PGA_SHAPE_QUAD && p.shape_type != PGA_SHAPE_DYNAMIC_CONVEX_POLYGON &&
+                p.shape_type != PGA_SHAPE_DYNAMIC_POLYGON)
+                return fail(ctx, PGA_ERR_UNSUPPORTED, at + "Extrude needs a planar shape (Quad or dynamic polygon)");
             d.kind = pga::K_EXTRUDE; need_p = 2; need_s = 1; break;
         case PGA_OP_COMPSPLIT:
-            if (p.shape_type != PGA_SHAPE_BOX) return fail(ctx, PGA_ERR_UNSUPPORTED, at + "ComponentSplit needs a Box procedure");
+            if (is_prism(p.shape_type)) {
+                if (p.flags & PGA_PROC_PARALLEL)
+                    return fail(ctx, PGA_ERR_UNSUPPORTED, at + "the parallel ComponentSplit splits Boxes only");
+                d.kind = pga::K_PRISMSPLIT; need_p = 0; need_s = 3; break;
+            }
+            if (p.shape_type != PGA_SHAPE_BOX)
+                return fail(ctx, PGA_ERR_UNSUPPORTED, at + "ComponentSplit needs a Box or a dynamic right prism");
             d.kind = pga::K_COMPSPLIT; need_p = 0; need_s = 3; break;
+        case PGA_OP_SET_AS_DYNAMIC_CONVEX_POLYGON:
+        case PGA_OP_SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM:
+        case PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON:
+        case PGA_OP_SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM:
+            // SetAs<DynParams> (SetAs.cuh:112-150): one vertex per parameter;
+            // above PGA_MAX_NUM_SIDES the reference writes past its vertex array
+            if (e.num_parameters > PGA_MAX_NUM_SIDES)
+                return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "SetAs takes at most 16 vertices");
+            d.kind = pga::K_SETAS; need_p = 3; need_s = 1; break;
         case PGA_OP_SUBDIV:
             d.kind = pga::K_SUBDIV; need_p = 1;
             need_s = e.num_parameters > 0 ? e.num_parameters - 1 : 0;
@@ -326,24 +355,34 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
             need_s = e.num_parameters > 0 ? e.num_parameters - 1 : 0;
             break;
         case PGA_OP_REPLICATE: d.kind = pga::K_REPLICATE; break;
-        case PGA_OP_COLLIDER: d.kind = pga::K_COLLIDER; need_p = 1; need_s = 1; collision = true; break;
-        case PGA_OP_IFCOLLIDES:   // decides on the model matrix in the count pass too
-            d.kind = pga::K_IFCOLLIDES; need_p = 1; need_s = 2; collision = true; count_geom = true; break;
+        case PGA_OP_COLLIDER:
+        case PGA_OP_IFCOLLIDES:
+            if (has_vertices(p.shape_type))
+                return fail(ctx, PGA_ERR_UNSUPPORTED, at + "collision tests of Box and Quad shapes only");
+            if (p.op_type == PGA_OP_COLLIDER) {
+                d.kind = pga::K_COLLIDER; need_p = 1; need_s = 1; collision = true;
+            } else {   // decides on the model matrix in the count pass too
+                d.kind = pga::K_IFCOLLIDES; need_p = 1; need_s = 2; collision = true; count_geom = true;
+            }
+            break;
         case PGA_OP_GENERATE:
             // generation function 0 (triangle soup) is derived into instance
             // records as well; pga_tessellate turns them into its geometry
             if (p.gen_func_idx != 0 && p.gen_func_idx != 1)
                 return fail(ctx, PGA_ERR_UNSUPPORTED, at + "only generation functions 0 and 1 are supported");
+            if (p.gen_func_idx == 0 && has_vertices(p.shape_type))
+                return fail(ctx, PGA_ERR_UNSUPPORTED, at + "generation function 0 of Box and Quad shapes only");
             d.kind = pga::K_GENERATE; need_p = 1; break;
         default:
             return fail(ctx, PGA_ERR_UNSUPPORTED,
                         at + "unsupported operator type " + std::to_string(p.op_type));
         }
-        if (p.shape_type != PGA_SHAPE_BOX && p.shape_type != PGA_SHAPE_QUAD)
-            return fail(ctx, PGA_ERR_UNSUPPORTED, at + "only Box and Quad shapes are supported");
+        if (p.shape_type != PGA_SHAPE_BOX && p.shape_type != PGA_SHAPE_QUAD && !has_vertices(p.shape_type))
+            return fail(ctx, PGA_ERR_UNSUPPORTED, at + "only Box, Quad and dynamic polygon and prism shapes are supported");
         if (e.num_parameters < need_p) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "too few parameters");
         if (e.num_successors < need_s) return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "too few successors");
         d.shape = (uint8_t)p.shape_type;
+        polygons |= has_vertices(p.shape_type);
         d.flags = (p.flags & PGA_PROC_STATIC) ? pga::EF_STATIC : 0;
         if (d.kind == pga::K_COMPSPLIT && (p.flags & PGA_PROC_PARALLEL)) d.flags |= pga::EF_PARALLEL;
         d.nparams = (uint8_t)e.num_parameters;
@@ -353,6 +392,19 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         for (uint32_t k = 0; k < e.num_parameters; ++k) {
             const pga_dispatch_parameter& q = e.parameters[k];
             DevParam dp{};
+            if (d.kind == pga::K_SETAS) {
+                // a vertex, dynToFloat2 (Parameters.cuh:714-742): a scalar is
+                // (value, 0); Vec2, Vec4 and Rand give their first two values
+                dp.type = PGA_PT_VEC2;
+                if (q.type == PGA_PT_SCALAR) { dp.a = q.value[0]; dp.b = 0.0f; }
+                else if (q.type == PGA_PT_VEC2 || q.type == PGA_PT_VEC4 || q.type == PGA_PT_RAND) {
+                    dp.a = q.value[0]; dp.b = q.value[1];
+                } else {
+                    return fail(ctx, PGA_ERR_BAD_PROGRAM, at + "SetAs vertex parameters are scalars, vectors or Rand pairs");
+                }
+                Pm.push_back(dp);
+                continue;
+            }
             dp.type = (uint32_t)q.type;
             switch (q.type) {
             case PGA_PT_SCALAR: dp.a = q.value[0]; break;
@@ -404,10 +456,26 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
     // successor shape consistency: the queue a child lands in holds its shape
     for (uint32_t i = 0; i < n; ++i) {
         const DevEntry& d = E[i];
-        uint8_t child_shape = d.kind == pga::K_COMPSPLIT ? (uint8_t)PGA_SHAPE_QUAD
-                            : d.kind == pga::K_EXTRUDE ? (uint8_t)PGA_SHAPE_BOX : d.shape;
+        uint8_t child_shape = d.shape;
+        if (d.kind == pga::K_COMPSPLIT) child_shape = PGA_SHAPE_QUAD;
+        if (d.kind == pga::K_EXTRUDE)   // Operator::nextShapeType: the extruded type
+            child_shape = d.shape == PGA_SHAPE_QUAD ? PGA_SHAPE_BOX
+                        : d.shape == PGA_SHAPE_DYNAMIC_CONVEX_POLYGON ? PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM
+                                                                      : PGA_SHAPE_DYNAMIC_RIGHT_PRISM;
+        if (d.kind == pga::K_SETAS) {
+            const int32_t op = procs[entries[i].operator_code].op_type;
+            child_shape = op == PGA_OP_SET_AS_DYNAMIC_CONVEX_POLYGON ? PGA_SHAPE_DYNAMIC_CONVEX_POLYGON
+                        : op == PGA_OP_SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM ? PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM
+                        : op == PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON ? PGA_SHAPE_DYNAMIC_POLYGON
+                                                                       : PGA_SHAPE_DYNAMIC_RIGHT_PRISM;
+        }
         for (uint32_t k = 0; k < d.nsucc; ++k) {
             int32_t s = Sc[d.sbase + k];
+            // a prism's caps (successors 0 and 1) are its polygons, its sides Quads
+            if (d.kind == pga::K_PRISMSPLIT)
+                child_shape = k == 2 ? (uint8_t)PGA_SHAPE_QUAD
+                            : d.shape == PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM ? (uint8_t)PGA_SHAPE_DYNAMIC_CONVEX_POLYGON
+                                                                               : (uint8_t)PGA_SHAPE_DYNAMIC_POLYGON;
             if (s >= 0 && E[s].shape != child_shape)
                 return fail(ctx, PGA_ERR_BAD_PROGRAM,
                             "entry " + std::to_string(i) + ": successor " + std::to_string(s) + " expects another shape type");
@@ -435,7 +503,7 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         return ok;
     };
     for (uint32_t i = 0; i < n; ++i) {
-        if (!is_fan(E[i].kind)) continue;
+        if (!is_fan(E[i].kind) || E[i].kind == pga::K_PRISMSPLIT) continue;
         bool leaf = true;
         for (uint32_t k = 0; k < E[i].nsucc && leaf; ++k) leaf = ends(Sc[E[i].sbase + k], 1);
         if (leaf) E[i].flags |= pga::EF_LEAF;
@@ -484,6 +552,50 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
             d.tslot = slots[t];
         }
     }
+    // dynamic polygons and prisms: the vertex list an entry's shapes carry is
+    // a fact of the table. SetAs gives its successor the list of its
+    // parameters; every other operator passes its own on to successors whose
+    // shapes have one (Extrude of a polygon, a prism's caps), and a list
+    // never changes on the way (operators move and size the shape only). An
+    // entry reached with two different lists is refused. A ComponentSplit of
+    // a prism keeps its list's first DevParam (bits) and vertex count in
+    // pv[0] and pv[1].
+    {
+        std::vector<int64_t> poly(n, -1);   // first DevParam | count << 32
+        auto same = [&](int64_t a, int64_t b) {
+            const uint32_t na = (uint32_t)(a >> 32), nb = (uint32_t)(b >> 32);
+            if (na != nb) return false;
+            for (uint32_t v = 0; v < na; ++v) {
+                const DevParam& x = Pm[(uint32_t)a + v];
+                const DevParam& y = Pm[(uint32_t)b + v];
+                if (std::memcmp(&x.a, &y.a, 8) != 0) return false;   // bitwise: -0 is not 0
+            }
+            return true;
+        };
+        std::vector<uint32_t> work;
+        std::string clash;
+        auto assign = [&](int32_t s, int64_t list) {
+            if (s < 0 || !has_vertices(E[s].shape)) return;
+            if (poly[s] < 0) { poly[s] = list; work.push_back((uint32_t)s); }
+            else if (!same(poly[s], list) && clash.empty())
+                clash = "entry " + std::to_string(s) + " is reached by two different vertex lists";
+        };
+        for (uint32_t i = 0; i < n; ++i)
+            if (E[i].kind == pga::K_SETAS) assign(Sc[E[i].sbase], (int64_t)E[i].pbase | ((int64_t)E[i].nparams << 32));
+        while (!work.empty()) {
+            const uint32_t i = work.back();
+            work.pop_back();
+            if (E[i].kind == pga::K_SETAS) continue;   // its successor takes the SetAs list
+            for (uint32_t k = 0; k < E[i].nsucc; ++k) assign(Sc[E[i].sbase + k], poly[i]);
+        }
+        if (!clash.empty()) return fail(ctx, PGA_ERR_UNSUPPORTED, clash);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (E[i].kind != pga::K_PRISMSPLIT) continue;
+            const uint32_t first = poly[i] < 0 ? 0u : (uint32_t)poly[i];
+            std::memcpy(&E[i].pv[0], &first, 4);
+            E[i].pv[1] = poly[i] < 0 ? 0.0f : (float)(uint32_t)(poly[i] >> 32);
+        }
+    }
     // direct leaf fan-outs: every child is a constant-terminal Generate or a
     // Discard, so a counting pass takes its terminal counts from a table
     for (uint32_t i = 0; i < n; ++i) {
@@ -499,6 +611,9 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
             succ_j.push_back(Sc[d.sbase]);
         } else if (d.kind == pga::K_COMPSPLIT) {
             for (uint32_t j = 0; j < 6; ++j) succ_j.push_back(Sc[d.sbase + (j < 4 ? 2u : (j == 4 ? 1u : 0u))]);
+        } else if (d.kind == pga::K_PRISMSPLIT) {
+            const uint32_t ns = (uint32_t)d.pv[1];
+            for (uint32_t j = 0; j < ns + 2; ++j) succ_j.push_back(Sc[d.sbase + (j < ns ? 2u : (j == ns ? 1u : 0u))]);
         } else {
             for (uint32_t k = 0; k < d.nsucc; ++k) succ_j.push_back(Sc[d.sbase + k]);
         }
@@ -589,6 +704,7 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
         ctx->max_param_slot = max_slot;
         ctx->nphases = nphases;
         ctx->collision = collision;
+        ctx->polygons = polygons;
         return PGA_OK;
     }
     ctx->collision = collision;
@@ -629,6 +745,7 @@ pga_status build_program(pga_context* ctx, const pga_procedure* procs, uint32_t 
     ctx->max_param_slot = max_slot;
     ctx->nphases = nphases;
     ctx->collision = collision;
+    ctx->polygons = polygons;
     ctx->initialized = true;
     return PGA_OK;
 }
@@ -1109,6 +1226,13 @@ static pga_status one_phase_only(pga_context* ctx, const char* what) {
                                               "IfCollides or more than one phase");
     return PGA_OK;
 }
+// Dynamic polygons and prisms run on the interpreting kernels (eagerly or
+// captured): the specialised kernel and the tape do not take them.
+static pga_status no_polygons(pga_context* ctx, const char* what) {
+    if (ctx->polygons)
+        return fail(ctx, PGA_ERR_UNSUPPORTED, std::string(what) + " does not run programs with dynamic polygons");
+    return PGA_OK;
+}
 
 // The derivation of a multi-phase program is one persistent launch per phase
 // (MultiPhaseEvaluator::execute, GPUEvaluator.cuh:416-440): every launch
@@ -1294,6 +1418,7 @@ PGA_API pga_status pga_specialize(pga_context* ctx, uint32_t block_size, int32_t
     }
     if (!ctx->initialized) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no program loaded");
     if (pga_status st = one_phase_only(ctx, "the specialised kernel")) return st;
+    if (pga_status st = no_polygons(ctx, "the specialised kernel")) return st;
     if (block_size == 0) block_size = kDefaultBlockSize;
     if (!spec_block_size_supported(block_size))
         return fail(ctx, PGA_ERR_INVALID_ARG, "unsupported workgroup size");
@@ -1350,6 +1475,7 @@ PGA_API pga_status pga_specialize_check(const pga_procedure* procedures, uint32_
     std::string msg;
     pga_status st = build_program(ctx.get(), procedures, num_procedures, entries, num_entries, false);
     if (st == PGA_OK) st = one_phase_only(ctx.get(), "the specialised kernel");
+    if (st == PGA_OK) st = no_polygons(ctx.get(), "the specialised kernel");
     if (st != PGA_OK) {
         msg = ctx->error;
     } else {
@@ -1531,6 +1657,7 @@ PGA_API pga_status pga_forward_tape(pga_context* ctx, uint32_t num_axioms, doubl
     pga_status st = check_launch(ctx, 0, num_axioms);
     if (st) return st;
     if ((st = one_phase_only(ctx, "the derivation tape"))) return st;
+    if ((st = no_polygons(ctx, "the derivation tape"))) return st;
     if (!ctx->d_tape_par) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no tape: call pga_tape_reserve first");
     if (num_axioms > ctx->tape_child_cap) return fail(ctx, PGA_ERR_OVERFLOW_QUEUE, "tape smaller than the axioms");
     if (ctx->tape_blocks == 0) {

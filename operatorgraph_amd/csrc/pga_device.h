@@ -33,6 +33,8 @@ enum Kind : uint8_t {
     K_REPLICATE,
     K_COLLIDER,       // stores the shape for IfCollides of later phases, one successor
     K_IFCOLLIDES,     // successor 0 if the shape collides with a stored collider, else 1
+    K_SETAS,          // the shape becomes a dynamic polygon or right prism, one successor
+    K_PRISMSPLIT,     // ComponentSplit of a dynamic right prism: sides, bottom, top
 };
 enum EntryFlags : uint8_t {
     EF_STATIC = 1,    // static-parameter operator form
@@ -48,7 +50,7 @@ enum EntryFlags : uint8_t {
 // always, so most operators cost a single table read.
 struct __align__(16) DevEntry {
     uint8_t kind;
-    uint8_t shape;      // pga_shape_type of the procedure (BOX/QUAD)
+    uint8_t shape;      // pga_shape_type of the procedure
     uint8_t flags;
     uint8_t nparams;
     uint8_t nsucc;
@@ -57,7 +59,8 @@ struct __align__(16) DevEntry {
     uint8_t ndirect;    // direct fan-out: (slot, count) pairs in aux
     uint32_t pbase;     // first DevParam
     uint32_t sbase;     // first successor in the successor pool
-    float pv[3];        // inline constant parameters
+    float pv[3];        // inline constant parameters; K_PRISMSPLIT: the bits of the
+                        // vertex list's first DevParam, and its vertex count
     int32_t sv[3];      // successors 0..2
     // fan-outs: aux[0] = skip mask, bit j: child j goes to a Discard or
     // nowhere (a Repeat's children all follow bit 0); direct fan-outs:
@@ -89,14 +92,16 @@ struct __align__(16) Rec {
     float s[3];         // size
     float seed;
     float custom;
-    uint32_t ed;        // entry | depth << 16; in multi-phase programs
-                        // entry | depth << 16 | phase << 28 (depth < 4096)
+    uint32_t ed;        // entry | invert << 15 | depth << 16; in multi-phase
+                        // programs also phase << 28 (depth < 4096); invert: a
+                        // dynamic polygon's or prism's vertex order is reversed
     uint32_t axiom;
     uint32_t path;
 };
 static_assert(sizeof(Rec) == 80, "Rec must be 80 bytes");
 
-__device__ __forceinline__ uint32_t rec_entry(const Rec& r) { return r.ed & 0xFFFFu; }
+const uint32_t kInvert = 0x8000u;   // Rec::ed bit 15
+__device__ __forceinline__ uint32_t rec_entry(const Rec& r) { return r.ed & 0x7FFFu; }
 
 struct Desc;
 

@@ -90,11 +90,25 @@ __device__ __forceinline__ void instance_rows(const Rec& r, f4& q0, f4& q1, f4& 
     const float b[16] = {r.s[0], 0.0f, 0.0f, 0.0f, 0.0f, r.s[1], 0.0f, 0.0f,
                          0.0f, 0.0f, r.s[2], 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
     float mm[12];
-    mul34x44(r.m, b, mm);
+    if (r.ed & kInvert) {
+        // an inverted polygon: getAdjustedModel (DynamicPolygon.cuh:68-75)
+        // negates the first two columns
+        float am[12];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            am[i * 4 + 0] = -r.m[i * 4 + 0];
+            am[i * 4 + 1] = -r.m[i * 4 + 1];
+            am[i * 4 + 2] = r.m[i * 4 + 2];
+            am[i * 4 + 3] = r.m[i * 4 + 3];
+        }
+        mul34x44(am, b, mm);
+    } else {
+        mul34x44(r.m, b, mm);
+    }
     q0 = f4{mm[0], mm[1], mm[2], mm[3]};
     q1 = f4{mm[4], mm[5], mm[6], mm[7]};
     q2 = f4{mm[8], mm[9], mm[10], mm[11]};
-    q3 = f4{r.custom, __uint_as_float(r.axiom), __uint_as_float(r.path), __uint_as_float(r.ed)};
+    q3 = f4{r.custom, __uint_as_float(r.axiom), __uint_as_float(r.path), __uint_as_float(r.ed & ~kInvert)};
 }
 
 __device__ __forceinline__ void emit_terminal(const Ctx& c, LevelLDS& L, f4* ST, bool wr, bool has, uint32_t slot,

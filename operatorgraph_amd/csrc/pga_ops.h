@@ -88,11 +88,11 @@ __device__ __forceinline__ bool advance(const Ctx& c, Rec& r, int32_t next, uint
         if (cph < pph || cph > c.phase) return false;
         // a shape of an earlier phase whose entry reaches no phase of this
         // launch (the entry's aux[1]) adds nothing here
-        if (cph < c.phase && c.prog.entries[(uint32_t)next & 0xFFFFu].aux[1] < c.phase) return false;
+        if (cph < c.phase && c.prog.entries[(uint32_t)next & 0x7FFFu].aux[1] < c.phase) return false;
     }
     uint32_t depth = rec_depth(c, r) + 1u;
     if (depth > c.dmask) { raise(c, PGA_ERR_MAX_DEPTH); return false; }
-    r.ed = ((uint32_t)next & 0xFFFFu) | (depth << 16) | (cph << 28);
+    r.ed = ((uint32_t)next & 0x7FFFu) | (r.ed & kInvert) | (depth << 16) | (cph << 28);
     r.path = path_mix(r.path, ordinal);
     return true;
 }
@@ -302,6 +302,9 @@ __device__ __forceinline__ int32_t apply_single(const Ctx& c, const E& e, Rec& r
                                                 CollMemo* memo = nullptr) {
     const uint32_t k = e.kind();
     *ordinal = 0;
+    // SetAs (SetAs.cuh:112-150): model, size, seed and custom attribute are
+    // kept; the vertex list is the entry's (a table fact, see build_program)
+    if (k == K_SETAS) { r.ed &= ~kInvert; return e.sv0; }
     if (!c.geom && (k == K_TRANSLATE || k == K_ROTATE)) return e.sv0;   // model only
     float v0, v1, v2;
     eval3(c, e, r, v0, v1, v2);
@@ -394,7 +397,7 @@ __device__ __forceinline__ int32_t apply_single(const Ctx& c, const E& e, Rec& r
 }
 
 __device__ __forceinline__ bool is_fanout(uint32_t k) {
-    return k == K_SUBDIV || k == K_REPEAT || k == K_COMPSPLIT || k == K_REPLICATE;
+    return k == K_SUBDIV || k == K_REPEAT || k == K_COMPSPLIT || k == K_REPLICATE || k == K_PRISMSPLIT;
 }
 
 // ---- fan-out operators ----
@@ -484,6 +487,8 @@ __device__ __forceinline__ void fan_init(const Ctx& c, const Rec& r, const E& e,
         f.off = 0.0f;
     } else if (kind == K_COMPSPLIT) {          // ComponentSplit.cuh:234-267
         f.n = 6;
+    } else if (kind == K_PRISMSPLIT) {         // the sides, then the bottom and top caps
+        f.n = (uint32_t)e.pv1 + 2u;
     } else {                                   // K_REPLICATE, Replicate.cuh:102-146
         f.n = e.nsucc();
     }
@@ -640,6 +645,59 @@ __device__ __forceinline__ bool build_child(const Ctx& c, const Rec& r, const E&
         const uint32_t mod = j < 4 ? j + 2u : (j == 4 ? 1u : 0u);
         if (c.seed) ch.seed = next_seed(r.seed, mod);
         return advance(c, ch, succ_of(c, e, j < 4 ? 2u : (j == 4 ? 1u : 0u)), j);
+    }
+    if (kind == K_PRISMSPLIT) {                // FaceSplitter<DynamicRightPrism> (ComponentSplit.cuh:23-75)
+        const uint32_t n = (uint32_t)e.pv1;
+        const uint32_t vb = __float_as_uint(e.pv0);
+        if (j < n) {
+            // side j, from vertex j-1 (n-1 for j = 0) to vertex j: getVertex
+            // reverses the order and mirrors y when inverted; both scaled by
+            // float2x2::scale(size.xz()) with the zero terms of the 2x2
+            // product, y negated, then the edge frame (facing)
+            const bool inv = (r.ed & kInvert) != 0u;
+            const uint32_t i0 = j == 0u ? n - 1u : j - 1u;
+            const DevParam q0 = c.prog.params[vb + (inv ? n - i0 - 1u : i0)];
+            const DevParam q1 = c.prog.params[vb + (inv ? n - j - 1u : j)];
+            const float a0 = q0.a, a1 = inv ? -q0.b : q0.b;
+            const float b0 = q1.a, b1 = inv ? -q1.b : q1.b;
+            const float sx = r.s[0], sz = r.s[2];
+            const float p0x = sx * a0 + 0.0f * a1, p0y = -(0.0f * a0 + sz * a1);
+            const float p1x = sx * b0 + 0.0f * b1, p1y = -(0.0f * b0 + sz * b1);
+            float dx = p1x - p0x, dy = p1y - p0y;
+            const float x = __builtin_sqrtf(dx * dx + dy * dy);
+            const float ox = p0x + (p1x - p0x) * 0.5f, oy = p0y + (p1y - p0y) * 0.5f;
+            const float rx = 1.0f / x;
+            dx = dx * rx;
+            dy = dy * rx;
+            if (c.geom) {
+                float tmp[12];
+                mul_translate(r.m, ox, 0.0f, oy, tmp);
+                const float b[16] = {dx, 0.0f, -dy, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f,
+                                     dy, 0.0f, dx, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+                mul34x44(tmp, b, ch.m);
+            }
+            ch.s[0] = x;
+            ch.s[2] = 1.0f;
+            if (c.seed) ch.seed = next_seed(r.seed, j + 2u);
+            ch.ed &= ~kInvert;
+            return advance(c, ch, succ_of(c, e, 2u), j);
+        }
+        // the bottom cap (j = n, inverted) and the top cap (j = n + 1):
+        // translate(0, size.y * sign * 0.5, 0), then the cap frame
+        const bool bottom = j == n;
+        const float sign = bottom ? -1.0f : 1.0f;
+        if (c.geom) {
+            float tmp[12];
+            mul_translate(r.m, 0.0f, (r.s[1] * sign) * 0.5f, 0.0f, tmp);
+            const float b[16] = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, sign, 0.0f,
+                                 0.0f, -sign, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+            mul34x44(tmp, b, ch.m);
+        }
+        ch.s[1] = r.s[2];
+        ch.s[2] = 1.0f;
+        if (c.seed) ch.seed = next_seed(r.seed, bottom ? 1u : 0u);
+        ch.ed = bottom ? (ch.ed | kInvert) : (ch.ed & ~kInvert);
+        return advance(c, ch, succ_of(c, e, bottom ? 1u : 0u), j);
     }
     // K_REPLICATE
     return advance(c, ch, succ_of(c, e, j), j);

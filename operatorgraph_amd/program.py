@@ -162,6 +162,11 @@ def Scalar(v):
     return Parameter(abi.PT_SCALAR, [v])
 
 
+def Vec2(x, y):
+    """A PT_VEC2 parameter: a vertex of SetAs* (Parameters.cuh:714-742)."""
+    return Parameter(abi.PT_VEC2, [x, y])
+
+
 def Rand(mn, mx):
     return Parameter(abi.PT_RAND, [mn, mx])
 

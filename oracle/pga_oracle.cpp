@@ -72,6 +72,7 @@ template <> struct S<float> {
     static float fabs_(float x) { return std::fabs(x); }
     static float fmod_(float x, float y) { return std::fmod(x, y); }
     static float deg2rad() { return 3.1415926535897932384626434f / 180.0f; } // Rotate.cuh:33-35
+    static float sqrt_(float x) { return std::sqrt(x); }                      // correctly rounded
 };
 template <> struct S<double> {
     static double c(double x) { return x; }
@@ -80,6 +81,7 @@ template <> struct S<double> {
     static double fabs_(double x) { return std::fabs(x); }
     static double fmod_(double x, double y) { return std::fmod(x, y); }
     static double deg2rad() { return 3.1415926535897932384626434 / 180.0; }
+    static double sqrt_(double x) { return std::sqrt(x); }
 };
 template <> struct S<Var> {
     static Var c(double x) { return Var(x); }
@@ -91,6 +93,10 @@ template <> struct S<Var> {
         return mk(std::fmod(x.v, y.v), x.id, 1.0, y.id, -q);
     }
     static Var deg2rad() { return Var(3.1415926535897932384626434 / 180.0); }
+    static Var sqrt_(Var x) {
+        const double r = std::sqrt(x.v);
+        return mk(r, x.id, r > 0.0 ? 0.5 / r : 0.0, -1, 0.0);
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -242,11 +248,40 @@ template <class T> struct Shape {
     float seed;
     float custom;
 };
+// The vertex list of a dynamic polygon or right prism (DynamicPolygon.cuh:
+// 16-100, DynamicRightPrism.cuh:18-130): numSides vertices in the unit
+// square, and `invert`, set on the bottom cap of a ComponentSplit, which
+// reverses the order, mirrors y and negates the first two model columns of
+// its instance.
+struct Poly {
+    uint32_t n = 0;
+    uint32_t invert = 0;
+    float v[2 * PGA_MAX_NUM_SIDES] = {};
+};
+inline bool is_polygon(int32_t st) {
+    return st == PGA_SHAPE_DYNAMIC_CONVEX_POLYGON || st == PGA_SHAPE_DYNAMIC_POLYGON;
+}
+inline bool is_prism(int32_t st) {
+    return st == PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM || st == PGA_SHAPE_DYNAMIC_RIGHT_PRISM;
+}
+inline bool has_vertices(int32_t st) { return is_polygon(st) || is_prism(st); }
+// getVertex: vertex i, or with `invert` vertex n-1-i with y negated
+inline void poly_vertex(const Poly& p, uint32_t i, float& x, float& y) {
+    if (p.invert) {
+        x = p.v[2 * (p.n - i - 1)];
+        y = -p.v[2 * (p.n - i - 1) + 1];
+    } else {
+        x = p.v[2 * i];
+        y = p.v[2 * i + 1];
+    }
+}
+
 struct Meta {
     int32_t entry;
     uint32_t axiom, path, depth;
     int32_t shape_type;
     uint32_t phase;    // multi-phase programs: the phase the shape is derived in
+    Poly poly;         // dynamic polygons and prisms
 };
 
 template <class T> void mul34x44(const T a[12], const T b[16], T c[12]) {
@@ -424,6 +459,8 @@ template <class T> struct Out {
     uint32_t terminal; // terminal index
     int32_t shape_type;
     Shape<T> s;
+    bool has_poly;     // child: `poly` replaces the parent's vertex list
+    Poly poly;
 };
 
 struct OpErr : std::runtime_error {
@@ -452,6 +489,7 @@ void push_child(std::vector<Out<T>>& out, const pga_dispatch_entry& e, uint32_t 
                 const Shape<T>& s) {
     Out<T> o;
     o.kind = 0; o.entry = succ(e, k); o.ordinal = ordinal; o.terminal = 0; o.shape_type = st; o.s = s;
+    o.has_poly = false;
     const int32_t ph = e.successors[k].phase_index;
     o.phase = ph == PGA_PHASE_INHERIT ? kPhaseInherit : (ph > 0 ? (uint32_t)ph : 0u);
     out.push_back(o);
@@ -781,10 +819,17 @@ void op_swapsize(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_disp
     push_child(out, e, 0, 0, m.shape_type, ns);
 }
 
+// Extrude.cuh:25-71 over a planar shape: Quad -> Box; a dynamic polygon ->
+// the right prism of the same vertex list and `invert` (fromPlanarType,
+// DynamicRightPrism.cuh:107-115), which the caller carries over.
 template <class T>
 void op_extrude(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e,
-                std::vector<Out<T>>& out) {             // Extrude.cuh:25-71
-    if (m.shape_type != PGA_SHAPE_QUAD) throw OpErr(PGA_ERR_UNSUPPORTED, "Extrude supports Quad shapes only");
+                std::vector<Out<T>>& out) {
+    int32_t to;
+    if (m.shape_type == PGA_SHAPE_QUAD) to = PGA_SHAPE_BOX;
+    else if (m.shape_type == PGA_SHAPE_DYNAMIC_CONVEX_POLYGON) to = PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM;
+    else if (m.shape_type == PGA_SHAPE_DYNAMIC_POLYGON) to = PGA_SHAPE_DYNAMIC_RIGHT_PRISM;
+    else throw OpErr(PGA_ERR_UNSUPPORTED, "Extrude supports planar shapes only");
     T extent = P(c, m, sh, e, 1);
     Shape<T> ns;
     T r[16];
@@ -793,7 +838,103 @@ void op_extrude(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_dispa
     ns.size[0] = sh.size[0]; ns.size[1] = extent; ns.size[2] = sh.size[1];
     ns.seed = next_seed(sh.seed, 0u);
     ns.custom = sh.custom;
-    push_child(out, e, 0, 0, PGA_SHAPE_BOX, ns);
+    push_child(out, e, 0, 0, to, ns);
+}
+
+// SetAs<DynParams, NewShape> (SetAs.cuh:112-150): the shape becomes a dynamic
+// polygon or right prism with numSides = the entry's parameter count and
+// vertex i = dynToFloat2(parameter i) (Parameters.cuh:714-742: a scalar is
+// (value, 0); Vec2, Vec4 and Rand give their first two values, unevaluated);
+// model, size, seed and custom attribute are kept, `invert` is cleared.
+// Fewer than 3 or more than PGA_MAX_NUM_SIDES vertices are refused (the
+// reference writes past its vertex array above MaxNumSides).
+template <class T>
+void op_setas(const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e, int32_t op,
+              std::vector<Out<T>>& out) {
+    (void)m;
+    const uint32_t n = e.num_parameters;
+    if (n < 3 || n > PGA_MAX_NUM_SIDES) throw OpErr(PGA_ERR_BAD_PROGRAM, "SetAs needs 3 to 16 vertices");
+    Poly p;
+    p.n = n;
+    for (uint32_t i = 0; i < n; ++i) {
+        const pga_dispatch_parameter& q = e.parameters[i];
+        if (q.type == PGA_PT_SCALAR) { p.v[2 * i] = q.value[0]; p.v[2 * i + 1] = 0.0f; }
+        else if (q.type == PGA_PT_VEC2 || q.type == PGA_PT_VEC4 || q.type == PGA_PT_RAND) {
+            p.v[2 * i] = q.value[0]; p.v[2 * i + 1] = q.value[1];
+        } else {
+            throw OpErr(PGA_ERR_BAD_PROGRAM, "SetAs vertex parameters are scalars, vectors or Rand pairs");
+        }
+    }
+    const int32_t to = op == PGA_OP_SET_AS_DYNAMIC_CONVEX_POLYGON ? PGA_SHAPE_DYNAMIC_CONVEX_POLYGON
+                     : op == PGA_OP_SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM ? PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM
+                     : op == PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON ? PGA_SHAPE_DYNAMIC_POLYGON
+                     : PGA_SHAPE_DYNAMIC_RIGHT_PRISM;
+    push_child(out, e, 0, 0, to, sh);
+    out.back().has_poly = true;
+    out.back().poly = p;
+}
+
+// ComponentSplit<false, Top, Bottom, Sides> over a dynamic right prism with
+// the generic FaceSplitter (ComponentSplit.cuh:23-75, 234-267): side f =
+// 0..n-1 runs from vertex f-1 (n-1 for f = 0) to vertex f, both scaled by
+// (size.x, size.z) with a 2x2 product and y negated; the side's frame turns
+// x onto that edge (facing), its origin is the edge's midpoint, its size
+// (|edge|, size.y, 1) and its seed modifier f + 2. Then the bottom cap
+// (inverted, modifier 1, Bottom) and the top cap (modifier 0, Top): the
+// model translated by +-size.y / 2 and turned by the cap frame, size
+// (size.x, size.z, 1), the prism's raw vertex list.
+template <class T>
+void op_compsplit_prism(const Meta& m, const Shape<T>& sh, const pga_dispatch_entry& e, std::vector<Out<T>>& out) {
+    (void)succ(e, 0); (void)succ(e, 1); (void)succ(e, 2);
+    const T zero = S<T>::c(0.0), one = S<T>::c(1.0), half = S<T>::c(0.5);
+    const uint32_t n = m.poly.n;
+    const T sx = sh.size[0], sz = sh.size[2];
+    for (uint32_t f = 0; f < n; ++f) {
+        float ax, ay, bx, by;
+        poly_vertex(m.poly, f == 0 ? n - 1 : f - 1, ax, ay);
+        poly_vertex(m.poly, f, bx, by);
+        const T a0 = S<T>::c(ax), a1 = S<T>::c(ay), b0 = S<T>::c(bx), b1 = S<T>::c(by);
+        // float2x2::scale(size.xz()) * vertex (matrix.h:389-393)
+        T p0x = sx * a0 + zero * a1, p0y = zero * a0 + sz * a1;
+        T p1x = sx * b0 + zero * b1, p1y = zero * b0 + sz * b1;
+        p0y = -p0y;
+        p1y = -p1y;
+        T dx = p1x - p0x, dy = p1y - p0y;
+        const T x = S<T>::sqrt_(dx * dx + dy * dy);
+        const T ox = p0x + (p1x - p0x) * half, oy = p0y + (p1y - p0y) * half;
+        const T r = one / x;
+        dx = dx * r;
+        dy = dy * r;
+        const T facing[16] = {dx, zero, -dy, zero, zero, one, zero, zero, dy, zero, dx, zero, zero, zero, zero, one};
+        T t[16], tmp[12];
+        mat_translate(ox, zero, oy, t);
+        Shape<T> q;
+        mul34x44(sh.m, t, tmp);
+        mul34x44(tmp, facing, q.m);
+        q.size[0] = x; q.size[1] = sh.size[1]; q.size[2] = one;
+        q.seed = next_seed(sh.seed, f + 2u);
+        q.custom = sh.custom;
+        push_child(out, e, 2, f, PGA_SHAPE_QUAD, q);
+    }
+    const int32_t cap_shape = m.shape_type == PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM ? PGA_SHAPE_DYNAMIC_CONVEX_POLYGON
+                                                                                  : PGA_SHAPE_DYNAMIC_POLYGON;
+    for (uint32_t cap = 0; cap < 2; ++cap) {   // bottom, then top
+        const bool bottom = cap == 0;
+        const T sign = S<T>::c(bottom ? -1.0 : 1.0);   // 1 - 2 * isBottom
+        const T frame[16] = {one, zero, zero, zero, zero, zero, sign, zero, zero, -sign, zero, zero, zero, zero, zero, one};
+        T t[16], tmp[12];
+        mat_translate(zero, (sh.size[1] * sign) * half, zero, t);
+        Shape<T> q;
+        mul34x44(sh.m, t, tmp);
+        mul34x44(tmp, frame, q.m);
+        q.size[0] = sh.size[0]; q.size[1] = sh.size[2]; q.size[2] = one;
+        q.seed = next_seed(sh.seed, bottom ? 1u : 0u);
+        q.custom = sh.custom;
+        push_child(out, e, bottom ? 1u : 0u, n + cap, cap_shape, q);
+        out.back().has_poly = true;
+        out.back().poly = m.poly;
+        out.back().poly.invert = bottom ? 1u : 0u;
+    }
 }
 
 template <class T>
@@ -811,9 +952,25 @@ void op_generate(const Ctx& c, const Meta& m, const Shape<T>& sh, const pga_disp
     if (term >= PGA_MAX_TERMINALS) throw OpErr(PGA_ERR_BAD_PROGRAM, "terminal index out of range");
     Out<T> o;
     o.kind = 1; o.entry = m.entry; o.ordinal = 0; o.terminal = term; o.shape_type = m.shape_type;
+    o.has_poly = false;
     T sc[16];
     mat_scale(sh.size[0], sh.size[1], sh.size[2], sc);
     o.s = sh;
+    if (has_vertices(m.shape_type)) {
+        // getAdjustedModel (DynamicPolygon.cuh:68-75): the first two columns
+        // times sign = -1 when inverted, 1 otherwise
+        T am[12];
+        const T sign = S<T>::c(m.poly.invert ? -1.0 : 1.0);
+        for (int i = 0; i < 3; ++i) {
+            am[i * 4 + 0] = sign * sh.m[i * 4 + 0];
+            am[i * 4 + 1] = sign * sh.m[i * 4 + 1];
+            am[i * 4 + 2] = sh.m[i * 4 + 2];
+            am[i * 4 + 3] = sh.m[i * 4 + 3];
+        }
+        mul34x44(am, sc, o.s.m);
+        out.push_back(o);
+        return;
+    }
     mul34x44(sh.m, sc, o.s.m);
     out.push_back(o);
 }
@@ -971,8 +1128,21 @@ void run_op(const Ctx& c, const Meta& m, const Shape<T>& sh, Dec& d, std::vector
         op_repeat(c, m, sh, e, d, out);
         break;
     case PGA_OP_COMPSPLIT:
-        if (p.flags & PGA_PROC_PARALLEL) op_compsplit_parallel(m, sh, e, out);
-        else op_compsplit(m, sh, e, out);
+        if (is_prism(m.shape_type)) {
+            if (p.flags & PGA_PROC_PARALLEL)
+                throw OpErr(PGA_ERR_UNSUPPORTED, "the parallel ComponentSplit splits Boxes only");
+            op_compsplit_prism(m, sh, e, out);
+        } else if (p.flags & PGA_PROC_PARALLEL) {
+            op_compsplit_parallel(m, sh, e, out);
+        } else {
+            op_compsplit(m, sh, e, out);
+        }
+        break;
+    case PGA_OP_SET_AS_DYNAMIC_CONVEX_POLYGON:
+    case PGA_OP_SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM:
+    case PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON:
+    case PGA_OP_SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM:
+        op_setas(m, sh, e, p.op_type, out);
         break;
     case PGA_OP_IFSIZELESS: op_ifsizeless(c, m, sh, e, d, out); break;
     case PGA_OP_IF: op_if(c, m, sh, e, d, out); break;
@@ -984,11 +1154,17 @@ void run_op(const Ctx& c, const Meta& m, const Shape<T>& sh, Dec& d, std::vector
     case PGA_OP_EXTRUDE: op_extrude(c, m, sh, e, out); break;
     case PGA_OP_REPLICATE: op_replicate(m, sh, e, out); break;
     case PGA_OP_DISCARD: break;                               // Discard.cuh:15-31
-    case PGA_OP_COLLIDER: op_collider(c, m, sh, e, d, out); break;
-    case PGA_OP_IFCOLLIDES: op_ifcollides(c, m, sh, e, d, out); break;
+    case PGA_OP_COLLIDER:
+    case PGA_OP_IFCOLLIDES:
+        if (has_vertices(m.shape_type)) throw OpErr(PGA_ERR_UNSUPPORTED, "collision tests of Box and Quad shapes only");
+        if (p.op_type == PGA_OP_COLLIDER) op_collider(c, m, sh, e, d, out);
+        else op_ifcollides(c, m, sh, e, d, out);
+        break;
     case PGA_OP_GENERATE:
         if (p.gen_func_idx != 0 && p.gen_func_idx != 1)
             throw OpErr(PGA_ERR_UNSUPPORTED, "generation functions 0 and 1 only");
+        if (p.gen_func_idx == 0 && has_vertices(m.shape_type))
+            throw OpErr(PGA_ERR_UNSUPPORTED, "generation function 0 of Box and Quad shapes only");
         op_generate(c, m, sh, e, d, out);
         break;
     default: throw OpErr(PGA_ERR_UNSUPPORTED, "unsupported operator type " + std::to_string(p.op_type));
@@ -1136,6 +1312,8 @@ int pgao_derive(const pga_procedure* procs, uint32_t num_procs,
             Item it;
             it.meta.entry = ax.entry_index; it.meta.axiom = a; it.meta.path = 0; it.meta.depth = 0;
             it.meta.shape_type = ax.shape_type;
+            if (has_vertices(ax.shape_type))
+                throw OpErr(PGA_ERR_UNSUPPORTED, "axioms are Box or Quad shapes (no vertex list)");
             it.meta.phase = 0;
             for (int i = 0; i < 12; ++i) { it.f.m[i] = ax.model[i]; it.dd.m[i] = ax.model[i]; it.v.m[i] = Var(ax.model[i]); }
             for (int i = 0; i < 3; ++i) { it.f.size[i] = ax.size[i]; it.dd.size[i] = ax.size[i]; it.v.size[i] = Var(ax.size[i]); }
@@ -1195,6 +1373,8 @@ int pgao_derive(const pga_procedure* procs, uint32_t num_procs,
                         ch.meta.path = path_mix(it.meta.path, o.ordinal);
                         ch.meta.depth = it.meta.depth + 1;
                         ch.meta.shape_type = o.shape_type;
+                        if (o.has_poly) ch.meta.poly = o.poly;
+                        else if (has_vertices(o.shape_type)) ch.meta.poly = it.meta.poly;
                         ch.meta.phase = o.phase == kPhaseInherit ? it.meta.phase : o.phase;
                         // a later phase's shape waits for it; an earlier phase's is
                         // never derived (that phase has run); with phases, the depth

@@ -1,0 +1,114 @@
+"""GPU parity of dynamic polygons and right prisms (SetAs*, ComponentSplit of
+a prism, Extrude of a polygon, inverted caps; tests/polygon_cases.py): the
+interpreting kernels through the C ABI against the oracle, bit-exact
+instance multisets, eagerly and replayed from a captured graph; and the
+refusals of the specialised kernel and the tape."""
+import sys
+
+import numpy as np
+import pytest
+
+from operatorgraph_amd import abi
+from tests import polygon_cases as PC
+from tests.helpers import assert_same_multiset
+
+pytestmark = pytest.mark.gpu
+
+TECHNIQUES = [abi.KERNELS, abi.MEGAKERNEL]
+N_TERMS = 5
+
+
+def _note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def E():
+    from operatorgraph_amd import engine
+    return engine
+
+
+def check(E, program, ax, technique=abi.MEGAKERNEL, **cfg):
+    from oracle import oracle as O
+    ref = O.derive(program, ax)
+    eng = E.Engine(technique=technique, **cfg)
+    try:
+        eng.load(program).set_axioms(ax)
+        for t in range(N_TERMS):
+            eng.bind(t, max(1, len(ref.instances(t))))
+        eng.execute()
+        for t in range(N_TERMS):
+            assert_same_multiset(eng.instances(t), ref.instances(t), f"{program.name} terminal {t}")
+    finally:
+        eng.close()
+    return ref
+
+
+@pytest.mark.parametrize("technique", TECHNIQUES)
+@pytest.mark.parametrize("n", [3, 4, 5, 8, 16])
+@pytest.mark.parametrize("convex", [True, False])
+def test_regular_prisms(E, technique, n, convex):
+    _note(f"regular prism n={n} convex={convex} technique={technique}")
+    ref = check(E, PC.prism_program(PC.regular(n, phase=0.1 * n), convex=convex), PC.boxes(256, n), technique)
+    assert len(ref.instances(2)) == 256 * n
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_convex_prisms(E, seed):
+    rng = np.random.default_rng(100 + seed)
+    verts = PC.random_convex(int(rng.integers(3, 17)), rng)
+    _note(f"random convex prism seed={seed} sides={len(verts)}")
+    check(E, PC.prism_program(verts), PC.boxes(512, 200 + seed))
+
+
+@pytest.mark.parametrize("technique", TECHNIQUES)
+@pytest.mark.parametrize("name", ["roof", "recap", "polygon_ops"])
+def test_programs(E, technique, name):
+    _note(f"polygon program {name} technique={technique}")
+    prog = {"roof": PC.roof_program, "recap": lambda: PC.recap_program(PC.random_convex(7, np.random.default_rng(3))),
+            "polygon_ops": lambda: PC.polygon_ops_program(PC.regular(6))}[name]()
+    check(E, prog, PC.boxes(1024, 7))
+
+
+def test_captured_graph_replays(E):
+    _note("polygon program captured and replayed")
+    from oracle import oracle as O
+    prog = PC.recap_program(PC.regular(5))
+    ax = PC.boxes(128, 11)
+    ref = O.derive(prog, ax)
+    eng = E.Engine()
+    try:
+        eng.load(prog).set_axioms(ax)
+        for t in range(N_TERMS):
+            eng.bind(t, max(1, len(ref.instances(t))))
+        eng.execute()
+        with eng.capture() as g:
+            eng.execute_async()
+        for _ in range(3):
+            g.launch()
+            eng.synchronize()
+            for t in range(4):
+                assert_same_multiset(eng.instances(t), ref.instances(t), f"replay terminal {t}")
+    finally:
+        eng.close()
+
+
+def test_refusals(E):
+    _note("polygon programs: specialise and tape refused")
+    eng = E.Engine()
+    try:
+        eng.load(PC.roof_program()).set_axioms(PC.boxes(4, 0))
+        for t in range(2):
+            eng.bind(t, 64)
+        with pytest.raises(E.PGAError) as e:
+            eng.specialize()
+        assert e.value.status == abi.ERR_UNSUPPORTED
+        eng.tape_reserve(1 << 12, 1 << 12)
+        with pytest.raises(E.PGAError) as e:
+            eng.forward_tape()
+        assert e.value.status == abi.ERR_UNSUPPORTED
+        eng.execute()
+        assert int(eng.counts(2)[0]) == 8
+    finally:
+        eng.close()

@@ -146,14 +146,22 @@ def test_scene_derives_at_its_own_axioms(name):
 
 
 @pytest.mark.parametrize("name", POLYGON_SCENES)
-def test_polygon_scenes_are_refused(name):
+def test_polygon_scenes_derive(name):
+    """Balcony and SuburbanHouse turn Boxes into triangular right prisms
+    (SetAsDynamicConvexRightPrism) and split them: every prism gives two
+    RoofSide caps; the suburban porch's prism one PorchFront cap."""
     from oracle import oracle as O
     procs, _ = SceneProgram(name).arrays()
     assert any(abi.SET_AS_DYNAMIC_CONVEX_POLYGON <= p.op_type <= abi.SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM
                for p in procs)
-    with pytest.raises(O.OracleError) as e:
-        O.derive(SceneProgram(name), scene_axioms(name, 2))
-    assert e.value.status == abi.ERR_UNSUPPORTED
+    assert any(p.shape_type == abi.DYNAMIC_CONVEX_RIGHT_PRISM and p.op_type == abi.COMPSPLIT for p in procs)
+    lots = 4
+    r = O.derive(SceneProgram(name), scene_axioms(name, lots))
+    c = _counts(r)
+    if name == "Balcony":      # RoofSide (1, 7), Roof (1, 6): one roof prism a lot
+        assert c[7] == 2 * lots and c[6] >= 2 * lots
+    else:                      # RoofSide (1, 6): three roof prisms a lot; PorchFront (1, 3)
+        assert c[6] == 6 * lots and c[3] == lots
 
 
 @pytest.mark.parametrize("name", ["Commercial"] + list(POLYGON_SCENES))

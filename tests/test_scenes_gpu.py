@@ -3,7 +3,8 @@ src/test_runner/*.cuh generate, lowered through include/pga/core/Core.h):
 both techniques and the kernel specialised to each table derive the
 oracle's instances bit for bit (canonical sort; transforms within 1e-5 of
 its FP64 twin), at the axiom sizes each scene's AxiomTraits give. The
-polygon scenes are refused when the table is loaded."""
+polygon scenes (triangular right prisms) run on the interpreting kernels
+only: the specialised kernel refuses them."""
 import sys
 
 import pytest
@@ -55,13 +56,23 @@ def test_scene_specialised(name):
     spec_check(SceneProgram(name), ax, n_terms, cap=cap)
 
 
+@pytest.mark.parametrize("technique", TECHNIQUES)
 @pytest.mark.parametrize("name", POLYGON_SCENES)
-def test_polygon_scene_refused_at_load(E, name):
-    _note(f"scene {name}: refused at load")
+def test_polygon_scene_interpreter(E, name, technique):
+    ax = scene_axioms(name, 32)
+    n_terms, cap = _terminals(name, ax)
+    _note(f"scene {name}: {len(ax)} lots, technique {technique}")
+    check_against_oracle(E, SceneProgram(name), ax, n_terms, technique, cap=cap)
+
+
+@pytest.mark.parametrize("name", POLYGON_SCENES)
+def test_polygon_scene_not_specialised(E, name):
+    _note(f"scene {name}: the specialised kernel refuses it")
     eng = E.Engine()
     try:
+        eng.load(SceneProgram(name))
         with pytest.raises(E.PGAError) as e:
-            eng.load(SceneProgram(name))
+            eng.specialize()
         assert e.value.status == abi.ERR_UNSUPPORTED
     finally:
         eng.close()
