@@ -50,9 +50,28 @@ using namespace PGA::Operators;
 using namespace PGA::Parameters;
 using PGA::Shapes::Box;
 using PGA::Shapes::Quad;
+typedef PGA::Shapes::DynamicPolygon<PGA::Constants::MaxNumSides, true> DCPoly;
+typedef PGA::Shapes::DynamicPolygon<PGA::Constants::MaxNumSides, false> DPoly;
+typedef PGA::Shapes::DynamicRightPrism<PGA::Constants::MaxNumSides, true> DCRPrism;
+typedef PGA::Shapes::DynamicRightPrism<PGA::Constants::MaxNumSides, false> DRPrism;
+
+// the procedures every dynamic shape shares, after its first one or two
+// (PGAFacade.cu:397-470)
+#define PGA_INTERPRETER_COMMON(S)                                                                               \
+    Proc<S, Generate<false, 0, DynParam<0>>>, Proc<S, Generate<false, 1, DynParam<0>>>,                         \
+        Proc<S, If<DynParam<0>, DCall<0>, DCall<1>>>,                                                           \
+        Proc<S, IfSizeLess<DynParam<0>, DynParam<1>, DCall<0>, DCall<1>>>, Proc<S, RandomRule<DynParams>>,     \
+        Proc<S, Repeat<false, DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>, Proc<S, Replicate<DynParams>>, \
+        Proc<S, Rotate<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,                                        \
+        Proc<S, Scale<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,                                         \
+        Proc<S, SetAsDynamicConvexPolygon<DynParams>>, Proc<S, SetAsDynamicConvexRightPrism<DynParams>>,        \
+        Proc<S, SetAsDynamicConcavePolygon<DynParams>>, Proc<S, SetAsDynamicConcaveRightPrism<DynParams>>,      \
+        Proc<S, Subdivide<DynParams>>, Proc<S, SwapSize<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,      \
+        Proc<S, Translate<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>
 
 // Quad: Discard .. Scale, four SetAs, Subdivide, SwapSize, Translate;
-// Box: ComponentSplit first, no Extrude (PGAFacade.cu:354-392)
+// Box: ComponentSplit first, no Extrude (PGAFacade.cu:354-392); then the
+// dynamic polygons (Extrude) and prisms (ComponentSplit)
 struct ProcList
     : T::List<
           Proc<Quad, Discard>, Proc<Quad, Extrude<DynParam<0>, DynParam<1>, DCall<0>>>,
@@ -76,7 +95,15 @@ struct ProcList
           Proc<Box, SetAsDynamicConvexPolygon<DynParams>>, Proc<Box, SetAsDynamicConvexRightPrism<DynParams>>,
           Proc<Box, SetAsDynamicConcavePolygon<DynParams>>, Proc<Box, SetAsDynamicConcaveRightPrism<DynParams>>,
           Proc<Box, Subdivide<DynParams>>, Proc<Box, SwapSize<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
-          Proc<Box, Translate<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>> {};
+          Proc<Box, Translate<DynParam<0>, DynParam<1>, DynParam<2>, DCall<0>>>,
+          // the dynamic shapes of the list built with INTERPRETER_SUPPORT_ALL_SHAPES
+          Proc<DCPoly, Discard>, Proc<DCPoly, Extrude<DynParam<0>, DynParam<1>, DCall<0>>>, PGA_INTERPRETER_COMMON(DCPoly),
+          Proc<DPoly, Discard>, Proc<DPoly, Extrude<DynParam<0>, DynParam<1>, DCall<0>>>, PGA_INTERPRETER_COMMON(DPoly),
+          Proc<DCRPrism, ComponentSplit<false, DCall<0>, DCall<1>, DCall<2>>>, Proc<DCRPrism, Discard>,
+          PGA_INTERPRETER_COMMON(DCRPrism),
+          Proc<DRPrism, ComponentSplit<false, DCall<0>, DCall<1>, DCall<2>>>, Proc<DRPrism, Discard>,
+          PGA_INTERPRETER_COMMON(DRPrism)> {};
+#undef PGA_INTERPRETER_COMMON
 
 struct Configuration {
     static const PGA::GPU::Technique Technique = PGA::GPU::MEGAKERNEL;

@@ -73,8 +73,50 @@ const pga_procedure kInterpreterProcedures[] = {
     {PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON, PGA_SHAPE_BOX, 0, 0},
     {PGA_OP_SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM, PGA_SHAPE_BOX, 0, 0}, {PGA_OP_SUBDIV, PGA_SHAPE_BOX, 0, 0},
     {PGA_OP_SWAPSIZE, PGA_SHAPE_BOX, 0, 0}, {PGA_OP_TRANSLATE, PGA_SHAPE_BOX, 0, 0},
+    // the dynamic shapes of the list built with INTERPRETER_SUPPORT_ALL_SHAPES
+    // (PGAFacade.cu:65-130, 395-470), which follow Box there too
+#define PGA_PLANAR_PROCS(S)                                                                                       \
+    {PGA_OP_DISCARD, S, 0, 0}, {PGA_OP_EXTRUDE, S, 0, 0}, {PGA_OP_GENERATE, S, 0, 0}, {PGA_OP_GENERATE, S, 1, 0},   \
+        {PGA_OP_IF, S, 0, 0}, {PGA_OP_IFSIZELESS, S, 0, 0}, {PGA_OP_STOCHASTIC, S, 0, 0},                          \
+        {PGA_OP_REPEAT, S, 0, 0}, {PGA_OP_REPLICATE, S, 0, 0}, {PGA_OP_ROTATE, S, 0, 0}, {PGA_OP_SCALE, S, 0, 0}, \
+        {PGA_OP_SET_AS_DYNAMIC_CONVEX_POLYGON, S, 0, 0}, {PGA_OP_SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM, S, 0, 0},      \
+        {PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON, S, 0, 0}, {PGA_OP_SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM, S, 0, 0},    \
+        {PGA_OP_SUBDIV, S, 0, 0}, {PGA_OP_SWAPSIZE, S, 0, 0}, {PGA_OP_TRANSLATE, S, 0, 0}
+#define PGA_PRISM_PROCS(S)                                                                                        \
+    {PGA_OP_COMPSPLIT, S, 0, 0}, {PGA_OP_DISCARD, S, 0, 0}, {PGA_OP_GENERATE, S, 0, 0}, {PGA_OP_GENERATE, S, 1, 0}, \
+        {PGA_OP_IF, S, 0, 0}, {PGA_OP_IFSIZELESS, S, 0, 0}, {PGA_OP_STOCHASTIC, S, 0, 0},                          \
+        {PGA_OP_REPEAT, S, 0, 0}, {PGA_OP_REPLICATE, S, 0, 0}, {PGA_OP_ROTATE, S, 0, 0}, {PGA_OP_SCALE, S, 0, 0}, \
+        {PGA_OP_SET_AS_DYNAMIC_CONVEX_POLYGON, S, 0, 0}, {PGA_OP_SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM, S, 0, 0},      \
+        {PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON, S, 0, 0}, {PGA_OP_SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM, S, 0, 0},    \
+        {PGA_OP_SUBDIV, S, 0, 0}, {PGA_OP_SWAPSIZE, S, 0, 0}, {PGA_OP_TRANSLATE, S, 0, 0}
+    PGA_PLANAR_PROCS(PGA_SHAPE_DYNAMIC_CONVEX_POLYGON),
+    PGA_PLANAR_PROCS(PGA_SHAPE_DYNAMIC_POLYGON),
+    PGA_PRISM_PROCS(PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM),
+    PGA_PRISM_PROCS(PGA_SHAPE_DYNAMIC_RIGHT_PRISM),
+#undef PGA_PLANAR_PROCS
+#undef PGA_PRISM_PROCS
 };
 const uint32_t kNumInterpreterProcedures = sizeof(kInterpreterProcedures) / sizeof(kInterpreterProcedures[0]);
+
+// Operator::nextShapeType (Operator.cpp:20-130) for successor k: a
+// ComponentSplit of a Box gives Quads, of a dynamic prism its polygon
+// (the caps, successors 0 and 1) and Quads (the sides); Extrude gives a
+// planar shape's extruded type; SetAs its target
+int next_shape_of(int op, int shape, size_t k) {
+    const bool prism = shape == PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM || shape == PGA_SHAPE_DYNAMIC_RIGHT_PRISM;
+    if (op == PGA_OP_COMPSPLIT && shape == PGA_SHAPE_BOX) return PGA_SHAPE_QUAD;
+    if (op == PGA_OP_COMPSPLIT && prism)
+        return k == 2 ? PGA_SHAPE_QUAD
+             : shape == PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM ? PGA_SHAPE_DYNAMIC_CONVEX_POLYGON : PGA_SHAPE_DYNAMIC_POLYGON;
+    if (op == PGA_OP_EXTRUDE && shape == PGA_SHAPE_QUAD) return PGA_SHAPE_BOX;
+    if (op == PGA_OP_EXTRUDE && shape == PGA_SHAPE_DYNAMIC_CONVEX_POLYGON) return PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM;
+    if (op == PGA_OP_EXTRUDE && shape == PGA_SHAPE_DYNAMIC_POLYGON) return PGA_SHAPE_DYNAMIC_RIGHT_PRISM;
+    if (op == PGA_OP_SET_AS_DYNAMIC_CONVEX_POLYGON) return PGA_SHAPE_DYNAMIC_CONVEX_POLYGON;
+    if (op == PGA_OP_SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM) return PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM;
+    if (op == PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON) return PGA_SHAPE_DYNAMIC_POLYGON;
+    if (op == PGA_OP_SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM) return PGA_SHAPE_DYNAMIC_RIGHT_PRISM;
+    return shape;
+}
 
 struct CompileError : std::runtime_error {
     explicit CompileError(const std::string& m) : std::runtime_error(m) {}
@@ -653,11 +695,7 @@ private:
         if (queued_->insert(s).second) queue_->push_back(s);
         return e;
     }
-    static int next_shape(int op, int shape) {   // Operator::nextShapeType for Box and Quad
-        if (op == PGA_OP_COMPSPLIT && shape == PGA_SHAPE_BOX) return PGA_SHAPE_QUAD;
-        if (op == PGA_OP_EXTRUDE && shape == PGA_SHAPE_QUAD) return PGA_SHAPE_BOX;
-        return shape;
-    }
+    static int next_shape(int op, int shape, size_t k = 0) { return next_shape_of(op, shape, k); }
     int procedure(int op, int shape, int gen, int line) const {
         for (uint32_t i = 0; i < kNumInterpreterProcedures; ++i) {
             const pga_procedure& p = kInterpreterProcedures[i];
@@ -698,15 +736,14 @@ private:
             entries[idx] = e;
             return;
         }
-        if (op == PGA_OP_SET_AS_DYNAMIC_CONVEX_POLYGON || op == PGA_OP_SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM ||
-            op == PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON || op == PGA_OP_SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM ||
-            op == PGA_OP_IFCOLLIDES || op == PGA_OP_COLLIDER)
-            throw CompileError(at_line(ln) + operator_name(op) + " is not supported (no Box/Quad procedure)");
+        if (op == PGA_OP_IFCOLLIDES || op == PGA_OP_COLLIDER)
+            throw CompileError(at_line(ln) + operator_name(op) + " is not supported (no interpreter procedure)");
         e.operator_code = procedure(op, shape, 0, ln);
-        const int ns = next_shape(op, shape);
         std::vector<int> succ;
-        for (const SuccessorP& n : s->next) succ.push_back(n->is_symbol ? symbol_successor(n->symbol, ns, n->line)
-                                                                        : new_entry());
+        for (size_t k = 0; k < s->next.size(); ++k) {
+            const SuccessorP& n = s->next[k];
+            succ.push_back(n->is_symbol ? symbol_successor(n->symbol, next_shape(op, shape, k), n->line) : new_entry());
+        }
         auto need = [&](size_t np, size_t nsucc) {   // Operator::check (ParserOperator.cpp:216-231)
             if (s->params.size() < np)
                 throw CompileError(at_line(ln) + operator_name(op) + " requires at least " + std::to_string(np) +
@@ -780,6 +817,27 @@ private:
             if (!s->params.empty()) throw CompileError(at_line(ln) + "Replicate accepts no parameters");
             no_factors();
             break;
+        case PGA_OP_SET_AS_DYNAMIC_CONVEX_POLYGON: case PGA_OP_SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM:
+        case PGA_OP_SET_AS_DYNAMIC_CONCAVE_POLYGON: case PGA_OP_SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM: {
+            // buildSetAs*Entry (DispatchTableGenerator.cpp:442-517): each
+            // vertex (its first two values) as a Vec2, ordered counter-
+            // clockwise from the bottom-left one (GeometryUtils.h:54-71)
+            need(3, 1);
+            no_factors();
+            std::vector<std::pair<float, float>> v;
+            for (const Operand& o : s->params) {
+                const pga_dispatch_parameter q = to_param(o, ln);
+                v.push_back({q.value[0], q.value[1]});
+            }
+            for (const auto& p : order_ccw(v)) {
+                pga_dispatch_parameter q{};
+                q.type = PGA_PT_VEC2;
+                q.value[0] = p.first;
+                q.value[1] = p.second;
+                params.push_back(q);
+            }
+            break;
+        }
         default:
             throw CompileError(at_line(ln) + operator_name(op) + " is not supported");
         }
@@ -794,7 +852,28 @@ private:
         }
         entries[idx] = e;
         for (size_t i = 0; i < s->next.size(); ++i)
-            if (!s->next[i]->is_symbol) visit(succ[i], ns, s->next[i], rule);
+            if (!s->next[i]->is_symbol) visit(succ[i], next_shape(op, shape, i), s->next[i], rule);
+    }
+    // GeometryUtils::orderVertices_CCW: the anchor is the vertex of least y
+    // (then least x); the others follow by the angle acos(d.x / |d|) of
+    // d = v - anchor, ascending (FP32, as OrderByAngle computes it; equal
+    // angles keep their order)
+    static std::vector<std::pair<float, float>> order_ccw(const std::vector<std::pair<float, float>>& in) {
+        size_t a = 0;
+        for (size_t j = 1; j < in.size(); ++j)
+            if (in[j].second < in[a].second || (in[j].second == in[a].second && in[j].first < in[a].first)) a = j;
+        const float ax = in[a].first, ay = in[a].second;
+        auto angle = [&](const std::pair<float, float>& v) {
+            const float dx = v.first - ax, dy = v.second - ay;
+            const float l = std::sqrt(dx * dx + dy * dy);
+            return std::acos(dx / l);
+        };
+        std::vector<std::pair<float, float>> out;
+        for (size_t j = 0; j < in.size(); ++j)
+            if (j != a) out.push_back(in[j]);
+        std::stable_sort(out.begin(), out.end(), [&](const auto& p, const auto& q) { return angle(p) < angle(q); });
+        out.insert(out.begin(), in[a]);
+        return out;
     }
 };
 
@@ -937,11 +1016,6 @@ private:
         on_stack_.push_back(0);
         return (int)V.size() - 1;
     }
-    static int next_shape(int op, int shape) {
-        if (op == PGA_OP_COMPSPLIT && shape == PGA_SHAPE_BOX) return PGA_SHAPE_QUAD;
-        if (op == PGA_OP_EXTRUDE && shape == PGA_SHAPE_QUAD) return PGA_SHAPE_BOX;
-        return shape;
-    }
     int symbol_vertex(const std::string& name, int shape, int line) {
         const std::string s = resolve(name);
         auto it = named_.find(s);
@@ -970,7 +1044,7 @@ private:
         const int shape = V[v].shape;
         for (size_t i = 0; i < s->next.size(); ++i) {
             const SuccessorP& n = s->next[i];
-            const int ns = next_shape(s->op, shape);
+            const int ns = next_shape_of(s->op, shape, i);
             const int e = (int)E.size();
             E.push_back(GEdge());
             E[e].from = v;

@@ -18,7 +18,7 @@ from .program import (AxiomParam, DispatchTable, Entry, Parameter, Philox, Proce
 
 B, Q = abi.BOX, abi.QUAD
 
-# src/interpreter/PGAFacade.cu:24-62 (Box/Quad only)
+# src/interpreter/PGAFacade.cu:24-130: Quad, Box, then the dynamic polygons and prisms
 INTERPRETER_PROCEDURES = ProcedureList([
     # QUAD
     Procedure(abi.DISCARD, Q), Procedure(abi.EXTRUDE, Q), Procedure(abi.GENERATE, Q, 0),
@@ -40,6 +40,18 @@ INTERPRETER_PROCEDURES = ProcedureList([
     Procedure(abi.SET_AS_DYNAMIC_CONCAVE_POLYGON, B),
     Procedure(abi.SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM, B),
     Procedure(abi.SUBDIV, B), Procedure(abi.SWAPSIZE, B), Procedure(abi.TRANSLATE, B),
+] + [
+    # the dynamic shapes of the list built with INTERPRETER_SUPPORT_ALL_SHAPES
+    # (PGAFacade.cu:65-130, 395-470), after Box as there
+    Procedure(o, s, g)
+    for s, first in ((abi.DYNAMIC_CONVEX_POLYGON, abi.EXTRUDE), (abi.DYNAMIC_POLYGON, abi.EXTRUDE),
+                     (abi.DYNAMIC_CONVEX_RIGHT_PRISM, abi.COMPSPLIT), (abi.DYNAMIC_RIGHT_PRISM, abi.COMPSPLIT))
+    for o, g in ([(abi.COMPSPLIT, 0), (abi.DISCARD, 0)] if first == abi.COMPSPLIT else [(abi.DISCARD, 0), (abi.EXTRUDE, 0)])
+    + [(abi.GENERATE, 0), (abi.GENERATE, 1), (abi.IF, 0), (abi.IFSIZELESS, 0), (abi.STOCHASTIC, 0),
+       (abi.REPEAT, 0), (abi.REPLICATE, 0), (abi.ROTATE, 0), (abi.SCALE, 0),
+       (abi.SET_AS_DYNAMIC_CONVEX_POLYGON, 0), (abi.SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM, 0),
+       (abi.SET_AS_DYNAMIC_CONCAVE_POLYGON, 0), (abi.SET_AS_DYNAMIC_CONCAVE_RIGHT_PRISM, 0),
+       (abi.SUBDIV, 0), (abi.SWAPSIZE, 0), (abi.TRANSLATE, 0)]
 ])
 
 

@@ -86,6 +86,9 @@ def grammar_lots(g, name, n, seed=7):
         sizes = np.stack([rng.uniform(20, 40, n), rng.uniform(40, 90, n), rng.uniform(20, 40, n)], 1)
     elif name == "mc_skyscraper":
         sizes = np.stack([rng.uniform(20, 40, n), 3.0 * rng.integers(10, 25, n) + 2.75, rng.uniform(20, 40, n)], 1)
+    elif name in ("balcony", "suburban_house"):   # the compiled scenes' AxiomTraits sizes
+        sizes = np.broadcast_to(scene_fixture()["Balcony" if name == "balcony" else "SuburbanHouse"]["axiom"]["size"],
+                                (n, 3))
     else:
         sizes = np.ones((n, 3))
     return g.axiom_records(seeds, sizes=sizes, positions=pos)

@@ -26,7 +26,8 @@ from tests.helpers import (TREE_COUNTS, assert_same_multiset, fixture_grammar, g
                            grammar_lots)
 
 REF_GRAMMARS = "/root/reference/src/interpreter/grammars"
-has_reference = pytest.mark.skipif(not os.path.isdir(REF_GRAMMARS), reason="reference grammars not present")
+HAVE_REF = os.path.isdir(REF_GRAMMARS)
+has_reference = pytest.mark.skipif(not HAVE_REF, reason="reference grammars not present")
 
 B, Q = abi.BOX, abi.QUAD
 
@@ -47,7 +48,7 @@ def unit_axiom(g, size=(1.0, 1.0, 1.0), pos=(0.0, 0.0, 0.0), seed=0.25):
 def test_procedure_list_is_the_interpreters():
     got = [p.key() for p in compiler.interpreter_procedures().procedures]
     assert got == [p.key() for p in grammars.INTERPRETER_PROCEDURES.procedures]
-    assert len(got) == 36
+    assert len(got) == 36 + 4 * 18
 
 
 MENGER = """
@@ -304,7 +305,10 @@ def test_generate_in_rule_uses_its_terminal_declaration():
     ("axiom Box A; A = CompSplit() { Discard() | Discard() | CompSplit() { Discard() | Discard() | Discard() } };",
      "unsupported procedure (CompSplit on Quad"),
     ("axiom Box A; A = Extrude(1) Discard();", "unsupported procedure (Extrude on Box"),
-    ("axiom Box A; A = SetAsDynamicConvexRightPrism((0,1),(1,0),(0,0)) Discard();", "not supported"),
+    ("axiom Box A; A = SetAsDynamicConvexRightPrism((0,1),(1,0)) Discard();", "at least 3 parameters"),
+    ("axiom Box A; A = IfCollides(1) { Discard() | Discard() };", "not supported"),
+    ("axiom Box A; A = SetAsDynamicConvexPolygon((0,1),(1,0),(0,0)) CompSplit() { Discard() | Discard() | Discard() };",
+     "unsupported procedure (CompSplit on DynamicConvexPolygon"),
     ("axiom Box A; A = B; B = A;", "cycle"),
     ("axiom Box A; A = Translate(ShapeNormal(X), 0, 0) Discard();", "only valid inside Exp"),
     ("axiom Box A; /* open", "unterminated comment"),
@@ -350,10 +354,51 @@ SUPPORTED = sorted(k for k, v in FIXTURE.items() if "refused" not in v)
 
 
 def test_fixture_covers_the_interpreter_grammars():
-    assert SUPPORTED == ["3d_tree_12_2", "3d_tree_4_3", "3d_tree_8_3", "commercial", "mc_skyscraper",
-                         "mc_spaceship", "menger_sponge_3"]
-    for name in ("balcony", "suburban_house"):
-        assert "SetAsDynamicConvexRightPrism is not supported" in FIXTURE[name]["refused"]
+    # all nine, balcony and suburban_house with the dynamic-shape procedures
+    # of the list built with INTERPRETER_SUPPORT_ALL_SHAPES
+    assert SUPPORTED == ["3d_tree_12_2", "3d_tree_4_3", "3d_tree_8_3", "balcony", "commercial", "mc_skyscraper",
+                         "mc_spaceship", "menger_sponge_3", "suburban_house"]
+
+
+@pytest.mark.parametrize("name", ["balcony", "suburban_house"])
+def test_polygon_grammars_match_the_compiled_scenes(name):
+    """balcony and suburban_house derive what the reference's own code
+    generator made of them (src/test_runner/Balcony.cuh, SuburbanHouse.cuh)
+    on the scenes' lots, terminal by terminal. The generator wrote the
+    roof triangle as (0, 0.5), (-0.499, -0.499), (0.499, -0.499); with the
+    grammar's (-0.5, -0.5), (0.5, -0.5) the suburban roofs' exact normal
+    tests (ShapeNormal(Y) == -1) come out differently for two sides a lot,
+    so that grammar is compared with the generator's vertices."""
+    from tests.helpers import SceneProgram, scene_axioms
+    scene = {"balcony": "Balcony", "suburban_house": "SuburbanHouse"}[name]
+    text = open(os.path.join(REF_GRAMMARS, name, "grammar.pga"), encoding="utf-8").read() if HAVE_REF else None
+    g = fixture_grammar(name)
+    ax = scene_axioms(scene, 6)
+    ref = derive(type(g)(SceneProgram(scene), [(0, B)], b""), ax)
+    ax["entry_index"] = g.axioms[0][0]
+    got = derive(g, ax)
+    c_got = [got.count(t) for t in range(abi.MAX_TERMINALS)]
+    c_ref = [ref.count(t) for t in range(abi.MAX_TERMINALS)]
+    if name == "balcony":
+        assert c_got == c_ref
+    else:
+        roof = 5
+        assert [c for t, c in enumerate(c_got) if t != roof] == [c for t, c in enumerate(c_ref) if t != roof]
+        assert c_got[roof] < c_ref[roof]
+        if text is not None:
+            g2 = compile_grammar(text.replace("(0,0.5),(-0.5,-0.5),(0.5,-0.5)", "(0,0.5),(-0.499,-0.499),(0.499,-0.499)"))
+            ax["entry_index"] = g2.axioms[0][0]
+            assert [derive(g2, ax).count(t) for t in range(abi.MAX_TERMINALS)] == c_ref
+
+
+def test_setas_orders_vertices_counter_clockwise():
+    """buildSetAs*Entry orders the vertices with orderVertices_CCW: the
+    bottom-left one first, the rest by the angle from it."""
+    g = compile_grammar("axiom Box A; terminal T (1, 0);"
+                        "A = SetAsDynamicConvexRightPrism((0.5, 0.5), (-0.5, 0.5), (0, -0.5), (0.5, -0.25)) T;")
+    setas = [e for e in g.program.table.entries if len(e.parameters) == 4][0]
+    assert [tuple(p.values[:2]) for p in setas.parameters] == [(0.0, -0.5), (0.5, -0.25), (0.5, 0.5), (-0.5, 0.5)]
+    assert all(p.type == abi.PT_VEC2 for p in setas.parameters)
 
 
 @has_reference
