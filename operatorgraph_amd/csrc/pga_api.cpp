@@ -1227,7 +1227,7 @@ static pga_status one_phase_only(pga_context* ctx, const char* what) {
     return PGA_OK;
 }
 // Dynamic polygons and prisms run on the interpreting kernels (eagerly or
-// captured): the specialised kernel and the tape do not take them.
+// captured) and the specialised kernel; the tape does not take them.
 static pga_status no_polygons(pga_context* ctx, const char* what) {
     if (ctx->polygons)
         return fail(ctx, PGA_ERR_UNSUPPORTED, std::string(what) + " does not run programs with dynamic polygons");
@@ -1418,7 +1418,6 @@ PGA_API pga_status pga_specialize(pga_context* ctx, uint32_t block_size, int32_t
     }
     if (!ctx->initialized) return fail(ctx, PGA_ERR_NOT_INITIALIZED, "no program loaded");
     if (pga_status st = one_phase_only(ctx, "the specialised kernel")) return st;
-    if (pga_status st = no_polygons(ctx, "the specialised kernel")) return st;
     if (block_size == 0) block_size = kDefaultBlockSize;
     if (!spec_block_size_supported(block_size))
         return fail(ctx, PGA_ERR_INVALID_ARG, "unsupported workgroup size");
@@ -1475,7 +1474,6 @@ PGA_API pga_status pga_specialize_check(const pga_procedure* procedures, uint32_
     std::string msg;
     pga_status st = build_program(ctx.get(), procedures, num_procedures, entries, num_entries, false);
     if (st == PGA_OK) st = one_phase_only(ctx.get(), "the specialised kernel");
-    if (st == PGA_OK) st = no_polygons(ctx.get(), "the specialised kernel");
     if (st != PGA_OK) {
         msg = ctx->error;
     } else {

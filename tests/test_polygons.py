@@ -177,15 +177,15 @@ def test_oracle_refusals():
 
 def test_library_validation():
     """The library's validation (pga_specialize_check validates as
-    pga_initialize does, then refuses polygon programs for the specialised
-    kernel): a vertex-list clash and too few or too many vertices are
-    errors; the roof and the generic-operator program pass validation."""
+    pga_initialize does, then compiles the specialised kernel): a
+    vertex-list clash and too few or too many vertices are errors; the roof
+    and the generic-operator program compile."""
     from operatorgraph_amd import engine as E
     st, log, _ = E.specialize_check(_two_lists())
     assert st == abi.ERR_UNSUPPORTED and "two different vertex lists" in log
     for prog in (PC.roof_program(), PC.polygon_ops_program(PC.regular(5)), PC.recap_program(PC.regular(4))):
         st, log, _ = E.specialize_check(prog)
-        assert st == abi.ERR_UNSUPPORTED and "dynamic polygons" in log, log
+        assert st == abi.OK, log
     for nv in (2, 17):
         prog = Program(PC.PROCEDURES, DispatchTable([
             Entry(PC.op(abi.SET_AS_DYNAMIC_CONVEX_POLYGON, PC.B), [Vec2(0.1 * i, 0.0) for i in range(nv)], [(1, 0)]),

@@ -1,8 +1,8 @@
 """GPU parity of dynamic polygons and right prisms (SetAs*, ComponentSplit of
 a prism, Extrude of a polygon, inverted caps; tests/polygon_cases.py): the
 interpreting kernels through the C ABI against the oracle, bit-exact
-instance multisets, eagerly and replayed from a captured graph; and the
-refusals of the specialised kernel and the tape."""
+instance multisets, eagerly, replayed from a captured graph and in the
+kernel specialised to the program; the tape refuses them."""
 import sys
 
 import numpy as np
@@ -94,16 +94,23 @@ def test_captured_graph_replays(E):
         eng.close()
 
 
-def test_refusals(E):
-    _note("polygon programs: specialise and tape refused")
+@pytest.mark.parametrize("name", ["roof", "recap", "polygon_ops", "prism16"])
+def test_specialised_kernel(name):
+    from tests.test_spec_gpu import spec_check
+    _note(f"polygon program {name}: specialised kernel")
+    prog = {"roof": PC.roof_program, "recap": lambda: PC.recap_program(PC.random_convex(7, np.random.default_rng(3))),
+            "polygon_ops": lambda: PC.polygon_ops_program(PC.regular(6)),
+            "prism16": lambda: PC.prism_program(PC.regular(16, phase=0.2))}[name]()
+    spec_check(prog, PC.boxes(512, 9), N_TERMS)
+
+
+def test_tape_refused(E):
+    _note("polygon programs: the tape refused")
     eng = E.Engine()
     try:
         eng.load(PC.roof_program()).set_axioms(PC.boxes(4, 0))
         for t in range(2):
             eng.bind(t, 64)
-        with pytest.raises(E.PGAError) as e:
-            eng.specialize()
-        assert e.value.status == abi.ERR_UNSUPPORTED
         eng.tape_reserve(1 << 12, 1 << 12)
         with pytest.raises(E.PGAError) as e:
             eng.forward_tape()

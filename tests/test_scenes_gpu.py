@@ -3,8 +3,7 @@ src/test_runner/*.cuh generate, lowered through include/pga/core/Core.h):
 both techniques and the kernel specialised to each table derive the
 oracle's instances bit for bit (canonical sort; transforms within 1e-5 of
 its FP64 twin), at the axiom sizes each scene's AxiomTraits give. The
-polygon scenes (triangular right prisms) run on the interpreting kernels
-only: the specialised kernel refuses them."""
+polygon scenes (triangular right prisms) run the same way."""
 import sys
 
 import pytest
@@ -66,13 +65,8 @@ def test_polygon_scene_interpreter(E, name, technique):
 
 
 @pytest.mark.parametrize("name", POLYGON_SCENES)
-def test_polygon_scene_not_specialised(E, name):
-    _note(f"scene {name}: the specialised kernel refuses it")
-    eng = E.Engine()
-    try:
-        eng.load(SceneProgram(name))
-        with pytest.raises(E.PGAError) as e:
-            eng.specialize()
-        assert e.value.status == abi.ERR_UNSUPPORTED
-    finally:
-        eng.close()
+def test_polygon_scene_specialised(name):
+    ax = scene_axioms(name, 32)
+    n_terms, cap = _terminals(name, ax)
+    _note(f"scene {name}: {len(ax)} lots, specialised kernel")
+    spec_check(SceneProgram(name), ax, n_terms, cap=cap)
