@@ -443,6 +443,7 @@ __device__ __forceinline__ Ctx adj_ctx(const KArgs& a) {
     c.max_chain = a.max_chain;
     c.geom = 1u;
     c.seed = a.use_seed;
+    c.poly = 0u;             // the tape refuses dynamic polygons
     c.tape = 1u;
     c.prog_lds = 0u;
     c.ctxs = 0u;             // one-phase programs without Collider / IfCollides only

@@ -888,6 +888,7 @@ pga::KArgs make_args(pga_context* ctx, uint32_t first, uint32_t count) {
     a.max_chain = kMaxChain;
     a.count_geom = ctx->count_geom;
     a.use_seed = ctx->use_seed;
+    a.polygons = ctx->polygons ? 1u : 0u;
     a.dmask = ctx->nphases > 1 ? 0x0FFFu : 0xFFFFu;
     a.phase = 0;
     if (ctx->collision) {

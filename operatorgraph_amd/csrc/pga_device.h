@@ -124,6 +124,7 @@ struct Ctx {
     uint32_t max_chain;
     uint32_t geom;                      // 0: skip model-matrix arithmetic (counting only)
     uint32_t seed;                      // 0: no parameter reads a seed, the seed chain is skipped
+    uint32_t poly;                      // 1: the program has dynamic polygons (Rec::ed's invert bit)
     uint32_t tape;                      // 1: recording a tape (no in-place leaf expansion)
     uint32_t prog_lds;                  // 1: the program tables are the LDS copy (derivation kernels), 0: global
     // multi-phase programs: depth bits of Rec::ed (0xFFFF: one phase, 0x0FFF:

@@ -50,6 +50,7 @@ struct KArgs {
     uint32_t max_chain;
     uint32_t count_geom;                // 1: some decision reads the model matrix
     uint32_t use_seed;                  // 1: some parameter reads a shape seed
+    uint32_t polygons;                  // 1: the program has dynamic polygons or prisms
     // multi-phase programs (one launch per phase): Rec::ed depth bits
     // (0xFFFF: one phase), the phase derived, and the collider store with its
     // append counter and the count stored by earlier phases

@@ -86,11 +86,11 @@ __device__ __forceinline__ uint32_t claim(LevelLDS& L, bool wr, uint32_t k, unsi
 // The instance record of a terminal shape as four 16-byte rows: model4 *
 // scale(size) (rows 0-2), then custom attribute, axiom, path and entry |
 // depth << 16.
-__device__ __forceinline__ void instance_rows(const Rec& r, f4& q0, f4& q1, f4& q2, f4& q3) {
+__device__ __forceinline__ void instance_rows(const Rec& r, bool inv, f4& q0, f4& q1, f4& q2, f4& q3) {
     const float b[16] = {r.s[0], 0.0f, 0.0f, 0.0f, 0.0f, r.s[1], 0.0f, 0.0f,
                          0.0f, 0.0f, r.s[2], 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
     float mm[12];
-    if (r.ed & kInvert) {
+    if (inv) {
         // an inverted polygon: getAdjustedModel (DynamicPolygon.cuh:68-75)
         // negates the first two columns
         float am[12];
@@ -114,7 +114,9 @@ __device__ __forceinline__ void instance_rows(const Rec& r, f4& q0, f4& q1, f4& 
 __device__ __forceinline__ void emit_terminal(const Ctx& c, LevelLDS& L, f4* ST, bool wr, bool has, uint32_t slot,
                                               const Rec& r, unsigned long long& idx) {
     f4 q0 = {}, q1 = {}, q2 = {}, q3 = {};
-    if (wr && has) instance_rows(r, q0, q1, q2, q3);
+    // c.poly is a constant 0 in kernels specialised to a program without
+    // polygons: the inverted form is compiled out there
+    if (wr && has) instance_rows(r, c.poly && (r.ed & kInvert) != 0u, q0, q1, q2, q3);
     unsigned long long pending = __ballot(has);
     idx = 0;
     while (pending) {                          // one claim per distinct slot in the wave
@@ -287,6 +289,7 @@ __device__ __forceinline__ Ctx make_ctx(const KArgs& a, const LevelLDS& L, uint3
     c.max_chain = a.max_chain;
     c.geom = wr ? 1u : a.count_geom;
     c.seed = a.use_seed;
+    c.poly = a.polygons;
     c.ctxs = CS ? 1u : 0u;
     c.dmask = CS ? a.dmask : 0xFFFFu;
     c.phase = CS ? a.phase : 0u;

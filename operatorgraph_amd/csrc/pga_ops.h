@@ -92,7 +92,7 @@ __device__ __forceinline__ bool advance(const Ctx& c, Rec& r, int32_t next, uint
     }
     uint32_t depth = rec_depth(c, r) + 1u;
     if (depth > c.dmask) { raise(c, PGA_ERR_MAX_DEPTH); return false; }
-    r.ed = ((uint32_t)next & 0x7FFFu) | (r.ed & kInvert) | (depth << 16) | (cph << 28);
+    r.ed = ((uint32_t)next & 0x7FFFu) | (c.poly ? (r.ed & kInvert) : 0u) | (depth << 16) | (cph << 28);
     r.path = path_mix(r.path, ordinal);
     return true;
 }

@@ -109,8 +109,13 @@ std::string pga_spec_source(const std::vector<pga::DevEntry>& entries, uint32_t 
              ", sv2 = " + std::to_string(d.sv[2]) + ";\n};\n";
     }
     s += "struct Prog {\n";
+    bool polygons = false;   // a dynamic polygon or prism procedure: 6, 7, 14, 15
+    for (const pga::DevEntry& d : entries)
+        polygons |= d.shape == PGA_SHAPE_DYNAMIC_CONVEX_POLYGON || d.shape == PGA_SHAPE_DYNAMIC_POLYGON ||
+                    d.shape == PGA_SHAPE_DYNAMIC_CONVEX_RIGHT_PRISM || d.shape == PGA_SHAPE_DYNAMIC_RIGHT_PRISM;
     s += "  static constexpr uint32_t kCountGeom = " + std::to_string(count_geom) +
-         "u, kUseSeed = " + std::to_string(use_seed) + "u, kEntries = " + std::to_string(n) + "u;\n";
+         "u, kUseSeed = " + std::to_string(use_seed) + "u, kEntries = " + std::to_string(n) +
+         "u, kPolygons = " + (polygons ? "1u" : "0u") + ";\n";
     s += std::string("  static constexpr bool kGrouped = ") + (grouped ? "true" : "false") + ";\n";
     s += "  template <class F> __device__ __forceinline__ static void dispatch(const Ctx& c, uint64_t u, uint32_t i, "
          "F& f) {\n";

@@ -169,6 +169,7 @@ __device__ __forceinline__ void spec_pass(const KArgs& a, LevelLDS& L, float4* P
     Ctx c = make_ctx<false>(a, L, level, WR);   // one-phase programs without Collider / IfCollides
     c.geom = WR ? 1u : S::kCountGeom;     // constants: the count pass drops the unread arithmetic
     c.seed = S::kUseSeed;
+    c.poly = S::kPolygons;
     c.par_out = io.par_out;
     c.desc_out = io.desc_out;
     c.par_cap = io.par_cap;
