@@ -192,3 +192,17 @@ def test_library_validation():
             PC.gen(PC.CP, 0)]), "bad_setas")
         st, log, _ = E.specialize_check(prog)
         assert st == abi.ERR_BAD_PROGRAM, log
+
+
+def test_random_polygon_programs_derive():
+    """The seeded random polygon programs (tests/polygon_fuzz.py) derive in
+    the oracle; more than half of them split prisms."""
+    from oracle import oracle as O
+    from tests import polygon_fuzz as F
+    splits = 0
+    for seed in range(24):
+        p = F.random_program(seed)
+        O.derive(p, PC.boxes(16, seed))
+        splits += any(e.operator_code in (PC.op(abi.COMPSPLIT, PC.CR), PC.op(abi.COMPSPLIT, PC.R))
+                      for e in p.table.entries)
+    assert splits >= 12

@@ -119,3 +119,19 @@ def test_tape_refused(E):
         assert int(eng.counts(2)[0]) == 8
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("technique", TECHNIQUES)
+@pytest.mark.parametrize("seed", range(16))
+def test_random_polygon_programs(E, seed, technique):
+    from tests import polygon_fuzz as F
+    _note(f"random polygon program seed={seed} technique={technique}")
+    check(E, F.random_program(seed), PC.boxes(512, 300 + seed), technique)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_polygon_programs_specialised(seed):
+    from tests import polygon_fuzz as F
+    from tests.test_spec_gpu import spec_check
+    _note(f"random polygon program seed={seed}: specialised kernel")
+    spec_check(F.random_program(seed), PC.boxes(512, 300 + seed), N_TERMS)
