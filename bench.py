@@ -287,6 +287,37 @@ def context_workload(torch, engine, grammars, axioms, n_lots, steps):
             "cpu_oracle": {"lots": 256, "terminals": cpu_T, "terminals_per_s": round(cpu_T / cpu_sec), "cores": 1}}
 
 
+def houses_workload(torch, engine, grammars, axioms, n_lots, steps):
+    """Gabled houses (grammars.gabled_houses: dynamic-polygon roofs, a
+    triangular right prism split into gables and roof sides) over n_lots
+    city lots, autotuned over the interpreting and the specialised kernels;
+    the oracle's single-thread rate on 256 of the lots beside it."""
+    prog = grammars.gabled_houses()
+    lots = axioms.city_lots(n_lots)
+    eng = engine.Engine(queue_capacity=1 << 23)
+    eng.load(prog).set_axioms(lots)
+    for t in range(3):
+        eng.bind(t, 0)
+    eng.execute()
+    counts = [int(c) for c in eng.counts(3)]
+    for t in range(3):
+        eng.bind(t, counts[t])
+    tune = tune_engine(eng, 3)
+    sec = timed_steps(torch, eng, steps, tune["technique"])
+    eng.close()
+    T = sum(counts)
+    from oracle import oracle as O
+    sample = axioms.city_lots(256)
+    t0 = time.perf_counter()
+    r = O.derive(prog, sample)
+    cpu_sec = time.perf_counter() - t0
+    cpu_T = sum(len(r.instances(t)) for t in range(3))
+    return {"lots": n_lots, "walls": counts[0], "gables": counts[1], "roof_sides": counts[2],
+            "terminals_per_step": T, "ms_per_step": round(sec * 1e3, 4), "terminals_per_s": round(T / sec),
+            "kernel": kernel_name(tune), "autotune": tune_summary(tune),
+            "cpu_oracle": {"lots": 256, "terminals": cpu_T, "terminals_per_s": round(cpu_T / cpu_sec), "cores": 1}}
+
+
 def context_steps(torch, engine, grammars, axioms, n_lots, steps, warmup):
     """The context-sensitive city block alone, `steps` timed executes (for
     profilers)."""
@@ -784,6 +815,10 @@ def main():
     if args.context_lots:
         context = context_workload(torch, engine, grammars, axioms, args.context_lots, max(3, args.steps // 4))
 
+    houses = None
+    if args.context_lots:
+        houses = houses_workload(torch, engine, grammars, axioms, args.context_lots, max(3, args.steps // 4))
+
     grad = None
     if args.gradient_lots:
         grad = gradient_workload(torch, engine, grammars, axioms, args.gradient_lots, max(5, args.steps // 2))
@@ -822,6 +857,7 @@ def main():
         "mixed_fwd_bwd_menger5_16384_lbvh": mixed_grad,
         "triangle_soup_16384": mesh,
         "context_city_16384": context,
+        "gabled_houses_16384": houses,
         "cpu_baseline": cpu,
     }
     torch.cuda.synchronize()

@@ -14,7 +14,7 @@
 """
 from . import abi
 from .program import (AxiomParam, DispatchTable, Entry, Parameter, Philox, Procedure, Rand,
-                      ProcedureList, Program, Scalar)
+                      ProcedureList, Program, Scalar, ShapeAttr)
 
 B, Q = abi.BOX, abi.QUAD
 
@@ -256,6 +256,39 @@ def _cop(op_type, shape, gen_func=0):
     if idx < 0:
         raise KeyError((op_type, shape, gen_func))
     return idx
+
+
+def gabled_houses():
+    """A house per lot with a gabled roof, over the interpreter procedures:
+
+        Lot    = SubDiv(Y) { -0.7: Body | -0.3: Roof };
+        Body   = CompSplit() { Discard() | Discard() | Repeat(X, 2, ADJUST_TO_FILL) Wall };
+        Roof   = SetAsDynamicConvexRightPrism((-0.5, -0.5), (0.5, -0.5), (0, 0.5))
+                 Rotate(90, 0, 0) Scale(ShapeSize(X), ShapeSize(Z), ShapeSize(Y))
+                 CompSplit() { Gable | Gable | RoofSide };
+        Wall (1, 0); Gable (1, 1); RoofSide (1, 2)
+
+    (the roof of the balcony grammar, RoofStructure1, on the top 30 % of the
+    lot): every lot gives two gables and three roof sides, and
+    2 max(1, floor(w / 2)) + 2 max(1, floor(d / 2)) wall tiles."""
+    B_, Q_ = abi.BOX, abi.QUAD
+    CR, CP = abi.DYNAMIC_CONVEX_RIGHT_PRISM, abi.DYNAMIC_CONVEX_POLYGON
+    tri = [(-0.5, -0.5), (0.5, -0.5), (0.0, 0.5)]
+    e = [
+        subdiv_entry(B_, 1, [-0.7, -0.3], [1, 2]),                                              # 0 Lot
+        Entry(op(abi.COMPSPLIT, B_), [], [(3, 0), (3, 0), (4, 0)]),                             # 1 Body
+        Entry(op(abi.SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM, B_), [Parameter(abi.PT_VEC2, [x, y]) for x, y in tri],
+              [(6, 0)]),                                                                        # 2 Roof
+        Entry(op(abi.DISCARD, Q_), [], []),                                                     # 3
+        Entry(op(abi.REPEAT, Q_), [Scalar(0.0), Scalar(2.0), Scalar(float(abi.ADJUST_TO_FILL))], [(5, 0)]),  # 4
+        Entry(op(abi.GENERATE, Q_, 1), [Scalar(0.0)], []),                                      # 5 Wall
+        Entry(op(abi.ROTATE, CR), [Scalar(90.0), Scalar(0.0), Scalar(0.0)], [(7, 0)]),          # 6
+        Entry(op(abi.SCALE, CR), [ShapeAttr(12), ShapeAttr(14), ShapeAttr(13)], [(8, 0)]),      # 7
+        Entry(op(abi.COMPSPLIT, CR), [], [(9, 0), (9, 0), (10, 0)]),                            # 8
+        Entry(op(abi.GENERATE, CP, 1), [Scalar(1.0)], []),                                      # 9 Gable
+        Entry(op(abi.GENERATE, Q_, 1), [Scalar(2.0)], []),                                      # 10 RoofSide
+    ]
+    return Program(INTERPRETER_PROCEDURES, DispatchTable(e), "gabled_houses")
 
 
 def context_city(walls=False):

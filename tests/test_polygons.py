@@ -206,3 +206,21 @@ def test_random_polygon_programs_derive():
         splits += any(e.operator_code in (PC.op(abi.COMPSPLIT, PC.CR), PC.op(abi.COMPSPLIT, PC.R))
                       for e in p.table.entries)
     assert splits >= 12
+
+
+def test_gabled_houses():
+    """grammars.gabled_houses on city lots: 2 max(1, floor(w / 2)) +
+    2 max(1, floor(d / 2)) wall tiles, two gables and three roof sides a
+    lot; the gables stand at the lot's front and back faces, up to the top."""
+    from oracle import oracle as O
+    from operatorgraph_amd import axioms, grammars
+    lots = axioms.city_lots(64)
+    r = O.derive(grammars.gabled_houses(), lots)
+    w, d = lots["size"][:, 0], lots["size"][:, 2]
+    walls = int(np.sum(2 * np.maximum(1, np.floor(w / 2)) + 2 * np.maximum(1, np.floor(d / 2))))
+    assert [len(r.instances(t)) for t in range(3)] == [walls, 2 * 64, 3 * 64]
+    g = r.instances(1)
+    _, _, c2, c3 = cols(g)
+    a = g["axiom"]
+    np.testing.assert_allclose(np.abs(c2[:, 2]), 1.0, atol=1e-6)     # gables face along z
+    np.testing.assert_allclose(np.abs(c3[:, 2] - lots["model"][a, 11]), d[a] / 2, rtol=1e-5)

@@ -135,3 +135,18 @@ def test_random_polygon_programs_specialised(seed):
     from tests.test_spec_gpu import spec_check
     _note(f"random polygon program seed={seed}: specialised kernel")
     spec_check(F.random_program(seed), PC.boxes(512, 300 + seed), N_TERMS)
+
+
+@pytest.mark.parametrize("technique", TECHNIQUES)
+def test_gabled_houses(E, technique):
+    from operatorgraph_amd import axioms, grammars
+    _note(f"gabled houses, 1024 lots, technique={technique}")
+    ref = check(E, grammars.gabled_houses(), axioms.city_lots(1024), technique)
+    assert len(ref.instances(1)) == 2048
+
+
+def test_gabled_houses_specialised():
+    from operatorgraph_amd import axioms, grammars
+    from tests.test_spec_gpu import spec_check
+    _note("gabled houses, 1024 lots, specialised kernel")
+    spec_check(grammars.gabled_houses(), axioms.city_lots(1024), 3)
