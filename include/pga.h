@@ -558,8 +558,10 @@ PGA_API pga_status pga_tessellate_device_count(const pga_instance* instances, co
  *    (src/compiler/Parser.cpp:494-540, src/compiler/DispatchTableGenerator.cpp:555-609)
  *    as driven by the interpreter (src/interpreter/PGAFacade.cu:600-660).
  *    pga_interpreter_procedures copies the interpreter's procedure list
- *    (src/interpreter/PGAFacade.cu:24-62) to out (capacity entries; out may
- *    be NULL to query the count). pga_compile_grammar always sets *out to a
+ *    (src/interpreter/PGAFacade.cu:24-130: Quad and Box, then the dynamic
+ *    polygons and prisms of the list built with INTERPRETER_SUPPORT_ALL_SHAPES;
+ *    108 procedures) to out (capacity entries; out may be NULL to query the
+ *    count). pga_compile_grammar always sets *out to a
  *    grammar object that the caller destroys; on error it holds no entries
  *    and pga_grammar_error gives the message (with the line number). The
  *    entries and axioms stay valid until pga_grammar_destroy. An axiom whose
