@@ -95,7 +95,7 @@ def test_rotated_axioms_move_every_face_rigidly():
 @pytest.mark.parametrize("seed", range(4))
 def test_inverted_cap_splits_into_reversed_edges(seed):
     """The bottom cap's extrusion is an inverted prism: its sides run over
-    the same edges (the same widths), in reverse order and mirrored."""
+    the same edges (the same widths), in reverse order."""
     from oracle import oracle as O
     rng = np.random.default_rng(seed)
     verts = PC.random_convex(int(rng.integers(3, 17)), rng)
@@ -113,12 +113,20 @@ def test_inverted_cap_splits_into_reversed_edges(seed):
     wu = np.linalg.norm(c0[up], axis=1)
     wd = np.linalg.norm(c0[down], axis=1)
     np.testing.assert_allclose(np.sort(wu), np.sort(wd), rtol=1e-5)
-    # in derivation order side j of the inverted prism is edge n-1-j+1 of
-    # the other's (getVertex reverses the list): the width sequences are
-    # reverses of each other up to a rotation
-    seq_u = wu[np.argsort(sides["path"][up], kind="stable")]
-    seq_d = wd[np.argsort(sides["path"][down], kind="stable")]
-    assert len(seq_u) == len(seq_d)
+    # side j of each second-level split, by its path (path_mix of the
+    # ordinals: the SetAs child, cap n (bottom) or n + 1 (top), the
+    # extrusion, side j): the top cap's prism runs edge (j-1, j) as the
+    # first split did, the inverted one edge (n-1-j, n-j), since getVertex
+    # reverses the list, so its widths are the others' at (n - j) mod n
+    from oracle.oracle import path_mix
+    p1 = path_mix(0, 0)
+    width = dict(zip(sides["path"].tolist(), np.linalg.norm(c0, axis=1)))
+    top = [width[path_mix(path_mix(path_mix(p1, n + 1), 0), j)] for j in range(n)]
+    bottom = [width[path_mix(path_mix(path_mix(p1, n), 0), j)] for j in range(n)]
+    first = r.instances(0)
+    w_first = dict(zip(first["path"].tolist(), np.linalg.norm(cols(first)[0], axis=1)))
+    np.testing.assert_allclose(top, [w_first[path_mix(p1, j)] for j in range(n)], rtol=1e-6)
+    np.testing.assert_allclose(bottom, [top[(n - j) % n] for j in range(n)], rtol=1e-6)
 
 
 def test_roof_counts():
