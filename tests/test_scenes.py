@@ -168,12 +168,9 @@ def test_polygon_scenes_derive(name):
 def test_scene_tables_load_and_specialise_without_a_device(name):
     """The library's validation accepts the commercial scene, whose
     expression operands carry 12.99997 for attribute 13 (read rounded, as
-    the reference reads them), and compiles its specialised kernel; the
-    polygon scenes are refused as unsupported. (The GPU tests load and
-    specialise every scene.)"""
+    the reference reads them), and the polygon scenes (one vertex list per
+    prism entry), and compiles their specialised kernels. (The GPU tests
+    load and specialise every scene.)"""
     from operatorgraph_amd import engine
     st, log, _ = engine.specialize_check(SceneProgram(name), 512)
-    if name in POLYGON_SCENES:
-        assert st == abi.ERR_UNSUPPORTED, log
-    else:
-        assert st == 0, log
+    assert st == 0, log
