@@ -232,3 +232,30 @@ def test_gabled_houses():
     a = g["axiom"]
     np.testing.assert_allclose(np.abs(c2[:, 2]), 1.0, atol=1e-6)     # gables face along z
     np.testing.assert_allclose(np.abs(c3[:, 2] - lots["model"][a, 11]), d[a] / 2, rtol=1e-5)
+
+
+def test_refused_polygon_forms():
+    """Triangle soup (generation function 0) and the collision operators of
+    polygon shapes are refused on load (PGA_ERR_UNSUPPORTED), in the oracle
+    too; so is a parallel ComponentSplit of a prism."""
+    from oracle import oracle as O
+    from operatorgraph_amd import engine as E
+    from operatorgraph_amd.program import Procedure, ProcedureList
+    procs = ProcedureList([Procedure(abi.SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM, PC.B),
+                           Procedure(abi.GENERATE, PC.CR, 0), Procedure(abi.COLLIDER, PC.CR),
+                           Procedure(abi.GENERATE, PC.CR, 1), Procedure(abi.COMPSPLIT, PC.CR, 0, abi.PROC_PARALLEL),
+                           Procedure(abi.GENERATE, PC.CP, 1), Procedure(abi.GENERATE, PC.Q, 1)])
+    tri = [Vec2(0.0, 0.5), Vec2(-0.5, -0.5), Vec2(0.5, -0.5)]
+    cases = {
+        "generation function 0": [Entry(0, tri, [(1, 0)]), Entry(1, [Scalar(0.0)], [])],
+        "collision": [Entry(0, tri, [(1, 0)]), Entry(2, [Scalar(1.0)], [(2, 0)]), Entry(3, [Scalar(0.0)], [])],
+        "parallel": [Entry(0, tri, [(1, 0)]), Entry(4, [], [(2, 0), (2, 0), (3, 0)]), Entry(5, [Scalar(0.0)], []),
+                     Entry(6, [Scalar(1.0)], [])],
+    }
+    for what, entries in cases.items():
+        prog = Program(procs, DispatchTable(entries), what)
+        st, log, _ = E.specialize_check(prog)
+        assert st == abi.ERR_UNSUPPORTED, (what, log)
+        with pytest.raises(O.OracleError) as e:
+            O.derive(prog, PC.boxes(2, 0))
+        assert e.value.status == abi.ERR_UNSUPPORTED, what
