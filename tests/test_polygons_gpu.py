@@ -150,3 +150,38 @@ def test_gabled_houses_specialised():
     from tests.test_spec_gpu import spec_check
     _note("gabled houses, 1024 lots, specialised kernel")
     spec_check(grammars.gabled_houses(), axioms.city_lots(1024), 3)
+
+
+def _two_phase_roofs():
+    """Phase 0 stores each lot as a collider (tag 1) and emits it; phase 1
+    tests the lot against the colliders and, where it collides (its own
+    box always does), turns it into a triangular prism and splits it: the
+    invert bit of the bottom cap beside the phase bits of a multi-phase
+    record."""
+    from operatorgraph_amd.program import Procedure, ProcedureList
+    procs = ProcedureList([Procedure(abi.REPLICATE, PC.B), Procedure(abi.COLLIDER, PC.B),
+                           Procedure(abi.GENERATE, PC.B, 1), Procedure(abi.IFCOLLIDES, PC.B),
+                           Procedure(abi.SET_AS_DYNAMIC_CONVEX_RIGHT_PRISM, PC.B), Procedure(abi.DISCARD, PC.B),
+                           Procedure(abi.COMPSPLIT, PC.CR), Procedure(abi.GENERATE, PC.CP, 1),
+                           Procedure(abi.GENERATE, PC.Q, 1)])
+    from operatorgraph_amd.program import DispatchTable, Entry, Program, Scalar, Vec2
+    tri = [Vec2(0.0, 0.5), Vec2(-0.5, -0.5), Vec2(0.5, -0.5)]
+    return Program(procs, DispatchTable([
+        Entry(0, [], [(1, 0), (3, 1)]),                 # 0 Replicate: phase 0 | phase 1
+        Entry(1, [Scalar(1.0)], [(2, 0)]),              # 1 Collider(1)
+        Entry(2, [Scalar(0.0)], []),                    # 2 Generate 0
+        Entry(3, [Scalar(1.0)], [(4, 1), (5, 1)]),      # 3 IfCollides(1)
+        Entry(4, tri, [(6, abi.PHASE_INHERIT)]),        # 4 SetAs prism
+        Entry(5, [], []),                               # 5 Discard
+        Entry(6, [], [(7, abi.PHASE_INHERIT), (7, abi.PHASE_INHERIT), (8, abi.PHASE_INHERIT)]),  # 6 split
+        Entry(7, [Scalar(1.0)], []),                    # 7 caps
+        Entry(8, [Scalar(2.0)], []),                    # 8 sides
+    ]), "two_phase_roofs")
+
+
+@pytest.mark.parametrize("technique", TECHNIQUES)
+def test_polygons_in_a_later_phase(E, technique):
+    _note(f"two-phase roofs technique={technique}")
+    ref = check(E, _two_phase_roofs(), PC.boxes(256, 21), technique)
+    assert [len(ref.instances(t)) for t in range(3)] == [256, 512, 768]
+    assert np.all(ref.instances(1)["meta"] >> 28 == 1)
